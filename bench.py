@@ -1,0 +1,120 @@
+"""Benchmark: particle-updates/s of the plain FP64 explicit MPM step.
+
+Workload: dam break, 128x64x64 cells (dx = 1/64), 524 288 particles
+(block 0.5x1x0.5, 2x2x2 per cell), Tait EOS K = 2e6, gamma = 7, mu = 1e-3,
+cubic B-splines, no-slip walls, CFL 0.3, USL, FP64, synthetic scene.
+One step = the full time-loop body (dt, P2G, grid update + BCs, G2P, move,
+redistribute, volume, stress).  value = particles x steps / seconds of the
+timed region, inputs resident in HBM.
+
+The cpu_baseline leg times the CPU restatement of the reference step
+(oracle/, OpenMP, all allowed host cores) on a bounded sample of the same
+workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "particle-updates/sec per full MPM step; learned vs plain at equal error"
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic bytes (FP64, each persistent field read/written once per kernel)
+P2G_B_PER_P, P2G_B_PER_N = 112, 56
+GRID_B_PER_N = 104
+PART_B_PER_P, PART_B_PER_N = 312 + 216, 48
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--scheme", default="USL", choices=["USL", "MUSL"])
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import numpy as np
+    from exagoop_amd import capi, scenes
+
+    scheme = capi.USL if args.scheme == "USL" else capi.MUSL
+    s, pos, r, idata = scenes.dam_break(scheme=scheme)
+    npart = pos.shape[1]
+    nn = (s.ncells[0] + 1) * (s.ncells[1] + 1) * (s.ncells[2] + 1)
+    print(f"[bench] dam break {npart} particles, {nn} nodes, {args.scheme}", file=sys.stderr,
+          flush=True)
+    g = capi.Context(s)
+    g.upload(pos, r, idata)
+    g.init_internal_forces()
+    g.step(args.warmup)
+    t0 = time.perf_counter()
+    g.step(args.steps)  # returns after the stream is synchronised
+    wall = time.perf_counter() - t0
+    ev_ms, _ = g.last_step_timing()
+    secs = ev_ms / 1e3
+    value = npart * args.steps / secs
+    print(f"[bench] {args.steps} steps: {ev_ms:.3f} ms (events), {wall * 1e3:.3f} ms (wall)",
+          file=sys.stderr, flush=True)
+
+    # per-kernel averages on the library's stream (separate, profiled pass)
+    g.set_option(capi.OPT_PROFILE, 1)
+    g.step(min(args.steps, 20))
+    _, kms = g.last_step_timing()
+    g.set_option(capi.OPT_PROFILE, 0)
+    kern = {
+        "p2g": (kms[0], P2G_B_PER_P * npart + P2G_B_PER_N * nn),
+        "grid": (kms[1], GRID_B_PER_N * nn),
+        "g2p_move_stress": (kms[2], PART_B_PER_P * npart + PART_B_PER_N * nn),
+    }
+    dom = max(kern, key=lambda k: kern[k][0])
+    dms, dbytes = kern[dom]
+    achieved = dbytes / (dms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                "kernel": dom,
+                "kernels_ms": {k: round(v[0], 5) for k, v in kern.items()},
+                "kernels_gbs": {k: round(v[1] / (v[0] * 1e-3) / 1e9, 2) for k, v in kern.items()}}
+
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+        threads = min(16, os.cpu_count() or 1)
+        o = oracle.Oracle(s, threads=threads)
+        o.upload(pos, r, idata)
+        o.init_internal_forces()
+        o.step(1)
+        t0 = time.perf_counter()
+        o.step(args.cpu_steps)
+        ct = time.perf_counter() - t0
+        cpu = {"value": round(npart * args.cpu_steps / ct, 1), "unit": "particle-updates/s",
+               "cores": threads, "kind": "port",
+               "sample": f"full dam break ({npart} particles), {args.cpu_steps} steps after "
+                         f"1 warm-up step, oracle OpenMP {threads} threads, {ct:.1f} s"}
+        print(f"[bench] cpu baseline {cpu['value']:.0f} p-upd/s on {threads} threads",
+              file=sys.stderr, flush=True)
+
+    out = {
+        "metric": METRIC, "value": round(value, 1), "unit": "particle-updates/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ev_ms / args.steps, 5), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "dam break plain FP64", "cells": list(s.ncells),
+                   "particles": npart, "nodes": nn, "order": 3, "scheme": args.scheme,
+                   "constitutive": "Tait EOS K=2e6 gamma=7 mu=1e-3", "cfl": 0.3},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

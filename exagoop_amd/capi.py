@@ -1,0 +1,307 @@
+"""ctypes binding of the exagoop_amd C ABI (include/exagoop_amd.h).
+
+Loads the in-tree libexagoop_amd.so.  There is no fallback: a missing or
+unloadable library raises ImportError, and every non-zero status code raises
+MPMError with the library's error string.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libexagoop_amd.so")
+
+NREAL = 46
+NINT = 3
+
+# Particle real fields (mpm_specs.H:17-73)
+R_RADIUS, R_VEL, R_VELP, R_STRAINRATE, R_STRAIN, R_STRESS, R_DEFGRAD = 0, 1, 4, 7, 13, 19, 25
+R_VOLUME, R_MASS, R_DENSITY, R_JACOBIAN, R_PRESSURE, R_VOLINIT = 34, 35, 36, 37, 38, 39
+R_E, R_NU, R_BULK, R_GAMMA, R_VISC, R_YACC = 40, 41, 42, 43, 44, 45
+I_PHASE, I_RIGID_ID, I_MODEL = 0, 1, 2
+
+# Nodal components (constants.H:36-55)
+N_MASS, N_VEL, N_FRC, N_DVEL, N_MASS_OLD = 0, 1, 4, 7, 10
+
+BC_PERIODIC, BC_NOSLIP, BC_SLIP, BC_PARTIALSLIP, BC_OUTFLOW = 0, 1, 2, 3, 4
+USL, MUSL = 0, 1
+
+OPT_PROFILE, OPT_KEEP_FORCES = 1, 2
+
+STATUS = {0: "OK", -1: "E_ARG", -2: "E_BC", -3: "E_ORDER", -4: "E_MODEL", -5: "E_OOM",
+          -6: "E_HIP", -7: "E_NAN", -8: "E_UNSUPPORTED", -9: "E_NODEVICE"}
+
+_d3 = ctypes.c_double * 3
+_i3 = ctypes.c_int * 3
+_d9 = ctypes.c_double * 9
+
+
+class Specs(ctypes.Structure):
+    """mpm_specs (include/exagoop_amd.h); mirrors the MPMspecs step subset."""
+    _fields_ = [
+        ("plo", _d3), ("phi", _d3), ("ncells", _i3), ("periodic", _i3),
+        ("order_scheme", ctypes.c_int), ("num_redist", ctypes.c_int),
+        ("gravity", _d3), ("external_loads", ctypes.c_int),
+        ("force_slab_lo", _d3), ("force_slab_hi", _d3), ("extforce", _d3),
+        ("mass_tolerance", ctypes.c_double), ("alpha_pic_flip", ctypes.c_double),
+        ("stress_update_scheme", ctypes.c_int), ("fixed_timestep", ctypes.c_int),
+        ("timestep", ctypes.c_double), ("CFL", ctypes.c_double),
+        ("dt_min_limit", ctypes.c_double), ("dt_max_limit", ctypes.c_double),
+        ("applied_strainrate", ctypes.c_double), ("applied_strainrate_time", ctypes.c_double),
+        ("calculate_strain_based_on_delta", ctypes.c_int),
+        ("bclo", _i3), ("bchi", _i3), ("wall_mu_lo", _d3), ("wall_mu_hi", _d3),
+        ("wall_vel_lo", _d9), ("wall_vel_hi", _d9),
+    ]
+
+    def copy(self):
+        s = Specs()
+        ctypes.pointer(s)[0] = self
+        return s
+
+
+def default_specs():
+    """Reference MPMspecs defaults (mpm_specs.H:78-220), built in Python so the
+    oracle can be driven without the HIP library."""
+    s = Specs()
+    s.gravity[:] = [0.0, -9.81, 0.0]
+    s.num_redist = 10
+    s.order_scheme = 1
+    s.mass_tolerance = 1e-8
+    s.alpha_pic_flip = 0.95
+    s.stress_update_scheme = USL
+    s.CFL = 0.1
+    s.dt_min_limit = 1e-12
+    s.dt_max_limit = 1.0
+    for d in range(3):
+        s.bclo[d] = BC_SLIP
+        s.bchi[d] = BC_SLIP
+        s.wall_mu_lo[d] = 0.1
+        s.wall_mu_hi[d] = 0.1
+    return s
+
+
+UDF_FN = ctypes.CFUNCTYPE(None, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                          ctypes.c_double, ctypes.POINTER(ctypes.c_double))
+
+# Exported symbols and their signatures: name -> (restype, argtypes)
+_P = ctypes.c_void_p
+_DP = ctypes.POINTER(ctypes.c_double)
+_IP = ctypes.POINTER(ctypes.c_int)
+_LP = ctypes.POINTER(ctypes.c_int64)
+SIGNATURES = {
+    "mpm_specs_default": (None, [ctypes.POINTER(Specs)]),
+    "mpm_order_directional": (ctypes.c_int, [ctypes.POINTER(Specs), _IP]),
+    "mpm_create": (ctypes.c_int, [ctypes.POINTER(Specs), ctypes.POINTER(_P)]),
+    "mpm_destroy": (None, [_P]),
+    "mpm_upload_particles": (ctypes.c_int, [_P, ctypes.c_int64, _DP, _DP, _IP]),
+    "mpm_num_particles": (ctypes.c_int, [_P, _LP]),
+    "mpm_download_particles": (ctypes.c_int, [_P, _DP, _DP, _IP, _LP]),
+    "mpm_get_nodal": (ctypes.c_int, [_P, ctypes.c_int, _DP]),
+    "mpm_reset_nodal": (ctypes.c_int, [_P]),
+    "mpm_p2g": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "mpm_backup_velocity": (ctypes.c_int, [_P]),
+    "mpm_nodal_update": (ctypes.c_int, [_P, ctypes.c_double]),
+    "mpm_apply_nodal_bcs": (ctypes.c_int, [_P, ctypes.c_double, ctypes.c_double]),
+    "mpm_g2p": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_double]),
+    "mpm_move": (ctypes.c_int, [_P, ctypes.c_double, ctypes.c_int]),
+    "mpm_update_volume": (ctypes.c_int, [_P]),
+    "mpm_stress": (ctypes.c_int, [_P, ctypes.c_double, ctypes.c_double]),
+    "mpm_calculate_dt": (ctypes.c_int, [_P, _DP]),
+    "mpm_init_internal_forces": (ctypes.c_int, [_P]),
+    "mpm_step": (ctypes.c_int, [_P, ctypes.c_int]),
+    "mpm_get_time": (ctypes.c_int, [_P, _DP, _LP, _DP]),
+    "mpm_add_levelset_body": (ctypes.c_int, [_P, _DP, ctypes.c_int, ctypes.c_double, _DP]),
+    "mpm_set_wall_udf": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, UDF_FN]),
+    "mpm_energies": (ctypes.c_int, [_P, _DP]),
+    "mpm_set_option": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int]),
+    "mpm_last_step_timing": (ctypes.c_int, [_P, _DP, _DP]),
+    "mpm_last_error": (ctypes.c_char_p, []),
+}
+
+
+class MPMError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load():
+    """Load libexagoop_amd.so once; raise ImportError if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+    try:
+        lib = ctypes.CDLL(LIB_PATH)
+    except OSError as e:
+        raise ImportError(f"cannot load {LIB_PATH}: {e}") from e
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def dptr(a):
+    return a.ctypes.data_as(_DP)
+
+
+def iptr(a):
+    return a.ctypes.data_as(_IP)
+
+
+def lptr(a):
+    return a.ctypes.data_as(_LP)
+
+
+class _Base:
+    """Shared wrapper logic for the product context and the oracle context."""
+
+    prefix = "mpm_"
+
+    def _call(self, name, *args):
+        rc = getattr(self._lib, self.prefix + name)(*args)
+        if rc != 0:
+            raise MPMError(f"{self.prefix}{name} -> {STATUS.get(rc, rc)}: {self._errstr()}")
+        return rc
+
+    def _errstr(self):
+        return ""
+
+    def close(self):
+        if getattr(self, "_ctx", None):
+            getattr(self._lib, self.prefix + "destroy")(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, pos, rdata, idata):
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        rdata = np.ascontiguousarray(rdata, dtype=np.float64)
+        idata = np.ascontiguousarray(idata, dtype=np.int32)
+        np_ = pos.shape[1]
+        assert pos.shape == (3, np_) and rdata.shape == (NREAL, np_) and idata.shape == (NINT, np_)
+        self._call("upload_particles", self._ctx, np_, dptr(pos), dptr(rdata), iptr(idata))
+
+    def num_particles(self):
+        n = ctypes.c_int64(0)
+        self._call("num_particles", self._ctx, ctypes.byref(n))
+        return n.value
+
+    def download(self):
+        n = self.num_particles()
+        pos = np.zeros((3, n))
+        rdata = np.zeros((NREAL, n))
+        idata = np.zeros((NINT, n), dtype=np.int32)
+        ids = np.zeros(n, dtype=np.int64)
+        if n:
+            self._call("download_particles", self._ctx, dptr(pos), dptr(rdata), iptr(idata),
+                       lptr(ids))
+        return pos, rdata, idata, ids
+
+    def nodal(self, comp):
+        out = np.zeros(self.nn)
+        self._call("get_nodal", self._ctx, comp, dptr(out))
+        return out
+
+    def step(self, n=1):
+        self._call("step", self._ctx, n)
+
+    def init_internal_forces(self):
+        self._call("init_internal_forces", self._ctx)
+
+    def reset_nodal(self):
+        self._call("reset_nodal", self._ctx)
+
+    def p2g(self, um, uv, uf):
+        self._call("p2g", self._ctx, um, uv, uf)
+
+    def backup_velocity(self):
+        self._call("backup_velocity", self._ctx)
+
+    def nodal_update(self, dt):
+        self._call("nodal_update", self._ctx, dt)
+
+    def apply_nodal_bcs(self, dt, t):
+        self._call("apply_nodal_bcs", self._ctx, dt, t)
+
+    def g2p(self, uv, us, dt):
+        self._call("g2p", self._ctx, uv, us, dt)
+
+    def move(self, dt, step):
+        self._call("move", self._ctx, dt, step)
+
+    def update_volume(self):
+        self._call("update_volume", self._ctx)
+
+    def stress(self, dt, time):
+        self._call("stress", self._ctx, dt, time)
+
+    def calculate_dt(self):
+        v = ctypes.c_double(0)
+        self._call("calculate_dt", self._ctx, ctypes.byref(v))
+        return v.value
+
+    def time(self):
+        t = ctypes.c_double(0)
+        s = ctypes.c_int64(0)
+        dt = ctypes.c_double(0)
+        self._call("get_time", self._ctx, ctypes.byref(t), ctypes.byref(s), ctypes.byref(dt))
+        return t.value, s.value, dt.value
+
+    def add_levelset_body(self, phi, bc, mu, wall_vel=(0.0, 0.0, 0.0)):
+        phi = np.ascontiguousarray(phi, dtype=np.float64).ravel()
+        assert phi.size == self.nn
+        w = np.asarray(wall_vel, dtype=np.float64)
+        self._call("add_levelset_body", self._ctx, dptr(phi), bc, mu, dptr(w))
+
+    def set_wall_udf(self, d, hi, fn):
+        """fn(x, y, z, t) -> (vx, vy, vz); kept alive by this object."""
+        def _cb(x, y, z, t, out):
+            v = fn(x, y, z, t)
+            out[0], out[1], out[2] = v[0], v[1], v[2]
+        cb = UDF_FN(_cb)
+        self._udf_refs = getattr(self, "_udf_refs", [])
+        self._udf_refs.append(cb)
+        self._call("set_wall_udf", self._ctx, d, hi, cb)
+
+    def energies(self):
+        out = np.zeros(6)
+        self._call("energies", self._ctx, dptr(out))
+        return out
+
+
+class Context(_Base):
+    """An exagoop_amd context on the current HIP device."""
+
+    prefix = "mpm_"
+
+    def __init__(self, specs):
+        self._lib = load()
+        self.specs = specs.copy()
+        ctx = ctypes.c_void_p()
+        self._ctx = None
+        self._call("create", ctypes.byref(specs), ctypes.byref(ctx))
+        self._ctx = ctx
+        n = specs.ncells
+        self.nn = (n[0] + 1) * (n[1] + 1) * (n[2] + 1)
+
+    def _errstr(self):
+        return self._lib.mpm_last_error().decode()
+
+    def set_option(self, opt, value):
+        self._call("set_option", self._ctx, opt, value)
+
+    def last_step_timing(self):
+        tot = ctypes.c_double(0)
+        k = np.zeros(4)
+        self._call("last_step_timing", self._ctx, ctypes.byref(tot), dptr(k))
+        return tot.value, k

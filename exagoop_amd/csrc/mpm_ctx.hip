@@ -1,0 +1,669 @@
+// mpm_ctx.hip — context, device storage and the C ABI of exagoop_amd.h.
+//
+// The context owns particle SoA buffers (component-major, reference field
+// order), the nodal arrays of the valid nodal box, P2G accumulators and the
+// device step scalars (dt, time).  Every entry point enqueues on the
+// context's stream; dt and time stay on the device so a step needs no host
+// round trip unless a wall-velocity UDF is set (the reference evaluates UDFs
+// on the host every step too, nodal_data_ops.cpp:929-1059).
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <numeric>
+#include <vector>
+
+#include "mpm_internal.h"
+
+using namespace xmpm;
+
+namespace {
+
+thread_local char g_err[256] = "";
+
+int set_err(int code, const char *what) {
+    std::snprintf(g_err, sizeof(g_err), "%s", what);
+    return code;
+}
+
+#define HIPCHK(expr)                                                           \
+    do {                                                                       \
+        hipError_t e_ = (expr);                                                \
+        if (e_ != hipSuccess) {                                                \
+            std::snprintf(g_err, sizeof(g_err), "%s: %s", #expr,               \
+                          hipGetErrorString(e_));                              \
+            return e_ == hipErrorOutOfMemory ? MPM_E_OOM : MPM_E_HIP;         \
+        }                                                                      \
+    } while (0)
+
+}  // namespace
+
+struct mpm_ctx {
+    mpm_specs s;
+    GridP g;
+    PhysP ph;
+    LsP ls;
+    UdfP udf;
+    PState P;
+    NState N;
+    Scalars *sc = nullptr;
+    double *d_energy = nullptr;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[8];
+    std::vector<double *> ls_bufs;
+    double *udf_dev[3][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};
+    mpm_wall_udf_fn udf_fn[3][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};
+    bool any_udf = false;
+    bool any_per = false;
+    bool dirty = false;  // particles moved since the last redistribute
+    int64_t steps = 0;
+    double time_host = 0.0;
+    double last_dt = 0.0;
+    double step_ms = 0.0;
+    double kernel_ms[4] = {0, 0, 0, 0};
+    int profile = 0;
+    int keep_forces = 1;
+};
+
+namespace {
+
+void free_particles(mpm_ctx *c) {
+    (void)hipFree(c->P.pos);
+    (void)hipFree(c->P.r);
+    (void)hipFree(c->P.idata);
+    (void)hipFree(c->P.id);
+    (void)hipFree(c->P.vsnap);
+    c->P.pos = c->P.r = c->P.vsnap = nullptr;
+    c->P.idata = nullptr;
+    c->P.id = nullptr;
+    c->P.np = c->P.cap = 0;
+}
+
+int upload_udf_tables(mpm_ctx *c, double t) {
+    // compute_udf_wall_vel_at_nodes (nodal_data_ops.cpp:929-1059), 3-D.
+    for (int d = 0; d < 3; ++d)
+        for (int h = 0; h < 2; ++h) {
+            if (!c->udf_fn[d][h]) continue;
+            const int p0 = (d == 0) ? 1 : 0;
+            const int p1 = (d == 2) ? 1 : 2;
+            std::vector<double> tab((size_t)(c->g.n[p0] + 1) * (c->g.n[p1] + 1) * 3, 0.0);
+            for (int j = 0; j <= c->g.n[p0]; ++j)
+                for (int k = 0; k <= c->g.n[p1]; ++k) {
+                    double co[3] = {0, 0, 0};
+                    co[d] = h ? c->s.phi[d] : c->s.plo[d];
+                    co[p0] = c->s.plo[p0] + j * c->g.dx[p0];
+                    co[p1] = c->s.plo[p1] + k * c->g.dx[p1];
+                    double vel[3] = {0, 0, 0};
+                    c->udf_fn[d][h](co[0], co[1], co[2], t, vel);
+                    const size_t f = ((size_t)j * (c->g.n[p1] + 1) + k) * 3;
+                    for (int q = 0; q < 3; ++q) tab[f + q] = vel[q];
+                }
+            HIPCHK(hipMemcpyAsync(c->udf_dev[d][h], tab.data(), tab.size() * sizeof(double),
+                                  hipMemcpyHostToDevice, c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+        }
+    return MPM_OK;
+}
+
+int check_launch() {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        std::snprintf(g_err, sizeof(g_err), "kernel launch: %s", hipGetErrorString(e));
+        return MPM_E_HIP;
+    }
+    return MPM_OK;
+}
+
+int grid_flags_full(const mpm_ctx *c) {
+    return G_FINISH_MASS | G_FINISH_VEL | G_FINISH_FRC | G_BACKUP | G_UPDATE | G_BCS |
+           (c->keep_forces ? G_KEEP_FRC : 0);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *mpm_last_error(void) { return g_err; }
+
+void mpm_specs_default(mpm_specs *s) {
+    std::memset(s, 0, sizeof(*s));
+    // mpm_specs.H:78-220 and read_mpm_specs defaults (:223-565)
+    s->gravity[1] = -9.81;
+    s->num_redist = 10;
+    s->order_scheme = 1;
+    s->mass_tolerance = 1e-8;
+    s->alpha_pic_flip = 0.95;
+    s->stress_update_scheme = MPM_USL;
+    s->CFL = 0.1;
+    s->dt_min_limit = 1e-12;
+    s->dt_max_limit = 1.0;
+    for (int d = 0; d < 3; ++d) {
+        s->bclo[d] = MPM_BC_SLIP;
+        s->bchi[d] = MPM_BC_SLIP;
+        s->wall_mu_lo[d] = 0.1;
+        s->wall_mu_hi[d] = 0.1;
+    }
+}
+
+int mpm_order_directional(const mpm_specs *s, int od[3]) {
+    if (!s || !od) return MPM_E_ARG;
+    // Initialise_Domain (mpm_init.cpp:180-233)
+    for (int d = 0; d < 3; ++d) {
+        if (s->order_scheme == 1) {
+            od[d] = 1;
+        } else if (s->order_scheme == 2 || s->order_scheme == 3) {
+            const int nc = s->ncells[d];
+            od[d] = (s->periodic[d] == 0) ? ((nc < 5) ? 1 : s->order_scheme)
+                                          : ((nc < 3) ? 1 : s->order_scheme);
+        } else {
+            return set_err(MPM_E_ORDER, "order_scheme must be 1, 2 or 3");
+        }
+    }
+    return MPM_OK;
+}
+
+int mpm_create(const mpm_specs *s, mpm_ctx **out) {
+    if (!s || !out) return set_err(MPM_E_ARG, "null argument");
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return set_err(MPM_E_NODEVICE, "no HIP device");
+    int od[3];
+    int rc = mpm_order_directional(s, od);
+    if (rc) return rc;
+    for (int d = 0; d < 3; ++d) {
+        if (s->ncells[d] < 1 || !(s->phi[d] > s->plo[d]))
+            return set_err(MPM_E_ARG, "bad domain");
+        if (s->bclo[d] < 0 || s->bclo[d] > MPM_BC_OUTFLOW || s->bchi[d] < 0 ||
+            s->bchi[d] > MPM_BC_OUTFLOW)
+            return set_err(MPM_E_BC, "unknown boundary condition");
+        // applybc aborts on BC_OUTFLOW at the first nodal-BC pass
+        // (mpm_kernels.H:107-110).
+        if (s->bclo[d] == MPM_BC_OUTFLOW || s->bchi[d] == MPM_BC_OUTFLOW)
+            return set_err(MPM_E_BC, "outflow BC is not supported by applybc");
+    }
+    if (s->stress_update_scheme != MPM_USL && s->stress_update_scheme != MPM_MUSL)
+        return set_err(MPM_E_ARG, "stress_update_scheme must be USL or MUSL");
+    mpm_ctx *c = new mpm_ctx();
+    c->s = *s;
+    GridP &g = c->g;
+    std::memset(&g, 0, sizeof(g));
+    g.uniform = (od[0] == od[1] && od[1] == od[2]);
+    for (int d = 0; d < 3; ++d) {
+        g.plo[d] = s->plo[d];
+        g.phi[d] = s->phi[d];
+        g.n[d] = s->ncells[d];
+        g.hi[d] = s->ncells[d] - 1;
+        g.dx[d] = (s->phi[d] - s->plo[d]) / (double)s->ncells[d];
+        g.dxi[d] = 1.0 / g.dx[d];
+        g.per[d] = s->periodic[d] ? 1 : 0;
+        g.od[d] = od[d];
+        double rhi = s->phi[d];
+        while ((int)std::floor((rhi - s->plo[d]) * g.dxi[d]) >= g.n[d])
+            rhi = std::nextafter(rhi, -INFINITY);
+        g.rhi[d] = rhi;
+        c->any_per |= (g.per[d] != 0);
+    }
+    g.nn = (int64_t)(g.n[0] + 1) * (g.n[1] + 1) * (g.n[2] + 1);
+    PhysP &ph = c->ph;
+    std::memset(&ph, 0, sizeof(ph));
+    for (int d = 0; d < 3; ++d) {
+        ph.grav[d] = s->gravity[d];
+        ph.slab_lo[d] = s->force_slab_lo[d];
+        ph.slab_hi[d] = s->force_slab_hi[d];
+        ph.extforce[d] = s->extforce[d];
+        ph.bclo[d] = s->bclo[d];
+        ph.bchi[d] = s->bchi[d];
+        ph.mu_lo[d] = s->wall_mu_lo[d];
+        ph.mu_hi[d] = s->wall_mu_hi[d];
+    }
+    for (int q = 0; q < 9; ++q) {
+        ph.wv_lo[q] = s->wall_vel_lo[q];
+        ph.wv_hi[q] = s->wall_vel_hi[q];
+    }
+    ph.extloads = s->external_loads;
+    ph.mass_tol = s->mass_tolerance;
+    ph.alpha = s->alpha_pic_flip;
+    ph.applied_rate = s->applied_strainrate;
+    ph.applied_time = s->applied_strainrate_time;
+    ph.delta = s->calculate_strain_based_on_delta == 1;
+    ph.cfl = s->CFL;
+    ph.dt_min = s->dt_min_limit;
+    ph.dt_max = s->dt_max_limit;
+    std::memset(&c->ls, 0, sizeof(c->ls));
+    std::memset(&c->udf, 0, sizeof(c->udf));
+    std::memset(&c->P, 0, sizeof(c->P));
+    std::memset(&c->N, 0, sizeof(c->N));
+
+    auto fail = [&](int code) {
+        mpm_destroy(c);
+        return code;
+    };
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+        return fail(set_err(MPM_E_HIP, "hipStreamCreate failed"));
+    for (int q = 0; q < 8; ++q)
+        if (hipEventCreate(&c->ev[q]) != hipSuccess) return fail(set_err(MPM_E_HIP, "event"));
+    const int64_t nn = g.nn;
+    NState &N = c->N;
+    if (hipMalloc(&N.acc, 7 * nn * sizeof(double)) != hipSuccess ||
+        hipMalloc(&N.mass, nn * sizeof(double)) != hipSuccess ||
+        hipMalloc(&N.vel, 3 * nn * sizeof(double)) != hipSuccess ||
+        hipMalloc(&N.frc, 3 * nn * sizeof(double)) != hipSuccess ||
+        hipMalloc(&N.dvel, 3 * nn * sizeof(double)) != hipSuccess ||
+        hipMalloc(&N.mold, nn * sizeof(double)) != hipSuccess ||
+        hipMalloc(&c->sc, sizeof(Scalars)) != hipSuccess ||
+        hipMalloc(&c->d_energy, 8 * sizeof(double)) != hipSuccess)
+        return fail(set_err(MPM_E_OOM, "nodal allocation failed"));
+    launch_zero(N.acc, 7 * nn, c->stream);
+    launch_zero(N.mass, nn, c->stream);
+    launch_zero(N.vel, 3 * nn, c->stream);
+    launch_zero(N.frc, 3 * nn, c->stream);
+    launch_zero(N.dvel, 3 * nn, c->stream);
+    launch_zero(N.mold, nn, c->stream);
+    launch_zero(reinterpret_cast<double *>(c->sc), sizeof(Scalars) / sizeof(double), c->stream);
+    if (hipStreamSynchronize(c->stream) != hipSuccess)
+        return fail(set_err(MPM_E_HIP, "init sync failed"));
+    *out = c;
+    return MPM_OK;
+}
+
+void mpm_destroy(mpm_ctx *c) {
+    if (!c) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    free_particles(c);
+    (void)hipFree(c->N.acc);
+    (void)hipFree(c->N.mass);
+    (void)hipFree(c->N.vel);
+    (void)hipFree(c->N.frc);
+    (void)hipFree(c->N.dvel);
+    (void)hipFree(c->N.mold);
+    (void)hipFree(c->sc);
+    (void)hipFree(c->d_energy);
+    for (double *p : c->ls_bufs) (void)hipFree(p);
+    for (int d = 0; d < 3; ++d)
+        for (int h = 0; h < 2; ++h) (void)hipFree(c->udf_dev[d][h]);
+    for (int q = 0; q < 8; ++q)
+        if (c->ev[q]) (void)hipEventDestroy(c->ev[q]);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int mpm_upload_particles(mpm_ctx *c, int64_t np, const double *pos, const double *rdata,
+                         const int *idata) {
+    if (!c || np < 0 || (np > 0 && (!pos || !rdata || !idata)))
+        return set_err(MPM_E_ARG, "bad particle arrays");
+    for (int64_t i = 0; i < np; ++i) {
+        const int m = idata[MPM_I_MODEL * np + i];
+        if (m < 0 || m > 2) return set_err(MPM_E_MODEL, "constitutive model must be 0, 1 or 2");
+        if (c->s.calculate_strain_based_on_delta == 1 && m != 0 &&
+            idata[MPM_I_PHASE * np + i] == 0)
+            return set_err(MPM_E_UNSUPPORTED,
+                           "delta strain update is implemented for model 0 only");
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    free_particles(c);
+    const int64_t cap = np > 0 ? np : 1;
+    PState &P = c->P;
+    HIPCHK(hipMalloc(&P.pos, 3 * cap * sizeof(double)));
+    HIPCHK(hipMalloc(&P.r, (int64_t)MPM_NREAL * cap * sizeof(double)));
+    HIPCHK(hipMalloc(&P.idata, (int64_t)MPM_NINT * cap * sizeof(int)));
+    HIPCHK(hipMalloc(&P.id, cap * sizeof(int64_t)));
+    if (c->any_per && c->s.stress_update_scheme == MPM_MUSL)
+        HIPCHK(hipMalloc(&P.vsnap, 3 * cap * sizeof(double)));
+    P.np = np;
+    P.cap = cap;
+    if (np > 0) {
+        HIPCHK(hipMemcpy(P.pos, pos, 3 * np * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(P.r, rdata, (int64_t)MPM_NREAL * np * sizeof(double),
+                         hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(P.idata, idata, (int64_t)MPM_NINT * np * sizeof(int),
+                         hipMemcpyHostToDevice));
+        std::vector<int64_t> ids(np);
+        std::iota(ids.begin(), ids.end(), (int64_t)0);
+        HIPCHK(hipMemcpy(P.id, ids.data(), np * sizeof(int64_t), hipMemcpyHostToDevice));
+        // InitParticles ends with Redistribute() (mpm_init.cpp:1166).
+        launch_redistribute(c->g, P, c->stream);
+        int rc = check_launch();
+        if (rc) return rc;
+    }
+    c->dirty = false;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return MPM_OK;
+}
+
+int mpm_num_particles(mpm_ctx *c, int64_t *np) {
+    if (!c || !np) return set_err(MPM_E_ARG, "null argument");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    std::vector<int64_t> ids(c->P.np);
+    if (c->P.np)
+        HIPCHK(hipMemcpy(ids.data(), c->P.id, c->P.np * sizeof(int64_t), hipMemcpyDeviceToHost));
+    int64_t n = 0;
+    for (int64_t v : ids) n += (v >= 0);
+    *np = n;
+    return MPM_OK;
+}
+
+int mpm_download_particles(mpm_ctx *c, double *pos, double *rdata, int *idata, int64_t *id) {
+    if (!c) return set_err(MPM_E_ARG, "null context");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const int64_t np = c->P.np, cap = c->P.cap;
+    if (np == 0) return MPM_OK;
+    std::vector<int64_t> ids(np);
+    HIPCHK(hipMemcpy(ids.data(), c->P.id, np * sizeof(int64_t), hipMemcpyDeviceToHost));
+    std::vector<int64_t> order;
+    order.reserve(np);
+    for (int64_t i = 0; i < np; ++i)
+        if (ids[i] >= 0) order.push_back(i);
+    std::sort(order.begin(), order.end(),
+              [&](int64_t a, int64_t b) { return ids[a] < ids[b]; });
+    const int64_t nl = (int64_t)order.size();
+    if (pos) {
+        std::vector<double> h(3 * cap);
+        HIPCHK(hipMemcpy(h.data(), c->P.pos, 3 * cap * sizeof(double), hipMemcpyDeviceToHost));
+        for (int d = 0; d < 3; ++d)
+            for (int64_t q = 0; q < nl; ++q) pos[d * nl + q] = h[d * cap + order[q]];
+    }
+    if (rdata) {
+        std::vector<double> h((size_t)MPM_NREAL * cap);
+        HIPCHK(hipMemcpy(h.data(), c->P.r, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+        for (int f = 0; f < MPM_NREAL; ++f)
+            for (int64_t q = 0; q < nl; ++q) rdata[f * nl + q] = h[f * cap + order[q]];
+    }
+    if (idata) {
+        std::vector<int> h((size_t)MPM_NINT * cap);
+        HIPCHK(hipMemcpy(h.data(), c->P.idata, h.size() * sizeof(int), hipMemcpyDeviceToHost));
+        for (int f = 0; f < MPM_NINT; ++f)
+            for (int64_t q = 0; q < nl; ++q) idata[f * nl + q] = h[f * cap + order[q]];
+    }
+    if (id)
+        for (int64_t q = 0; q < nl; ++q) id[q] = ids[order[q]];
+    return MPM_OK;
+}
+
+int mpm_get_nodal(mpm_ctx *c, int comp, double *out) {
+    if (!c || !out || comp < 0 || comp >= MPM_N_NCOMP) return set_err(MPM_E_ARG, "bad comp");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const int64_t nn = c->g.nn;
+    const double *src = nullptr;
+    if (comp == MPM_N_MASS) src = c->N.mass;
+    else if (comp >= MPM_N_VEL && comp < MPM_N_VEL + 3) src = c->N.vel + (comp - MPM_N_VEL) * nn;
+    else if (comp >= MPM_N_FRC && comp < MPM_N_FRC + 3) src = c->N.frc + (comp - MPM_N_FRC) * nn;
+    else if (comp >= MPM_N_DVEL && comp < MPM_N_DVEL + 3)
+        src = c->N.dvel + (comp - MPM_N_DVEL) * nn;
+    else if (comp == MPM_N_MASS_OLD) src = c->N.mold;
+    if (!src) {
+        std::memset(out, 0, nn * sizeof(double));
+        return MPM_OK;
+    }
+    HIPCHK(hipMemcpy(out, src, nn * sizeof(double), hipMemcpyDeviceToHost));
+    return MPM_OK;
+}
+
+int mpm_reset_nodal(mpm_ctx *c) {
+    if (!c) return set_err(MPM_E_ARG, "null context");
+    const int64_t nn = c->g.nn;
+    launch_zero(c->N.mass, nn, c->stream);
+    launch_zero(c->N.vel, 3 * nn, c->stream);
+    launch_zero(c->N.frc, 3 * nn, c->stream);
+    launch_zero(c->N.dvel, 3 * nn, c->stream);
+    launch_zero(c->N.mold, nn, c->stream);
+    return check_launch();
+}
+
+int mpm_p2g(mpm_ctx *c, int um, int uv, int uf) {
+    if (!c) return set_err(MPM_E_ARG, "null context");
+    if (uf < 0 || uf > 1) return set_err(MPM_E_ARG, "update_forces must be 0 or 1");
+    // The reference zeroes the selected components first (:368-403); the
+    // accumulators are already zero.
+    launch_p2g(c->g, c->ph, c->P, c->N, um, uv, uf, nullptr, c->stream);
+    int flags = (um ? G_FINISH_MASS : 0) | (uv ? G_FINISH_VEL : 0) |
+                (uf ? (G_FINISH_FRC | G_KEEP_FRC) : 0);
+    if (flags) launch_grid(c->g, c->ph, c->ls, c->udf, c->N, flags, 0.0, nullptr, c->stream);
+    return check_launch();
+}
+
+int mpm_backup_velocity(mpm_ctx *c) {
+    if (!c) return set_err(MPM_E_ARG, "null context");
+    launch_grid(c->g, c->ph, c->ls, c->udf, c->N, G_BACKUP, 0.0, nullptr, c->stream);
+    return check_launch();
+}
+
+int mpm_nodal_update(mpm_ctx *c, double dt) {
+    if (!c) return set_err(MPM_E_ARG, "null context");
+    launch_grid(c->g, c->ph, c->ls, c->udf, c->N, G_UPDATE, dt, nullptr, c->stream);
+    return check_launch();
+}
+
+int mpm_apply_nodal_bcs(mpm_ctx *c, double dt, double t) {
+    if (!c) return set_err(MPM_E_ARG, "null context");
+    if (c->any_udf) {
+        int rc = upload_udf_tables(c, t);
+        if (rc) return rc;
+    }
+    launch_grid(c->g, c->ph, c->ls, c->udf, c->N, G_BCS, dt, nullptr, c->stream);
+    return check_launch();
+}
+
+int mpm_g2p(mpm_ctx *c, int uv, int us, double dt) {
+    if (!c) return set_err(MPM_E_ARG, "null context");
+    int flags = (uv ? P_G2P_VEL : 0) | (us ? P_G2P_GRAD : 0);
+    if (flags)
+        launch_particle(c->g, c->ph, c->ls, c->udf, c->P, c->N, flags, dt, 0.0, nullptr,
+                        c->stream);
+    return check_launch();
+}
+
+int mpm_move(mpm_ctx *c, double dt, int step) {
+    if (!c) return set_err(MPM_E_ARG, "null context");
+    const int nr = c->s.num_redist > 0 ? c->s.num_redist : 1;
+    const bool redist = (step % nr) == 0;
+    launch_particle(c->g, c->ph, c->ls, c->udf, c->P, c->N, P_MOVE | (redist ? P_REDIST : 0), dt,
+                    0.0, nullptr, c->stream);
+    c->dirty = !redist;
+    return check_launch();
+}
+
+int mpm_update_volume(mpm_ctx *c) {
+    if (!c) return set_err(MPM_E_ARG, "null context");
+    launch_particle(c->g, c->ph, c->ls, c->udf, c->P, c->N, P_VOLUME, 0.0, 0.0, nullptr,
+                    c->stream);
+    return check_launch();
+}
+
+int mpm_stress(mpm_ctx *c, double dt, double time) {
+    if (!c) return set_err(MPM_E_ARG, "null context");
+    launch_particle(c->g, c->ph, c->ls, c->udf, c->P, c->N, P_STRESS, dt, time, nullptr,
+                    c->stream);
+    return check_launch();
+}
+
+int mpm_calculate_dt(mpm_ctx *c, double *dt) {
+    if (!c || !dt) return set_err(MPM_E_ARG, "null argument");
+    launch_dt(c->g, c->ph, c->P, c->sc, c->s.fixed_timestep == 1, c->s.timestep, c->stream);
+    int rc = check_launch();
+    if (rc) return rc;
+    Scalars h;
+    HIPCHK(hipMemcpyAsync(&h, c->sc, sizeof(Scalars), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    *dt = h.dt;
+    return MPM_OK;
+}
+
+int mpm_init_internal_forces(mpm_ctx *c) {
+    if (!c) return set_err(MPM_E_ARG, "null context");
+    // Initialise_Internal_Forces (mpm_init.cpp:368-401): P2G(1,1,1), backup,
+    // G2P(update_vel=0, update_strainrate=1, dt=0), stress(dt=0, applied rate).
+    launch_p2g(c->g, c->ph, c->P, c->N, 1, 1, 1, nullptr, c->stream);
+    launch_grid(c->g, c->ph, c->ls, c->udf, c->N,
+                G_FINISH_MASS | G_FINISH_VEL | G_FINISH_FRC | G_KEEP_FRC | G_BACKUP, 0.0, nullptr,
+                c->stream);
+    launch_particle(c->g, c->ph, c->ls, c->udf, c->P, c->N, P_G2P_GRAD | P_STRESS_INIT, 0.0, 0.0,
+                    nullptr, c->stream);
+    return check_launch();
+}
+
+int mpm_step(mpm_ctx *c, int nsteps) {
+    if (!c || nsteps < 0) return set_err(MPM_E_ARG, "bad argument");
+    const int nr = c->s.num_redist > 0 ? c->s.num_redist : 1;
+    const bool musl = c->s.stress_update_scheme == MPM_MUSL;
+    hipStream_t st = c->stream;
+    const bool prof = c->profile != 0;
+    double kms[4] = {0, 0, 0, 0};
+    HIPCHK(hipEventRecord(c->ev[0], st));
+    for (int it = 0; it < nsteps; ++it) {
+        c->steps++;
+        const int step = (int)c->steps;
+        const bool redist = (step % nr) == 0;
+        // Redistribute_Fill_Update at the top of the step (main.cpp:88).
+        if (redist && c->dirty) launch_redistribute(c->g, c->P, st);
+        if (musl && c->P.vsnap) launch_snapshot_vel(c->P, st);
+        launch_dt(c->g, c->ph, c->P, c->sc, c->s.fixed_timestep == 1, c->s.timestep, st);
+        if (c->any_udf) {
+            Scalars h;
+            HIPCHK(hipMemcpyAsync(&h, c->sc, sizeof(Scalars), hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            int rc = upload_udf_tables(c, h.time);
+            if (rc) return rc;
+        }
+        if (prof) HIPCHK(hipEventRecord(c->ev[1], st));
+        launch_p2g(c->g, c->ph, c->P, c->N, 1, 1, 1, nullptr, st);
+        if (prof) HIPCHK(hipEventRecord(c->ev[2], st));
+        launch_grid(c->g, c->ph, c->ls, c->udf, c->N, grid_flags_full(c), 0.0, c->sc, st);
+        if (prof) HIPCHK(hipEventRecord(c->ev[3], st));
+        const int tail = P_MOVE | (redist ? P_REDIST : 0) | P_VOLUME | P_STRESS;
+        if (!musl) {
+            launch_particle(c->g, c->ph, c->ls, c->udf, c->P, c->N, P_G2P_VEL | P_G2P_GRAD | tail,
+                            0.0, 0.0, c->sc, st);
+        } else {
+            launch_particle(c->g, c->ph, c->ls, c->udf, c->P, c->N, P_G2P_VEL, 0.0, 0.0, c->sc, st);
+            launch_p2g(c->g, c->ph, c->P, c->N, 0, 1, 0, c->P.vsnap, st);
+            launch_grid(c->g, c->ph, c->ls, c->udf, c->N, G_FINISH_VEL | G_BCS, 0.0, c->sc, st);
+            launch_particle(c->g, c->ph, c->ls, c->udf, c->P, c->N, P_G2P_GRAD | tail, 0.0, 0.0,
+                            c->sc, st);
+        }
+        if (prof) HIPCHK(hipEventRecord(c->ev[4], st));
+        launch_advance_time(c->sc, st);
+        c->dirty = !redist;
+        int rc = check_launch();
+        if (rc) return rc;
+        if (prof) {
+            HIPCHK(hipEventSynchronize(c->ev[4]));
+            float a = 0, b = 0, d = 0;
+            (void)hipEventElapsedTime(&a, c->ev[1], c->ev[2]);
+            (void)hipEventElapsedTime(&b, c->ev[2], c->ev[3]);
+            (void)hipEventElapsedTime(&d, c->ev[3], c->ev[4]);
+            kms[0] += a;
+            kms[1] += b;
+            kms[2] += d;
+        }
+    }
+    HIPCHK(hipEventRecord(c->ev[5], st));
+    HIPCHK(hipEventSynchronize(c->ev[5]));
+    float tot = 0;
+    (void)hipEventElapsedTime(&tot, c->ev[0], c->ev[5]);
+    c->step_ms = tot;
+    for (int q = 0; q < 4; ++q) c->kernel_ms[q] = nsteps ? kms[q] / nsteps : 0.0;
+    Scalars h;
+    HIPCHK(hipMemcpy(&h, c->sc, sizeof(Scalars), hipMemcpyDeviceToHost));
+    c->time_host = h.time;
+    c->last_dt = h.dt;
+    return MPM_OK;
+}
+
+int mpm_get_time(mpm_ctx *c, double *time, int64_t *steps, double *last_dt) {
+    if (!c) return set_err(MPM_E_ARG, "null context");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    Scalars h;
+    HIPCHK(hipMemcpy(&h, c->sc, sizeof(Scalars), hipMemcpyDeviceToHost));
+    if (time) *time = h.time;
+    if (steps) *steps = c->steps;
+    if (last_dt) *last_dt = h.dt;
+    return MPM_OK;
+}
+
+int mpm_add_levelset_body(mpm_ctx *c, const double *phi, int bc, double mu,
+                          const double w[3]) {
+    if (!c || !phi) return set_err(MPM_E_ARG, "null argument");
+    if (c->ls.nb >= MPM_MAX_LS_BODIES) return set_err(MPM_E_ARG, "too many level-set bodies");
+    if (bc < 0 || bc > MPM_BC_OUTFLOW) return set_err(MPM_E_BC, "unknown body BC");
+    const GridP &g = c->g;
+    const int G0 = g.n[0] + 3, G1 = g.n[1] + 3, G2 = g.n[2] + 3;
+    // One ghost layer: 1.0 outside non-periodic faces, periodic image otherwise
+    // (nodal_data_ops.cpp:144-153).
+    std::vector<double> h((size_t)G0 * G1 * G2, 1.0);
+    for (int k = -1; k <= g.n[2] + 1; ++k)
+        for (int j = -1; j <= g.n[1] + 1; ++j)
+            for (int i = -1; i <= g.n[0] + 1; ++i) {
+                int I[3] = {i, j, k}, S[3];
+                bool ok = true;
+                for (int d = 0; d < 3; ++d) {
+                    S[d] = I[d];
+                    if (I[d] < 0 || I[d] > g.n[d]) {
+                        if (!g.per[d]) ok = false;
+                        else S[d] = I[d] < 0 ? I[d] + g.n[d] : I[d] - g.n[d];
+                    }
+                }
+                if (!ok) continue;
+                h[((size_t)(k + 1) * G1 + (j + 1)) * G0 + (i + 1)] =
+                    phi[((size_t)S[2] * (g.n[1] + 1) + S[1]) * (g.n[0] + 1) + S[0]];
+            }
+    double *d = nullptr;
+    HIPCHK(hipMalloc(&d, h.size() * sizeof(double)));
+    HIPCHK(hipMemcpy(d, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+    c->ls_bufs.push_back(d);
+    const int b = c->ls.nb++;
+    c->ls.phi[b] = d;
+    c->ls.bc[b] = bc;
+    c->ls.mu[b] = mu;
+    for (int q = 0; q < 3; ++q) c->ls.w[b][q] = w ? w[q] : 0.0;
+    return MPM_OK;
+}
+
+int mpm_set_wall_udf(mpm_ctx *c, int dir, int hi, mpm_wall_udf_fn fn) {
+    if (!c || dir < 0 || dir > 2 || hi < 0 || hi > 1) return set_err(MPM_E_ARG, "bad face");
+    c->udf_fn[dir][hi] = fn;
+    if (fn && !c->udf_dev[dir][hi]) {
+        const int p0 = (dir == 0) ? 1 : 0;
+        const int p1 = (dir == 2) ? 1 : 2;
+        size_t n = (size_t)(c->g.n[p0] + 1) * (c->g.n[p1] + 1) * 3;
+        HIPCHK(hipMalloc(&c->udf_dev[dir][hi], n * sizeof(double)));
+        HIPCHK(hipMemset(c->udf_dev[dir][hi], 0, n * sizeof(double)));
+    }
+    c->udf.tab[dir][hi] = fn ? c->udf_dev[dir][hi] : nullptr;
+    c->any_udf = false;
+    for (int d = 0; d < 3; ++d)
+        for (int h = 0; h < 2; ++h) c->any_udf |= (c->udf_fn[d][h] != nullptr);
+    return MPM_OK;
+}
+
+int mpm_energies(mpm_ctx *c, double out[6]) {
+    if (!c || !out) return set_err(MPM_E_ARG, "null argument");
+    launch_zero(c->d_energy, 6, c->stream);
+    launch_energies(c->P, c->d_energy, c->stream);
+    int rc = check_launch();
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(out, c->d_energy, 6 * sizeof(double), hipMemcpyDeviceToHost,
+                          c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return MPM_OK;
+}
+
+int mpm_set_option(mpm_ctx *c, int option, int value) {
+    if (!c) return set_err(MPM_E_ARG, "null context");
+    if (option == MPM_OPT_PROFILE) c->profile = value;
+    else if (option == MPM_OPT_KEEP_FORCES) c->keep_forces = value;
+    else return set_err(MPM_E_ARG, "unknown option");
+    return MPM_OK;
+}
+
+int mpm_last_step_timing(mpm_ctx *c, double *total_ms, double kernel_ms[4]) {
+    if (!c) return set_err(MPM_E_ARG, "null context");
+    if (total_ms) *total_ms = c->step_ms;
+    if (kernel_ms)
+        for (int q = 0; q < 4; ++q) kernel_ms[q] = c->kernel_ms[q];
+    return MPM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,119 @@
+// mpm_internal.h — device-side state and kernel parameter blocks shared by
+// the kernels (mpm_kernels.hip) and the context / C ABI (mpm_ctx.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/exagoop_amd.h"
+
+namespace xmpm {
+
+// Grid geometry (one box covering the domain, lo = 0).
+struct GridP {
+    double plo[3], phi[3], dx[3], dxi[3];
+    double rhi[3];  // largest position whose cell is n-1 (periodic wrap bound)
+    int n[3];       // cells per direction
+    int hi[3];      // n - 1
+    int per[3];
+    int od[3];      // per-direction order after the small-grid fallback
+    int uniform;    // all od equal
+    int64_t nn;     // nodes (n0+1)(n1+1)(n2+1)
+};
+
+// Step physics and wall data (MPMspecs subset).
+struct PhysP {
+    double grav[3];
+    int extloads;
+    double slab_lo[3], slab_hi[3], extforce[3];
+    double mass_tol;
+    double alpha;
+    int bclo[3], bchi[3];
+    double mu_lo[3], mu_hi[3];
+    double wv_lo[9], wv_hi[9];
+    double applied_rate, applied_time;
+    int delta;
+    double cfl, dt_min, dt_max;
+};
+
+struct LsP {
+    int nb;
+    const double *phi[MPM_MAX_LS_BODIES];  // (n0+3)(n1+3)(n2+3), one ghost layer
+    int bc[MPM_MAX_LS_BODIES];
+    double mu[MPM_MAX_LS_BODIES];
+    double w[MPM_MAX_LS_BODIES][3];
+};
+
+struct UdfP {
+    const double *tab[3][2];  // per face, ((n[p0]+1)(n[p1]+1)) x 3, or null
+};
+
+// Particle SoA: component-major with stride cap.
+struct PState {
+    double *pos;    // 3 * cap
+    double *r;      // MPM_NREAL * cap
+    int *idata;     // MPM_NINT * cap
+    int64_t *id;    // cap; < 0 = removed
+    double *vsnap;  // 3 * cap velocity at the last neighbour fill (MUSL images)
+    int64_t np;
+    int64_t cap;
+};
+
+// Nodal data on the valid nodal box, component-major with stride nn.
+struct NState {
+    double *acc;   // 7 * nn: P2G accumulators m, p0..2, f0..2 (kept zero between uses)
+    double *mass;  // nn
+    double *vel;   // 3 * nn
+    double *frc;   // 3 * nn
+    double *dvel;  // 3 * nn
+    double *mold;  // nn
+};
+
+// Device scalars of the step.
+struct Scalars {
+    double dt;
+    double time;
+    unsigned long long dtmin_bits;  // running min of the per-particle dt (positive doubles)
+    int err;
+    int pad;
+};
+
+// Grid-kernel stage flags.
+enum {
+    G_FINISH_MASS = 1,
+    G_FINISH_VEL = 2,
+    G_FINISH_FRC = 4,
+    G_BACKUP = 8,
+    G_UPDATE = 16,
+    G_BCS = 32,
+    G_KEEP_FRC = 64
+};
+
+// Particle-kernel stage flags.
+enum {
+    P_G2P_VEL = 1,
+    P_G2P_GRAD = 2,
+    P_MOVE = 4,
+    P_REDIST = 8,
+    P_VOLUME = 16,
+    P_STRESS = 32,
+    P_STRESS_INIT = 64  // apply_constitutive_model(dt=0, applied rate) of the init pass
+};
+
+// Launchers (mpm_kernels.hip).
+void launch_zero(double *p, int64_t n, hipStream_t s);
+void launch_p2g(const GridP &g, const PhysP &ph, const PState &P, const NState &N, int um,
+                int uv, int uf, const double *vimg, hipStream_t s);
+void launch_grid(const GridP &g, const PhysP &ph, const LsP &ls, const UdfP &udf,
+                 const NState &N, int flags, double dt, const Scalars *sc, hipStream_t s);
+void launch_particle(const GridP &g, const PhysP &ph, const LsP &ls, const UdfP &udf,
+                     const PState &P, const NState &N, int flags, double dt, double time,
+                     const Scalars *sc, hipStream_t s);
+void launch_dt(const GridP &g, const PhysP &ph, const PState &P, Scalars *sc, int fixed,
+               double fixed_dt, hipStream_t s);
+void launch_advance_time(Scalars *sc, hipStream_t s);
+void launch_energies(const PState &P, double *out6, hipStream_t s);
+void launch_snapshot_vel(const PState &P, hipStream_t s);
+void launch_redistribute(const GridP &g, const PState &P, hipStream_t s);
+
+}  // namespace xmpm

@@ -1,0 +1,259 @@
+/*
+ * exagoop_amd.h — C ABI of the MI355X-native explicit MPM step.
+ *
+ * Drop-in boundary for ExaGOOP's MPMParticleContainer hot path.  Every entry
+ * point below replaces one reference method or free function (cited per
+ * declaration, paths relative to the ExaGOOP source tree).  Plain C types
+ * only: pointers, sizes and status codes.  All entry points return
+ * MPM_OK (0) or a negative MPM_E* code instead of aborting the process
+ * (the reference calls amrex::Abort on the same conditions).
+ *
+ * Threading: one host thread per context; work is enqueued on the context's
+ * HIP stream and is stream-ordered.  Host buffers passed in are borrowed only
+ * for the duration of the call.  Device state is owned by the context.
+ */
+#ifndef EXAGOOP_AMD_H_
+#define EXAGOOP_AMD_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Status codes                                                             */
+/* ------------------------------------------------------------------------ */
+#define MPM_OK 0
+#define MPM_E_ARG -1        /* bad argument / null pointer / size mismatch */
+#define MPM_E_BC -2         /* unknown boundary condition (mpm_kernels.H:107-110) */
+#define MPM_E_ORDER -3      /* unsupported order_scheme (mpm_particle_grid_ops.cpp:293-296) */
+#define MPM_E_MODEL -4      /* invalid constitutive model (mpm_particle_timestep.cpp:76-80) */
+#define MPM_E_OOM -5        /* device allocation failed */
+#define MPM_E_HIP -6        /* HIP runtime error */
+#define MPM_E_NAN -7        /* non-finite state detected */
+#define MPM_E_UNSUPPORTED -8 /* combination the reference aborts on (e.g. delta model for fluids) */
+#define MPM_E_NODEVICE -9   /* no HIP device */
+
+/* ------------------------------------------------------------------------ */
+/* Particle record layout (mpm_specs.H:17-73, 3-D, USE_TEMP off)            */
+/* Exchange format is component-major: field f of particle i at [f*np + i]. */
+/* ------------------------------------------------------------------------ */
+#define MPM_NREAL 46
+#define MPM_NINT 3
+
+enum mpm_real_field {
+    MPM_R_RADIUS = 0,
+    MPM_R_VEL = 1,        /* 1..3  */
+    MPM_R_VELP = 4,       /* 4..6  grid velocity v' used for advection */
+    MPM_R_STRAINRATE = 7, /* 7..12 packed XX,XY,XZ,YY,YZ,ZZ */
+    MPM_R_STRAIN = 13,    /* 13..18 */
+    MPM_R_STRESS = 19,    /* 19..24 */
+    MPM_R_DEFGRAD = 25,   /* 25..33 row-major 3x3 */
+    MPM_R_VOLUME = 34,
+    MPM_R_MASS = 35,
+    MPM_R_DENSITY = 36,
+    MPM_R_JACOBIAN = 37,
+    MPM_R_PRESSURE = 38,
+    MPM_R_VOLINIT = 39,
+    MPM_R_E = 40,
+    MPM_R_NU = 41,
+    MPM_R_BULK = 42,
+    MPM_R_GAMMA = 43,
+    MPM_R_VISC = 44,
+    MPM_R_YACC = 45
+};
+
+enum mpm_int_field { MPM_I_PHASE = 0, MPM_I_RIGID_ID = 1, MPM_I_MODEL = 2 };
+
+/* Nodal component indices (constants.H:36-55). */
+enum mpm_nodal_comp {
+    MPM_N_MASS = 0,
+    MPM_N_VEL = 1,   /* 1..3 */
+    MPM_N_FRC = 4,   /* 4..6 */
+    MPM_N_DVEL = 7,  /* 7..9 */
+    MPM_N_MASS_OLD = 10,
+    MPM_N_NCOMP = 20
+};
+
+/* Boundary condition codes (constants.H:89-93). */
+enum mpm_bc {
+    MPM_BC_PERIODIC = 0,
+    MPM_BC_NOSLIP = 1,
+    MPM_BC_SLIP = 2,
+    MPM_BC_PARTIALSLIP = 3,
+    MPM_BC_OUTFLOW = 4
+};
+
+enum mpm_scheme { MPM_USL = 0, MPM_MUSL = 1 };
+
+/* ------------------------------------------------------------------------ */
+/* Problem specification: the subset of MPMspecs (mpm_specs.H:75-220) the    */
+/* step reads.  mpm_specs_default() fills the reference defaults.           */
+/* ------------------------------------------------------------------------ */
+typedef struct mpm_specs {
+    double plo[3];
+    double phi[3];
+    int ncells[3];
+    int periodic[3];
+    int order_scheme; /* 1 linear, 2 quadratic B-spline, 3 cubic B-spline */
+    int num_redist;   /* particle redistribution interval (utilities.cpp:420-434) */
+    double gravity[3];
+    int external_loads;
+    double force_slab_lo[3];
+    double force_slab_hi[3];
+    double extforce[3];
+    double mass_tolerance;
+    double alpha_pic_flip;
+    int stress_update_scheme; /* MPM_USL / MPM_MUSL */
+    int fixed_timestep;
+    double timestep;
+    double CFL;
+    double dt_min_limit;
+    double dt_max_limit;
+    double applied_strainrate;
+    double applied_strainrate_time;
+    int calculate_strain_based_on_delta;
+    int bclo[3];
+    int bchi[3];
+    double wall_mu_lo[3];
+    double wall_mu_hi[3];
+    double wall_vel_lo[9]; /* [dir*3 + comp] */
+    double wall_vel_hi[9];
+} mpm_specs;
+
+/* Wall-velocity user function, same signature as WallVelUDFFuncPtr
+ * (mpm_specs.H:12): f(x, y, z, t, vel[3]). */
+typedef void (*mpm_wall_udf_fn)(double, double, double, double, double *);
+
+typedef struct mpm_ctx mpm_ctx;
+
+/* Reference defaults of MPMspecs (mpm_specs.H:78-220). */
+void mpm_specs_default(mpm_specs *s);
+
+/* Per-direction order after the small-grid fallback (mpm_init.cpp:180-233):
+ * writes order_scheme_directional[3]; returns MPM_E_ORDER for order not in
+ * {1,2,3}. */
+int mpm_order_directional(const mpm_specs *s, int out_order[3]);
+
+/* ------------------------------------------------------------------------ */
+/* Context lifetime                                                         */
+/* ------------------------------------------------------------------------ */
+
+/* Replaces Initialise_Domain (mpm_init.cpp:108-299) + the
+ * MPMParticleContainer constructor (main.cpp:59): validates specs, picks
+ * per-direction orders, allocates nodal storage on the current device. */
+int mpm_create(const mpm_specs *specs, mpm_ctx **out);
+void mpm_destroy(mpm_ctx *ctx);
+
+/* Replaces InitParticles' hand-off to the container (mpm_init.cpp:942-1167,
+ * the particle tile fill + Redistribute).  pos is [3*np], rdata
+ * [MPM_NREAL*np], idata [MPM_NINT*np], all component-major. */
+int mpm_upload_particles(mpm_ctx *ctx, int64_t np, const double *pos,
+                         const double *rdata, const int *idata);
+/* Number of live particles (removed ones excluded). */
+int mpm_num_particles(mpm_ctx *ctx, int64_t *np);
+/* Copies live particles back in the upload's order (ids preserved):
+ * pos/rdata/idata as in upload, id[np] the upload index of each particle. */
+int mpm_download_particles(mpm_ctx *ctx, double *pos, double *rdata,
+                           int *idata, int64_t *id);
+/* Copies nodal component comp (mpm_nodal_comp) of the valid nodal box,
+ * (n0+1)(n1+1)(n2+1) values, i fastest. */
+int mpm_get_nodal(mpm_ctx *ctx, int comp, double *out);
+
+/* ------------------------------------------------------------------------ */
+/* MPMParticleContainer methods (mpm_particle_container.H:150-577)          */
+/* ------------------------------------------------------------------------ */
+
+/* Reset_Nodaldata_to_Zero (nodal_data_ops.cpp:924-927). */
+int mpm_reset_nodal(mpm_ctx *ctx);
+
+/* deposit_onto_grid_momentum (mpm_particle_grid_ops.cpp:333-628), including
+ * the v = p/m normalisation.  Flags as in the reference. */
+int mpm_p2g(mpm_ctx *ctx, int update_mass, int update_vel, int update_forces);
+
+/* backup_current_velocity (nodal_data_ops.cpp:57-81). */
+int mpm_backup_velocity(mpm_ctx *ctx);
+
+/* Nodal_Time_Update_Momentum (nodal_data_ops.cpp:451-482). */
+int mpm_nodal_update(mpm_ctx *ctx, double dt);
+
+/* Apply_Nodal_BCs (utilities.cpp:183-204): wall BCs, UDF wall BCs,
+ * level-set BCs, then store_delta_velocity.  t is the time the UDFs are
+ * evaluated at. */
+int mpm_apply_nodal_bcs(mpm_ctx *ctx, double dt, double t);
+
+/* interpolate_from_grid (mpm_particle_grid_ops.cpp:924-1098). */
+int mpm_g2p(mpm_ctx *ctx, int update_vel, int update_strainrate, double dt);
+
+/* moveParticles (mpm_particle_timestep.cpp:226-578) followed by the
+ * Redistribute_Fill_Update that follows it in the time loop
+ * (main.cpp:142, utilities.cpp:420-434): periodic wrap and removal of
+ * particles outside a non-periodic domain when step % num_redist == 0. */
+int mpm_move(mpm_ctx *ctx, double dt, int step);
+
+/* updateVolume (mpm_particle_timestep.cpp:131-182). */
+int mpm_update_volume(mpm_ctx *ctx);
+
+/* Calculate_MP_Stress_Strain (utilities.cpp:372-399) → apply_constitutive_model
+ * (mpm_particle_container.cpp:43-142) or _delta (:178-272). */
+int mpm_stress(mpm_ctx *ctx, double dt, double time);
+
+/* Calculate_time_step (mpm_particle_timestep.cpp:38-111). */
+int mpm_calculate_dt(mpm_ctx *ctx, double *dt);
+
+/* Initialise_Internal_Forces (mpm_init.cpp:368-401). */
+int mpm_init_internal_forces(mpm_ctx *ctx);
+
+/* nsteps iterations of the time loop body (main.cpp:83-187) without output
+ * and diagnostics: dt, P2G, grid update, BCs, USL/MUSL G2P, move,
+ * redistribute, volume, stress, time += dt. */
+int mpm_step(mpm_ctx *ctx, int nsteps);
+
+/* Current simulation time / step count kept by the context. */
+int mpm_get_time(mpm_ctx *ctx, double *time, int64_t *steps, double *last_dt);
+
+/* ------------------------------------------------------------------------ */
+/* Boundary data                                                            */
+/* ------------------------------------------------------------------------ */
+
+/* Level-set body (mpm_eb.H:176-288, nodal_data_ops.cpp:129-220): phi is the
+ * body's signed distance on the valid nodal box ((n0+1)(n1+1)(n2+1), i
+ * fastest, lsref = 1).  Up to MPM_MAX_LS_BODIES bodies. */
+#define MPM_MAX_LS_BODIES 8
+int mpm_add_levelset_body(mpm_ctx *ctx, const double *phi, int bc, double mu,
+                          const double wall_vel[3]);
+
+/* Wall-velocity UDF for face (dir, hi) (mpm_specs.H:452-485,
+ * nodal_data_ops.cpp:929-1175, mpm_particle_timestep.cpp:417-475).  The
+ * context evaluates it on the host at each BC pass as the reference does. */
+int mpm_set_wall_udf(mpm_ctx *ctx, int dir, int hi, mpm_wall_udf_fn fn);
+
+/* ------------------------------------------------------------------------ */
+/* Diagnostics (mpm_diagnostics.cpp:23-127)                                 */
+/* ------------------------------------------------------------------------ */
+
+/* out[0]=TKE, out[1]=TSE, out[2..4]=sum m*v, out[5]=sum m (all particles). */
+int mpm_energies(mpm_ctx *ctx, double out[6]);
+
+/* Context options without a reference counterpart (tooling for callers that
+ * measure or inspect the step): MPM_OPT_PROFILE records HIP events around
+ * each kernel of mpm_step (read back with mpm_last_step_timing);
+ * MPM_OPT_KEEP_FORCES stores nodal forces for mpm_get_nodal (default 1). */
+#define MPM_OPT_PROFILE 1
+#define MPM_OPT_KEEP_FORCES 2
+int mpm_set_option(mpm_ctx *ctx, int option, int value);
+
+/* Last HIP error string (static storage). */
+const char *mpm_last_error(void);
+
+/* Milliseconds of the last mpm_step call measured with HIP events on the
+ * context's stream, and per-kernel average durations (ms) over that call:
+ * kernel order is p2g, grid, g2p (fused particle kernel), sort/other. */
+int mpm_last_step_timing(mpm_ctx *ctx, double *total_ms, double kernel_ms[4]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* EXAGOOP_AMD_H_ */
