@@ -82,7 +82,8 @@ def main():
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
                 "kernel": dom,
                 "kernels_ms": {k: round(v[0], 5) for k, v in kern.items()},
-                "kernels_gbs": {k: round(v[1] / (v[0] * 1e-3) / 1e9, 2) for k, v in kern.items()}}
+                "kernels_gbs": {k: round(v[1] / (v[0] * 1e-3) / 1e9, 2) for k, v in kern.items()},
+                "sort_ms": round(kms[3], 5)}
 
     cpu = None
     if not args.no_cpu:

@@ -46,8 +46,16 @@ struct mpm_ctx {
     PhysP ph;
     LsP ls;
     UdfP udf;
-    PState P;
+    PState P;      // current particle buffers
+    PState Q;      // second buffer the sorted step writes into
     NState N;
+    SortState S;
+    // Cold particle fields (never read by the step), by upload index.
+    std::vector<double> cold_radius, cold_yacc;
+    std::vector<int> cold_rigid;
+    bool skip_ok = true;    // idle node tiles hold the all-zero massless state
+    bool all_dirty = true;  // next step's grid pass must visit every node
+    int sorted = 1;         // use the cell-sorted step
     Scalars *sc = nullptr;
     double *d_energy = nullptr;
     hipStream_t stream = nullptr;
@@ -69,16 +77,36 @@ struct mpm_ctx {
 
 namespace {
 
+void free_pstate(PState &P) {
+    (void)hipFree(P.pos);
+    (void)hipFree(P.r);
+    (void)hipFree(P.idata);
+    (void)hipFree(P.id);
+    P.pos = P.r = nullptr;
+    P.idata = nullptr;
+    P.id = nullptr;
+    P.np = P.cap = 0;
+}
+
 void free_particles(mpm_ctx *c) {
-    (void)hipFree(c->P.pos);
-    (void)hipFree(c->P.r);
-    (void)hipFree(c->P.idata);
-    (void)hipFree(c->P.id);
     (void)hipFree(c->P.vsnap);
-    c->P.pos = c->P.r = c->P.vsnap = nullptr;
-    c->P.idata = nullptr;
-    c->P.id = nullptr;
-    c->P.np = c->P.cap = 0;
+    c->P.vsnap = nullptr;
+    free_pstate(c->P);
+    free_pstate(c->Q);
+    (void)hipFree(c->S.key);
+    (void)hipFree(c->S.rank);
+    (void)hipFree(c->S.perm);
+    c->S.key = c->S.rank = c->S.perm = nullptr;
+}
+
+hipError_t alloc_pstate(PState &P, int64_t cap) {
+    hipError_t e;
+    if ((e = hipMalloc(&P.pos, 3 * cap * sizeof(double))) != hipSuccess) return e;
+    if ((e = hipMalloc(&P.r, (int64_t)MPM_NREAL * cap * sizeof(double))) != hipSuccess) return e;
+    if ((e = hipMalloc(&P.idata, (int64_t)MPM_NINT * cap * sizeof(int))) != hipSuccess) return e;
+    if ((e = hipMalloc(&P.id, cap * sizeof(int64_t))) != hipSuccess) return e;
+    P.cap = cap;
+    return hipSuccess;
 }
 
 int upload_udf_tables(mpm_ctx *c, double t) {
@@ -207,6 +235,10 @@ int mpm_create(const mpm_specs *s, mpm_ctx **out) {
         c->any_per |= (g.per[d] != 0);
     }
     g.nn = (int64_t)(g.n[0] + 1) * (g.n[1] + 1) * (g.n[2] + 1);
+    for (int d = 0; d < 3; ++d) {
+        g.ntc[d] = (g.n[d] + kTileCells - 1) / kTileCells;
+        g.ntn[d] = g.n[d] / kTileCells + 1;
+    }
     PhysP &ph = c->ph;
     std::memset(&ph, 0, sizeof(ph));
     for (int d = 0; d < 3; ++d) {
@@ -235,7 +267,13 @@ int mpm_create(const mpm_specs *s, mpm_ctx **out) {
     std::memset(&c->ls, 0, sizeof(c->ls));
     std::memset(&c->udf, 0, sizeof(c->udf));
     std::memset(&c->P, 0, sizeof(c->P));
+    std::memset(&c->Q, 0, sizeof(c->Q));
     std::memset(&c->N, 0, sizeof(c->N));
+    std::memset(&c->S, 0, sizeof(c->S));
+    for (int d = 0; d < 3; ++d)
+        c->skip_ok &= (s->wall_vel_lo[3 * d] == 0.0 && s->wall_vel_lo[3 * d + 1] == 0.0 &&
+                       s->wall_vel_lo[3 * d + 2] == 0.0 && s->wall_vel_hi[3 * d] == 0.0 &&
+                       s->wall_vel_hi[3 * d + 1] == 0.0 && s->wall_vel_hi[3 * d + 2] == 0.0);
 
     auto fail = [&](int code) {
         mpm_destroy(c);
@@ -256,6 +294,24 @@ int mpm_create(const mpm_specs *s, mpm_ctx **out) {
         hipMalloc(&c->sc, sizeof(Scalars)) != hipSuccess ||
         hipMalloc(&c->d_energy, 8 * sizeof(double)) != hipSuccess)
         return fail(set_err(MPM_E_OOM, "nodal allocation failed"));
+    SortState &S = c->S;
+    S.nkeys = g.ntc[0] * g.ntc[1] * g.ntc[2] * kTileCells * kTileCells * kTileCells;
+    S.scan_bytes = sort_scan_bytes(S.nkeys);
+    const int ntn = g.ntn[0] * g.ntn[1] * g.ntn[2];
+    if (hipMalloc(&S.count, (S.nkeys + 1) * sizeof(int)) != hipSuccess ||
+        hipMalloc(&S.cell_start, (S.nkeys + 1) * sizeof(int)) != hipSuccess ||
+        hipMalloc(&S.scan_tmp, S.scan_bytes > 0 ? S.scan_bytes : 16) != hipSuccess ||
+        hipMalloc(&S.tile_epoch, ntn * sizeof(int)) != hipSuccess)
+        return fail(set_err(MPM_E_OOM, "sort allocation failed"));
+    if (hipMemsetAsync(S.count, 0, (S.nkeys + 1) * sizeof(int), c->stream) != hipSuccess ||
+        hipMemsetAsync(S.cell_start, 0, (S.nkeys + 1) * sizeof(int), c->stream) != hipSuccess)
+        return fail(set_err(MPM_E_HIP, "sort init failed"));
+    {
+        std::vector<int> h(ntn, -1000);
+        if (hipMemcpy(S.tile_epoch, h.data(), ntn * sizeof(int), hipMemcpyHostToDevice) !=
+            hipSuccess)
+            return fail(set_err(MPM_E_HIP, "sort init failed"));
+    }
     launch_zero(N.acc, 7 * nn, c->stream);
     launch_zero(N.mass, nn, c->stream);
     launch_zero(N.vel, 3 * nn, c->stream);
@@ -281,6 +337,10 @@ void mpm_destroy(mpm_ctx *c) {
     (void)hipFree(c->N.mold);
     (void)hipFree(c->sc);
     (void)hipFree(c->d_energy);
+    (void)hipFree(c->S.count);
+    (void)hipFree(c->S.cell_start);
+    (void)hipFree(c->S.scan_tmp);
+    (void)hipFree(c->S.tile_epoch);
     for (double *p : c->ls_bufs) (void)hipFree(p);
     for (int d = 0; d < 3; ++d)
         for (int h = 0; h < 2; ++h) (void)hipFree(c->udf_dev[d][h]);
@@ -304,16 +364,24 @@ int mpm_upload_particles(mpm_ctx *c, int64_t np, const double *pos, const double
     }
     HIPCHK(hipStreamSynchronize(c->stream));
     free_particles(c);
+    if (np >= (int64_t)1 << 31) return set_err(MPM_E_ARG, "at most 2^31-1 particles");
     const int64_t cap = np > 0 ? np : 1;
     PState &P = c->P;
-    HIPCHK(hipMalloc(&P.pos, 3 * cap * sizeof(double)));
-    HIPCHK(hipMalloc(&P.r, (int64_t)MPM_NREAL * cap * sizeof(double)));
-    HIPCHK(hipMalloc(&P.idata, (int64_t)MPM_NINT * cap * sizeof(int)));
-    HIPCHK(hipMalloc(&P.id, cap * sizeof(int64_t)));
+    HIPCHK(alloc_pstate(P, cap));
+    HIPCHK(alloc_pstate(c->Q, cap));
+    HIPCHK(hipMalloc(&c->S.key, cap * sizeof(int)));
+    HIPCHK(hipMalloc(&c->S.rank, cap * sizeof(int)));
+    HIPCHK(hipMalloc(&c->S.perm, cap * sizeof(int)));
     if (c->any_per && c->s.stress_update_scheme == MPM_MUSL)
         HIPCHK(hipMalloc(&P.vsnap, 3 * cap * sizeof(double)));
+    c->Q.vsnap = P.vsnap;
     P.np = np;
-    P.cap = cap;
+    c->Q.np = np;
+    c->cold_radius.assign(rdata + (size_t)MPM_R_RADIUS * np, rdata + (size_t)MPM_R_RADIUS * np + np);
+    c->cold_yacc.assign(rdata + (size_t)MPM_R_YACC * np, rdata + (size_t)MPM_R_YACC * np + np);
+    c->cold_rigid.assign(idata + (size_t)MPM_I_RIGID_ID * np,
+                         idata + (size_t)MPM_I_RIGID_ID * np + np);
+    c->all_dirty = true;
     if (np > 0) {
         HIPCHK(hipMemcpy(P.pos, pos, 3 * np * sizeof(double), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(P.r, rdata, (int64_t)MPM_NREAL * np * sizeof(double),
@@ -370,12 +438,19 @@ int mpm_download_particles(mpm_ctx *c, double *pos, double *rdata, int *idata, i
         HIPCHK(hipMemcpy(h.data(), c->P.r, h.size() * sizeof(double), hipMemcpyDeviceToHost));
         for (int f = 0; f < MPM_NREAL; ++f)
             for (int64_t q = 0; q < nl; ++q) rdata[f * nl + q] = h[f * cap + order[q]];
+        // cold fields from the host copies, by upload index
+        for (int64_t q = 0; q < nl; ++q) {
+            rdata[MPM_R_RADIUS * nl + q] = c->cold_radius[ids[order[q]]];
+            rdata[MPM_R_YACC * nl + q] = c->cold_yacc[ids[order[q]]];
+        }
     }
     if (idata) {
         std::vector<int> h((size_t)MPM_NINT * cap);
         HIPCHK(hipMemcpy(h.data(), c->P.idata, h.size() * sizeof(int), hipMemcpyDeviceToHost));
         for (int f = 0; f < MPM_NINT; ++f)
             for (int64_t q = 0; q < nl; ++q) idata[f * nl + q] = h[f * cap + order[q]];
+        for (int64_t q = 0; q < nl; ++q)
+            idata[MPM_I_RIGID_ID * nl + q] = c->cold_rigid[ids[order[q]]];
     }
     if (id)
         for (int64_t q = 0; q < nl; ++q) id[q] = ids[order[q]];
@@ -403,6 +478,7 @@ int mpm_get_nodal(mpm_ctx *c, int comp, double *out) {
 
 int mpm_reset_nodal(mpm_ctx *c) {
     if (!c) return set_err(MPM_E_ARG, "null context");
+    c->all_dirty = true;
     const int64_t nn = c->g.nn;
     launch_zero(c->N.mass, nn, c->stream);
     launch_zero(c->N.vel, 3 * nn, c->stream);
@@ -415,6 +491,7 @@ int mpm_reset_nodal(mpm_ctx *c) {
 int mpm_p2g(mpm_ctx *c, int um, int uv, int uf) {
     if (!c) return set_err(MPM_E_ARG, "null context");
     if (uf < 0 || uf > 1) return set_err(MPM_E_ARG, "update_forces must be 0 or 1");
+    c->all_dirty = true;
     // The reference zeroes the selected components first (:368-403); the
     // accumulators are already zero.
     launch_p2g(c->g, c->ph, c->P, c->N, um, uv, uf, nullptr, c->stream);
@@ -426,18 +503,21 @@ int mpm_p2g(mpm_ctx *c, int um, int uv, int uf) {
 
 int mpm_backup_velocity(mpm_ctx *c) {
     if (!c) return set_err(MPM_E_ARG, "null context");
+    c->all_dirty = true;
     launch_grid(c->g, c->ph, c->ls, c->udf, c->N, G_BACKUP, 0.0, nullptr, c->stream);
     return check_launch();
 }
 
 int mpm_nodal_update(mpm_ctx *c, double dt) {
     if (!c) return set_err(MPM_E_ARG, "null context");
+    c->all_dirty = true;
     launch_grid(c->g, c->ph, c->ls, c->udf, c->N, G_UPDATE, dt, nullptr, c->stream);
     return check_launch();
 }
 
 int mpm_apply_nodal_bcs(mpm_ctx *c, double dt, double t) {
     if (!c) return set_err(MPM_E_ARG, "null context");
+    c->all_dirty = true;
     if (c->any_udf) {
         int rc = upload_udf_tables(c, t);
         if (rc) return rc;
@@ -493,6 +573,7 @@ int mpm_calculate_dt(mpm_ctx *c, double *dt) {
 
 int mpm_init_internal_forces(mpm_ctx *c) {
     if (!c) return set_err(MPM_E_ARG, "null context");
+    c->all_dirty = true;
     // Initialise_Internal_Forces (mpm_init.cpp:368-401): P2G(1,1,1), backup,
     // G2P(update_vel=0, update_strainrate=1, dt=0), stress(dt=0, applied rate).
     launch_p2g(c->g, c->ph, c->P, c->N, 1, 1, 1, nullptr, c->stream);
@@ -527,36 +608,57 @@ int mpm_step(mpm_ctx *c, int nsteps) {
             int rc = upload_udf_tables(c, h.time);
             if (rc) return rc;
         }
+        const bool sorted = c->sorted != 0;
         if (prof) HIPCHK(hipEventRecord(c->ev[1], st));
-        launch_p2g(c->g, c->ph, c->P, c->N, 1, 1, 1, nullptr, st);
+        if (sorted) launch_sort(c->g, c->P, c->S, st);
         if (prof) HIPCHK(hipEventRecord(c->ev[2], st));
-        launch_grid(c->g, c->ph, c->ls, c->udf, c->N, grid_flags_full(c), 0.0, c->sc, st);
+        if (sorted) launch_p2g_tiles(c->g, c->ph, c->P, c->S, c->N, step, 0, st);
+        else launch_p2g(c->g, c->ph, c->P, c->N, 1, 1, 1, nullptr, st);
         if (prof) HIPCHK(hipEventRecord(c->ev[3], st));
-        const int tail = P_MOVE | (redist ? P_REDIST : 0) | P_VOLUME | P_STRESS;
-        if (!musl) {
-            launch_particle(c->g, c->ph, c->ls, c->udf, c->P, c->N, P_G2P_VEL | P_G2P_GRAD | tail,
-                            0.0, 0.0, c->sc, st);
-        } else {
-            launch_particle(c->g, c->ph, c->ls, c->udf, c->P, c->N, P_G2P_VEL, 0.0, 0.0, c->sc, st);
-            launch_p2g(c->g, c->ph, c->P, c->N, 0, 1, 0, c->P.vsnap, st);
-            launch_grid(c->g, c->ph, c->ls, c->udf, c->N, G_FINISH_VEL | G_BCS, 0.0, c->sc, st);
-            launch_particle(c->g, c->ph, c->ls, c->udf, c->P, c->N, P_G2P_GRAD | tail, 0.0, 0.0,
-                            c->sc, st);
-        }
+        // Idle node tiles are skipped when their massless state is all-zero;
+        // a pass after an out-of-step nodal call visits every node.
+        const int *te = (sorted && c->skip_ok && !c->any_udf && !c->all_dirty) ? c->S.tile_epoch
+                                                                                : nullptr;
+        launch_grid(c->g, c->ph, c->ls, c->udf, c->N, grid_flags_full(c), 0.0, c->sc, st, te,
+                    step);
+        c->all_dirty = false;
         if (prof) HIPCHK(hipEventRecord(c->ev[4], st));
+        const int tail = P_MOVE | (redist ? P_REDIST : 0) | P_VOLUME | P_STRESS;
+        int last = P_G2P_VEL | P_G2P_GRAD | tail;
+        if (musl) {
+            launch_particle(c->g, c->ph, c->ls, c->udf, c->P, c->N, P_G2P_VEL, 0.0, 0.0, c->sc, st);
+            // Images of the re-projection P2G carry the velocities of the
+            // neighbour fill at the top of the step (updateNeighbors is not
+            // called in between, main.cpp:120-140).
+            if (c->P.vsnap || !sorted) launch_p2g(c->g, c->ph, c->P, c->N, 0, 1, 0, c->P.vsnap, st);
+            else launch_p2g_tiles(c->g, c->ph, c->P, c->S, c->N, step, 1, st);
+            launch_grid(c->g, c->ph, c->ls, c->udf, c->N, G_FINISH_VEL | G_BCS, 0.0, c->sc, st, te,
+                        step);
+            last = P_G2P_GRAD | tail;
+        }
+        if (sorted) {
+            launch_particle_sorted(c->g, c->ph, c->ls, c->udf, c->P, c->Q, c->S, c->N, last, c->sc,
+                                   st);
+            std::swap(c->P, c->Q);
+        } else {
+            launch_particle(c->g, c->ph, c->ls, c->udf, c->P, c->N, last, 0.0, 0.0, c->sc, st);
+        }
+        if (prof) HIPCHK(hipEventRecord(c->ev[6], st));
         launch_advance_time(c->sc, st);
         c->dirty = !redist;
         int rc = check_launch();
         if (rc) return rc;
         if (prof) {
-            HIPCHK(hipEventSynchronize(c->ev[4]));
-            float a = 0, b = 0, d = 0;
-            (void)hipEventElapsedTime(&a, c->ev[1], c->ev[2]);
-            (void)hipEventElapsedTime(&b, c->ev[2], c->ev[3]);
-            (void)hipEventElapsedTime(&d, c->ev[3], c->ev[4]);
+            HIPCHK(hipEventSynchronize(c->ev[6]));
+            float a = 0, b = 0, d = 0, e = 0;
+            (void)hipEventElapsedTime(&a, c->ev[2], c->ev[3]);
+            (void)hipEventElapsedTime(&b, c->ev[3], c->ev[4]);
+            (void)hipEventElapsedTime(&d, c->ev[4], c->ev[6]);
+            (void)hipEventElapsedTime(&e, c->ev[1], c->ev[2]);
             kms[0] += a;
             kms[1] += b;
             kms[2] += d;
+            kms[3] += e;
         }
     }
     HIPCHK(hipEventRecord(c->ev[5], st));
@@ -565,6 +667,11 @@ int mpm_step(mpm_ctx *c, int nsteps) {
     (void)hipEventElapsedTime(&tot, c->ev[0], c->ev[5]);
     c->step_ms = tot;
     for (int q = 0; q < 4; ++q) c->kernel_ms[q] = nsteps ? kms[q] / nsteps : 0.0;
+    if (c->sorted && nsteps > 0 && c->P.np > 0) {
+        int nlive = 0;
+        HIPCHK(hipMemcpy(&nlive, c->S.cell_start + c->S.nkeys, sizeof(int), hipMemcpyDeviceToHost));
+        c->P.np = c->Q.np = nlive;  // removed particles were dropped by the reorder
+    }
     Scalars h;
     HIPCHK(hipMemcpy(&h, c->sc, sizeof(Scalars), hipMemcpyDeviceToHost));
     c->time_host = h.time;
@@ -654,6 +761,10 @@ int mpm_set_option(mpm_ctx *c, int option, int value) {
     if (!c) return set_err(MPM_E_ARG, "null context");
     if (option == MPM_OPT_PROFILE) c->profile = value;
     else if (option == MPM_OPT_KEEP_FORCES) c->keep_forces = value;
+    else if (option == MPM_OPT_SORTED) {
+        c->sorted = value;
+        c->all_dirty = true;
+    }
     else return set_err(MPM_E_ARG, "unknown option");
     return MPM_OK;
 }

@@ -19,7 +19,14 @@ struct GridP {
     int od[3];      // per-direction order after the small-grid fallback
     int uniform;    // all od equal
     int64_t nn;     // nodes (n0+1)(n1+1)(n2+1)
+    int ntc[3];     // cell tiles per direction (kTileCells cells each)
+    int ntn[3];     // node tiles per direction (n / kTileCells + 1)
 };
+
+// Cells per tile edge for the sorted step: particles are binned into
+// kTileCells^3-cell tiles; a P2G workgroup owns one tile and accumulates
+// into an LDS block of (kTileCells + 3)^3 nodes (stencil offsets -1..+2).
+constexpr int kTileCells = 4;
 
 // Step physics and wall data (MPMspecs subset).
 struct PhysP {
@@ -105,7 +112,8 @@ void launch_zero(double *p, int64_t n, hipStream_t s);
 void launch_p2g(const GridP &g, const PhysP &ph, const PState &P, const NState &N, int um,
                 int uv, int uf, const double *vimg, hipStream_t s);
 void launch_grid(const GridP &g, const PhysP &ph, const LsP &ls, const UdfP &udf,
-                 const NState &N, int flags, double dt, const Scalars *sc, hipStream_t s);
+                 const NState &N, int flags, double dt, const Scalars *sc, hipStream_t s,
+                 const int *tile_epoch = nullptr, int epoch = 0);
 void launch_particle(const GridP &g, const PhysP &ph, const LsP &ls, const UdfP &udf,
                      const PState &P, const NState &N, int flags, double dt, double time,
                      const Scalars *sc, hipStream_t s);
@@ -115,5 +123,25 @@ void launch_advance_time(Scalars *sc, hipStream_t s);
 void launch_energies(const PState &P, double *out6, hipStream_t s);
 void launch_snapshot_vel(const PState &P, hipStream_t s);
 void launch_redistribute(const GridP &g, const PState &P, hipStream_t s);
+
+// Sorted step (mpm_sort.hip).
+struct SortState {
+    int *count;       // nkeys + 1 (last slot: removed particles)
+    int *cell_start;  // nkeys + 1, exclusive scan of count
+    int *key;         // cap
+    int *rank;        // cap
+    int *perm;        // cap: sorted position -> current index
+    void *scan_tmp;
+    size_t scan_bytes;
+    int nkeys;
+    int *tile_epoch;  // node tiles: last epoch a deposit landed there
+};
+size_t sort_scan_bytes(int nkeys);
+void launch_sort(const GridP &g, const PState &P, SortState &S, hipStream_t s);
+void launch_p2g_tiles(const GridP &g, const PhysP &ph, const PState &P, const SortState &S,
+                      const NState &N, int epoch, int momentum_only, hipStream_t s);
+void launch_particle_sorted(const GridP &g, const PhysP &ph, const LsP &ls, const UdfP &udf,
+                            const PState &Pin, const PState &Pout, const SortState &S,
+                            const NState &N, int flags, const Scalars *sc, hipStream_t s);
 
 }  // namespace xmpm

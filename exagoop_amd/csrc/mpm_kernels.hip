@@ -9,66 +9,13 @@
 // Particle: one thread per particle, fused G2P / F / D / move /
 // redistribute / volume / stress.
 
-#include "mpm_internal.h"
-#include "mpm_math.h"
+#include "mpm_device.h"
 
 namespace xmpm {
 
 namespace {
 
 constexpr int kBlock = 256;
-
-__device__ __forceinline__ int64_t nidx(const GridP &g, int I, int J, int K) {
-    return ((int64_t)K * (g.n[1] + 1) + J) * (g.n[0] + 1) + I;
-}
-
-// Deposit targets of an unwrapped node index along one direction.
-__device__ __forceinline__ int targets(int I, int n, int per, int t[2], bool img[2]) {
-    int c = 0;
-    if (I >= 0 && I <= n) {
-        t[c] = I;
-        img[c] = false;
-        ++c;
-    }
-    if (per) {
-        if (I <= 0 && I + n >= 0) {
-            t[c] = I + n;
-            img[c] = true;
-            ++c;
-        } else if (I >= n && I - n <= n) {
-            t[c] = I - n;
-            img[c] = true;
-            ++c;
-        }
-    }
-    return c;
-}
-
-// Node index read by G2P for an unwrapped index: ghost nodes of periodic
-// directions hold the value one period away (FillBoundary); ghost nodes of
-// non-periodic directions hold zero.
-__device__ __forceinline__ int wrap_read(int I, int n, int per, bool &ok) {
-    if (I >= 0 && I <= n) return I;
-    if (!per) {
-        ok = false;
-        return 0;
-    }
-    const int t = I < 0 ? I + n : I - n;
-    if (t < 0 || t > n) {
-        ok = false;  // more than one period away: not a ghost node
-        return 0;
-    }
-    return t;
-}
-
-__device__ __forceinline__ double nodal_read(const GridP &g, const double *arr, int I, int J,
-                                             int K) {
-    bool ok = true;
-    int a = wrap_read(I, g.n[0], g.per[0], ok);
-    int b = wrap_read(J, g.n[1], g.per[1], ok);
-    int c = wrap_read(K, g.n[2], g.per[2], ok);
-    return ok ? arr[nidx(g, a, b, c)] : 0.0;
-}
 
 // ---------------------------------------------------------------------------
 __global__ void k_zero(double *p, int64_t n) {
@@ -183,58 +130,6 @@ __global__ __launch_bounds__(kBlock) void k_p2g(GridP g, PhysP ph, PState P, dou
     }
 }
 
-// Level-set value / gradient (mpm_eb.H:176-288), lsref = 1, one ghost layer.
-__device__ double ls_value(const GridP &g, const double *phi, const double x[3]) {
-    int idx[3];
-    for (int d = 0; d < 3; ++d) idx[d] = (int)floor((x[d] - g.plo[d] + kTiny) / g.dx[d]);
-    double r[3];
-    for (int d = 0; d < 3; ++d) r[d] = (x[d] - (g.plo[d] + idx[d] * g.dx[d])) / g.dx[d];
-    const int G0 = g.n[0] + 3, G1 = g.n[1] + 3;
-    double v = 0.0;
-    for (int corner = 0; corner < 8; ++corner) {
-        int off[3] = {corner & 1, (corner >> 1) & 1, (corner >> 2) & 1};
-        int ii[3];
-        double w = 1.0;
-        for (int d = 0; d < 3; ++d) {
-            ii[d] = idx[d] + off[d];
-            ii[d] = ii[d] < -1 ? -1 : (ii[d] > g.n[d] + 1 ? g.n[d] + 1 : ii[d]);
-            w *= (off[d] == 0) ? (1.0 - r[d]) : r[d];
-        }
-        v += phi[((int64_t)(ii[2] + 1) * G1 + (ii[1] + 1)) * G0 + (ii[0] + 1)] * w;
-    }
-    return v;
-}
-
-__device__ void ls_grad(const GridP &g, const double *phi, const double x[3], double gr[3]) {
-    int idx[3];
-    for (int d = 0; d < 3; ++d) idx[d] = (int)floor((x[d] - g.plo[d] + kTiny) / g.dx[d]);
-    double r[3], dinv[3];
-    for (int d = 0; d < 3; ++d) {
-        r[d] = (x[d] - (g.plo[d] + idx[d] * g.dx[d])) / g.dx[d];
-        dinv[d] = 1.0 / g.dx[d];
-    }
-    const int G0 = g.n[0] + 3, G1 = g.n[1] + 3;
-    gr[0] = gr[1] = gr[2] = 0.0;
-    for (int corner = 0; corner < 8; ++corner) {
-        int off[3] = {corner & 1, (corner >> 1) & 1, (corner >> 2) & 1};
-        int ii[3];
-        for (int d = 0; d < 3; ++d) {
-            ii[d] = idx[d] + off[d];
-            ii[d] = ii[d] < -1 ? -1 : (ii[d] > g.n[d] + 1 ? g.n[d] + 1 : ii[d]);
-        }
-        const double ph = phi[((int64_t)(ii[2] + 1) * G1 + (ii[1] + 1)) * G0 + (ii[0] + 1)];
-        for (int dir = 0; dir < 3; ++dir) {
-            double prod = 1.0;
-            for (int d = 0; d < 3; ++d) {
-                double val = (off[d] == 0) ? (1.0 - r[d]) : r[d];
-                double der = (off[d] == 0) ? -dinv[d] : dinv[d];
-                prod *= (d == dir) ? der : val;
-            }
-            gr[dir] += ph * prod;
-        }
-    }
-}
-
 // Nodal pipeline: P2G normalisation (mpm_particle_grid_ops.cpp:590-627),
 // backup_current_velocity (nodal_data_ops.cpp:57-81),
 // Nodal_Time_Update_Momentum (:451-482), nodal_bcs (:663-766),
@@ -242,7 +137,8 @@ __device__ void ls_grad(const GridP &g, const double *phi, const double x[3], do
 // store_delta_velocity (:368-392).
 __global__ __launch_bounds__(kBlock) void k_grid(GridP g, PhysP ph, LsP ls, UdfP udf,
                                                  NState N, int flags, double dt_val,
-                                                 const Scalars *sc) {
+                                                 const Scalars *sc, const int *tile_epoch,
+                                                 int epoch) {
     const int64_t node = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (node >= g.nn) return;
     const int64_t nn = g.nn;
@@ -250,6 +146,24 @@ __global__ __launch_bounds__(kBlock) void k_grid(GridP g, PhysP ph, LsP ls, UdfP
     const int I = (int)(node % (g.n[0] + 1));
     const int J = (int)((node / (g.n[0] + 1)) % (g.n[1] + 1));
     const int K = (int)(node / ((int64_t)(g.n[0] + 1) * (g.n[1] + 1)));
+    if (tile_epoch) {
+        // Node tiles no particle deposited on this step hold the massless
+        // state, which is all-zero when walls are at rest and no UDF is set:
+        // written once when a tile goes idle, skipped afterwards.
+        const int t = tile_epoch[node_tile(g, I, J, K)];
+        if (t != epoch) {
+            if (t == epoch - 1 && (flags & G_FINISH_MASS)) {
+                N.mass[node] = 0.0;
+                N.mold[node] = 0.0;
+                for (int d = 0; d < 3; ++d) {
+                    N.vel[d * nn + node] = 0.0;
+                    N.dvel[d * nn + node] = 0.0;
+                    if (flags & G_KEEP_FRC) N.frc[d * nn + node] = 0.0;
+                }
+            }
+            return;
+        }
+    }
     double m = N.mass[node];
     double v[3], dv[3];
     for (int d = 0; d < 3; ++d) {
@@ -377,10 +291,11 @@ __global__ __launch_bounds__(kBlock) void k_grid(GridP g, PhysP ph, LsP ls, UdfP
     }
 }
 
-// Per-particle pipeline: interpolate_from_grid (mpm_particle_grid_ops.cpp:
-// 967-1096), moveParticles (mpm_particle_timestep.cpp:329-576), the
-// Redistribute that follows it, updateVolume (:144-180) and
-// apply_constitutive_model[_delta] (mpm_particle_container.cpp:43-272).
+// Per-particle pipeline, in place: interpolate_from_grid
+// (mpm_particle_grid_ops.cpp:967-1096), moveParticles
+// (mpm_particle_timestep.cpp:329-576), the Redistribute that follows it,
+// updateVolume (:144-180) and apply_constitutive_model[_delta]
+// (mpm_particle_container.cpp:43-272).
 __global__ __launch_bounds__(kBlock) void k_particle(GridP g, PhysP ph, LsP ls, UdfP udf,
                                                      PState P, NState N, int flags,
                                                      double dt_val, double time_val,
@@ -388,315 +303,20 @@ __global__ __launch_bounds__(kBlock) void k_particle(GridP g, PhysP ph, LsP ls, 
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P.np) return;
     if (P.id[i] < 0) return;
-    const int64_t cap = P.cap;
-    const int64_t nn = g.nn;
     const double dt = sc ? sc->dt : dt_val;
     const double time = sc ? sc->time : time_val;
-    double *R = P.r;
-    const int phase = P.idata[MPM_I_PHASE * cap + i];
-    double x[3] = {P.pos[i], P.pos[cap + i], P.pos[2 * cap + i]};
-
-    if ((flags & (P_G2P_VEL | P_G2P_GRAD)) && phase == 0) {
-        int iv[3], mn[3], mx[3];
-        for (int d = 0; d < 3; ++d) {
-            iv[d] = (int)floor((x[d] - g.plo[d]) * g.dxi[d]);
-            bounds(iv[d], 0, g.hi[d], g.od[d], g.per[d], mn[d], mx[d]);
-        }
-        double L[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-        const bool grad = flags & P_G2P_GRAD;
-        double bv[3][4], bd[3][4];
-        if (grad)
-            for (int d = 0; d < 3; ++d)
-                for (int l = mn[d]; l < mx[d]; ++l)
-                    basis1d(g.od[d], g.per[d], iv[d], l, 0, g.hi[d], x[d], g.plo[d], g.dx[d],
-                            bv[d][l - mn[d]], bd[d][l - mn[d]]);
-        bool grad_done = false;
-        if (flags & P_G2P_VEL) {
-            double vals0[3] = {0, 0, 0}, vals1[3] = {0, 0, 0};
-            const double alpha = ph.alpha;
-            if (g.uniform && g.od[0] >= 2) {
-                const int O = g.od[0];
-                double wv[3][4];
-                for (int d = 0; d < 3; ++d)
-                    for (int l = mn[d]; l < mx[d]; ++l)
-                        wv[d][l - mn[d]] =
-                            interp1d(O, g.per[d], iv[d] + l, 0, g.hi[d], x[d], g.plo[d], g.dx[d]);
-                for (int n = mn[2]; n < mx[2]; ++n)
-                    for (int mm = mn[1]; mm < mx[1]; ++mm)
-                        for (int l = mn[0]; l < mx[0]; ++l) {
-                            const int I = iv[0] + l, J = iv[1] + mm, K = iv[2] + n;
-                            bool ok = true;
-                            const int a = wrap_read(I, g.n[0], g.per[0], ok);
-                            const int b = wrap_read(J, g.n[1], g.per[1], ok);
-                            const int c = wrap_read(K, g.n[2], g.per[2], ok);
-                            const int64_t q = nidx(g, a, b, c);
-                            double nv[3], nd[3];
-                            for (int d = 0; d < 3; ++d) {
-                                nv[d] = ok ? N.vel[d * nn + q] : 0.0;
-                                nd[d] = ok ? N.dvel[d * nn + q] : 0.0;
-                            }
-                            const double w = wv[0][l - mn[0]] * wv[1][mm - mn[1]] * wv[2][n - mn[2]];
-                            for (int d = 0; d < 3; ++d) {
-                                vals0[d] += w * nv[d];
-                                vals1[d] += w * nd[d];
-                            }
-                            if (grad) {
-                                const double vx = bv[0][l - mn[0]], vy = bv[1][mm - mn[1]],
-                                             vz = bv[2][n - mn[2]];
-                                const double gr[3] = {bd[0][l - mn[0]] * vy * vz,
-                                                      vx * bd[1][mm - mn[1]] * vz,
-                                                      vx * vy * bd[2][n - mn[2]]};
-                                for (int d1 = 0; d1 < 3; ++d1)
-                                    for (int d2 = 0; d2 < 3; ++d2) L[d1][d2] += nv[d1] * gr[d2];
-                            }
-                        }
-                grad_done = grad;
-            } else {
-                for (int dim = 0; dim < 3; ++dim) {
-                    const double *A0 = N.vel + dim * nn;
-                    const double *A1 = N.dvel + dim * nn;
-                    const int O = g.od[dim];
-                    if (O == 1) {
-                        // bilin_interp_two (interpolants.H:583-656), trilinear in all dirs
-                        const int ii = iv[0], jj = iv[1], kk = iv[2];
-                        const double a = (x[0] - (g.plo[0] + ii * g.dx[0])) / g.dx[0];
-                        const double b = (x[1] - (g.plo[1] + jj * g.dx[1])) / g.dx[1];
-                        const double c = (x[2] - (g.plo[2] + kk * g.dx[2])) / g.dx[2];
-                        const double w000 = (1.0 - a) * (1.0 - b) * (1.0 - c);
-                        const double w100 = a * (1.0 - b) * (1.0 - c);
-                        const double w010 = (1.0 - a) * b * (1.0 - c);
-                        const double w110 = a * b * (1.0 - c);
-                        const double w001 = (1.0 - a) * (1.0 - b) * c;
-                        const double w101 = a * (1.0 - b) * c;
-                        const double w011 = (1.0 - a) * b * c;
-                        const double w111 = a * b * c;
-                        for (int q = 0; q < 2; ++q) {
-                            const double *A = q ? A1 : A0;
-                            double s = w000 * nodal_read(g, A, ii, jj, kk) +
-                                       w100 * nodal_read(g, A, ii + 1, jj, kk) +
-                                       w010 * nodal_read(g, A, ii, jj + 1, kk) +
-                                       w110 * nodal_read(g, A, ii + 1, jj + 1, kk) +
-                                       w001 * nodal_read(g, A, ii, jj, kk + 1) +
-                                       w101 * nodal_read(g, A, ii + 1, jj, kk + 1) +
-                                       w011 * nodal_read(g, A, ii, jj + 1, kk + 1) +
-                                       w111 * nodal_read(g, A, ii + 1, jj + 1, kk + 1);
-                            if (q) vals1[dim] = s;
-                            else vals0[dim] = s;
-                        }
-                    } else {
-                        double wv[3][4];
-                        for (int d = 0; d < 3; ++d)
-                            for (int l = mn[d]; l < mx[d]; ++l)
-                                wv[d][l - mn[d]] = interp1d(O, g.per[d], iv[d] + l, 0, g.hi[d], x[d],
-                                                            g.plo[d], g.dx[d]);
-                        for (int n = mn[2]; n < mx[2]; ++n)
-                            for (int mm = mn[1]; mm < mx[1]; ++mm)
-                                for (int l = mn[0]; l < mx[0]; ++l) {
-                                    const double w =
-                                        wv[0][l - mn[0]] * wv[1][mm - mn[1]] * wv[2][n - mn[2]];
-                                    vals0[dim] += w * nodal_read(g, A0, iv[0] + l, iv[1] + mm, iv[2] + n);
-                                    vals1[dim] += w * nodal_read(g, A1, iv[0] + l, iv[1] + mm, iv[2] + n);
-                                }
-                    }
-                }
-            }
-            for (int d = 0; d < 3; ++d) {
-                R[(MPM_R_VELP + d) * cap + i] = vals0[d];
-                double vo = R[(MPM_R_VEL + d) * cap + i];
-                R[(MPM_R_VEL + d) * cap + i] = alpha * vo + alpha * vals1[d] + (1 - alpha) * vals0[d];
-            }
-        }
-        if (grad) {
-            if (!grad_done) {
-                for (int n = mn[2]; n < mx[2]; ++n)
-                    for (int mm = mn[1]; mm < mx[1]; ++mm)
-                        for (int l = mn[0]; l < mx[0]; ++l) {
-                            const double vx = bv[0][l - mn[0]], vy = bv[1][mm - mn[1]],
-                                         vz = bv[2][n - mn[2]];
-                            const double gr[3] = {bd[0][l - mn[0]] * vy * vz,
-                                                  vx * bd[1][mm - mn[1]] * vz,
-                                                  vx * vy * bd[2][n - mn[2]]};
-                            for (int d1 = 0; d1 < 3; ++d1) {
-                                const double v1 = nodal_read(g, N.vel + d1 * nn, iv[0] + l,
-                                                             iv[1] + mm, iv[2] + n);
-                                for (int d2 = 0; d2 < 3; ++d2) L[d1][d2] += v1 * gr[d2];
-                            }
-                        }
-            }
-            // get_deformation_gradient_tensor (interpolants.H:1732-1787)
-            double nrm = 0.0;
-            for (int a = 0; a < 3; a++)
-                for (int b = 0; b < 3; b++) nrm += L[a][b] * L[a][b];
-            if (!(nrm >= 1.0e10)) {
-                double Lp[3][3], Fp[3][3];
-                for (int a = 0; a < 3; a++)
-                    for (int b = 0; b < 3; b++) {
-                        Lp[a][b] = (a == b) ? 1.0 + L[a][b] * dt : L[a][b] * dt;
-                        Fp[a][b] = R[(MPM_R_DEFGRAD + 3 * a + b) * cap + i];
-                    }
-                for (int a = 0; a < 3; ++a)
-                    for (int b = 0; b < 3; ++b) {
-                        double acc = 0.0;
-                        for (int q = 0; q < 3; ++q) acc += Lp[a][q] * Fp[q][b];
-                        R[(MPM_R_DEFGRAD + a * 3 + b) * cap + i] = acc;
-                    }
-            }
-            int ind = 0;
-            for (int d1 = 0; d1 < 3; ++d1)
-                for (int d2 = d1; d2 < 3; ++d2) {
-                    R[(MPM_R_STRAINRATE + ind) * cap + i] = 0.5 * (L[d1][d2] + L[d2][d1]);
-                    ind++;
-                }
-        }
-    }
-
+    Rec R;
+    load_all(P, i, R);
+    if ((flags & (P_G2P_VEL | P_G2P_GRAD)) && R.phase == 0)
+        stage_g2p(g, ph, N, R, flags & P_G2P_VEL, flags & P_G2P_GRAD, dt);
     if (flags & P_MOVE) {
-        for (int d = 0; d < 3; ++d) x[d] += R[(MPM_R_VELP + d) * cap + i] * dt;
-        double rin[3], rout[3];
-        for (int d = 0; d < 3; ++d) {
-            rin[d] = R[(MPM_R_VEL + d) * cap + i];
-            rout[d] = rin[d];
-        }
-        if (ls.nb > 0 && phase == 0) {
-            int hit = -1;
-            double minphi = kTiny;
-            for (int b = 0; b < ls.nb; ++b) {
-                double ph_ = ls_value(g, ls.phi[b], x);
-                if (ph_ < minphi) {
-                    minphi = ph_;
-                    hit = b;
-                }
-            }
-            if (hit >= 0) {
-                double nrm[3];
-                ls_grad(g, ls.phi[hit], x, nrm);
-                double gm = 0.0;
-                for (int d = 0; d < 3; ++d) gm += nrm[d] * nrm[d];
-                gm = sqrt(gm);
-                for (int d = 0; d < 3; ++d) nrm[d] /= (gm + kTiny);
-                int mod = applybc(rin, rout, ls.mu[hit], nrm, ls.bc[hit]);
-                if (mod > 0)
-                    for (int d = 0; d < 3; ++d) x[d] += 2.0 * fabs(minphi) * nrm[d];
-                for (int d = 0; d < 3; ++d) R[(MPM_R_VEL + d) * cap + i] = rout[d];
-            }
-        }
-        double wv[3] = {0, 0, 0};
-        for (int dir = 0; dir < 3; ++dir) {
-            int side = -1;
-            if (x[dir] < g.plo[dir]) side = 0;
-            else if (x[dir] > g.phi[dir]) side = 1;
-            if (side < 0) continue;
-            const double *wvel = side ? ph.wv_hi : ph.wv_lo;
-            for (int d = 0; d < 3; ++d) wv[d] = wvel[dir * 3 + d];
-            const double *tab = udf.tab[dir][side];
-            if (tab) {
-                const int p0 = (dir == 0) ? 1 : 0;
-                const int p1 = (dir == 2) ? 1 : 2;
-                double f0 = (x[p0] - g.plo[p0]) / g.dx[p0];
-                double f1 = (x[p1] - g.plo[p1]) / g.dx[p1];
-                f0 = fmax(0.0, fmin((double)g.n[p0], f0));
-                f1 = fmax(0.0, fmin((double)g.n[p1], f1));
-                const int j0 = (int)f0, j1 = min(j0 + 1, g.n[p0]);
-                const int k0 = (int)f1, k1 = min(k0 + 1, g.n[p1]);
-                const double wx1 = f0 - j0, wx0 = 1.0 - wx1;
-                const double wy1 = f1 - k0, wy0 = 1.0 - wy1;
-                const int n1 = g.n[p1] + 1;
-                for (int q = 0; q < 3; ++q)
-                    wv[q] = wx0 * wy0 * tab[(j0 * n1 + k0) * 3 + q] +
-                            wx0 * wy1 * tab[(j0 * n1 + k1) * 3 + q] +
-                            wx1 * wy0 * tab[(j1 * n1 + k0) * 3 + q] +
-                            wx1 * wy1 * tab[(j1 * n1 + k1) * 3 + q];
-            }
-            for (int d = 0; d < 3; ++d) rin[d] -= wv[d];
-            double nrm[3] = {0, 0, 0};
-            nrm[dir] = side ? -1.0 : 1.0;
-            int mod = applybc(rin, rout, side ? ph.mu_hi[dir] : ph.mu_lo[dir], nrm,
-                              side ? ph.bchi[dir] : ph.bclo[dir]);
-            if (mod > 0) x[dir] = 2.0 * (side ? g.phi[dir] : g.plo[dir]) - x[dir];
-            for (int q = 0; q < 3; ++q) R[(MPM_R_VEL + q) * cap + i] = rout[q] + wv[q];
-        }
-        if (flags & P_REDIST) {
-            // Redistribute: periodic wrap (AMReX enforcePeriodic) and removal
-            // of particles whose cell is outside a non-periodic domain.
-            for (int d = 0; d < 3; ++d) {
-                if (!g.per[d]) continue;
-                const double L_ = g.phi[d] - g.plo[d];
-                if (x[d] > g.rhi[d]) {
-                    while (x[d] > g.rhi[d]) x[d] -= L_;
-                    if (x[d] < g.plo[d]) x[d] = g.plo[d];
-                } else if (x[d] < g.plo[d]) {
-                    while (x[d] < g.plo[d]) x[d] += L_;
-                    if (x[d] > g.rhi[d]) x[d] = g.rhi[d];
-                }
-            }
-            for (int d = 0; d < 3; ++d) {
-                if (g.per[d]) continue;
-                const int c = (int)floor((x[d] - g.plo[d]) * g.dxi[d]);
-                if (c < 0 || c > g.hi[d]) P.id[i] = -1 - P.id[i];
-            }
-        }
-        for (int d = 0; d < 3; ++d) P.pos[d * cap + i] = x[d];
+        stage_move(g, ph, ls, udf, R, dt);
+        if (flags & P_REDIST) stage_redistribute(g, R.x, R.id);
     }
-
-    if ((flags & P_VOLUME) && phase == 0) {
-        double F[9];
-        for (int q = 0; q < 9; ++q) F[q] = R[(MPM_R_DEFGRAD + q) * cap + i];
-        const double det = det3(F);
-        R[MPM_R_JACOBIAN * cap + i] = det;
-        const double V = R[MPM_R_VOLINIT * cap + i] * det;
-        R[MPM_R_VOLUME * cap + i] = V;
-        R[MPM_R_DENSITY * cap + i] = R[MPM_R_MASS * cap + i] / V;
-    }
-
-    if ((flags & (P_STRESS | P_STRESS_INIT)) && phase == 0) {
-        const bool init = flags & P_STRESS_INIT;
-        const double sdt = init ? 0.0 : dt;
-        const double rate =
-            init ? ph.applied_rate : ((time < ph.applied_time) ? ph.applied_rate : 0.0);
-        const bool delta = !init && ph.delta;
-        const int model = P.idata[MPM_I_MODEL * cap + i];
-        double eps[6], D[6];
-        for (int q = 0; q < 6; ++q) {
-            D[q] = R[(MPM_R_STRAINRATE + q) * cap + i];
-            eps[q] = R[(MPM_R_STRAIN + q) * cap + i] + sdt * D[q];
-        }
-        eps[XX] += sdt * rate;
-        eps[YY] += sdt * rate;
-        eps[ZZ] += sdt * rate;
-        for (int q = 0; q < 6; ++q) R[(MPM_R_STRAIN + q) * cap + i] = eps[q];
-        const double E = R[MPM_R_E * cap + i], nu = R[MPM_R_NU * cap + i];
-        double s[6];
-        if (delta) {
-            // apply_constitutive_model_delta: model 0 only (others rejected on the host)
-            double de[6];
-            for (int q = 0; q < 6; ++q) de[q] = sdt * D[q];
-            de[XX] += sdt * rate;
-            de[YY] += sdt * rate;
-            de[ZZ] += sdt * rate;
-            double ds[6];
-            linear_elastic(de, ds, E, nu);
-            for (int q = 0; q < 6; ++q)
-                R[(MPM_R_STRESS + q) * cap + i] = R[(MPM_R_STRESS + q) * cap + i] + ds[q];
-        } else {
-            if (model == 0) {
-                linear_elastic(eps, s, E, nu);
-            } else if (model == 1) {
-                const double pinf = 0.0;
-                const double pr = R[MPM_R_BULK * cap + i] *
-                                      (pow(1.0 / R[MPM_R_JACOBIAN * cap + i],
-                                           R[MPM_R_GAMMA * cap + i]) -
-                                       1.0) +
-                                  pinf;
-                R[MPM_R_PRESSURE * cap + i] = pr;
-                newtonian(D, s, R[MPM_R_VISC * cap + i], pr);
-            } else {
-                double F[9];
-                for (int q = 0; q < 9; ++q) F[q] = R[(MPM_R_DEFGRAD + q) * cap + i];
-                neo_hookean(F, s, E, nu);
-            }
-            for (int q = 0; q < 6; ++q) R[(MPM_R_STRESS + q) * cap + i] = s[q];
-        }
-    }
+    if ((flags & P_VOLUME) && R.phase == 0) stage_volume(R);
+    if ((flags & (P_STRESS | P_STRESS_INIT)) && R.phase == 0)
+        stage_stress(ph, R, dt, time, flags & P_STRESS_INIT);
+    store_all(P, i, R);
 }
 
 // Calculate_time_step (mpm_particle_timestep.cpp:38-111): per-particle
@@ -799,23 +419,10 @@ __global__ void k_redistribute(GridP g, PState P) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P.np || P.id[i] < 0) return;
     double x[3] = {P.pos[i], P.pos[P.cap + i], P.pos[2 * P.cap + i]};
-    for (int d = 0; d < 3; ++d) {
-        if (!g.per[d]) continue;
-        const double L_ = g.phi[d] - g.plo[d];
-        if (x[d] > g.rhi[d]) {
-            while (x[d] > g.rhi[d]) x[d] -= L_;
-            if (x[d] < g.plo[d]) x[d] = g.plo[d];
-        } else if (x[d] < g.plo[d]) {
-            while (x[d] < g.plo[d]) x[d] += L_;
-            if (x[d] > g.rhi[d]) x[d] = g.rhi[d];
-        }
-    }
-    for (int d = 0; d < 3; ++d) {
-        P.pos[d * P.cap + i] = x[d];
-        if (g.per[d]) continue;
-        const int c = (int)floor((x[d] - g.plo[d]) * g.dxi[d]);
-        if (c < 0 || c > g.hi[d]) P.id[i] = -1 - P.id[i];
-    }
+    int64_t id = P.id[i];
+    stage_redistribute(g, x, id);
+    for (int d = 0; d < 3; ++d) P.pos[d * P.cap + i] = x[d];
+    P.id[i] = id;
 }
 
 inline unsigned blocks_for(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
@@ -841,9 +448,10 @@ void launch_p2g(const GridP &g, const PhysP &ph, const PState &P, const NState &
 }
 
 void launch_grid(const GridP &g, const PhysP &ph, const LsP &ls, const UdfP &udf,
-                 const NState &N, int flags, double dt, const Scalars *sc, hipStream_t s) {
+                 const NState &N, int flags, double dt, const Scalars *sc, hipStream_t s,
+                 const int *tile_epoch, int epoch) {
     hipLaunchKernelGGL(k_grid, dim3(blocks_for(g.nn)), dim3(kBlock), 0, s, g, ph, ls, udf, N,
-                       flags, dt, sc);
+                       flags, dt, sc, tile_epoch, epoch);
 }
 
 void launch_particle(const GridP &g, const PhysP &ph, const LsP &ls, const UdfP &udf,

@@ -65,11 +65,12 @@ def make_records(pos, vol_particle, density, model, E=0.0, nu=0.0, K=0.0, gamma=
     return r, idata
 
 
-def elastic_disks(nx=64, ny=64, nz=4, ppc=2, steps=200):
+def elastic_disks(nx=64, ny=64, nz=4, ppc=2, c1=(0.3, 0.3), c2=(0.7, 0.7), speed=0.1):
     """Two linear-elastic disks colliding, extruded 3-D: unit square,
     64x64x4 cells (dx = 1/64), z periodic, 2x2x2 particles per cell, r = 0.2
     at (0.3,0.3) and (0.7,0.7), v = +-(0.1,0.1,0), E = 1000, nu = 0.3,
-    rho = 1000, quadratic B-splines, fixed dt = 1e-4."""
+    rho = 1000, quadratic B-splines, fixed dt = 1e-4.  With these values the
+    disks first touch at t ~ 0.59 (gap 0.166 closing at 0.283)."""
     s = default_specs()
     s.plo[:] = [0.0, 0.0, 0.0]
     s.phi[:] = [1.0, ny / nx, nz / nx]
@@ -84,15 +85,15 @@ def elastic_disks(nx=64, ny=64, nz=4, ppc=2, steps=200):
     s.stress_update_scheme = USL
 
     def inside(X, Y, Z):
-        a = (X - 0.3) ** 2 + (Y - 0.3) ** 2 <= 0.2 ** 2
-        b = (X - 0.7) ** 2 + (Y - 0.7) ** 2 <= 0.2 ** 2
+        a = (X - c1[0]) ** 2 + (Y - c1[1]) ** 2 <= 0.2 ** 2
+        b = (X - c2[0]) ** 2 + (Y - c2[1]) ** 2 <= 0.2 ** 2
         return a | b
 
     pos, vp = lattice(s, (ppc, ppc, ppc), inside)
     vel = np.zeros((3, pos.shape[1]))
-    first = (pos[0] - 0.3) ** 2 + (pos[1] - 0.3) ** 2 <= 0.2 ** 2
-    vel[0] = np.where(first, 0.1, -0.1)
-    vel[1] = np.where(first, 0.1, -0.1)
+    first = (pos[0] - c1[0]) ** 2 + (pos[1] - c1[1]) ** 2 <= 0.2 ** 2
+    vel[0] = np.where(first, speed, -speed)
+    vel[1] = np.where(first, speed, -speed)
     r, idata = make_records(pos, vp, 1000.0, 0, E=1000.0, nu=0.3, vel=vel)
     return s, pos, r, idata
 

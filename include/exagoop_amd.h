@@ -239,9 +239,12 @@ int mpm_energies(mpm_ctx *ctx, double out[6]);
 /* Context options without a reference counterpart (tooling for callers that
  * measure or inspect the step): MPM_OPT_PROFILE records HIP events around
  * each kernel of mpm_step (read back with mpm_last_step_timing);
- * MPM_OPT_KEEP_FORCES stores nodal forces for mpm_get_nodal (default 1). */
+ * MPM_OPT_KEEP_FORCES stores nodal forces for mpm_get_nodal (default 1);
+ * MPM_OPT_SORTED selects the cell-sorted step (default 1) or the unsorted
+ * one-thread-per-particle atomic step (0). */
 #define MPM_OPT_PROFILE 1
 #define MPM_OPT_KEEP_FORCES 2
+#define MPM_OPT_SORTED 3
 int mpm_set_option(mpm_ctx *ctx, int option, int value);
 
 /* Last HIP error string (static storage). */

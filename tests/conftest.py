@@ -1,3 +1,4 @@
+import ctypes
 import os
 import sys
 
@@ -12,12 +13,19 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs a HIP device (MI355X)")
 
 
+_HAS_GPU = None
+
+
 def has_gpu():
-    try:
-        import torch
-        return torch.cuda.is_available()
-    except Exception:
-        return False
+    global _HAS_GPU
+    if _HAS_GPU is None:
+        try:
+            hip = ctypes.CDLL("libamdhip64.so")
+            n = ctypes.c_int(0)
+            _HAS_GPU = hip.hipGetDeviceCount(ctypes.byref(n)) == 0 and n.value > 0
+        except OSError:
+            _HAS_GPU = False
+    return _HAS_GPU
 
 
 @pytest.fixture(scope="session")
