@@ -27,7 +27,7 @@ N_MASS, N_VEL, N_FRC, N_DVEL, N_MASS_OLD = 0, 1, 4, 7, 10
 BC_PERIODIC, BC_NOSLIP, BC_SLIP, BC_PARTIALSLIP, BC_OUTFLOW = 0, 1, 2, 3, 4
 USL, MUSL = 0, 1
 
-OPT_PROFILE, OPT_KEEP_FORCES, OPT_SORTED = 1, 2, 3
+OPT_PROFILE, OPT_KEEP_FORCES, OPT_SORTED, OPT_FAST, OPT_FUSED_DT = 1, 2, 3, 4, 5
 
 STATUS = {0: "OK", -1: "E_ARG", -2: "E_BC", -3: "E_ORDER", -4: "E_MODEL", -5: "E_OOM",
           -6: "E_HIP", -7: "E_NAN", -8: "E_UNSUPPORTED", -9: "E_NODEVICE"}

@@ -56,6 +56,9 @@ struct mpm_ctx {
     bool skip_ok = true;    // idle node tiles hold the all-zero massless state
     bool all_dirty = true;  // next step's grid pass must visit every node
     int sorted = 1;         // use the cell-sorted step
+    int fast = 1;           // use the specialised particle kernel where it applies
+    int fuse_dt = 1;        // the specialised kernel reduces the next step's dt
+    bool dt_next_valid = false;  // the last step reduced the next step's dt candidates
     Scalars *sc = nullptr;
     double *d_energy = nullptr;
     hipStream_t stream = nullptr;
@@ -382,6 +385,7 @@ int mpm_upload_particles(mpm_ctx *c, int64_t np, const double *pos, const double
     c->cold_rigid.assign(idata + (size_t)MPM_I_RIGID_ID * np,
                          idata + (size_t)MPM_I_RIGID_ID * np + np);
     c->all_dirty = true;
+    c->dt_next_valid = false;
     if (np > 0) {
         HIPCHK(hipMemcpy(P.pos, pos, 3 * np * sizeof(double), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(P.r, rdata, (int64_t)MPM_NREAL * np * sizeof(double),
@@ -528,6 +532,7 @@ int mpm_apply_nodal_bcs(mpm_ctx *c, double dt, double t) {
 
 int mpm_g2p(mpm_ctx *c, int uv, int us, double dt) {
     if (!c) return set_err(MPM_E_ARG, "null context");
+    c->dt_next_valid = false;
     int flags = (uv ? P_G2P_VEL : 0) | (us ? P_G2P_GRAD : 0);
     if (flags)
         launch_particle(c->g, c->ph, c->ls, c->udf, c->P, c->N, flags, dt, 0.0, nullptr,
@@ -537,6 +542,7 @@ int mpm_g2p(mpm_ctx *c, int uv, int us, double dt) {
 
 int mpm_move(mpm_ctx *c, double dt, int step) {
     if (!c) return set_err(MPM_E_ARG, "null context");
+    c->dt_next_valid = false;
     const int nr = c->s.num_redist > 0 ? c->s.num_redist : 1;
     const bool redist = (step % nr) == 0;
     launch_particle(c->g, c->ph, c->ls, c->udf, c->P, c->N, P_MOVE | (redist ? P_REDIST : 0), dt,
@@ -547,6 +553,7 @@ int mpm_move(mpm_ctx *c, double dt, int step) {
 
 int mpm_update_volume(mpm_ctx *c) {
     if (!c) return set_err(MPM_E_ARG, "null context");
+    c->dt_next_valid = false;
     launch_particle(c->g, c->ph, c->ls, c->udf, c->P, c->N, P_VOLUME, 0.0, 0.0, nullptr,
                     c->stream);
     return check_launch();
@@ -554,6 +561,7 @@ int mpm_update_volume(mpm_ctx *c) {
 
 int mpm_stress(mpm_ctx *c, double dt, double time) {
     if (!c) return set_err(MPM_E_ARG, "null context");
+    c->dt_next_valid = false;
     launch_particle(c->g, c->ph, c->ls, c->udf, c->P, c->N, P_STRESS, dt, time, nullptr,
                     c->stream);
     return check_launch();
@@ -574,6 +582,7 @@ int mpm_calculate_dt(mpm_ctx *c, double *dt) {
 int mpm_init_internal_forces(mpm_ctx *c) {
     if (!c) return set_err(MPM_E_ARG, "null context");
     c->all_dirty = true;
+    c->dt_next_valid = false;
     // Initialise_Internal_Forces (mpm_init.cpp:368-401): P2G(1,1,1), backup,
     // G2P(update_vel=0, update_strainrate=1, dt=0), stress(dt=0, applied rate).
     launch_p2g(c->g, c->ph, c->P, c->N, 1, 1, 1, nullptr, c->stream);
@@ -598,9 +607,14 @@ int mpm_step(mpm_ctx *c, int nsteps) {
         const int step = (int)c->steps;
         const bool redist = (step % nr) == 0;
         // Redistribute_Fill_Update at the top of the step (main.cpp:88).
+        // The previous fast particle kernel already reduced this step's dt
+        // candidates unless a redistribution (which may remove particles)
+        // runs first.
+        const bool fused_dt = c->dt_next_valid && !(redist && c->dirty);
         if (redist && c->dirty) launch_redistribute(c->g, c->P, st);
         if (musl && c->P.vsnap) launch_snapshot_vel(c->P, st);
-        launch_dt(c->g, c->ph, c->P, c->sc, c->s.fixed_timestep == 1, c->s.timestep, st);
+        launch_dt(c->g, c->ph, c->P, c->sc, c->s.fixed_timestep == 1, c->s.timestep, st,
+                  fused_dt ? 1 : 0);
         if (c->any_udf) {
             Scalars h;
             HIPCHK(hipMemcpyAsync(&h, c->sc, sizeof(Scalars), hipMemcpyDeviceToHost, st));
@@ -610,9 +624,9 @@ int mpm_step(mpm_ctx *c, int nsteps) {
         }
         const bool sorted = c->sorted != 0;
         if (prof) HIPCHK(hipEventRecord(c->ev[1], st));
-        if (sorted) launch_sort(c->g, c->P, c->S, st);
+        if (sorted) launch_sort(c->g, c->P, c->S, c->Q.id, st);
         if (prof) HIPCHK(hipEventRecord(c->ev[2], st));
-        if (sorted) launch_p2g_tiles(c->g, c->ph, c->P, c->S, c->N, step, 0, st);
+        if (sorted) launch_p2g_tiles(c->g, c->ph, c->P, c->S, c->N, step, 0, nr > 1, c->sc, st);
         else launch_p2g(c->g, c->ph, c->P, c->N, 1, 1, 1, nullptr, st);
         if (prof) HIPCHK(hipEventRecord(c->ev[3], st));
         // Idle node tiles are skipped when their massless state is all-zero;
@@ -631,17 +645,26 @@ int mpm_step(mpm_ctx *c, int nsteps) {
             // neighbour fill at the top of the step (updateNeighbors is not
             // called in between, main.cpp:120-140).
             if (c->P.vsnap || !sorted) launch_p2g(c->g, c->ph, c->P, c->N, 0, 1, 0, c->P.vsnap, st);
-            else launch_p2g_tiles(c->g, c->ph, c->P, c->S, c->N, step, 1, st);
+            else launch_p2g_tiles(c->g, c->ph, c->P, c->S, c->N, step, 1, nr > 1, c->sc, st);
             launch_grid(c->g, c->ph, c->ls, c->udf, c->N, G_FINISH_VEL | G_BCS, 0.0, c->sc, st, te,
                         step);
             last = P_G2P_GRAD | tail;
         }
-        if (sorted) {
+        // The tile kernel needs every particle inside its tile: particles
+        // redistributed every step.
+        if (sorted && c->fast && nr == 1 && particle_fast_ok(c->g, c->ls, c->udf)) {
+            launch_particle_fast(c->g, c->ph, c->P, c->Q, c->S, c->N, musl ? 0 : 1, redist ? 1 : 0,
+                                 c->sc, st);
+            std::swap(c->P, c->Q);
+            c->dt_next_valid = c->fuse_dt != 0;
+        } else if (sorted) {
             launch_particle_sorted(c->g, c->ph, c->ls, c->udf, c->P, c->Q, c->S, c->N, last, c->sc,
                                    st);
             std::swap(c->P, c->Q);
+            c->dt_next_valid = false;
         } else {
             launch_particle(c->g, c->ph, c->ls, c->udf, c->P, c->N, last, 0.0, 0.0, c->sc, st);
+            c->dt_next_valid = false;
         }
         if (prof) HIPCHK(hipEventRecord(c->ev[6], st));
         launch_advance_time(c->sc, st);
@@ -676,6 +699,11 @@ int mpm_step(mpm_ctx *c, int nsteps) {
     HIPCHK(hipMemcpy(&h, c->sc, sizeof(Scalars), hipMemcpyDeviceToHost));
     c->time_host = h.time;
     c->last_dt = h.dt;
+    if (h.err) {
+        int zero = 0;
+        HIPCHK(hipMemcpy(&c->sc->err, &zero, sizeof(int), hipMemcpyHostToDevice));
+        return set_err(MPM_E_UNSUPPORTED, "a particle lay outside its sorted tile");
+    }
     return MPM_OK;
 }
 
@@ -764,6 +792,13 @@ int mpm_set_option(mpm_ctx *c, int option, int value) {
     else if (option == MPM_OPT_SORTED) {
         c->sorted = value;
         c->all_dirty = true;
+        c->dt_next_valid = false;
+    } else if (option == MPM_OPT_FAST) {
+        c->fast = value;
+        c->dt_next_valid = false;
+    } else if (option == MPM_OPT_FUSED_DT) {
+        c->fuse_dt = value;
+        c->dt_next_valid = false;
     }
     else return set_err(MPM_E_ARG, "unknown option");
     return MPM_OK;

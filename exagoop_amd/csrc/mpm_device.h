@@ -426,11 +426,13 @@ __device__ __forceinline__ void stage_redistribute(const GridP &g, double x[3], 
     for (int d = 0; d < 3; ++d) {
         if (!g.per[d]) continue;
         const double L_ = g.phi[d] - g.plo[d];
+        // bounded shifts: a non-finite or runaway position is left outside
+        // (and removed below only in non-periodic directions)
         if (x[d] > g.rhi[d]) {
-            while (x[d] > g.rhi[d]) x[d] -= L_;
+            for (int k = 0; k < 64 && x[d] > g.rhi[d]; ++k) x[d] -= L_;
             if (x[d] < g.plo[d]) x[d] = g.plo[d];
         } else if (x[d] < g.plo[d]) {
-            while (x[d] < g.plo[d]) x[d] += L_;
+            for (int k = 0; k < 64 && x[d] < g.plo[d]; ++k) x[d] += L_;
             if (x[d] > g.rhi[d]) x[d] = g.rhi[d];
         }
     }

@@ -81,6 +81,7 @@ struct Scalars {
     double dt;
     double time;
     unsigned long long dtmin_bits;  // running min of the per-particle dt (positive doubles)
+    unsigned long long dtmin_next_bits;  // same, accumulated by the step for the next step
     int err;
     int pad;
 };
@@ -117,8 +118,9 @@ void launch_grid(const GridP &g, const PhysP &ph, const LsP &ls, const UdfP &udf
 void launch_particle(const GridP &g, const PhysP &ph, const LsP &ls, const UdfP &udf,
                      const PState &P, const NState &N, int flags, double dt, double time,
                      const Scalars *sc, hipStream_t s);
+// fused: take the minimum the previous step's particle kernel accumulated.
 void launch_dt(const GridP &g, const PhysP &ph, const PState &P, Scalars *sc, int fixed,
-               double fixed_dt, hipStream_t s);
+               double fixed_dt, hipStream_t s, int fused = 0);
 void launch_advance_time(Scalars *sc, hipStream_t s);
 void launch_energies(const PState &P, double *out6, hipStream_t s);
 void launch_snapshot_vel(const PState &P, hipStream_t s);
@@ -137,11 +139,23 @@ struct SortState {
     int *tile_epoch;  // node tiles: last epoch a deposit landed there
 };
 size_t sort_scan_bytes(int nkeys);
-void launch_sort(const GridP &g, const PState &P, SortState &S, hipStream_t s);
+// out_id: id array of the buffer the step writes into (slots past the live
+// count are marked free), or null.
+void launch_sort(const GridP &g, const PState &P, SortState &S, int64_t *out_id, hipStream_t s);
+// fallback: particles may lie outside their tile (not redistributed every
+// step); they deposit straight to global memory.  Without it such a particle
+// sets Scalars::err.
 void launch_p2g_tiles(const GridP &g, const PhysP &ph, const PState &P, const SortState &S,
-                      const NState &N, int epoch, int momentum_only, hipStream_t s);
+                      const NState &N, int epoch, int momentum_only, int fallback, Scalars *sc,
+                      hipStream_t s);
 void launch_particle_sorted(const GridP &g, const PhysP &ph, const LsP &ls, const UdfP &udf,
                             const PState &Pin, const PState &Pout, const SortState &S,
                             const NState &N, int flags, const Scalars *sc, hipStream_t s);
+// Specialised particle kernel (uniform order, no level set, no UDF); also
+// accumulates the next step's dt minimum.
+bool particle_fast_ok(const GridP &g, const LsP &ls, const UdfP &udf);
+void launch_particle_fast(const GridP &g, const PhysP &ph, const PState &Pin, const PState &Pout,
+                          const SortState &S, const NState &N, int vel, int redist, Scalars *sc,
+                          hipStream_t s);
 
 }  // namespace xmpm

@@ -363,12 +363,15 @@ __global__ void k_dt_init(Scalars *sc) {
     sc->dtmin_bits = (unsigned long long)__double_as_longlong(__builtin_huge_val());
 }
 
-__global__ void k_dt_final(Scalars *sc, PhysP ph, int fixed, double fixed_dt) {
+__global__ void k_dt_final(Scalars *sc, PhysP ph, int fixed, double fixed_dt, int fused) {
+    const unsigned long long bits = fused ? sc->dtmin_next_bits : sc->dtmin_bits;
+    // the step about to run accumulates the following step's minimum
+    sc->dtmin_next_bits = (unsigned long long)__double_as_longlong(__builtin_huge_val());
     if (fixed) {
         sc->dt = fixed_dt;
         return;
     }
-    double dt = __longlong_as_double((long long)sc->dtmin_bits);
+    double dt = __longlong_as_double((long long)bits);
     if (dt == __builtin_huge_val()) dt = 1.79769313486231570815e+308;  // numeric_limits max
     dt = ph.cfl * dt;
     dt = fmax(fmin(dt, ph.dt_max), ph.dt_min);
@@ -463,13 +466,14 @@ void launch_particle(const GridP &g, const PhysP &ph, const LsP &ls, const UdfP 
 }
 
 void launch_dt(const GridP &g, const PhysP &ph, const PState &P, Scalars *sc, int fixed,
-               double fixed_dt, hipStream_t s) {
-    if (!fixed) {
+               double fixed_dt, hipStream_t s, int fused) {
+    if (!fixed && !fused) {
         hipLaunchKernelGGL(k_dt_init, dim3(1), dim3(1), 0, s, sc);
         if (P.np > 0)
             hipLaunchKernelGGL(k_dt, dim3(blocks_for(P.np)), dim3(kBlock), 0, s, g, P, sc);
     }
-    hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(1), 0, s, sc, ph, fixed, fixed_dt);
+    hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(1), 0, s, sc, ph, fixed, fixed_dt,
+                       (fixed || !fused) ? 0 : 1);
 }
 
 void launch_advance_time(Scalars *sc, hipStream_t s) {

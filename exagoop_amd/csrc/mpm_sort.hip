@@ -2,17 +2,23 @@
 //
 // Each step the live particles are counting-sorted by cell in tile-major
 // order (kTileCells^3-cell tiles, cells x-fastest inside a tile):
-//   k_keys     key + rank within the cell (atomic increment returns the rank)
+//   k_keys     key + rank within the cell (one atomic per run of equal keys
+//              in a wavefront; particles arrive nearly sorted)
 //   scan       exclusive sum of the per-cell counts (hipCUB)
 //   k_scatter  perm[start[key] + rank] = current index
-// P2G then runs one workgroup per tile: particles are visited cell-major
-// across the 64 lanes of a wavefront (lane = cell, so the 64 lanes of one
-// ds_add hit 64 distinct nodes), deposits go to an LDS block of
-// (kTileCells+3)^3 nodes x 7 components, and the block is flushed with one
-// global FP64 atomic per non-zero node component.
-// The fused particle kernel gathers each particle through perm and writes it
-// to the other buffer in sorted order, so particle state stays cell-sorted
-// and the next step's sort is close to the identity.
+// P2G (k_p2g_groups) runs one workgroup per tile.  A wavefront takes the 16
+// cells of one z-slice of the tile, four cells (one x-row) at a time, 16
+// lanes per cell: lane (ky, kz) owns the four stencil nodes along x of one
+// (y, z) stencil row and accumulates the cell's particles into registers;
+// the 12 1-D weights of a particle are computed once (one per lane) and
+// shared by lane shuffles.  Per-cell sums go into a private LDS slab of the
+// wavefront with plain read-modify-write (the 64 lanes always address 64
+// distinct nodes), the four slabs are summed at the end and flushed with one
+// global FP64 atomic per non-zero node component.  No LDS atomics.
+// The fused particle kernel (k_particle_tile) stages the tile's node block
+// (v, dv) in LDS, runs G2P / F / D / move / volume / stress per particle
+// and writes the particle to the other buffer in sorted order, so particle
+// state stays cell-sorted and the next sort is close to the identity.
 
 #include <hipcub/hipcub.hpp>
 
@@ -23,9 +29,12 @@ namespace xmpm {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kTN = kTileCells + 3;          // LDS nodes per edge
-constexpr int kTNN = kTN * kTN * kTN;        // LDS nodes per tile
+constexpr int kTN = kTileCells + 3;                    // LDS nodes per tile edge (offsets -1..+2)
+constexpr int kTNN = kTN * kTN * kTN;                  // LDS nodes per tile
 constexpr int kCellsPerTile = kTileCells * kTileCells * kTileCells;
+constexpr int kSlab = kTN * kTN * kTileCells;          // nodes of one z-slice's slab (7 x 7 x 4)
+
+static_assert(kTileCells == 4 && kBlock == 256, "wavefront/cell mapping assumes 4^3 tiles");
 
 // Blocks are dispatched round-robin over the 8 XCDs; give each XCD a
 // contiguous range of tiles so neighbouring tiles (which share halo nodes
@@ -46,36 +55,61 @@ __device__ __forceinline__ int cell_key(const GridP &g, const int c[3]) {
 
 __global__ __launch_bounds__(kBlock) void k_keys(GridP g, PState P, SortState S) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= P.np) return;
-    int key = S.nkeys;  // removed particles
-    if (P.id[i] >= 0) {
-        int c[3];
-        for (int d = 0; d < 3; ++d) {
-            // cell as getParticleCell computes it; a particle outside the
-            // domain (not yet redistributed) is binned with the nearest cell
-            // and deposits through the global path of the tile kernel.
-            int v = (int)floor((P.pos[d * P.cap + i] - g.plo[d]) * g.dxi[d]);
-            c[d] = v < 0 ? 0 : (v > g.hi[d] ? g.hi[d] : v);
+    const bool valid = i < P.np;
+    int key = -1;
+    if (valid) {
+        key = S.nkeys;  // removed particles
+        if (P.id[i] >= 0) {
+            int c[3];
+            for (int d = 0; d < 3; ++d) {
+                // cell as getParticleCell computes it; a particle outside the
+                // domain (not yet redistributed) is binned with the nearest
+                // cell and deposits through the global path.
+                int v = (int)floor((P.pos[d * P.cap + i] - g.plo[d]) * g.dxi[d]);
+                c[d] = v < 0 ? 0 : (v > g.hi[d] ? g.hi[d] : v);
+            }
+            key = cell_key(g, c);
         }
-        key = cell_key(g, c);
     }
+    const int lane = threadIdx.x & 63;
+    const int prev = __shfl_up(key, 1, 64);
+    const bool head = valid && (lane == 0 || prev != key);
+    const unsigned long long heads = __ballot(head);
+    const unsigned long long upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
+    const int hl = 63 - __builtin_clzll((heads & upto) | 1ull);  // this run's head lane
+    const unsigned long long above = heads & ~upto;
+    const int next = above ? __builtin_ctzll(above) : 64;
+    int base = 0;
+    if (head) {
+        int end = next;
+        const int64_t wave0 = i - lane;
+        if (wave0 + end > P.np) end = (int)(P.np - wave0);  // lanes past np are in no run
+        base = atomicAdd(&S.count[key], end - lane);
+    }
+    base = __shfl(base, hl, 64);
+    if (!valid) return;
     S.key[i] = key;
-    S.rank[i] = atomicAdd(&S.count[key], 1);
+    S.rank[i] = base + (lane - hl);
 }
 
-__global__ __launch_bounds__(kBlock) void k_scatter(PState P, SortState S) {
+// perm[start[key] + rank] = i; re-zero the counts; mark output slots past the
+// live count as free (removed particles are dropped by the reorder).
+__global__ __launch_bounds__(kBlock) void k_scatter(PState P, SortState S, int64_t *out_id) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P.np) return;
     const int key = S.key[i];
     if (key < S.nkeys) S.perm[S.cell_start[key] + S.rank[i]] = (int)i;
-    S.count[key] = 0;  // ready for the next sort
+    S.count[key] = 0;
+    if (out_id && i >= S.cell_start[S.nkeys]) out_id[i] = -1;
 }
 
 // One particle's deposit straight to the global accumulators (particles
-// outside their tile's LDS block).
-__device__ void deposit_global(const GridP &g, const PhysP &ph, const double x[3], double m,
-                               const double vel[3], double V, const double T[9], bool forces,
-                               bool mass, double *acc, int *tile_epoch, int epoch) {
+// outside their tile's node block).
+__device__ __attribute__((noinline)) void deposit_global(const GridP &g, const PhysP &ph,
+                                                         const double x[3], double m,
+                                                         const double vel[3], double V,
+                                                         const double T[9], bool forces,
+                                                         double *acc, int *tile_epoch, int epoch) {
     int iv[3], mn[3], mx[3];
     for (int d = 0; d < 3; ++d) {
         iv[d] = (int)floor((x[d] - g.plo[d]) * g.dxi[d]);
@@ -124,7 +158,7 @@ __device__ void deposit_global(const GridP &g, const PhysP &ph, const double x[3
                     for (int qy = 0; qy < cy; ++qy)
                         for (int qx = 0; qx < cx; ++qx) {
                             const int64_t node = nidx(g, tx[qx], ty[qy], tz[qz]);
-                            for (int c = mass ? 0 : 1; c < (forces ? 7 : 4); ++c)
+                            for (int c = forces ? 0 : 1; c < (forces ? 7 : 4); ++c)
                                 unsafeAtomicAdd(&acc[c * nn + node], val[c]);
                             tile_epoch[node_tile(g, tx[qx], ty[qy], tz[qz])] = epoch;
                         }
@@ -135,156 +169,197 @@ __device__ void deposit_global(const GridP &g, const PhysP &ph, const double x[3
 
 // deposit_onto_grid_momentum (mpm_particle_grid_ops.cpp:333-587) over one
 // cell tile.  FORCES: mass, momentum and forces (P2G(1,1,1)); otherwise
-// momentum only (the MUSL re-projection P2G(0,1,0)).
-template <bool FORCES>
-__global__ __launch_bounds__(kBlock) void k_p2g_tile(GridP g, PhysP ph, PState P, SortState S,
-                                                     double *acc, int epoch) {
+// momentum only (the MUSL re-projection P2G(0,1,0)).  Stencil offsets
+// -1..2 in every direction; offsets compute_bounds excludes get zero weight
+// (they only address nodes outside the domain, which the reference clips).
+template <bool FORCES, bool FALLBACK>
+__global__ __launch_bounds__(kBlock) void k_p2g_groups(GridP g, PhysP ph, PState P, SortState S,
+                                                       double *acc, int epoch, Scalars *sc) {
     constexpr int NC = FORCES ? 7 : 3;
-    __shared__ double lds[NC * kTNN];
+    __shared__ double slab[4][NC * kSlab];  // one private slab per wavefront
     __shared__ int cs[kCellsPerTile + 1];
-    __shared__ int maxc;
     const int tile = xcd_remap(blockIdx.x, gridDim.x);
     const int tid = threadIdx.x;
     if (tid <= kCellsPerTile) cs[tid] = S.cell_start[tile * kCellsPerTile + tid];
-    if (tid == 0) maxc = 0;
     __syncthreads();
     if (cs[kCellsPerTile] == cs[0]) return;  // empty tile (uniform over the block)
-    for (int e = tid; e < NC * kTNN; e += kBlock) lds[e] = 0.0;
-    if (tid < kCellsPerTile) atomicMax(&maxc, cs[tid + 1] - cs[tid]);
+    const int wave = tid >> 6, lane = tid & 63;
+    double *my = slab[wave];
+    for (int e = lane; e < NC * kSlab; e += 64) my[e] = 0.0;
     const int tx = tile % g.ntc[0];
     const int ty = (tile / g.ntc[0]) % g.ntc[1];
     const int tz = tile / (g.ntc[0] * g.ntc[1]);
     const int T0[3] = {tx * kTileCells, ty * kTileCells, tz * kTileCells};
     __syncthreads();
     const int64_t cap = P.cap;
-    const int cell = tid % kCellsPerTile;
-    const int cnt = cs[cell + 1] - cs[cell];
-    for (int r = tid / kCellsPerTile; r < maxc; r += kBlock / kCellsPerTile) {
-        if (r >= cnt) continue;
-        const int64_t j = S.perm[cs[cell] + r];
-        if (P.idata[MPM_I_PHASE * cap + j] != 0) continue;
-        double x[3];
-        int iv[3], mn[3], mx[3];
-        bool inside = true;
-        for (int d = 0; d < 3; ++d) {
-            x[d] = P.pos[d * cap + j];
-            iv[d] = (int)floor((x[d] - g.plo[d]) * g.dxi[d]);
-            inside &= (iv[d] >= T0[d] && iv[d] < T0[d] + kTileCells);
-            bounds(iv[d], 0, g.hi[d], g.od[d], g.per[d], mn[d], mx[d]);
-        }
-        const double m = P.r[MPM_R_MASS * cap + j];
-        double vel[3];
-        for (int d = 0; d < 3; ++d) vel[d] = P.r[(MPM_R_VEL + d) * cap + j];
-        double T[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-        double V = 0.0;
-        if (FORCES) {
-            int ind = 0;
-            for (int a = 0; a < 3; a++)
-                for (int b = a; b < 3; b++) {
-                    const double s = P.r[(MPM_R_STRESS + ind) * cap + j];
-                    T[a * 3 + b] = s;
-                    T[b * 3 + a] = s;
-                    ind++;
+    const int q = lane >> 4;    // cell x inside the tile
+    const int gl = lane & 15;   // lane inside the cell's group
+    const int ky = gl & 3, kz = gl >> 2;
+    const int gbase = lane & ~15;
+    const int e_dim = gl < 4 ? 0 : (gl < 8 ? 1 : 2);  // the 1-D factor this lane evaluates
+    const int e_l = (gl & 3) - 1;
+    const int cz = wave;
+    for (int cy = 0; cy < kTileCells; ++cy) {
+        const int cell = q + kTileCells * (cy + kTileCells * cz);
+        const int beg = cs[cell], cnt = cs[cell + 1] - beg;
+        int maxcnt = cnt;
+        maxcnt = max(maxcnt, __shfl_xor(maxcnt, 16, 64));
+        maxcnt = max(maxcnt, __shfl_xor(maxcnt, 32, 64));
+        double a4[4][NC];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+#pragma unroll
+            for (int c = 0; c < NC; ++c) a4[k][c] = 0.0;
+        for (int t = 0; t < maxcnt; ++t) {
+            const bool act = t < cnt;
+            const int64_t j = act ? S.perm[beg + t] : 0;
+            bool use = act && P.idata[MPM_I_PHASE * cap + j] == 0;
+            double x[3] = {0, 0, 0}, vel[3] = {0, 0, 0}, m = 0.0, V = 0.0;
+            double T[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+            int iv[3] = {T0[0], T0[1], T0[2]};
+            bool inside = true;
+            if (use) {
+                for (int d = 0; d < 3; ++d) {
+                    x[d] = P.pos[d * cap + j];
+                    vel[d] = P.r[(MPM_R_VEL + d) * cap + j];
+                    iv[d] = (int)floor((x[d] - g.plo[d]) * g.dxi[d]);
+                    inside &= (iv[d] >= T0[d] && iv[d] < T0[d] + kTileCells);
                 }
-            V = P.r[MPM_R_VOLUME * cap + j];
-        }
-        if (!inside) {
-            deposit_global(g, ph, x, m, vel, V, T, FORCES, FORCES, acc, S.tile_epoch, epoch);
-            continue;
-        }
-        double bv[3][4], bd[3][4];
-        for (int d = 0; d < 3; ++d)
-            for (int l = mn[d]; l < mx[d]; ++l)
-                basis1d(g.od[d], g.per[d], iv[d], l, 0, g.hi[d], x[d], g.plo[d], g.dx[d],
-                        bv[d][l - mn[d]], bd[d][l - mn[d]]);
-        // Per-particle factors: momentum m v, body force m g (+ slab load),
-        // internal-force operator B = -V sigma.
-        double mv[3], a[3], B[9];
-        const bool slab = FORCES && ph.extloads && x[0] > ph.slab_lo[0] &&
-                          x[0] < ph.slab_hi[0] && x[1] > ph.slab_lo[1] && x[1] < ph.slab_hi[1] &&
-                          x[2] > ph.slab_lo[2] && x[2] < ph.slab_hi[2];
-        for (int d = 0; d < 3; ++d) {
-            mv[d] = m * vel[d];
-            a[d] = m * ph.grav[d] + (slab ? ph.extforce[d] : 0.0);
-        }
-        for (int q = 0; q < 9; ++q) B[q] = -V * T[q];
-        const int lx = iv[0] - T0[0] + 1, ly = iv[1] - T0[1] + 1, lz = iv[2] - T0[2] + 1;
-        for (int n = mn[2]; n < mx[2]; ++n) {
-            const double wz = bv[2][n - mn[2]], dwz = bd[2][n - mn[2]];
-            for (int mm = mn[1]; mm < mx[1]; ++mm) {
-                const double wy = bv[1][mm - mn[1]], dwy = bd[1][mm - mn[1]];
-                const double wyz = wy * wz, dwyz = dwy * wz, wydz = wy * dwz;
+                m = P.r[MPM_R_MASS * cap + j];
+                if (FORCES) {
+                    int ind = 0;
+                    for (int a = 0; a < 3; a++)
+                        for (int b = a; b < 3; b++) {
+                            const double s = P.r[(MPM_R_STRESS + ind) * cap + j];
+                            T[a * 3 + b] = s;
+                            T[b * 3 + a] = s;
+                            ind++;
+                        }
+                    V = P.r[MPM_R_VOLUME * cap + j];
+                }
+                if (!inside) {
+                    // only particles not redistributed this step can lie
+                    // outside their tile
+                    if (FALLBACK) {
+                        if (gl == 0)
+                            deposit_global(g, ph, x, m, vel, V, T, FORCES, acc, S.tile_epoch,
+                                           epoch);
+                    } else {
+                        sc->err = 1;
+                    }
+                    use = false;
+                }
+            }
+            // this lane's 1-D factor (value, derivative), zero where unused
+            double ev = 0.0, ed = 0.0;
+            {
+                const int d = e_dim;
+                int mn_, mx_;
+                bounds(iv[d], 0, g.hi[d], g.od[d], g.per[d], mn_, mx_);
+                double v_, d_;
+                basis1d(g.od[d], g.per[d], iv[d], e_l, 0, g.hi[d], x[d], g.plo[d], g.dx[d], v_, d_);
+                const bool u = use && e_l >= mn_ && e_l < mx_;
+                ev = u ? v_ : 0.0;
+                ed = u ? d_ : 0.0;
+            }
+            double wx[4], dwx[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                wx[k] = __shfl(ev, gbase + k, 64);
+                dwx[k] = __shfl(ed, gbase + k, 64);
+            }
+            const double wy = __shfl(ev, gbase + 4 + ky, 64), dwy = __shfl(ed, gbase + 4 + ky, 64);
+            const double wz = __shfl(ev, gbase + 8 + kz, 64), dwz = __shfl(ed, gbase + 8 + kz, 64);
+            {
+#pragma clang fp contract(fast)
+                // momentum m v, body force m g (+ slab load), internal-force
+                // operator B = -V sigma; row factors for this lane's (y, z)
+                const bool sl = FORCES && ph.extloads && x[0] > ph.slab_lo[0] &&
+                                x[0] < ph.slab_hi[0] && x[1] > ph.slab_lo[1] &&
+                                x[1] < ph.slab_hi[1] && x[2] > ph.slab_lo[2] &&
+                                x[2] < ph.slab_hi[2];
+                const double wyz = wy * wz;
                 const double Mr = m * wyz;
                 double Pr[3], C[3], E[3];
-                for (int d = 0; d < 3; ++d) {
-                    Pr[d] = mv[d] * wyz;
-                    if (FORCES) {
-                        C[d] = a[d] * wyz + B[d * 3 + 1] * dwyz + B[d * 3 + 2] * wydz;
-                        E[d] = B[d * 3 + 0] * wyz;
+                for (int d = 0; d < 3; ++d) Pr[d] = m * vel[d] * wyz;
+                if (FORCES) {
+                    const double dwyz = dwy * wz, wydz = wy * dwz;
+                    for (int d = 0; d < 3; ++d) {
+                        const double ad = m * ph.grav[d] + (sl ? ph.extforce[d] : 0.0);
+                        C[d] = ad * wyz - V * (T[d * 3 + 1] * dwyz + T[d * 3 + 2] * wydz);
+                        E[d] = -V * T[d * 3 + 0] * wyz;
                     }
                 }
-                double *row = lds + ((lz + n) * kTN + (ly + mm)) * kTN + lx;
-                for (int l = mn[0]; l < mx[0]; ++l) {
-                    const double wx = bv[0][l - mn[0]], dwx = bd[0][l - mn[0]];
-                    double *p = row + l;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
                     if (FORCES) {
-                        atomicAdd(p, wx * Mr);
-                        atomicAdd(p + 1 * kTNN, wx * Pr[0]);
-                        atomicAdd(p + 2 * kTNN, wx * Pr[1]);
-                        atomicAdd(p + 3 * kTNN, wx * Pr[2]);
-                        atomicAdd(p + 4 * kTNN, wx * C[0] + dwx * E[0]);
-                        atomicAdd(p + 5 * kTNN, wx * C[1] + dwx * E[1]);
-                        atomicAdd(p + 6 * kTNN, wx * C[2] + dwx * E[2]);
+                        a4[k][0] += wx[k] * Mr;
+                        for (int d = 0; d < 3; ++d) a4[k][1 + d] += wx[k] * Pr[d];
+                        for (int d = 0; d < 3; ++d) a4[k][4 + d] += wx[k] * C[d] + dwx[k] * E[d];
                     } else {
-                        atomicAdd(p, wx * Pr[0]);
-                        atomicAdd(p + 1 * kTNN, wx * Pr[1]);
-                        atomicAdd(p + 2 * kTNN, wx * Pr[2]);
+                        for (int d = 0; d < 3; ++d) a4[k][d] += wx[k] * Pr[d];
                     }
                 }
             }
         }
+        // For a given k the 64 lanes address 64 distinct slab nodes
+        // (x = q + k, y = cy + ky, z = kz), so the read-modify-write needs no
+        // atomics; but node q + k of step k is node (q+1) + (k-1) of the next
+        // group's step k-1.  LDS operations of one wavefront execute in issue
+        // order, so it suffices that the compiler keeps the steps apart.
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int idx = (kz * kTN + (cy + ky)) * kTN + (q + k);
+#pragma unroll
+            for (int c = 0; c < NC; ++c) my[c * kSlab + idx] += a4[k][c];
+            asm volatile("" ::: "memory");
+        }
     }
     __syncthreads();
-    // Flush: one global atomic per non-zero node component, periodic image
-    // targets as in deposit_global; mark the node tiles that received data.
+    // Sum the four slabs per tile node and flush: one global atomic per
+    // non-zero node component; periodic image targets as in deposit_global;
+    // mark the node tiles that received data.
     const int64_t nn = g.nn;
     for (int e = tid; e < kTNN; e += kBlock) {
-        double v[NC];
-        bool any = false;
-        for (int c = 0; c < NC; ++c) {
-            v[c] = lds[c * kTNN + e];
-            any |= (v[c] != 0.0);
-        }
-        if (!any) continue;
         const int ax = e % kTN, ay = (e / kTN) % kTN, az = e / (kTN * kTN);
+        double v[NC];
+        for (int c = 0; c < NC; ++c) v[c] = 0.0;
+        for (int w = 0; w < 4; ++w) {
+            const int k = az - w;
+            if (k < 0 || k >= kTileCells) continue;
+            const int idx = (k * kTN + ay) * kTN + ax;
+            for (int c = 0; c < NC; ++c) v[c] += slab[w][c * kSlab + idx];
+        }
+        bool any = false;
+        for (int c = 0; c < NC; ++c) any |= (v[c] != 0.0);
+        if (!any) continue;
         int tx_[2], ty_[2], tz_[2];
         bool ix[2], iy[2], iz[2];
         const int cx = targets(T0[0] + ax - 1, g.n[0], g.per[0], tx_, ix);
         const int cy = targets(T0[1] + ay - 1, g.n[1], g.per[1], ty_, iy);
-        const int cz = targets(T0[2] + az - 1, g.n[2], g.per[2], tz_, iz);
-        for (int qz = 0; qz < cz; ++qz)
+        const int cz2 = targets(T0[2] + az - 1, g.n[2], g.per[2], tz_, iz);
+        for (int qz = 0; qz < cz2; ++qz)
             for (int qy = 0; qy < cy; ++qy)
                 for (int qx = 0; qx < cx; ++qx) {
                     const int64_t node = nidx(g, tx_[qx], ty_[qy], tz_[qz]);
                     for (int c = 0; c < NC; ++c)
-                        if (v[c] != 0.0) unsafeAtomicAdd(&acc[(FORCES ? c : c + 1) * nn + node], v[c]);
+                        if (v[c] != 0.0)
+                            unsafeAtomicAdd(&acc[(FORCES ? c : c + 1) * nn + node], v[c]);
                     S.tile_epoch[node_tile(g, tx_[qx], ty_[qy], tz_[qz])] = epoch;
                 }
     }
 }
 
-// Fused per-particle pipeline of the sorted step: gather particle perm[i]
-// from Pin, run the stages, write it to Pout at i.
+// Fused per-particle pipeline of the sorted step (generic: level sets,
+// UDFs, mixed orders): gather particle perm[i] from Pin, run the stages,
+// write it to Pout at i.
 __global__ __launch_bounds__(kBlock) void k_particle_sorted(GridP g, PhysP ph, LsP ls, UdfP udf,
                                                             PState Pin, PState Pout, SortState S,
                                                             NState N, int flags,
                                                             const Scalars *sc) {
     const int nlive = S.cell_start[S.nkeys];
     const int64_t i = (int64_t)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (i >= nlive) {
-        if (i < Pin.np) Pout.id[i] = -1;  // slot freed by a removed particle
-        return;
-    }
+    if (i >= nlive) return;
     const int64_t j = S.perm[i];
     const double dt = sc->dt;
     const double time = sc->time;
@@ -301,6 +376,281 @@ __global__ __launch_bounds__(kBlock) void k_particle_sorted(GridP g, PhysP ph, L
     store_all(Pout, i, R);
 }
 
+// Calculate_time_step's per-particle candidate (mpm_particle_timestep.cpp:
+// 51-102) from the state a step leaves behind.
+__device__ __forceinline__ double dt_candidate(const GridP &g, int model, double K, double E,
+                                               double nu, double rho, const double v[3]) {
+    double Cs;
+    if (model == 1) {
+        Cs = sqrt(K / rho);
+    } else {
+        const double lam = E * nu / ((1 + nu) * (1 - 2.0 * nu));
+        const double mu = E / (2.0 * (1 + nu));
+        Cs = sqrt((lam + 2.0 * mu) / rho);
+    }
+    double vm = 0.0;
+    for (int d = 0; d < 3; ++d) vm += v[d] * v[d];
+    vm = sqrt(vm);
+    double dxmin = g.dx[0];
+    for (int d = 1; d < 3; ++d) dxmin = fmin(dxmin, g.dx[d]);
+    return dxmin / (Cs + vm);
+}
+
+// Wavefront min and one atomic-min per wavefront into the next step's
+// running minimum.  Every lane of the wavefront must call it.
+__device__ __forceinline__ void dt_wave_min(double c, Scalars *sc) {
+    for (int off = 32; off > 0; off >>= 1) c = fmin(c, __shfl_xor(c, off, 64));
+    if ((threadIdx.x & 63) == 0 && c < __builtin_huge_val())
+        atomicMin(&sc->dtmin_next_bits, (unsigned long long)__double_as_longlong(c));
+}
+
+// Fused per-particle pipeline for the common case (every direction order O,
+// no level-set body, no wall UDF), one workgroup per cell tile: the tile's
+// 7^3 node block of v (and dv) is staged in LDS, then each thread takes one
+// particle of the tile: G2P over a fixed-width unrolled stencil (offsets
+// outside compute_bounds are skipped; the reference never reads them), F, D,
+// move + wall BCs, redistribution, volume, stress, written in sorted order.
+// The next step's dt candidates are reduced on the way out.
+template <int O, bool VEL, bool DELTA>
+__global__ __launch_bounds__(kBlock) void k_particle_tile(GridP g, PhysP ph, PState Pin,
+                                                          PState Pout, SortState S, NState N,
+                                                          int redist, Scalars *sc) {
+    constexpr int W = (O == 1) ? 2 : 4;
+    constexpr int L0 = (O == 1) ? 0 : -1;
+    constexpr int NCV = VEL ? 6 : 3;
+    __shared__ double nodes[kTNN * NCV];
+    const int tile = xcd_remap(blockIdx.x, gridDim.x);
+    const int tid = threadIdx.x;
+    const int beg = S.cell_start[tile * kCellsPerTile];
+    const int end = S.cell_start[tile * kCellsPerTile + kCellsPerTile];
+    if (beg == end) return;  // uniform over the block
+    const int tx = tile % g.ntc[0];
+    const int ty = (tile / g.ntc[0]) % g.ntc[1];
+    const int tz = tile / (g.ntc[0] * g.ntc[1]);
+    const int T0[3] = {tx * kTileCells, ty * kTileCells, tz * kTileCells};
+    const int64_t nn = g.nn;
+    for (int e = tid; e < kTNN; e += kBlock) {
+        const int a[3] = {e % kTN, (e / kTN) % kTN, e / (kTN * kTN)};
+        bool ok = true;
+        int I[3];
+        for (int d = 0; d < 3; ++d) I[d] = wrap_read(T0[d] - 1 + a[d], g.n[d], g.per[d], ok);
+        const int64_t qn = nidx(g, I[0], I[1], I[2]);
+        for (int c = 0; c < 3; ++c) {
+            nodes[e * NCV + c] = ok ? N.vel[c * nn + qn] : 0.0;
+            if (VEL) nodes[e * NCV + 3 + c] = ok ? N.dvel[c * nn + qn] : 0.0;
+        }
+    }
+    __syncthreads();
+    const int64_t ci = Pin.cap, co = Pout.cap;
+    const double dt = sc->dt;
+    const double time = sc->time;
+    double cmin = __builtin_huge_val();
+    const int niter = (end - beg + kBlock - 1) / kBlock;
+    for (int it = 0; it < niter; ++it) {
+        const int64_t i = beg + (int64_t)it * kBlock + tid;
+        if (i >= end) continue;
+        const int64_t j = S.perm[i];
+        int64_t id = Pin.id[j];
+        const int phase = Pin.idata[MPM_I_PHASE * ci + j];
+        double x[3], v[3], vp[3], D[6], F[9];
+        for (int d = 0; d < 3; ++d) {
+            x[d] = Pin.pos[d * ci + j];
+            v[d] = Pin.r[(MPM_R_VEL + d) * ci + j];
+        }
+        if (phase == 0) {
+            int iv[3], mn[3], mx[3];
+            bool inside = true;
+            for (int d = 0; d < 3; ++d) {
+                iv[d] = (int)floor((x[d] - g.plo[d]) * g.dxi[d]);
+                bounds(iv[d], 0, g.hi[d], O, g.per[d], mn[d], mx[d]);
+                inside &= (iv[d] >= T0[d] && iv[d] < T0[d] + kTileCells);
+            }
+            double wi[3][W], bv[3][W], bd[3][W];
+            bool use[3][W];
+            int ni[3][W];
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+#pragma unroll
+                for (int k = 0; k < W; ++k) {
+                    const int l = L0 + k;
+                    use[d][k] = (l >= mn[d] && l < mx[d]);
+                    basis1d(O, g.per[d], iv[d], l, 0, g.hi[d], x[d], g.plo[d], g.dx[d], bv[d][k],
+                            bd[d][k]);
+                    // bilin_interp_two's weights are the linear basis values
+                    wi[d][k] = (O == 1) ? bv[d][k]
+                                        : interp1d(O, g.per[d], iv[d] + l, 0, g.hi[d], x[d],
+                                                   g.plo[d], g.dx[d]);
+                    ni[d][k] = iv[d] + l - T0[d] + 1;  // LDS block index
+                }
+            double vals0[3] = {0, 0, 0}, vals1[3] = {0, 0, 0};
+            double L[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+#pragma unroll
+            for (int kz = 0; kz < W; ++kz)
+#pragma unroll
+                for (int ky = 0; ky < W; ++ky)
+#pragma unroll
+                    for (int kx = 0; kx < W; ++kx) {
+                        if (!(use[2][kz] && use[1][ky] && use[0][kx])) continue;
+                        if (!inside) continue;  // handled by g2p_global below
+                        double nv[3], nd[3] = {0, 0, 0};
+                        const int e = (ni[2][kz] * kTN + ni[1][ky]) * kTN + ni[0][kx];
+                        for (int d = 0; d < 3; ++d) {
+                            nv[d] = nodes[e * NCV + d];
+                            if (VEL) nd[d] = nodes[e * NCV + 3 + d];
+                        }
+                        if (VEL) {
+                            const double w = wi[0][kx] * wi[1][ky] * wi[2][kz];
+                            for (int d = 0; d < 3; ++d) {
+                                vals0[d] += w * nv[d];
+                                vals1[d] += w * nd[d];
+                            }
+                        }
+                        const double vx = bv[0][kx], vy = bv[1][ky], vz = bv[2][kz];
+                        const double gr[3] = {bd[0][kx] * vy * vz, vx * bd[1][ky] * vz,
+                                              vx * vy * bd[2][kz]};
+                        for (int d1 = 0; d1 < 3; ++d1)
+                            for (int d2 = 0; d2 < 3; ++d2) L[d1][d2] += nv[d1] * gr[d2];
+                    }
+            // Every particle lies in its tile when particles are redistributed
+            // each step (the only configuration this kernel is launched for).
+            if (!inside) sc->err = 1;
+            if (VEL) {
+                const double alpha = ph.alpha;
+                for (int d = 0; d < 3; ++d) {
+                    vp[d] = vals0[d];
+                    v[d] = alpha * v[d] + alpha * vals1[d] + (1 - alpha) * vals0[d];
+                }
+            } else {
+                for (int d = 0; d < 3; ++d) vp[d] = Pin.r[(MPM_R_VELP + d) * ci + j];
+            }
+            for (int q = 0; q < 9; ++q) F[q] = Pin.r[(MPM_R_DEFGRAD + q) * ci + j];
+            // get_deformation_gradient_tensor (interpolants.H:1732-1787)
+            double nrm = 0.0;
+            for (int a = 0; a < 3; a++)
+                for (int b = 0; b < 3; b++) nrm += L[a][b] * L[a][b];
+            if (!(nrm >= 1.0e10)) {
+                double Fp[9];
+                for (int q = 0; q < 9; ++q) Fp[q] = F[q];
+                for (int a = 0; a < 3; ++a)
+                    for (int b = 0; b < 3; ++b) {
+                        double acc = 0.0;
+                        for (int q = 0; q < 3; ++q) {
+                            const double lp = (a == q) ? 1.0 + L[a][q] * dt : L[a][q] * dt;
+                            acc += lp * Fp[q * 3 + b];
+                        }
+                        F[a * 3 + b] = acc;
+                    }
+            }
+            int ind = 0;
+            for (int d1 = 0; d1 < 3; ++d1)
+                for (int d2 = d1; d2 < 3; ++d2) {
+                    D[ind] = 0.5 * (L[d1][d2] + L[d2][d1]);
+                    ind++;
+                }
+        } else {
+            for (int d = 0; d < 3; ++d) vp[d] = Pin.r[(MPM_R_VELP + d) * ci + j];
+            for (int q = 0; q < 6; ++q) D[q] = Pin.r[(MPM_R_STRAINRATE + q) * ci + j];
+            for (int q = 0; q < 9; ++q) F[q] = Pin.r[(MPM_R_DEFGRAD + q) * ci + j];
+        }
+        // moveParticles without level sets or UDF tables
+        // (mpm_particle_timestep.cpp:329-576), then Redistribute.
+        {
+            for (int d = 0; d < 3; ++d) x[d] += vp[d] * dt;
+            double rin[3], rout[3], wv[3] = {0, 0, 0};
+            for (int d = 0; d < 3; ++d) rin[d] = rout[d] = v[d];
+            for (int dir = 0; dir < 3; ++dir) {
+                int side = -1;
+                if (x[dir] < g.plo[dir]) side = 0;
+                else if (x[dir] > g.phi[dir]) side = 1;
+                if (side < 0) continue;
+                const double *wvel = side ? ph.wv_hi : ph.wv_lo;
+                for (int d = 0; d < 3; ++d) {
+                    wv[d] = wvel[dir * 3 + d];
+                    rin[d] -= wv[d];
+                }
+                double nrm_[3] = {0, 0, 0};
+                nrm_[dir] = side ? -1.0 : 1.0;
+                const int mod = applybc(rin, rout, side ? ph.mu_hi[dir] : ph.mu_lo[dir], nrm_,
+                                        side ? ph.bchi[dir] : ph.bclo[dir]);
+                if (mod > 0) x[dir] = 2.0 * (side ? g.phi[dir] : g.plo[dir]) - x[dir];
+                for (int q = 0; q < 3; ++q) v[q] = rout[q] + wv[q];
+            }
+            if (redist) stage_redistribute(g, x, id);
+        }
+        // remaining state, updateVolume and the constitutive update
+        const int model = Pin.idata[MPM_I_MODEL * ci + j];
+        const double m = Pin.r[MPM_R_MASS * ci + j];
+        const double V0 = Pin.r[MPM_R_VOLINIT * ci + j];
+        const double E = Pin.r[MPM_R_E * ci + j], nu = Pin.r[MPM_R_NU * ci + j];
+        const double K = Pin.r[MPM_R_BULK * ci + j];
+        const double gam = Pin.r[MPM_R_GAMMA * ci + j], mu = Pin.r[MPM_R_VISC * ci + j];
+        double eps[6], sig[6], J, V, rho, p;
+        for (int q = 0; q < 6; ++q) eps[q] = Pin.r[(MPM_R_STRAIN + q) * ci + j];
+        if (phase == 0) {
+            J = det3(F);
+            V = V0 * J;
+            rho = m / V;
+            const double rate = (time < ph.applied_time) ? ph.applied_rate : 0.0;
+            for (int q = 0; q < 6; ++q) eps[q] = eps[q] + dt * D[q];
+            eps[XX] += dt * rate;
+            eps[YY] += dt * rate;
+            eps[ZZ] += dt * rate;
+            p = (model == 1) ? 0.0 : Pin.r[MPM_R_PRESSURE * ci + j];
+            if (DELTA) {
+                double de[6], ds[6];
+                for (int q = 0; q < 6; ++q) de[q] = dt * D[q];
+                de[XX] += dt * rate;
+                de[YY] += dt * rate;
+                de[ZZ] += dt * rate;
+                linear_elastic(de, ds, E, nu);
+                for (int q = 0; q < 6; ++q) sig[q] = Pin.r[(MPM_R_STRESS + q) * ci + j] + ds[q];
+            } else if (model == 0) {
+                linear_elastic(eps, sig, E, nu);
+            } else if (model == 1) {
+                const double pinf = 0.0;
+                p = K * (pow(1.0 / J, gam) - 1.0) + pinf;
+                newtonian(D, sig, mu, p);
+            } else {
+                neo_hookean(F, sig, E, nu);
+            }
+        } else {
+            J = Pin.r[MPM_R_JACOBIAN * ci + j];
+            V = Pin.r[MPM_R_VOLUME * ci + j];
+            rho = Pin.r[MPM_R_DENSITY * ci + j];
+            p = Pin.r[MPM_R_PRESSURE * ci + j];
+            for (int q = 0; q < 6; ++q) sig[q] = Pin.r[(MPM_R_STRESS + q) * ci + j];
+        }
+        double *R = Pout.r;
+        for (int d = 0; d < 3; ++d) {
+            Pout.pos[d * co + i] = x[d];
+            R[(MPM_R_VEL + d) * co + i] = v[d];
+            R[(MPM_R_VELP + d) * co + i] = vp[d];
+        }
+        for (int q = 0; q < 6; ++q) {
+            R[(MPM_R_STRAINRATE + q) * co + i] = D[q];
+            R[(MPM_R_STRAIN + q) * co + i] = eps[q];
+            R[(MPM_R_STRESS + q) * co + i] = sig[q];
+        }
+        for (int q = 0; q < 9; ++q) R[(MPM_R_DEFGRAD + q) * co + i] = F[q];
+        R[MPM_R_VOLUME * co + i] = V;
+        R[MPM_R_MASS * co + i] = m;
+        R[MPM_R_DENSITY * co + i] = rho;
+        R[MPM_R_JACOBIAN * co + i] = J;
+        R[MPM_R_PRESSURE * co + i] = p;
+        R[MPM_R_VOLINIT * co + i] = V0;
+        R[MPM_R_E * co + i] = E;
+        R[MPM_R_NU * co + i] = nu;
+        R[MPM_R_BULK * co + i] = K;
+        R[MPM_R_GAMMA * co + i] = gam;
+        R[MPM_R_VISC * co + i] = mu;
+        Pout.idata[MPM_I_PHASE * co + i] = phase;
+        Pout.idata[MPM_I_MODEL * co + i] = model;
+        Pout.id[i] = id;
+        if (id >= 0 && phase == 0) cmin = fmin(cmin, dt_candidate(g, model, K, E, nu, rho, v));
+    }
+    dt_wave_min(cmin, sc);  // every lane reaches this point
+}
+
 inline unsigned blocks_for(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
 }  // namespace
@@ -312,25 +662,28 @@ size_t sort_scan_bytes(int nkeys) {
     return bytes;
 }
 
-void launch_sort(const GridP &g, const PState &P, SortState &S, hipStream_t s) {
+void launch_sort(const GridP &g, const PState &P, SortState &S, int64_t *out_id, hipStream_t s) {
     if (P.np <= 0) return;
     hipLaunchKernelGGL(k_keys, dim3(blocks_for(P.np)), dim3(kBlock), 0, s, g, P, S);
     size_t bytes = S.scan_bytes;
     (void)hipcub::DeviceScan::ExclusiveSum(S.scan_tmp, bytes, S.count, S.cell_start, S.nkeys + 1,
                                            s);
-    hipLaunchKernelGGL(k_scatter, dim3(blocks_for(P.np)), dim3(kBlock), 0, s, P, S);
+    hipLaunchKernelGGL(k_scatter, dim3(blocks_for(P.np)), dim3(kBlock), 0, s, P, S, out_id);
 }
 
 void launch_p2g_tiles(const GridP &g, const PhysP &ph, const PState &P, const SortState &S,
-                      const NState &N, int epoch, int momentum_only, hipStream_t s) {
+                      const NState &N, int epoch, int momentum_only, int fallback, Scalars *sc,
+                      hipStream_t s) {
     if (P.np <= 0) return;
-    const unsigned ntiles = (unsigned)(g.ntc[0] * g.ntc[1] * g.ntc[2]);
-    if (momentum_only)
-        hipLaunchKernelGGL(k_p2g_tile<false>, dim3(ntiles), dim3(kBlock), 0, s, g, ph, P, S,
-                           N.acc, epoch);
-    else
-        hipLaunchKernelGGL(k_p2g_tile<true>, dim3(ntiles), dim3(kBlock), 0, s, g, ph, P, S, N.acc,
-                           epoch);
+    const dim3 gr((unsigned)(g.ntc[0] * g.ntc[1] * g.ntc[2])), bl(kBlock);
+#define XMPM_PG(f, fb)                                                                        \
+    hipLaunchKernelGGL((k_p2g_groups<f, fb>), gr, bl, 0, s, g, ph, P, S, N.acc, epoch, sc)
+    if (momentum_only) {
+        if (fallback) XMPM_PG(false, true); else XMPM_PG(false, false);
+    } else {
+        if (fallback) XMPM_PG(true, true); else XMPM_PG(true, false);
+    }
+#undef XMPM_PG
 }
 
 void launch_particle_sorted(const GridP &g, const PhysP &ph, const LsP &ls, const UdfP &udf,
@@ -339,6 +692,34 @@ void launch_particle_sorted(const GridP &g, const PhysP &ph, const LsP &ls, cons
     if (Pin.np <= 0) return;
     hipLaunchKernelGGL(k_particle_sorted, dim3(blocks_for(Pin.np)), dim3(kBlock), 0, s, g, ph,
                        ls, udf, Pin, Pout, S, N, flags, sc);
+}
+
+bool particle_fast_ok(const GridP &g, const LsP &ls, const UdfP &udf) {
+    if (!g.uniform || ls.nb > 0) return false;
+    for (int d = 0; d < 3; ++d)
+        if (udf.tab[d][0] || udf.tab[d][1]) return false;
+    return true;
+}
+
+void launch_particle_fast(const GridP &g, const PhysP &ph, const PState &Pin, const PState &Pout,
+                          const SortState &S, const NState &N, int vel, int redist, Scalars *sc,
+                          hipStream_t s) {
+    if (Pin.np <= 0) return;
+    const dim3 gr((unsigned)(g.ntc[0] * g.ntc[1] * g.ntc[2])), bl(kBlock);
+    const int O = g.od[0];
+#define XMPM_PT(o, v, dl)                                                                     \
+    hipLaunchKernelGGL((k_particle_tile<o, v, dl>), gr, bl, 0, s, g, ph, Pin, Pout, S, N,   \
+                       redist, sc)
+    if (ph.delta) {
+        if (O == 1) { if (vel) XMPM_PT(1, true, true); else XMPM_PT(1, false, true); }
+        else if (O == 2) { if (vel) XMPM_PT(2, true, true); else XMPM_PT(2, false, true); }
+        else { if (vel) XMPM_PT(3, true, true); else XMPM_PT(3, false, true); }
+    } else {
+        if (O == 1) { if (vel) XMPM_PT(1, true, false); else XMPM_PT(1, false, false); }
+        else if (O == 2) { if (vel) XMPM_PT(2, true, false); else XMPM_PT(2, false, false); }
+        else { if (vel) XMPM_PT(3, true, false); else XMPM_PT(3, false, false); }
+    }
+#undef XMPM_PT
 }
 
 }  // namespace xmpm

@@ -241,10 +241,16 @@ int mpm_energies(mpm_ctx *ctx, double out[6]);
  * each kernel of mpm_step (read back with mpm_last_step_timing);
  * MPM_OPT_KEEP_FORCES stores nodal forces for mpm_get_nodal (default 1);
  * MPM_OPT_SORTED selects the cell-sorted step (default 1) or the unsorted
- * one-thread-per-particle atomic step (0). */
+ * one-thread-per-particle atomic step (0); MPM_OPT_FAST selects the
+ * specialised particle kernel where it applies (default 1) or the generic
+ * one (0); MPM_OPT_FUSED_DT lets the specialised particle kernel reduce the
+ * next step's dt candidates on its way out (default 1) instead of a separate
+ * pass over the particles at the top of the next step (0). */
 #define MPM_OPT_PROFILE 1
 #define MPM_OPT_KEEP_FORCES 2
 #define MPM_OPT_SORTED 3
+#define MPM_OPT_FAST 4
+#define MPM_OPT_FUSED_DT 5
 int mpm_set_option(mpm_ctx *ctx, int option, int value);
 
 /* Last HIP error string (static storage). */

@@ -416,11 +416,12 @@ void redistribute_one(orc_ctx *c, Part &p) {
         while ((int)std::floor((rhi - plo) * c->dxi[d]) >= c->n[d])
             rhi = std::nextafter(rhi, -INFINITY);
         double L = phi - plo;
+        // bounded shifts (a non-finite position would never settle)
         if (p.pos[d] > rhi) {
-            while (p.pos[d] > rhi) p.pos[d] -= L;
+            for (int k = 0; k < 64 && p.pos[d] > rhi; ++k) p.pos[d] -= L;
             if (p.pos[d] < plo) p.pos[d] = plo;
         } else if (p.pos[d] < plo) {
-            while (p.pos[d] < plo) p.pos[d] += L;
+            for (int k = 0; k < 64 && p.pos[d] < plo; ++k) p.pos[d] += L;
             if (p.pos[d] > rhi) p.pos[d] = rhi;
         }
     }
