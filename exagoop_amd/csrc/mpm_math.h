@@ -86,46 +86,51 @@ __constant__ __device__ static const SplineTables kTab = {
      {0, 0, 1, 1, 2, 2},
      {0, 0, 1, 1, 2, -1}}};
 
-// interval_idx_quadratic (interpolants.H:187-199)
-__device__ __forceinline__ int quad_seg(int t, double z) {
+// interval_idx_quadratic (interpolants.H:187-199).  The evaluators take the
+// coefficient table as an argument so a kernel can pass a copy staged in LDS.
+__device__ __forceinline__ int quad_seg(const SplineTables &T, int t, double z) {
     double shift = (t == 2) ? 1.0 : 1.5;
     int raw = (int)floor((z + shift) * 2);
     if (raw < 0 || raw >= 6) return -1;
-    return kTab.qmap[t][raw];
+    return T.qmap[t][raw];
 }
 
 // quadraticspline_1d / _der (interpolants.H:266-304)
-__device__ __forceinline__ double quad_val(int t, double z) {
+__device__ __forceinline__ double quad_val(const SplineTables &T, int t, double z) {
     double z2 = z * z;
-    int s = quad_seg(t, z);
+    int s = quad_seg(T, t, z);
     if (s < 0) return 0.0;
-    const double *c = kTab.qval[t][s];
+    const double *c = T.qval[t][s];
     return c[0] * z2 + c[1] * z + c[2];
 }
-__device__ __forceinline__ double quad_der(int t, double z) {
-    int s = quad_seg(t, z);
+__device__ __forceinline__ double quad_der(const SplineTables &T, int t, double z) {
+    int s = quad_seg(T, t, z);
     if (s < 0) return 0.0;
-    const double *c = kTab.qder[t][s];
+    const double *c = T.qder[t][s];
     return c[0] * z + c[1];
 }
 
 // cubicspline_1d / _der (interpolants.H:215-250); a segment index outside
 // [0,3] (|z| >= 2 from rounding at the support edge) evaluates to 0.
-__device__ __forceinline__ double cubic_val(int t, double z) {
+__device__ __forceinline__ double cubic_val(const SplineTables &T, int t, double z) {
     double z3 = z * z * z;
     double z2 = z * z;
     int s = (int)floor(z) + 2;
     if (s < 0 || s > 3) return 0.0;
-    const double *c = kTab.cval[t][s];
+    const double *c = T.cval[t][s];
     return c[0] * z3 + c[1] * z2 + c[2] * z + c[3];
 }
-__device__ __forceinline__ double cubic_der(int t, double z) {
+__device__ __forceinline__ double cubic_der(const SplineTables &T, int t, double z) {
     double z2 = z * z;
     int s = (int)floor(z) + 2;
     if (s < 0 || s > 3) return 0.0;
-    const double *c = kTab.cder[t][s];
+    const double *c = T.cder[t][s];
     return c[0] * z2 + c[1] * z + c[2];
 }
+__device__ __forceinline__ double quad_val(int t, double z) { return quad_val(kTab, t, z); }
+__device__ __forceinline__ double quad_der(int t, double z) { return quad_der(kTab, t, z); }
+__device__ __forceinline__ double cubic_val(int t, double z) { return cubic_val(kTab, t, z); }
+__device__ __forceinline__ double cubic_der(int t, double z) { return cubic_der(kTab, t, z); }
 
 // Node shape-function type, arithmetic form of basisval*
 // (interpolants.H:1419-1427) and if-chain form of *_interp_two (:946-956).

@@ -379,7 +379,7 @@ __global__ __launch_bounds__(kBlock) void k_particle_sorted(GridP g, PhysP ph, L
 // Calculate_time_step's per-particle candidate (mpm_particle_timestep.cpp:
 // 51-102) from the state a step leaves behind.
 __device__ __forceinline__ double dt_candidate(const GridP &g, int model, double K, double E,
-                                               double nu, double rho, const double v[3]) {
+                                               double nu, double rho, double vmag2) {
     double Cs;
     if (model == 1) {
         Cs = sqrt(K / rho);
@@ -388,9 +388,7 @@ __device__ __forceinline__ double dt_candidate(const GridP &g, int model, double
         const double mu = E / (2.0 * (1 + nu));
         Cs = sqrt((lam + 2.0 * mu) / rho);
     }
-    double vm = 0.0;
-    for (int d = 0; d < 3; ++d) vm += v[d] * v[d];
-    vm = sqrt(vm);
+    const double vm = sqrt(vmag2);
     double dxmin = g.dx[0];
     for (int d = 1; d < 3; ++d) dxmin = fmin(dxmin, g.dx[d]);
     return dxmin / (Cs + vm);
@@ -404,21 +402,123 @@ __device__ __forceinline__ void dt_wave_min(double c, Scalars *sc) {
         atomicMin(&sc->dtmin_next_bits, (unsigned long long)__double_as_longlong(c));
 }
 
+// 1-D factors of stencil offset l of cell i in one direction: value b and
+// derivative d of basisval_and_grad (interpolants.H:1592-1632) and the
+// *_interp_two weight w (interpolants.H:943-1005, 1295-1365), all zero for
+// an offset outside compute_bounds.  r is formed with the reciprocal cell
+// size, so the basis and interpolation weights of one node type coincide.
+template <int O>
+__device__ __forceinline__ void tile_factors(const SplineTables &T, int per, int i, int l,
+                                             int hi, double x, double plo, double dx,
+                                             double dxi, bool use, double &w, double &b,
+                                             double &d) {
+    if (O == 1) {
+        const double r = (x - (plo + i * dx)) * dxi;
+        b = (l == 0) ? 1.0 - r : r;
+        d = (l == 0) ? -dxi : dxi;
+        w = b;
+    } else {
+        const int idx = i + l;
+        const int ta = per ? 3 : type_arith(idx, 0, hi);
+        const int tc = per ? 3 : type_chain(idx, 0, hi);
+        const double r = (x - (plo + idx * dx)) * dxi;
+        if (O == 2) {
+            b = quad_val(T, ta, r);
+            d = quad_der(T, ta, r) * dxi;
+            w = (ta == tc) ? b : quad_val(T, tc, r);
+        } else {
+            b = cubic_val(T, ta, r);
+            d = cubic_der(T, ta, r) * dxi;
+            w = (ta == tc) ? b : cubic_val(T, tc, r);
+        }
+    }
+    if (!use) w = b = d = 0.0;
+}
+
+// G2P of one particle from the LDS node block of its tile: PIC velocity
+// vals0 (and FLIP increment vals1 when VEL) and the velocity gradient L
+// (row-major).  The z offsets run as a loop and the W x W (y, x) offsets
+// of a plane are unrolled; the x and y factors stay in registers.  Returns
+// false, with zero sums, for a particle outside the tile.
+template <int O, bool VEL>
+__device__ __forceinline__ bool g2p_block(const GridP &g, const SplineTables &T,
+                                          const double *nodes, const int T0[3],
+                                          const double x[3], double vals0[3], double vals1[3],
+                                          double L[9]) {
+#pragma clang fp contract(fast)
+    constexpr int W = (O == 1) ? 2 : 4;
+    constexpr int L0 = (O == 1) ? 0 : -1;
+    constexpr int NCV = VEL ? 6 : 3;
+    int iv[3], mn[3], mx[3];
+    bool inside = true;
+    for (int d = 0; d < 3; ++d) {
+        iv[d] = (int)floor((x[d] - g.plo[d]) * g.dxi[d]);
+        bounds(iv[d], 0, g.hi[d], O, g.per[d], mn[d], mx[d]);
+        inside &= (iv[d] >= T0[d] && iv[d] < T0[d] + kTileCells);
+    }
+    for (int d = 0; d < 3; ++d) vals0[d] = vals1[d] = 0.0;
+    for (int q = 0; q < 9; ++q) L[q] = 0.0;
+    if (!inside) return false;
+    double fw[2][W], fb[2][W], fd[2][W];
+#pragma unroll
+    for (int d = 0; d < 2; ++d)
+#pragma unroll
+        for (int k = 0; k < W; ++k) {
+            const int l = L0 + k;
+            tile_factors<O>(T, g.per[d], iv[d], l, g.hi[d], x[d], g.plo[d], g.dx[d], g.dxi[d],
+                            l >= mn[d] && l < mx[d], fw[d][k], fb[d][k], fd[d][k]);
+        }
+    const double *blk =
+        nodes + (((iv[2] - T0[2] + L0 + 1) * kTN + (iv[1] - T0[1] + L0 + 1)) * kTN +
+                 (iv[0] - T0[0] + L0 + 1)) * NCV;
+#pragma unroll 1
+    for (int kz = 0; kz < W; ++kz) {
+        const int l = L0 + kz;
+        double wz, bz, dz;
+        tile_factors<O>(T, g.per[2], iv[2], l, g.hi[2], x[2], g.plo[2], g.dx[2], g.dxi[2],
+                        l >= mn[2] && l < mx[2], wz, bz, dz);
+        const double *pz = blk + kz * (kTN * kTN * NCV);
+#pragma unroll
+        for (int ky = 0; ky < W; ++ky) {
+            const double wyz = fw[1][ky] * wz;
+            const double p0 = fb[1][ky] * bz, p1 = fd[1][ky] * bz, p2 = fb[1][ky] * dz;
+            const double *py = pz + ky * (kTN * NCV);
+#pragma unroll
+            for (int kx = 0; kx < W; ++kx) {
+                const double *q = py + kx * NCV;
+                const double gr0 = fd[0][kx] * p0, gr1 = fb[0][kx] * p1, gr2 = fb[0][kx] * p2;
+                for (int a = 0; a < 3; ++a) {
+                    const double nv = q[a];
+                    L[a * 3 + 0] += nv * gr0;
+                    L[a * 3 + 1] += nv * gr1;
+                    L[a * 3 + 2] += nv * gr2;
+                }
+                if (VEL) {
+                    const double w = fw[0][kx] * wyz;
+                    for (int a = 0; a < 3; ++a) {
+                        vals0[a] += w * q[a];
+                        vals1[a] += w * q[3 + a];
+                    }
+                }
+            }
+        }
+    }
+    return true;
+}
+
 // Fused per-particle pipeline for the common case (every direction order O,
 // no level-set body, no wall UDF), one workgroup per cell tile: the tile's
-// 7^3 node block of v (and dv) is staged in LDS, then each thread takes one
-// particle of the tile: G2P over a fixed-width unrolled stencil (offsets
-// outside compute_bounds are skipped; the reference never reads them), F, D,
-// move + wall BCs, redistribution, volume, stress, written in sorted order.
-// The next step's dt candidates are reduced on the way out.
+// 7^3 node block of v (and dv) and the spline coefficient tables are staged
+// in LDS, then each thread takes one particle of the tile: G2P (g2p_block),
+// F, D, move + wall BCs, redistribution, volume, stress, written in sorted
+// order.  The next step's dt candidates are reduced on the way out.
 template <int O, bool VEL, bool DELTA>
 __global__ __launch_bounds__(kBlock) void k_particle_tile(GridP g, PhysP ph, PState Pin,
                                                           PState Pout, SortState S, NState N,
                                                           int redist, Scalars *sc) {
-    constexpr int W = (O == 1) ? 2 : 4;
-    constexpr int L0 = (O == 1) ? 0 : -1;
     constexpr int NCV = VEL ? 6 : 3;
     __shared__ double nodes[kTNN * NCV];
+    __shared__ SplineTables tab;
     const int tile = xcd_remap(blockIdx.x, gridDim.x);
     const int tid = threadIdx.x;
     const int beg = S.cell_start[tile * kCellsPerTile];
@@ -440,6 +540,13 @@ __global__ __launch_bounds__(kBlock) void k_particle_tile(GridP g, PhysP ph, PSt
             if (VEL) nodes[e * NCV + 3 + c] = ok ? N.dvel[c * nn + qn] : 0.0;
         }
     }
+    if (O > 1) {
+        static_assert(sizeof(SplineTables) % sizeof(int) == 0, "table copy by words");
+        const int *src = reinterpret_cast<const int *>(&kTab);
+        int *dst = reinterpret_cast<int *>(&tab);
+        for (int e = tid; e < (int)(sizeof(SplineTables) / sizeof(int)); e += kBlock)
+            dst[e] = src[e];
+    }
     __syncthreads();
     const int64_t ci = Pin.cap, co = Pout.cap;
     const double dt = sc->dt;
@@ -450,70 +557,17 @@ __global__ __launch_bounds__(kBlock) void k_particle_tile(GridP g, PhysP ph, PSt
         const int64_t i = beg + (int64_t)it * kBlock + tid;
         if (i >= end) continue;
         const int64_t j = S.perm[i];
-        int64_t id = Pin.id[j];
         const int phase = Pin.idata[MPM_I_PHASE * ci + j];
         double x[3], v[3], vp[3], D[6], F[9];
-        for (int d = 0; d < 3; ++d) {
-            x[d] = Pin.pos[d * ci + j];
-            v[d] = Pin.r[(MPM_R_VEL + d) * ci + j];
-        }
+        for (int d = 0; d < 3; ++d) x[d] = Pin.pos[d * ci + j];
         if (phase == 0) {
-            int iv[3], mn[3], mx[3];
-            bool inside = true;
-            for (int d = 0; d < 3; ++d) {
-                iv[d] = (int)floor((x[d] - g.plo[d]) * g.dxi[d]);
-                bounds(iv[d], 0, g.hi[d], O, g.per[d], mn[d], mx[d]);
-                inside &= (iv[d] >= T0[d] && iv[d] < T0[d] + kTileCells);
-            }
-            double wi[3][W], bv[3][W], bd[3][W];
-            bool use[3][W];
-            int ni[3][W];
-#pragma unroll
-            for (int d = 0; d < 3; ++d)
-#pragma unroll
-                for (int k = 0; k < W; ++k) {
-                    const int l = L0 + k;
-                    use[d][k] = (l >= mn[d] && l < mx[d]);
-                    basis1d(O, g.per[d], iv[d], l, 0, g.hi[d], x[d], g.plo[d], g.dx[d], bv[d][k],
-                            bd[d][k]);
-                    // bilin_interp_two's weights are the linear basis values
-                    wi[d][k] = (O == 1) ? bv[d][k]
-                                        : interp1d(O, g.per[d], iv[d] + l, 0, g.hi[d], x[d],
-                                                   g.plo[d], g.dx[d]);
-                    ni[d][k] = iv[d] + l - T0[d] + 1;  // LDS block index
-                }
-            double vals0[3] = {0, 0, 0}, vals1[3] = {0, 0, 0};
-            double L[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-#pragma unroll
-            for (int kz = 0; kz < W; ++kz)
-#pragma unroll
-                for (int ky = 0; ky < W; ++ky)
-#pragma unroll
-                    for (int kx = 0; kx < W; ++kx) {
-                        if (!(use[2][kz] && use[1][ky] && use[0][kx])) continue;
-                        if (!inside) continue;  // handled by g2p_global below
-                        double nv[3], nd[3] = {0, 0, 0};
-                        const int e = (ni[2][kz] * kTN + ni[1][ky]) * kTN + ni[0][kx];
-                        for (int d = 0; d < 3; ++d) {
-                            nv[d] = nodes[e * NCV + d];
-                            if (VEL) nd[d] = nodes[e * NCV + 3 + d];
-                        }
-                        if (VEL) {
-                            const double w = wi[0][kx] * wi[1][ky] * wi[2][kz];
-                            for (int d = 0; d < 3; ++d) {
-                                vals0[d] += w * nv[d];
-                                vals1[d] += w * nd[d];
-                            }
-                        }
-                        const double vx = bv[0][kx], vy = bv[1][ky], vz = bv[2][kz];
-                        const double gr[3] = {bd[0][kx] * vy * vz, vx * bd[1][ky] * vz,
-                                              vx * vy * bd[2][kz]};
-                        for (int d1 = 0; d1 < 3; ++d1)
-                            for (int d2 = 0; d2 < 3; ++d2) L[d1][d2] += nv[d1] * gr[d2];
-                    }
-            // Every particle lies in its tile when particles are redistributed
-            // each step (the only configuration this kernel is launched for).
-            if (!inside) sc->err = 1;
+            double vals0[3], vals1[3], L[9];
+            // Every particle lies in its tile when particles are
+            // redistributed each step (the only configuration this kernel
+            // is launched for).
+            if (!g2p_block<O, VEL>(g, O > 1 ? tab : kTab, nodes, T0, x, vals0, vals1, L))
+                sc->err = 1;
+            for (int d = 0; d < 3; ++d) v[d] = Pin.r[(MPM_R_VEL + d) * ci + j];
             if (VEL) {
                 const double alpha = ph.alpha;
                 for (int d = 0; d < 3; ++d) {
@@ -523,11 +577,10 @@ __global__ __launch_bounds__(kBlock) void k_particle_tile(GridP g, PhysP ph, PSt
             } else {
                 for (int d = 0; d < 3; ++d) vp[d] = Pin.r[(MPM_R_VELP + d) * ci + j];
             }
-            for (int q = 0; q < 9; ++q) F[q] = Pin.r[(MPM_R_DEFGRAD + q) * ci + j];
             // get_deformation_gradient_tensor (interpolants.H:1732-1787)
+            for (int q = 0; q < 9; ++q) F[q] = Pin.r[(MPM_R_DEFGRAD + q) * ci + j];
             double nrm = 0.0;
-            for (int a = 0; a < 3; a++)
-                for (int b = 0; b < 3; b++) nrm += L[a][b] * L[a][b];
+            for (int q = 0; q < 9; q++) nrm += L[q] * L[q];
             if (!(nrm >= 1.0e10)) {
                 double Fp[9];
                 for (int q = 0; q < 9; ++q) Fp[q] = F[q];
@@ -535,7 +588,7 @@ __global__ __launch_bounds__(kBlock) void k_particle_tile(GridP g, PhysP ph, PSt
                     for (int b = 0; b < 3; ++b) {
                         double acc = 0.0;
                         for (int q = 0; q < 3; ++q) {
-                            const double lp = (a == q) ? 1.0 + L[a][q] * dt : L[a][q] * dt;
+                            const double lp = (a == q) ? 1.0 + L[a * 3 + q] * dt : L[a * 3 + q] * dt;
                             acc += lp * Fp[q * 3 + b];
                         }
                         F[a * 3 + b] = acc;
@@ -544,16 +597,22 @@ __global__ __launch_bounds__(kBlock) void k_particle_tile(GridP g, PhysP ph, PSt
             int ind = 0;
             for (int d1 = 0; d1 < 3; ++d1)
                 for (int d2 = d1; d2 < 3; ++d2) {
-                    D[ind] = 0.5 * (L[d1][d2] + L[d2][d1]);
+                    D[ind] = 0.5 * (L[d1 * 3 + d2] + L[d2 * 3 + d1]);
                     ind++;
                 }
         } else {
-            for (int d = 0; d < 3; ++d) vp[d] = Pin.r[(MPM_R_VELP + d) * ci + j];
+            for (int d = 0; d < 3; ++d) {
+                v[d] = Pin.r[(MPM_R_VEL + d) * ci + j];
+                vp[d] = Pin.r[(MPM_R_VELP + d) * ci + j];
+            }
             for (int q = 0; q < 6; ++q) D[q] = Pin.r[(MPM_R_STRAINRATE + q) * ci + j];
             for (int q = 0; q < 9; ++q) F[q] = Pin.r[(MPM_R_DEFGRAD + q) * ci + j];
         }
+        int64_t id = Pin.id[j];
+        double *R = Pout.r;
         // moveParticles without level sets or UDF tables
-        // (mpm_particle_timestep.cpp:329-576), then Redistribute.
+        // (mpm_particle_timestep.cpp:329-576), then Redistribute.  Each
+        // output is stored as soon as it is final to keep live ranges short.
         {
             for (int d = 0; d < 3; ++d) x[d] += vp[d] * dt;
             double rin[3], rout[3], wv[3] = {0, 0, 0};
@@ -577,24 +636,55 @@ __global__ __launch_bounds__(kBlock) void k_particle_tile(GridP g, PhysP ph, PSt
             }
             if (redist) stage_redistribute(g, x, id);
         }
-        // remaining state, updateVolume and the constitutive update
+        for (int d = 0; d < 3; ++d) {
+            Pout.pos[d * co + i] = x[d];
+            R[(MPM_R_VEL + d) * co + i] = v[d];
+            R[(MPM_R_VELP + d) * co + i] = vp[d];
+        }
+        Pout.id[i] = id;
+        Pout.idata[MPM_I_PHASE * co + i] = phase;
+        const double vmag2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+        for (int q = 0; q < 6; ++q) R[(MPM_R_STRAINRATE + q) * co + i] = D[q];
+        // updateVolume and the constitutive update
         const int model = Pin.idata[MPM_I_MODEL * ci + j];
         const double m = Pin.r[MPM_R_MASS * ci + j];
         const double V0 = Pin.r[MPM_R_VOLINIT * ci + j];
-        const double E = Pin.r[MPM_R_E * ci + j], nu = Pin.r[MPM_R_NU * ci + j];
-        const double K = Pin.r[MPM_R_BULK * ci + j];
-        const double gam = Pin.r[MPM_R_GAMMA * ci + j], mu = Pin.r[MPM_R_VISC * ci + j];
-        double eps[6], sig[6], J, V, rho, p;
-        for (int q = 0; q < 6; ++q) eps[q] = Pin.r[(MPM_R_STRAIN + q) * ci + j];
+        R[MPM_R_MASS * co + i] = m;
+        R[MPM_R_VOLINIT * co + i] = V0;
+        Pout.idata[MPM_I_MODEL * co + i] = model;
+        double J, V, rho;
         if (phase == 0) {
             J = det3(F);
             V = V0 * J;
             rho = m / V;
+        } else {
+            J = Pin.r[MPM_R_JACOBIAN * ci + j];
+            V = Pin.r[MPM_R_VOLUME * ci + j];
+            rho = Pin.r[MPM_R_DENSITY * ci + j];
+        }
+        R[MPM_R_VOLUME * co + i] = V;
+        R[MPM_R_DENSITY * co + i] = rho;
+        R[MPM_R_JACOBIAN * co + i] = J;
+        for (int q = 0; q < 9; ++q) R[(MPM_R_DEFGRAD + q) * co + i] = F[q];
+        const double E = Pin.r[MPM_R_E * ci + j], nu = Pin.r[MPM_R_NU * ci + j];
+        const double K = Pin.r[MPM_R_BULK * ci + j];
+        R[MPM_R_E * co + i] = E;
+        R[MPM_R_NU * co + i] = nu;
+        R[MPM_R_BULK * co + i] = K;
+        // The dt candidate depends only on state final at this point.
+        if (id >= 0 && phase == 0) cmin = fmin(cmin, dt_candidate(g, model, K, E, nu, rho, vmag2));
+        const double gam = Pin.r[MPM_R_GAMMA * ci + j], mu = Pin.r[MPM_R_VISC * ci + j];
+        R[MPM_R_GAMMA * co + i] = gam;
+        R[MPM_R_VISC * co + i] = mu;
+        double eps[6], sig[6], p;
+        for (int q = 0; q < 6; ++q) eps[q] = Pin.r[(MPM_R_STRAIN + q) * ci + j];
+        if (phase == 0) {
             const double rate = (time < ph.applied_time) ? ph.applied_rate : 0.0;
             for (int q = 0; q < 6; ++q) eps[q] = eps[q] + dt * D[q];
             eps[XX] += dt * rate;
             eps[YY] += dt * rate;
             eps[ZZ] += dt * rate;
+            for (int q = 0; q < 6; ++q) R[(MPM_R_STRAIN + q) * co + i] = eps[q];
             p = (model == 1) ? 0.0 : Pin.r[MPM_R_PRESSURE * ci + j];
             if (DELTA) {
                 double de[6], ds[6];
@@ -614,39 +704,12 @@ __global__ __launch_bounds__(kBlock) void k_particle_tile(GridP g, PhysP ph, PSt
                 neo_hookean(F, sig, E, nu);
             }
         } else {
-            J = Pin.r[MPM_R_JACOBIAN * ci + j];
-            V = Pin.r[MPM_R_VOLUME * ci + j];
-            rho = Pin.r[MPM_R_DENSITY * ci + j];
+            for (int q = 0; q < 6; ++q) R[(MPM_R_STRAIN + q) * co + i] = eps[q];
             p = Pin.r[MPM_R_PRESSURE * ci + j];
             for (int q = 0; q < 6; ++q) sig[q] = Pin.r[(MPM_R_STRESS + q) * ci + j];
         }
-        double *R = Pout.r;
-        for (int d = 0; d < 3; ++d) {
-            Pout.pos[d * co + i] = x[d];
-            R[(MPM_R_VEL + d) * co + i] = v[d];
-            R[(MPM_R_VELP + d) * co + i] = vp[d];
-        }
-        for (int q = 0; q < 6; ++q) {
-            R[(MPM_R_STRAINRATE + q) * co + i] = D[q];
-            R[(MPM_R_STRAIN + q) * co + i] = eps[q];
-            R[(MPM_R_STRESS + q) * co + i] = sig[q];
-        }
-        for (int q = 0; q < 9; ++q) R[(MPM_R_DEFGRAD + q) * co + i] = F[q];
-        R[MPM_R_VOLUME * co + i] = V;
-        R[MPM_R_MASS * co + i] = m;
-        R[MPM_R_DENSITY * co + i] = rho;
-        R[MPM_R_JACOBIAN * co + i] = J;
+        for (int q = 0; q < 6; ++q) R[(MPM_R_STRESS + q) * co + i] = sig[q];
         R[MPM_R_PRESSURE * co + i] = p;
-        R[MPM_R_VOLINIT * co + i] = V0;
-        R[MPM_R_E * co + i] = E;
-        R[MPM_R_NU * co + i] = nu;
-        R[MPM_R_BULK * co + i] = K;
-        R[MPM_R_GAMMA * co + i] = gam;
-        R[MPM_R_VISC * co + i] = mu;
-        Pout.idata[MPM_I_PHASE * co + i] = phase;
-        Pout.idata[MPM_I_MODEL * co + i] = model;
-        Pout.id[i] = id;
-        if (id >= 0 && phase == 0) cmin = fmin(cmin, dt_candidate(g, model, K, E, nu, rho, v));
     }
     dt_wave_min(cmin, sc);  // every lane reaches this point
 }
