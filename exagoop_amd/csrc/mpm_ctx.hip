@@ -99,7 +99,10 @@ void free_particles(mpm_ctx *c) {
     (void)hipFree(c->S.key);
     (void)hipFree(c->S.rank);
     (void)hipFree(c->S.perm);
+    (void)hipFree(c->S.pkt);
     c->S.key = c->S.rank = c->S.perm = nullptr;
+    c->S.pkt = nullptr;
+    c->S.pcap = 0;
 }
 
 hipError_t alloc_pstate(PState &P, int64_t cap) {
@@ -298,13 +301,15 @@ int mpm_create(const mpm_specs *s, mpm_ctx **out) {
         hipMalloc(&c->d_energy, 8 * sizeof(double)) != hipSuccess)
         return fail(set_err(MPM_E_OOM, "nodal allocation failed"));
     SortState &S = c->S;
-    S.nkeys = g.ntc[0] * g.ntc[1] * g.ntc[2] * kTileCells * kTileCells * kTileCells;
+    S.ntiles = g.ntc[0] * g.ntc[1] * g.ntc[2];
+    S.nkeys = S.ntiles * kTileCells * kTileCells * kTileCells;
     S.scan_bytes = sort_scan_bytes(S.nkeys);
     const int ntn = g.ntn[0] * g.ntn[1] * g.ntn[2];
     if (hipMalloc(&S.count, (S.nkeys + 1) * sizeof(int)) != hipSuccess ||
         hipMalloc(&S.cell_start, (S.nkeys + 1) * sizeof(int)) != hipSuccess ||
         hipMalloc(&S.scan_tmp, S.scan_bytes > 0 ? S.scan_bytes : 16) != hipSuccess ||
-        hipMalloc(&S.tile_epoch, ntn * sizeof(int)) != hipSuccess)
+        hipMalloc(&S.tile_epoch, ntn * sizeof(int)) != hipSuccess ||
+        hipMalloc(&S.tiles, (S.ntiles + 1) * sizeof(int)) != hipSuccess)
         return fail(set_err(MPM_E_OOM, "sort allocation failed"));
     if (hipMemsetAsync(S.count, 0, (S.nkeys + 1) * sizeof(int), c->stream) != hipSuccess ||
         hipMemsetAsync(S.cell_start, 0, (S.nkeys + 1) * sizeof(int), c->stream) != hipSuccess)
@@ -344,6 +349,7 @@ void mpm_destroy(mpm_ctx *c) {
     (void)hipFree(c->S.cell_start);
     (void)hipFree(c->S.scan_tmp);
     (void)hipFree(c->S.tile_epoch);
+    (void)hipFree(c->S.tiles);
     for (double *p : c->ls_bufs) (void)hipFree(p);
     for (int d = 0; d < 3; ++d)
         for (int h = 0; h < 2; ++h) (void)hipFree(c->udf_dev[d][h]);
@@ -375,6 +381,8 @@ int mpm_upload_particles(mpm_ctx *c, int64_t np, const double *pos, const double
     HIPCHK(hipMalloc(&c->S.key, cap * sizeof(int)));
     HIPCHK(hipMalloc(&c->S.rank, cap * sizeof(int)));
     HIPCHK(hipMalloc(&c->S.perm, cap * sizeof(int)));
+    HIPCHK(hipMalloc(&c->S.pkt, kPktComps * cap * sizeof(double)));
+    c->S.pcap = cap;
     if (c->any_per && c->s.stress_update_scheme == MPM_MUSL)
         HIPCHK(hipMalloc(&P.vsnap, 3 * cap * sizeof(double)));
     c->Q.vsnap = P.vsnap;
@@ -624,7 +632,7 @@ int mpm_step(mpm_ctx *c, int nsteps) {
         }
         const bool sorted = c->sorted != 0;
         if (prof) HIPCHK(hipEventRecord(c->ev[1], st));
-        if (sorted) launch_sort(c->g, c->P, c->S, c->Q.id, st);
+        if (sorted) launch_sort(c->g, c->ph, c->P, c->S, c->Q.id, 1, st);
         if (prof) HIPCHK(hipEventRecord(c->ev[2], st));
         if (sorted) launch_p2g_tiles(c->g, c->ph, c->P, c->S, c->N, step, 0, nr > 1, c->sc, st);
         else launch_p2g(c->g, c->ph, c->P, c->N, 1, 1, 1, nullptr, st);

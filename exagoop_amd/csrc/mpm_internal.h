@@ -137,11 +137,23 @@ struct SortState {
     size_t scan_bytes;
     int nkeys;
     int *tile_epoch;  // node tiles: last epoch a deposit landed there
+    // P2G packets by sorted position, component-major with stride pcap:
+    // x[3], m, m v[3], a[3] = m g (+ slab load), B[6] = -V sigma (xx, xy,
+    // xz, yy, yz, zz); all but x zero for a particle with phase != 0.
+    double *pkt;
+    int64_t pcap;
+    // Non-empty cell tiles after the sort: tiles[0] = count, then the tile
+    // indices in increasing order (ntiles + 1 entries).
+    int *tiles;
+    int ntiles;
 };
+constexpr int kPktComps = 16;
 size_t sort_scan_bytes(int nkeys);
 // out_id: id array of the buffer the step writes into (slots past the live
 // count are marked free), or null.
-void launch_sort(const GridP &g, const PState &P, SortState &S, int64_t *out_id, hipStream_t s);
+// packets: also build the P2G packets of the sorted particles.
+void launch_sort(const GridP &g, const PhysP &ph, const PState &P, SortState &S,
+                 int64_t *out_id, int packets, hipStream_t s);
 // fallback: particles may lie outside their tile (not redistributed every
 // step); they deposit straight to global memory.  Without it such a particle
 // sets Scalars::err.

@@ -364,9 +364,13 @@ __global__ void k_dt_init(Scalars *sc) {
 }
 
 __global__ void k_dt_final(Scalars *sc, PhysP ph, int fixed, double fixed_dt, int fused) {
-    const unsigned long long bits = fused ? sc->dtmin_next_bits : sc->dtmin_bits;
-    // the step about to run accumulates the following step's minimum
-    sc->dtmin_next_bits = (unsigned long long)__double_as_longlong(__builtin_huge_val());
+    // The step about to run accumulates the following step's minimum, so the
+    // running value is reset here.  Read and reset are one atomic exchange: a
+    // plain load of a uniform address becomes a scalar-cache load, which the
+    // hardware does not order against the vector store of the reset.
+    const unsigned long long nxt = atomicExch(
+        &sc->dtmin_next_bits, (unsigned long long)__double_as_longlong(__builtin_huge_val()));
+    const unsigned long long bits = fused ? nxt : sc->dtmin_bits;
     if (fixed) {
         sc->dt = fixed_dt;
         return;

@@ -45,6 +45,20 @@ __device__ __forceinline__ int xcd_remap(int b, int nb) {
     return x * per + min(x, rem) + k;
 }
 
+// Workgroup b of a tile-kernel launch takes position p of the non-empty
+// tile list: each XCD (b mod 8) gets a contiguous eighth of the list, so the
+// work is spread evenly over the XCDs whatever part of the domain holds
+// particles, and neighbouring tiles (which share halo nodes) meet in the
+// same L2.  Returns -1 for a workgroup without a tile.
+__device__ __forceinline__ int active_tile(const SortState &S, int b) {
+    const int na = S.tiles[0];
+    const int per = (na + 7) >> 3;
+    const int x = b & 7, k = b >> 3;
+    if (k >= per) return -1;
+    const int p = x * per + k;
+    return p < na ? S.tiles[1 + p] : -1;
+}
+
 __device__ __forceinline__ int cell_key(const GridP &g, const int c[3]) {
     const int t[3] = {c[0] / kTileCells, c[1] / kTileCells, c[2] / kTileCells};
     const int tile = (t[2] * g.ntc[1] + t[1]) * g.ntc[0] + t[0];
@@ -93,23 +107,77 @@ __global__ __launch_bounds__(kBlock) void k_keys(GridP g, PState P, SortState S)
 }
 
 // perm[start[key] + rank] = i; re-zero the counts; mark output slots past the
-// live count as free (removed particles are dropped by the reorder).
-__global__ __launch_bounds__(kBlock) void k_scatter(PState P, SortState S, int64_t *out_id) {
+// live count as free (removed particles are dropped by the reorder).  With
+// packets, also write particle i's P2G packet at its sorted position.
+__global__ __launch_bounds__(kBlock) void k_scatter(PState P, SortState S, int64_t *out_id,
+                                                    PhysP ph, int packets) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P.np) return;
     const int key = S.key[i];
-    if (key < S.nkeys) S.perm[S.cell_start[key] + S.rank[i]] = (int)i;
+    if (key < S.nkeys) {
+        const int slot = S.cell_start[key] + S.rank[i];
+        S.perm[slot] = (int)i;
+        if (packets) {
+            const int64_t cap = P.cap, pc = S.pcap;
+            double x[3];
+            for (int d = 0; d < 3; ++d) x[d] = P.pos[d * cap + i];
+            double v[13];
+            for (int e = 0; e < 13; ++e) v[e] = 0.0;
+            if (P.idata[MPM_I_PHASE * cap + i] == 0) {
+                const double m = P.r[MPM_R_MASS * cap + i];
+                v[0] = m;
+                for (int d = 0; d < 3; ++d) v[1 + d] = m * P.r[(MPM_R_VEL + d) * cap + i];
+                const bool sl = ph.extloads && x[0] > ph.slab_lo[0] && x[0] < ph.slab_hi[0] &&
+                                x[1] > ph.slab_lo[1] && x[1] < ph.slab_hi[1] &&
+                                x[2] > ph.slab_lo[2] && x[2] < ph.slab_hi[2];
+                for (int d = 0; d < 3; ++d) v[4 + d] = m * ph.grav[d] + (sl ? ph.extforce[d] : 0.0);
+                const double V = P.r[MPM_R_VOLUME * cap + i];
+                for (int e = 0; e < 6; ++e) v[7 + e] = -V * P.r[(MPM_R_STRESS + e) * cap + i];
+            }
+            for (int d = 0; d < 3; ++d) S.pkt[d * pc + slot] = x[d];
+            for (int e = 0; e < 13; ++e) S.pkt[(3 + e) * pc + slot] = v[e];
+        }
+    }
     S.count[key] = 0;
     if (out_id && i >= S.cell_start[S.nkeys]) out_id[i] = -1;
 }
 
+// Non-empty tiles in increasing order into S.tiles[1..], their number into
+// S.tiles[0] (one workgroup of 1024 threads; deterministic order).
+__global__ __launch_bounds__(1024) void k_tile_list(SortState S) {
+    __shared__ int wsum[16];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    int base = 0;
+    for (int t0 = 0; t0 < S.ntiles; t0 += 1024) {
+        const int t = t0 + tid;
+        const bool f = t < S.ntiles && S.cell_start[(t + 1) * kCellsPerTile] !=
+                                           S.cell_start[t * kCellsPerTile];
+        const unsigned long long b = __ballot(f);
+        const int inwave = __popcll(b & ((1ull << lane) - 1ull));
+        if (lane == 0) wsum[w] = __popcll(b);
+        __syncthreads();
+        int off = 0, tot = 0;
+        for (int q = 0; q < 16; ++q) {
+            if (q < w) off += wsum[q];
+            tot += wsum[q];
+        }
+        if (f) S.tiles[1 + base + off + inwave] = t;
+        base += tot;
+        __syncthreads();
+    }
+    if (tid == 0) S.tiles[0] = base;
+}
+
 // One particle's deposit straight to the global accumulators (particles
-// outside their tile's node block).
-__device__ __attribute__((noinline)) void deposit_global(const GridP &g, const PhysP &ph,
-                                                         const double x[3], double m,
-                                                         const double vel[3], double V,
-                                                         const double T[9], bool forces,
+// outside their tile's node block), from its P2G packet values: mass m,
+// momentum mv, body force density a and B = -V sigma (xx, xy, xz, yy, yz,
+// zz).  Per node: m N, mv N and a N + B grad N.
+__device__ __attribute__((noinline)) void deposit_global(const GridP &g, const double x[3],
+                                                         const double *vals, bool forces,
                                                          double *acc, int *tile_epoch, int epoch) {
+    const double m = vals[0];
+    const double *mv = vals + 1, *a = vals + 4, *B = vals + 7;
+    const int sym[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};
     int iv[3], mn[3], mx[3];
     for (int d = 0; d < 3; ++d) {
         iv[d] = (int)floor((x[d] - g.plo[d]) * g.dxi[d]);
@@ -120,9 +188,6 @@ __device__ __attribute__((noinline)) void deposit_global(const GridP &g, const P
         for (int l = mn[d]; l < mx[d]; ++l)
             basis1d(g.od[d], g.per[d], iv[d], l, 0, g.hi[d], x[d], g.plo[d], g.dx[d],
                     bv[d][l - mn[d]], bd[d][l - mn[d]]);
-    const bool slab = ph.extloads && x[0] > ph.slab_lo[0] && x[0] < ph.slab_hi[0] &&
-                      x[1] > ph.slab_lo[1] && x[1] < ph.slab_hi[1] && x[2] > ph.slab_lo[2] &&
-                      x[2] < ph.slab_hi[2];
     const int64_t nn = g.nn;
     for (int n = mn[2]; n < mx[2]; ++n) {
         int tz[2];
@@ -141,17 +206,15 @@ __device__ __attribute__((noinline)) void deposit_global(const GridP &g, const P
                 const double N = vx * vy * vz;
                 double val[7];
                 val[0] = m * N;
-                for (int d = 0; d < 3; ++d) val[1 + d] = m * vel[d] * N;
+                for (int d = 0; d < 3; ++d) val[1 + d] = mv[d] * N;
                 for (int d = 0; d < 3; ++d) val[4 + d] = 0.0;
                 if (forces) {
                     const double gr[3] = {bd[0][l - mn[0]] * vy * vz, vx * bd[1][mm - mn[1]] * vz,
                                           vx * vy * bd[2][n - mn[2]]};
-                    for (int a = 0; a < 3; ++a) {
-                        double bf = m * ph.grav[a] * N;
-                        if (slab) bf += ph.extforce[a] * N;
+                    for (int d = 0; d < 3; ++d) {
                         double tv = 0.0;
-                        for (int b = 0; b < 3; ++b) tv += T[a * 3 + b] * gr[b];
-                        val[4 + a] = bf + (-V * tv);
+                        for (int b = 0; b < 3; ++b) tv += B[sym[d * 3 + b]] * gr[b];
+                        val[4 + d] = a[d] * N + tv;
                     }
                 }
                 for (int qz = 0; qz < cz; ++qz)
@@ -167,153 +230,322 @@ __device__ __attribute__((noinline)) void deposit_global(const GridP &g, const P
     }
 }
 
+// The four factor pairs (value, derivative) of offsets -1..2 of cell i in
+// direction d for P2G: basisval_and_grad (interpolants.H:1592-1632), zero
+// for offsets outside compute_bounds (mpm_particle_grid_ops.cpp:255-299).  O
+// is the direction's order, or 0 to read it from the grid (mixed orders).
+// Away from non-periodic walls every node of a cubic stencil has shape type 3
+// and offset k lies in segment 3 - k, so the coefficients are constants;
+// the table is read only when a node type or segment differs.
+template <int O>
+__device__ __forceinline__ void p2g_factors4(const SplineTables &T, const GridP &g, int d, int i,
+                                             double x, double f[8]) {
+    const int od = O ? O : g.od[d];
+    const double dx = g.dx[d], dxi = g.dxi[d], plo = g.plo[d];
+    const int hi = g.hi[d], per = g.per[d];
+    if (od == 1) {
+        const double r = (x - (plo + i * dx)) * dxi;
+        f[0] = f[1] = f[6] = f[7] = 0.0;
+        f[2] = 1.0 - r;
+        f[3] = -dxi;
+        f[4] = r;
+        f[5] = dxi;
+        return;
+    }
+    int mn, mx;
+    bounds(i, 0, hi, od, per, mn, mx);
+    const bool interior = per || (i >= 3 && i <= hi - 3);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int idx = i + k - 1;
+        const double r = (x - (plo + idx * dx)) * dxi;
+        double b, dv;
+        if (od == 3) {
+            const int s = (int)floor(r) + 2;
+            if (interior && s == 3 - k) {
+                const double *c = kTab.cval[3][3 - k];
+                const double *e = kTab.cder[3][3 - k];
+                const double r2 = r * r, r3 = r * r * r;
+                b = c[0] * r3 + c[1] * r2 + c[2] * r + c[3];
+                dv = (e[0] * r2 + e[1] * r + e[2]) * dxi;
+            } else {
+                const int t = per ? 3 : type_arith(idx, 0, hi);
+                b = cubic_val(T, t, r);
+                dv = cubic_der(T, t, r) * dxi;
+            }
+        } else {
+            const int t = per ? 3 : type_arith(idx, 0, hi);
+            b = quad_val(T, t, r);
+            dv = quad_der(T, t, r) * dxi;
+        }
+        const bool u = (k - 1 >= mn && k - 1 < mx);
+        f[2 * k] = u ? b : 0.0;
+        f[2 * k + 1] = u ? dv : 0.0;
+    }
+}
+
+// Particle packet of the P2G staging area (doubles): 12 factor pairs
+// (value, derivative) for directions x, y, z and offsets -1..2, then m, m v,
+// the body force density a = m g (+ slab load) and B = -V sigma (xx, xy, xz,
+// yy, yz, zz): packet component c >= 3 of the sorted packet array lands at
+// kPkM - 3 + c.  All zero for a slot without a depositing particle.
+constexpr int kPkFac = 0, kPkM = 24, kPkMV = 25, kPkA = 28, kPkB = 31, kPk = 38;
+constexpr int kSlots = 16;  // packets per wavefront and round: 4 cells x 4 particles
+
+__device__ __forceinline__ int sym_idx(int a, int b) {
+    // XX XY XZ / XY YY YZ / XZ YZ ZZ
+    return (a == 0) ? b : (a == 1 ? (b == 0 ? 1 : b + 2) : (b == 0 ? 2 : b + 3));
+}
+
+// One staging lane's share of a packet: the position component of its
+// direction and up to four value components.
+struct StageRaw {
+    double xd;
+    double val[4];
+    int64_t slot;  // sorted position (FORCES) or particle index (momentum only)
+    bool act;
+};
+
 // deposit_onto_grid_momentum (mpm_particle_grid_ops.cpp:333-587) over one
-// cell tile.  FORCES: mass, momentum and forces (P2G(1,1,1)); otherwise
-// momentum only (the MUSL re-projection P2G(0,1,0)).  Stencil offsets
-// -1..2 in every direction; offsets compute_bounds excludes get zero weight
-// (they only address nodes outside the domain, which the reference clips).
-template <bool FORCES, bool FALLBACK>
-__global__ __launch_bounds__(kBlock) void k_p2g_groups(GridP g, PhysP ph, PState P, SortState S,
-                                                       double *acc, int epoch, Scalars *sc) {
+// cell tile.  FORCES: mass, momentum and forces (P2G(1,1,1)) from the sorted
+// packets the sort wrote; otherwise momentum only (the MUSL re-projection
+// P2G(0,1,0)) from the particle state through perm.
+//
+// Wavefront w takes the 16 cells of z-slice w of the tile, one row of four
+// cells (x = 0..3) per value of cy.  Each row's particles are staged in rounds
+// of four per cell: 16 packets are built by the 64 lanes (lane part d < 3
+// evaluates the four factor pairs of direction d; every part copies a share
+// of the value components) into the wavefront's LDS staging area, the next
+// round's loads already in flight.  Then 16 lanes per cell, lane (ky, kz)
+// owning the four stencil nodes along x of one (y, z) stencil row, read the
+// packets and accumulate into registers; offsets compute_bounds excludes
+// have zero factors (they only address nodes outside the domain, which the
+// reference clips).  After a row the sums go into the wavefront's private
+// LDS slab with plain read-modify-write (the 64 lanes address 64 distinct
+// nodes), and at the end the four slabs are summed per node and flushed with
+// one global FP64 atomic per non-zero node component.  FALLBACK: a particle
+// outside its cell (not redistributed this step) deposits through
+// deposit_global; without it such a particle raises the error flag.
+template <int O, bool FORCES, bool FALLBACK>
+__global__ __launch_bounds__(kBlock) void k_p2g_cells(GridP g, PhysP ph, PState P, SortState S,
+                                                      double *acc, int epoch, Scalars *sc) {
     constexpr int NC = FORCES ? 7 : 3;
     __shared__ double slab[4][NC * kSlab];  // one private slab per wavefront
+    __shared__ double pk[4][kSlots * kPk];  // staging area per wavefront
     __shared__ int cs[kCellsPerTile + 1];
-    const int tile = xcd_remap(blockIdx.x, gridDim.x);
+    __shared__ SplineTables tab;
+    const int tile = active_tile(S, blockIdx.x);
+    if (tile < 0) return;  // uniform over the block
     const int tid = threadIdx.x;
     if (tid <= kCellsPerTile) cs[tid] = S.cell_start[tile * kCellsPerTile + tid];
+    if (O != 1) {
+        const int *src = reinterpret_cast<const int *>(&kTab);
+        int *dst = reinterpret_cast<int *>(&tab);
+        for (int e = tid; e < (int)(sizeof(SplineTables) / sizeof(int)); e += kBlock)
+            dst[e] = src[e];
+    }
     __syncthreads();
     if (cs[kCellsPerTile] == cs[0]) return;  // empty tile (uniform over the block)
     const int wave = tid >> 6, lane = tid & 63;
     double *my = slab[wave];
+    double *pkw = pk[wave];
     for (int e = lane; e < NC * kSlab; e += 64) my[e] = 0.0;
     const int tx = tile % g.ntc[0];
     const int ty = (tile / g.ntc[0]) % g.ntc[1];
     const int tz = tile / (g.ntc[0] * g.ntc[1]);
     const int T0[3] = {tx * kTileCells, ty * kTileCells, tz * kTileCells};
-    __syncthreads();
-    const int64_t cap = P.cap;
-    const int q = lane >> 4;    // cell x inside the tile
-    const int gl = lane & 15;   // lane inside the cell's group
-    const int ky = gl & 3, kz = gl >> 2;
-    const int gbase = lane & ~15;
-    const int e_dim = gl < 4 ? 0 : (gl < 8 ? 1 : 2);  // the 1-D factor this lane evaluates
-    const int e_l = (gl & 3) - 1;
+    const int64_t cap = P.cap, pc = S.pcap;
+    const SplineTables &TT = (O != 1) ? tab : kTab;
     const int cz = wave;
-    for (int cy = 0; cy < kTileCells; ++cy) {
-        const int cell = q + kTileCells * (cy + kTileCells * cz);
-        const int beg = cs[cell], cnt = cs[cell + 1] - beg;
-        int maxcnt = cnt;
-        maxcnt = max(maxcnt, __shfl_xor(maxcnt, 16, 64));
-        maxcnt = max(maxcnt, __shfl_xor(maxcnt, 32, 64));
-        double a4[4][NC];
+    // accumulation role: group q (cell x inside the tile), lane (ky, kz)
+    const int q = lane >> 4;
+    const int gl = lane & 15;
+    const int ky = gl & 3, kz = gl >> 2;
+    // staging role: packet slot s (cell x = s >> 2, particle s & 3 of the
+    // round), part 0..3; part d < 3 also evaluates direction d's factors.
+    // Value components 3..15 of a packet are split 4/3/3/3 over the parts.
+    const int s_slot = lane & 15, s_part = lane >> 4;
+    const int s_cell = s_slot >> 2, s_t = s_slot & 3;
+    const int v_first = (s_part == 0) ? 3 : 4 + 3 * s_part;
+    const int v_n = FORCES ? ((s_part == 0) ? 4 : 3) : ((s_part == 0) ? 4 : 0);
+    // largest cell count of each row over the wavefront's four cells
+    int mc[kTileCells];
 #pragma unroll
-        for (int k = 0; k < 4; ++k)
-#pragma unroll
-            for (int c = 0; c < NC; ++c) a4[k][c] = 0.0;
-        for (int t = 0; t < maxcnt; ++t) {
-            const bool act = t < cnt;
-            const int64_t j = act ? S.perm[beg + t] : 0;
-            bool use = act && P.idata[MPM_I_PHASE * cap + j] == 0;
-            double x[3] = {0, 0, 0}, vel[3] = {0, 0, 0}, m = 0.0, V = 0.0;
-            double T[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-            int iv[3] = {T0[0], T0[1], T0[2]};
-            bool inside = true;
-            if (use) {
-                for (int d = 0; d < 3; ++d) {
-                    x[d] = P.pos[d * cap + j];
-                    vel[d] = P.r[(MPM_R_VEL + d) * cap + j];
-                    iv[d] = (int)floor((x[d] - g.plo[d]) * g.dxi[d]);
-                    inside &= (iv[d] >= T0[d] && iv[d] < T0[d] + kTileCells);
-                }
-                m = P.r[MPM_R_MASS * cap + j];
-                if (FORCES) {
-                    int ind = 0;
-                    for (int a = 0; a < 3; a++)
-                        for (int b = a; b < 3; b++) {
-                            const double s = P.r[(MPM_R_STRESS + ind) * cap + j];
-                            T[a * 3 + b] = s;
-                            T[b * 3 + a] = s;
-                            ind++;
-                        }
-                    V = P.r[MPM_R_VOLUME * cap + j];
-                }
-                if (!inside) {
-                    // only particles not redistributed this step can lie
-                    // outside their tile
-                    if (FALLBACK) {
-                        if (gl == 0)
-                            deposit_global(g, ph, x, m, vel, V, T, FORCES, acc, S.tile_epoch,
-                                           epoch);
-                    } else {
-                        sc->err = 1;
+    for (int r = 0; r < kTileCells; ++r) {
+        const int c0 = kTileCells * (r + kTileCells * cz) + q;
+        int m = cs[c0 + 1] - cs[c0];
+        m = max(m, __shfl_xor(m, 16, 64));
+        m = max(m, __shfl_xor(m, 32, 64));
+        mc[r] = m;
+    }
+    auto load = [&](int ry, int base) {
+        StageRaw R;
+        const int c0 = kTileCells * (ry + kTileCells * cz) + s_cell;
+        const int sbeg = cs[c0], scnt = cs[c0 + 1] - sbeg;
+        const int t = base + s_t;
+        R.act = t < scnt;
+        R.xd = 0.0;
+        for (int e = 0; e < 4; ++e) R.val[e] = 0.0;
+        R.slot = 0;
+        if (R.act) {
+            if (FORCES) {
+                R.slot = sbeg + t;
+                if (s_part < 3) R.xd = S.pkt[s_part * pc + R.slot];
+                for (int e = 0; e < 4; ++e)
+                    if (e < v_n) R.val[e] = S.pkt[(v_first + e) * pc + R.slot];
+            } else {
+                const int64_t j = S.perm[sbeg + t];
+                R.slot = j;
+                if (P.idata[MPM_I_PHASE * cap + j] != 0) {
+                    R.act = false;
+                } else {
+                    if (s_part < 3) R.xd = P.pos[s_part * cap + j];
+                    if (s_part == 0) {
+                        R.val[0] = P.r[MPM_R_MASS * cap + j];
+                        for (int d = 0; d < 3; ++d)
+                            R.val[1 + d] = R.val[0] * P.r[(MPM_R_VEL + d) * cap + j];
                     }
-                    use = false;
                 }
             }
-            // this lane's 1-D factor (value, derivative), zero where unused
-            double ev = 0.0, ed = 0.0;
-            {
-                const int d = e_dim;
-                int mn_, mx_;
-                bounds(iv[d], 0, g.hi[d], g.od[d], g.per[d], mn_, mx_);
-                double v_, d_;
-                basis1d(g.od[d], g.per[d], iv[d], e_l, 0, g.hi[d], x[d], g.plo[d], g.dx[d], v_, d_);
-                const bool u = use && e_l >= mn_ && e_l < mx_;
-                ev = u ? v_ : 0.0;
-                ed = u ? d_ : 0.0;
+        }
+        return R;
+    };
+    auto mc_of = [&](int r) {
+        return r == 0 ? mc[0] : (r == 1 ? mc[1] : (r == 2 ? mc[2] : mc[3]));
+    };
+    auto advance = [&](int &ry, int &base) {
+        base += 4;
+        while (ry < kTileCells && base >= mc_of(ry)) {
+            ++ry;
+            base = 0;
+        }
+    };
+    int ry = 0, base = -4;
+    advance(ry, base);
+    double a4[4][NC];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int c = 0; c < NC; ++c) a4[k][c] = 0.0;
+    StageRaw cur;
+    if (ry < kTileCells) cur = load(ry, base);
+    while (ry < kTileCells) {
+        int nry = ry, nbase = base;
+        advance(nry, nbase);
+        StageRaw nxt;
+        if (nry < kTileCells) nxt = load(nry, nbase);
+        // ---- staging: this round's 16 packets
+        {
+            const int ic = (s_part == 0) ? T0[0] + s_cell
+                                         : ((s_part == 1) ? T0[1] + ry : T0[2] + cz);
+            const bool out =
+                s_part < 3 && cur.act &&
+                (int)floor((cur.xd - g.plo[s_part]) * g.dxi[s_part]) != ic;
+            const unsigned long long ob = __ballot(out);
+            const bool slot_out = ((ob >> s_slot) | (ob >> (s_slot + 16)) |
+                                   (ob >> (s_slot + 32))) & 1ull;
+            bool use = cur.act;
+            if (slot_out) {
+                if (FALLBACK) {
+                    if (s_part == 3 && cur.act) {
+                        double x[3], vals[13];
+                        if (FORCES) {
+                            for (int d = 0; d < 3; ++d) x[d] = S.pkt[d * pc + cur.slot];
+                            for (int e = 0; e < 13; ++e) vals[e] = S.pkt[(3 + e) * pc + cur.slot];
+                        } else {
+                            const int64_t j = cur.slot;
+                            for (int d = 0; d < 3; ++d) x[d] = P.pos[d * cap + j];
+                            for (int e = 0; e < 13; ++e) vals[e] = 0.0;
+                            vals[0] = P.r[MPM_R_MASS * cap + j];
+                            for (int d = 0; d < 3; ++d)
+                                vals[1 + d] = vals[0] * P.r[(MPM_R_VEL + d) * cap + j];
+                        }
+                        deposit_global(g, x, vals, FORCES, acc, S.tile_epoch, epoch);
+                    }
+                } else {
+                    sc->err = 1;
+                }
+                use = false;
             }
+            double *pw = pkw + s_slot * kPk;
+            if (s_part < 3) {
+                double f[8];
+                if (use) {
+                    p2g_factors4<O>(TT, g, s_part, ic, cur.xd, f);
+                } else {
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) f[e] = 0.0;
+                }
+#pragma unroll
+                for (int e = 0; e < 8; ++e) pw[kPkFac + s_part * 8 + e] = f[e];
+            }
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+                if (e < v_n) pw[kPkM - 3 + v_first + e] = use ? cur.val[e] : 0.0;
+        }
+        // The staging area is private to the wavefront and LDS operations of
+        // one wavefront execute in issue order: the compiler only has to keep
+        // the packet writes ahead of the reads below.
+        asm volatile("" ::: "memory");
+        // ---- accumulation: particle base + tt of each of the row's cells
+        const int nt = min(4, mc_of(ry) - base);
+        for (int tt = 0; tt < nt; ++tt) {
+#pragma clang fp contract(fast)
+            const double *pr = pkw + (q * 4 + tt) * kPk;
             double wx[4], dwx[4];
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
-                wx[k] = __shfl(ev, gbase + k, 64);
-                dwx[k] = __shfl(ed, gbase + k, 64);
+                wx[k] = pr[kPkFac + 2 * k];
+                dwx[k] = pr[kPkFac + 2 * k + 1];
             }
-            const double wy = __shfl(ev, gbase + 4 + ky, 64), dwy = __shfl(ed, gbase + 4 + ky, 64);
-            const double wz = __shfl(ev, gbase + 8 + kz, 64), dwz = __shfl(ed, gbase + 8 + kz, 64);
-            {
-#pragma clang fp contract(fast)
-                // momentum m v, body force m g (+ slab load), internal-force
-                // operator B = -V sigma; row factors for this lane's (y, z)
-                const bool sl = FORCES && ph.extloads && x[0] > ph.slab_lo[0] &&
-                                x[0] < ph.slab_hi[0] && x[1] > ph.slab_lo[1] &&
-                                x[1] < ph.slab_hi[1] && x[2] > ph.slab_lo[2] &&
-                                x[2] < ph.slab_hi[2];
-                const double wyz = wy * wz;
-                const double Mr = m * wyz;
+            const double wy = pr[kPkFac + 8 + 2 * ky], dwy = pr[kPkFac + 8 + 2 * ky + 1];
+            const double wz = pr[kPkFac + 16 + 2 * kz], dwz = pr[kPkFac + 16 + 2 * kz + 1];
+            const double wyz = wy * wz;
+            if (FORCES) {
+                const double Mr = pr[kPkM] * wyz;
+                const double dwyz = dwy * wz, wydz = wy * dwz;
                 double Pr[3], C[3], E[3];
-                for (int d = 0; d < 3; ++d) Pr[d] = m * vel[d] * wyz;
-                if (FORCES) {
-                    const double dwyz = dwy * wz, wydz = wy * dwz;
-                    for (int d = 0; d < 3; ++d) {
-                        const double ad = m * ph.grav[d] + (sl ? ph.extforce[d] : 0.0);
-                        C[d] = ad * wyz - V * (T[d * 3 + 1] * dwyz + T[d * 3 + 2] * wydz);
-                        E[d] = -V * T[d * 3 + 0] * wyz;
-                    }
+                for (int d = 0; d < 3; ++d) {
+                    Pr[d] = pr[kPkMV + d] * wyz;
+                    C[d] = pr[kPkA + d] * wyz + pr[kPkB + sym_idx(d, 1)] * dwyz +
+                           pr[kPkB + sym_idx(d, 2)] * wydz;
+                    E[d] = pr[kPkB + sym_idx(d, 0)] * wyz;
                 }
 #pragma unroll
                 for (int k = 0; k < 4; ++k) {
-                    if (FORCES) {
-                        a4[k][0] += wx[k] * Mr;
-                        for (int d = 0; d < 3; ++d) a4[k][1 + d] += wx[k] * Pr[d];
-                        for (int d = 0; d < 3; ++d) a4[k][4 + d] += wx[k] * C[d] + dwx[k] * E[d];
-                    } else {
-                        for (int d = 0; d < 3; ++d) a4[k][d] += wx[k] * Pr[d];
-                    }
+                    a4[k][0] += wx[k] * Mr;
+                    for (int d = 0; d < 3; ++d) a4[k][1 + d] += wx[k] * Pr[d];
+                    for (int d = 0; d < 3; ++d) a4[k][4 + d] += wx[k] * C[d] + dwx[k] * E[d];
                 }
+            } else {
+                double Pr[3];
+                for (int d = 0; d < 3; ++d) Pr[d] = pr[kPkMV + d] * wyz;
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+                    for (int d = 0; d < 3; ++d) a4[k][d] += wx[k] * Pr[d];
             }
         }
-        // For a given k the 64 lanes address 64 distinct slab nodes
-        // (x = q + k, y = cy + ky, z = kz), so the read-modify-write needs no
-        // atomics; but node q + k of step k is node (q+1) + (k-1) of the next
-        // group's step k-1.  LDS operations of one wavefront execute in issue
-        // order, so it suffices that the compiler keeps the steps apart.
+        asm volatile("" ::: "memory");  // reads of this round before the next writes
+        if (nry != ry) {
+            // Row done.  For a given k the 64 lanes address 64 distinct slab
+            // nodes (x = q + k, y = ry + ky, z = kz), so the read-modify-write
+            // needs no atomics; but node q + k of step k is node (q+1) + (k-1)
+            // of the next group's step k-1, so the compiler must keep the
+            // steps apart (LDS operations of a wavefront execute in order).
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const int idx = (kz * kTN + (cy + ky)) * kTN + (q + k);
+            for (int k = 0; k < 4; ++k) {
+                const int idx = (kz * kTN + (ry + ky)) * kTN + (q + k);
 #pragma unroll
-            for (int c = 0; c < NC; ++c) my[c * kSlab + idx] += a4[k][c];
-            asm volatile("" ::: "memory");
+                for (int c = 0; c < NC; ++c) {
+                    my[c * kSlab + idx] += a4[k][c];
+                    a4[k][c] = 0.0;
+                }
+                asm volatile("" ::: "memory");
+            }
         }
+        ry = nry;
+        base = nbase;
+        cur = nxt;
     }
     __syncthreads();
     // Sum the four slabs per tile node and flush: one global atomic per
@@ -519,7 +751,8 @@ __global__ __launch_bounds__(kBlock) void k_particle_tile(GridP g, PhysP ph, PSt
     constexpr int NCV = VEL ? 6 : 3;
     __shared__ double nodes[kTNN * NCV];
     __shared__ SplineTables tab;
-    const int tile = xcd_remap(blockIdx.x, gridDim.x);
+    const int tile = active_tile(S, blockIdx.x);
+    if (tile < 0) return;  // uniform over the block
     const int tid = threadIdx.x;
     const int beg = S.cell_start[tile * kCellsPerTile];
     const int end = S.cell_start[tile * kCellsPerTile + kCellsPerTile];
@@ -725,27 +958,38 @@ size_t sort_scan_bytes(int nkeys) {
     return bytes;
 }
 
-void launch_sort(const GridP &g, const PState &P, SortState &S, int64_t *out_id, hipStream_t s) {
+void launch_sort(const GridP &g, const PhysP &ph, const PState &P, SortState &S,
+                 int64_t *out_id, int packets, hipStream_t s) {
     if (P.np <= 0) return;
     hipLaunchKernelGGL(k_keys, dim3(blocks_for(P.np)), dim3(kBlock), 0, s, g, P, S);
     size_t bytes = S.scan_bytes;
     (void)hipcub::DeviceScan::ExclusiveSum(S.scan_tmp, bytes, S.count, S.cell_start, S.nkeys + 1,
                                            s);
-    hipLaunchKernelGGL(k_scatter, dim3(blocks_for(P.np)), dim3(kBlock), 0, s, P, S, out_id);
+    hipLaunchKernelGGL(k_scatter, dim3(blocks_for(P.np)), dim3(kBlock), 0, s, P, S, out_id, ph,
+                       packets);
+    hipLaunchKernelGGL(k_tile_list, dim3(1), dim3(1024), 0, s, S);
 }
 
 void launch_p2g_tiles(const GridP &g, const PhysP &ph, const PState &P, const SortState &S,
                       const NState &N, int epoch, int momentum_only, int fallback, Scalars *sc,
                       hipStream_t s) {
     if (P.np <= 0) return;
-    const dim3 gr((unsigned)(g.ntc[0] * g.ntc[1] * g.ntc[2])), bl(kBlock);
-#define XMPM_PG(f, fb)                                                                        \
-    hipLaunchKernelGGL((k_p2g_groups<f, fb>), gr, bl, 0, s, g, ph, P, S, N.acc, epoch, sc)
+    const dim3 gr((unsigned)((S.ntiles + 7) / 8 * 8)), bl(kBlock);
+#define XMPM_PG(o, f, fb)                                                                     \
+    hipLaunchKernelGGL((k_p2g_cells<o, f, fb>), gr, bl, 0, s, g, ph, P, S, N.acc, epoch, sc)
+#define XMPM_PGO(f, fb)                                                                       \
+    do {                                                                                      \
+        if (!g.uniform) XMPM_PG(0, f, fb);                                                    \
+        else if (g.od[0] == 1) XMPM_PG(1, f, fb);                                             \
+        else if (g.od[0] == 2) XMPM_PG(2, f, fb);                                             \
+        else XMPM_PG(3, f, fb);                                                               \
+    } while (0)
     if (momentum_only) {
-        if (fallback) XMPM_PG(false, true); else XMPM_PG(false, false);
+        if (fallback) XMPM_PGO(false, true); else XMPM_PGO(false, false);
     } else {
-        if (fallback) XMPM_PG(true, true); else XMPM_PG(true, false);
+        if (fallback) XMPM_PGO(true, true); else XMPM_PGO(true, false);
     }
+#undef XMPM_PGO
 #undef XMPM_PG
 }
 
@@ -768,7 +1012,7 @@ void launch_particle_fast(const GridP &g, const PhysP &ph, const PState &Pin, co
                           const SortState &S, const NState &N, int vel, int redist, Scalars *sc,
                           hipStream_t s) {
     if (Pin.np <= 0) return;
-    const dim3 gr((unsigned)(g.ntc[0] * g.ntc[1] * g.ntc[2])), bl(kBlock);
+    const dim3 gr((unsigned)((S.ntiles + 7) / 8 * 8)), bl(kBlock);
     const int O = g.od[0];
 #define XMPM_PT(o, v, dl)                                                                     \
     hipLaunchKernelGGL((k_particle_tile<o, v, dl>), gr, bl, 0, s, g, ph, Pin, Pout, S, N,   \
