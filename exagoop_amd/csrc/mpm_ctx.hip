@@ -621,8 +621,10 @@ int mpm_step(mpm_ctx *c, int nsteps) {
         const bool fused_dt = c->dt_next_valid && !(redist && c->dirty);
         if (redist && c->dirty) launch_redistribute(c->g, c->P, st);
         if (musl && c->P.vsnap) launch_snapshot_vel(c->P, st);
+        // The previous step of this call ends here: its dt advances the time
+        // inside the dt launch; the last step's advance follows the loop.
         launch_dt(c->g, c->ph, c->P, c->sc, c->s.fixed_timestep == 1, c->s.timestep, st,
-                  fused_dt ? 1 : 0);
+                  fused_dt ? 1 : 0, it > 0 ? 1 : 0);
         if (c->any_udf) {
             Scalars h;
             HIPCHK(hipMemcpyAsync(&h, c->sc, sizeof(Scalars), hipMemcpyDeviceToHost, st));
@@ -675,7 +677,6 @@ int mpm_step(mpm_ctx *c, int nsteps) {
             c->dt_next_valid = false;
         }
         if (prof) HIPCHK(hipEventRecord(c->ev[6], st));
-        launch_advance_time(c->sc, st);
         c->dirty = !redist;
         int rc = check_launch();
         if (rc) return rc;
@@ -692,6 +693,7 @@ int mpm_step(mpm_ctx *c, int nsteps) {
             kms[3] += e;
         }
     }
+    if (nsteps > 0) launch_advance_time(c->sc, st);
     HIPCHK(hipEventRecord(c->ev[5], st));
     HIPCHK(hipEventSynchronize(c->ev[5]));
     float tot = 0;

@@ -363,7 +363,8 @@ __global__ void k_dt_init(Scalars *sc) {
     sc->dtmin_bits = (unsigned long long)__double_as_longlong(__builtin_huge_val());
 }
 
-__global__ void k_dt_final(Scalars *sc, PhysP ph, int fixed, double fixed_dt, int fused) {
+__global__ void k_dt_final(Scalars *sc, PhysP ph, int fixed, double fixed_dt, int fused,
+                           int advance) {
     // The step about to run accumulates the following step's minimum, so the
     // running value is reset here.  Read and reset are one atomic exchange: a
     // plain load of a uniform address becomes a scalar-cache load, which the
@@ -371,15 +372,20 @@ __global__ void k_dt_final(Scalars *sc, PhysP ph, int fixed, double fixed_dt, in
     const unsigned long long nxt = atomicExch(
         &sc->dtmin_next_bits, (unsigned long long)__double_as_longlong(__builtin_huge_val()));
     const unsigned long long bits = fused ? nxt : sc->dtmin_bits;
+    double dt;
     if (fixed) {
-        sc->dt = fixed_dt;
-        return;
+        dt = fixed_dt;
+    } else {
+        dt = __longlong_as_double((long long)bits);
+        if (dt == __builtin_huge_val()) dt = 1.79769313486231570815e+308;  // numeric_limits max
+        dt = ph.cfl * dt;
+        dt = fmax(fmin(dt, ph.dt_max), ph.dt_min);
     }
-    double dt = __longlong_as_double((long long)bits);
-    if (dt == __builtin_huge_val()) dt = 1.79769313486231570815e+308;  // numeric_limits max
-    dt = ph.cfl * dt;
-    dt = fmax(fmin(dt, ph.dt_max), ph.dt_min);
-    sc->dt = dt;
+    // Same exchange for dt: the previous step's dt advances the time
+    // (the end of that step, folded into this launch).
+    const unsigned long long old = atomicExch(reinterpret_cast<unsigned long long *>(&sc->dt),
+                                              (unsigned long long)__double_as_longlong(dt));
+    if (advance) sc->time = sc->time + __longlong_as_double((long long)old);
 }
 
 __global__ void k_advance_time(Scalars *sc) { sc->time = sc->time + sc->dt; }
@@ -470,14 +476,14 @@ void launch_particle(const GridP &g, const PhysP &ph, const LsP &ls, const UdfP 
 }
 
 void launch_dt(const GridP &g, const PhysP &ph, const PState &P, Scalars *sc, int fixed,
-               double fixed_dt, hipStream_t s, int fused) {
+               double fixed_dt, hipStream_t s, int fused, int advance) {
     if (!fixed && !fused) {
         hipLaunchKernelGGL(k_dt_init, dim3(1), dim3(1), 0, s, sc);
         if (P.np > 0)
             hipLaunchKernelGGL(k_dt, dim3(blocks_for(P.np)), dim3(kBlock), 0, s, g, P, sc);
     }
     hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(1), 0, s, sc, ph, fixed, fixed_dt,
-                       (fixed || !fused) ? 0 : 1);
+                       (fixed || !fused) ? 0 : 1, advance);
 }
 
 void launch_advance_time(Scalars *sc, hipStream_t s) {

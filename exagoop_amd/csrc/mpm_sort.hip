@@ -69,6 +69,7 @@ __device__ __forceinline__ int cell_key(const GridP &g, const int c[3]) {
 
 __global__ __launch_bounds__(kBlock) void k_keys(GridP g, PState P, SortState S) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) S.tiles[0] = 0;  // non-empty tile count, filled by k_tile_list
     const bool valid = i < P.np;
     int key = -1;
     if (valid) {
@@ -142,30 +143,29 @@ __global__ __launch_bounds__(kBlock) void k_scatter(PState P, SortState S, int64
     if (out_id && i >= S.cell_start[S.nkeys]) out_id[i] = -1;
 }
 
-// Non-empty tiles in increasing order into S.tiles[1..], their number into
-// S.tiles[0] (one workgroup of 1024 threads; deterministic order).
-__global__ __launch_bounds__(1024) void k_tile_list(SortState S) {
-    __shared__ int wsum[16];
+// Non-empty tiles into S.tiles[1..], their number into S.tiles[0] (zeroed
+// by k_keys): each workgroup compacts 256 consecutive tiles in order and
+// reserves its range with one atomic, so runs of neighbouring tiles stay
+// together while the order of the runs depends on scheduling.
+__global__ __launch_bounds__(kBlock) void k_tile_list(SortState S) {
+    __shared__ int wsum[kBlock / 64];
+    __shared__ int base;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    int base = 0;
-    for (int t0 = 0; t0 < S.ntiles; t0 += 1024) {
-        const int t = t0 + tid;
-        const bool f = t < S.ntiles && S.cell_start[(t + 1) * kCellsPerTile] !=
-                                           S.cell_start[t * kCellsPerTile];
-        const unsigned long long b = __ballot(f);
-        const int inwave = __popcll(b & ((1ull << lane) - 1ull));
-        if (lane == 0) wsum[w] = __popcll(b);
-        __syncthreads();
-        int off = 0, tot = 0;
-        for (int q = 0; q < 16; ++q) {
-            if (q < w) off += wsum[q];
-            tot += wsum[q];
-        }
-        if (f) S.tiles[1 + base + off + inwave] = t;
-        base += tot;
-        __syncthreads();
+    const int t = blockIdx.x * kBlock + tid;
+    const bool f = t < S.ntiles &&
+                   S.cell_start[(t + 1) * kCellsPerTile] != S.cell_start[t * kCellsPerTile];
+    const unsigned long long b = __ballot(f);
+    const int inwave = __popcll(b & ((1ull << lane) - 1ull));
+    if (lane == 0) wsum[w] = __popcll(b);
+    __syncthreads();
+    int off = 0, tot = 0;
+    for (int q = 0; q < kBlock / 64; ++q) {
+        if (q < w) off += wsum[q];
+        tot += wsum[q];
     }
-    if (tid == 0) S.tiles[0] = base;
+    if (tid == 0) base = tot ? atomicAdd(&S.tiles[0], tot) : 0;
+    __syncthreads();
+    if (f) S.tiles[1 + base + off + inwave] = t;
 }
 
 // One particle's deposit straight to the global accumulators (particles
@@ -927,14 +927,16 @@ __global__ __launch_bounds__(kBlock) void k_particle_tile(GridP g, PhysP ph, PSt
                 de[ZZ] += dt * rate;
                 linear_elastic(de, ds, E, nu);
                 for (int q = 0; q < 6; ++q) sig[q] = Pin.r[(MPM_R_STRESS + q) * ci + j] + ds[q];
+            } else if (model == 2) {
+                // (the Tait branch goes last: its pow expansion is the widest
+                // point of the kernel and F, eps, E, nu are dead there)
+                neo_hookean(F, sig, E, nu);
             } else if (model == 0) {
                 linear_elastic(eps, sig, E, nu);
-            } else if (model == 1) {
+            } else {
                 const double pinf = 0.0;
                 p = K * (pow(1.0 / J, gam) - 1.0) + pinf;
                 newtonian(D, sig, mu, p);
-            } else {
-                neo_hookean(F, sig, E, nu);
             }
         } else {
             for (int q = 0; q < 6; ++q) R[(MPM_R_STRAIN + q) * co + i] = eps[q];
@@ -967,7 +969,7 @@ void launch_sort(const GridP &g, const PhysP &ph, const PState &P, SortState &S,
                                            s);
     hipLaunchKernelGGL(k_scatter, dim3(blocks_for(P.np)), dim3(kBlock), 0, s, P, S, out_id, ph,
                        packets);
-    hipLaunchKernelGGL(k_tile_list, dim3(1), dim3(1024), 0, s, S);
+    hipLaunchKernelGGL(k_tile_list, dim3((S.ntiles + kBlock - 1) / kBlock), dim3(kBlock), 0, s, S);
 }
 
 void launch_p2g_tiles(const GridP &g, const PhysP &ph, const PState &P, const SortState &S,
