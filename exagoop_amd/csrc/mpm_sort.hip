@@ -289,8 +289,14 @@ __device__ __forceinline__ void p2g_factors4(const SplineTables &T, const GridP 
 // the body force density a = m g (+ slab load) and B = -V sigma (xx, xy, xz,
 // yy, yz, zz): packet component c >= 3 of the sorted packet array lands at
 // kPkM - 3 + c.  All zero for a slot without a depositing particle.
-constexpr int kPkFac = 0, kPkM = 24, kPkMV = 25, kPkA = 28, kPkB = 31, kPk = 38;
+constexpr int kPkFac = 0, kPkM = 24, kPkMV = 25, kPkA = 28, kPkB = 31, kPk = 40;
 constexpr int kSlots = 16;  // packets per wavefront and round: 4 cells x 4 particles
+// Packet slot s (cell s >> 2) starts at pk_base(s): the stride of 40 doubles
+// keeps 16-byte alignment, the extra 2 doubles per cell put the four cells'
+// packets (read by four lane groups at once, each group broadcasting one
+// address) in four different LDS bank quads.
+constexpr int kPkArea = kSlots * kPk + 2 * (kSlots / 4 - 1);
+__device__ __forceinline__ int pk_base(int s) { return s * kPk + 2 * (s >> 2); }
 
 __device__ __forceinline__ int sym_idx(int a, int b) {
     // XX XY XZ / XY YY YZ / XZ YZ ZZ
@@ -331,7 +337,7 @@ __global__ __launch_bounds__(kBlock) void k_p2g_cells(GridP g, PhysP ph, PState 
                                                       double *acc, int epoch, Scalars *sc) {
     constexpr int NC = FORCES ? 7 : 3;
     __shared__ double slab[4][NC * kSlab];  // one private slab per wavefront
-    __shared__ double pk[4][kSlots * kPk];  // staging area per wavefront
+    __shared__ double pk[4][kPkArea];  // staging area per wavefront
     __shared__ int cs[kCellsPerTile + 1];
     __shared__ SplineTables tab;
     const int tile = active_tile(S, blockIdx.x);
@@ -467,7 +473,7 @@ __global__ __launch_bounds__(kBlock) void k_p2g_cells(GridP g, PhysP ph, PState 
                 }
                 use = false;
             }
-            double *pw = pkw + s_slot * kPk;
+            double *pw = pkw + pk_base(s_slot);
             if (s_part < 3) {
                 double f[8];
                 if (use) {
@@ -491,7 +497,7 @@ __global__ __launch_bounds__(kBlock) void k_p2g_cells(GridP g, PhysP ph, PState 
         const int nt = min(4, mc_of(ry) - base);
         for (int tt = 0; tt < nt; ++tt) {
 #pragma clang fp contract(fast)
-            const double *pr = pkw + (q * 4 + tt) * kPk;
+            const double *pr = pkw + pk_base(q * 4 + tt);
             double wx[4], dwx[4];
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
