@@ -10,7 +10,7 @@ import subprocess
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libexagoop_amd.so")
-SOURCES = ["mpm_kernels.hip", "mpm_sort.hip", "mpm_ctx.hip"]
+SOURCES = ["mpm_kernels.hip", "mpm_sort.hip", "mpm_ctx.hip", "mpm_mlp.hip", "mpm_learn.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

@@ -28,6 +28,8 @@ BC_PERIODIC, BC_NOSLIP, BC_SLIP, BC_PARTIALSLIP, BC_OUTFLOW = 0, 1, 2, 3, 4
 USL, MUSL = 0, 1
 
 OPT_PROFILE, OPT_KEEP_FORCES, OPT_SORTED, OPT_FAST, OPT_FUSED_DT = 1, 2, 3, 4, 5
+MLP_NPARAMS = 11331
+MLP_IN = 128
 
 STATUS = {0: "OK", -1: "E_ARG", -2: "E_BC", -3: "E_ORDER", -4: "E_MODEL", -5: "E_OOM",
           -6: "E_HIP", -7: "E_NAN", -8: "E_UNSUPPORTED", -9: "E_NODEVICE"}
@@ -89,6 +91,7 @@ _P = ctypes.c_void_p
 _DP = ctypes.POINTER(ctypes.c_double)
 _IP = ctypes.POINTER(ctypes.c_int)
 _LP = ctypes.POINTER(ctypes.c_int64)
+_FP = ctypes.POINTER(ctypes.c_float)
 SIGNATURES = {
     "mpm_specs_default": (None, [ctypes.POINTER(Specs)]),
     "mpm_order_directional": (ctypes.c_int, [ctypes.POINTER(Specs), _IP]),
@@ -117,6 +120,24 @@ SIGNATURES = {
     "mpm_set_option": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int]),
     "mpm_last_step_timing": (ctypes.c_int, [_P, _DP, _DP]),
     "mpm_last_error": (ctypes.c_char_p, []),
+    "mpm_mlp_create": (ctypes.c_int, [ctypes.POINTER(_P)]),
+    "mpm_mlp_destroy": (None, [_P]),
+    "mpm_mlp_set_params": (ctypes.c_int, [_P, _FP, _DP]),
+    "mpm_mlp_get_params": (ctypes.c_int, [_P, _FP, _DP]),
+    "mpm_mlp_forward": (ctypes.c_int, [_P, ctypes.c_int64, _FP, _FP]),
+    "mpm_mlp_gradient": (ctypes.c_int, [_P, ctypes.c_int64, _FP, _FP, _FP, _DP]),
+    "mpm_mlp_adam": (ctypes.c_int, [_P, _FP, ctypes.c_float, ctypes.c_int]),
+    "mpm_dataset_create": (ctypes.c_int, [ctypes.c_int64, ctypes.POINTER(_P)]),
+    "mpm_dataset_destroy": (None, [_P]),
+    "mpm_dataset_size": (ctypes.c_int, [_P, _LP]),
+    "mpm_dataset_add": (ctypes.c_int, [_P, ctypes.c_int64, _FP, _FP]),
+    "mpm_dataset_target_rms": (ctypes.c_int, [_P, _DP]),
+    "mpm_learn_features": (ctypes.c_int, [_P, _P, ctypes.c_int64, _LP, _FP]),
+    "mpm_learn_collect": (ctypes.c_int, [_P, _P, _P, ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_uint64]),
+    "mpm_mlp_train": (ctypes.c_int, [_P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_float,
+                                     ctypes.c_uint64, _DP]),
+    "mpm_step_learned": (ctypes.c_int, [_P, _P, ctypes.c_int, ctypes.c_int, _DP, _IP]),
 }
 
 
@@ -305,3 +326,139 @@ class Context(_Base):
         k = np.zeros(4)
         self._call("last_step_timing", self._ctx, ctypes.byref(tot), dptr(k))
         return tot.value, k
+
+
+def fptr(a):
+    return a.ctypes.data_as(_FP)
+
+
+def _check(lib, rc, what):
+    if rc != 0:
+        raise MPMError(f"{what} -> {STATUS.get(rc, rc)}: {lib.mpm_last_error().decode()}")
+
+
+class MLP:
+    """The learned correction's network on the device."""
+
+    def __init__(self, params=None, scales=(1.0, 1.0, 1.0)):
+        self._lib = load()
+        p = _P()
+        _check(self._lib, self._lib.mpm_mlp_create(ctypes.byref(p)), "mpm_mlp_create")
+        self._p = p
+        if params is not None:
+            self.set_params(params, scales)
+
+    def close(self):
+        if getattr(self, "_p", None):
+            self._lib.mpm_mlp_destroy(self._p)
+            self._p = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, params, scales):
+        params = np.ascontiguousarray(params, dtype=np.float32)
+        assert params.size == MLP_NPARAMS
+        sc = np.ascontiguousarray(scales, dtype=np.float64)
+        _check(self._lib, self._lib.mpm_mlp_set_params(self._p, fptr(params), dptr(sc)),
+               "mpm_mlp_set_params")
+
+    def get_params(self):
+        params = np.zeros(MLP_NPARAMS, dtype=np.float32)
+        sc = np.zeros(3)
+        _check(self._lib, self._lib.mpm_mlp_get_params(self._p, fptr(params), dptr(sc)),
+               "mpm_mlp_get_params")
+        return params, sc
+
+    def forward(self, X):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        Y = np.zeros((X.shape[0], 3), dtype=np.float32)
+        _check(self._lib, self._lib.mpm_mlp_forward(self._p, X.shape[0], fptr(X), fptr(Y)),
+               "mpm_mlp_forward")
+        return Y
+
+    def gradient(self, X, T):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        T = np.ascontiguousarray(T, dtype=np.float32)
+        g = np.zeros(MLP_NPARAMS, dtype=np.float32)
+        loss = ctypes.c_double(0)
+        _check(self._lib, self._lib.mpm_mlp_gradient(self._p, X.shape[0], fptr(X), fptr(T),
+                                                     fptr(g), ctypes.byref(loss)),
+               "mpm_mlp_gradient")
+        return loss.value, g
+
+    def adam(self, grad, lr, t):
+        grad = np.ascontiguousarray(grad, dtype=np.float32)
+        _check(self._lib, self._lib.mpm_mlp_adam(self._p, fptr(grad), lr, t), "mpm_mlp_adam")
+
+    def train(self, dataset, epochs, batch, lr, seed=0):
+        loss = np.zeros(max(epochs, 1))
+        _check(self._lib, self._lib.mpm_mlp_train(self._p, dataset._p, epochs, batch, lr, seed,
+                                                  dptr(loss)), "mpm_mlp_train")
+        return loss[:epochs]
+
+
+class Dataset:
+    """Training rows on the device."""
+
+    def __init__(self, capacity):
+        self._lib = load()
+        p = _P()
+        _check(self._lib, self._lib.mpm_dataset_create(capacity, ctypes.byref(p)),
+               "mpm_dataset_create")
+        self._p = p
+
+    def close(self):
+        if getattr(self, "_p", None):
+            self._lib.mpm_dataset_destroy(self._p)
+            self._p = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def size(self):
+        n = ctypes.c_int64(0)
+        _check(self._lib, self._lib.mpm_dataset_size(self._p, ctypes.byref(n)),
+               "mpm_dataset_size")
+        return n.value
+
+    def add(self, X, T):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        T = np.ascontiguousarray(T, dtype=np.float32)
+        _check(self._lib, self._lib.mpm_dataset_add(self._p, X.shape[0], fptr(X), fptr(T)),
+               "mpm_dataset_add")
+
+    def target_rms(self):
+        v = ctypes.c_double(0)
+        _check(self._lib, self._lib.mpm_dataset_target_rms(self._p, ctypes.byref(v)),
+               "mpm_dataset_target_rms")
+        return v.value
+
+
+def learn_features(ctx, mlp, nodes):
+    nodes = np.ascontiguousarray(nodes, dtype=np.int64)
+    X = np.zeros((nodes.size, MLP_IN), dtype=np.float32)
+    _check(ctx._lib, ctx._lib.mpm_learn_features(ctx._ctx, mlp._p, nodes.size, lptr(nodes),
+                                                  fptr(X)), "mpm_learn_features")
+    return X
+
+
+def learn_collect(ctx, mlp, dataset, nfine, max_rows, seed):
+    _check(ctx._lib, ctx._lib.mpm_learn_collect(ctx._ctx, mlp._p, dataset._p, nfine, max_rows,
+                                                 seed), "mpm_learn_collect")
+
+
+def step_learned(ctx, mlp, ncoarse, nfine, guards=(0.02, 0.25)):
+    """Returns the number of coarse steps that fell back to plain steps."""
+    g = np.ascontiguousarray(guards, dtype=np.float64)
+    fb = ctypes.c_int(0)
+    _check(ctx._lib, ctx._lib.mpm_step_learned(ctx._ctx, mlp._p if mlp is not None else None,
+                                                ncoarse, nfine, dptr(g), ctypes.byref(fb)),
+           "mpm_step_learned")
+    return fb.value

@@ -1,0 +1,83 @@
+// mpm_context.h — the context behind the C ABI (mpm_ctx.hip) and the
+// internal helpers the entry points share (error text, launch checks).
+#pragma once
+
+#include <vector>
+
+#include "mpm_internal.h"
+
+struct mpm_ctx {
+    using GridP = xmpm::GridP;
+    using PhysP = xmpm::PhysP;
+    using LsP = xmpm::LsP;
+    using UdfP = xmpm::UdfP;
+    using PState = xmpm::PState;
+    using NState = xmpm::NState;
+    using SortState = xmpm::SortState;
+    using Scalars = xmpm::Scalars;
+
+    mpm_specs s;
+    GridP g;
+    PhysP ph;
+    LsP ls;
+    UdfP udf;
+    PState P;      // current particle buffers
+    PState Q;      // second buffer the sorted step writes into
+    NState N;
+    SortState S;
+    // Cold particle fields (never read by the step), by upload index.
+    std::vector<double> cold_radius, cold_yacc;
+    std::vector<int> cold_rigid;
+    bool skip_ok = true;    // idle node tiles hold the all-zero massless state
+    bool all_dirty = true;  // next step's grid pass must visit every node
+    int sorted = 1;         // use the cell-sorted step
+    int fast = 1;           // use the specialised particle kernel where it applies
+    int fuse_dt = 1;        // the specialised kernel reduces the next step's dt
+    bool dt_next_valid = false;  // the last step reduced the next step's dt candidates
+    Scalars *sc = nullptr;
+    double *d_energy = nullptr;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[8];
+    std::vector<double *> ls_bufs;
+    double *udf_dev[3][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};
+    mpm_wall_udf_fn udf_fn[3][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};
+    bool any_udf = false;
+    bool any_per = false;
+    bool dirty = false;  // particles moved since the last redistribute
+    int64_t steps = 0;
+    double time_host = 0.0;
+    double last_dt = 0.0;
+    double step_ms = 0.0;
+    double kernel_ms[4] = {0, 0, 0, 0};
+    int profile = 0;
+    int keep_forces = 1;
+    // Learned-correction work buffers (allocated by its first call, nodal sized).
+    int *ml_nodes = nullptr;      // active node list / selected rows
+    int *ml_count = nullptr;      // [0] active count, [1] selected count, [2] appended count
+    __bf16 *ml_x = nullptr;       // feature rows, padded to 32 rows
+    float *ml_y = nullptr;        // network outputs [row][3]
+    double *ml_vc = nullptr;      // coarse nodal velocity of selected rows [row][3]
+    double *ml_guard = nullptr;   // guard sums
+    int64_t ml_cap = 0;
+};
+
+namespace xmpm {
+
+// Record the message of a failed call and return its status.
+int set_err(int code, const char *what);
+int hip_err(hipError_t e, const char *expr);
+// Status of the last kernel launch.
+int check_launch();
+
+inline int grid_flags_full(const mpm_ctx *c) {
+    return G_FINISH_MASS | G_FINISH_VEL | G_FINISH_FRC | G_BACKUP | G_UPDATE | G_BCS |
+           (c->keep_forces ? G_KEEP_FRC : 0);
+}
+
+}  // namespace xmpm
+
+#define HIPCHK(expr)                                             \
+    do {                                                         \
+        hipError_t e_ = (expr);                                  \
+        if (e_ != hipSuccess) return ::xmpm::hip_err(e_, #expr); \
+    } while (0)

@@ -15,11 +15,11 @@
 #include <numeric>
 #include <vector>
 
-#include "mpm_internal.h"
+#include "mpm_context.h"
 
 using namespace xmpm;
 
-namespace {
+namespace xmpm {
 
 thread_local char g_err[256] = "";
 
@@ -28,55 +28,22 @@ int set_err(int code, const char *what) {
     return code;
 }
 
-#define HIPCHK(expr)                                                           \
-    do {                                                                       \
-        hipError_t e_ = (expr);                                                \
-        if (e_ != hipSuccess) {                                                \
-            std::snprintf(g_err, sizeof(g_err), "%s: %s", #expr,               \
-                          hipGetErrorString(e_));                              \
-            return e_ == hipErrorOutOfMemory ? MPM_E_OOM : MPM_E_HIP;         \
-        }                                                                      \
-    } while (0)
+int hip_err(hipError_t e, const char *expr) {
+    std::snprintf(g_err, sizeof(g_err), "%s: %s", expr, hipGetErrorString(e));
+    return e == hipErrorOutOfMemory ? MPM_E_OOM : MPM_E_HIP;
+}
 
-}  // namespace
+int check_launch() {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        std::snprintf(g_err, sizeof(g_err), "kernel launch: %s", hipGetErrorString(e));
+        return MPM_E_HIP;
+    }
+    return MPM_OK;
+}
 
-struct mpm_ctx {
-    mpm_specs s;
-    GridP g;
-    PhysP ph;
-    LsP ls;
-    UdfP udf;
-    PState P;      // current particle buffers
-    PState Q;      // second buffer the sorted step writes into
-    NState N;
-    SortState S;
-    // Cold particle fields (never read by the step), by upload index.
-    std::vector<double> cold_radius, cold_yacc;
-    std::vector<int> cold_rigid;
-    bool skip_ok = true;    // idle node tiles hold the all-zero massless state
-    bool all_dirty = true;  // next step's grid pass must visit every node
-    int sorted = 1;         // use the cell-sorted step
-    int fast = 1;           // use the specialised particle kernel where it applies
-    int fuse_dt = 1;        // the specialised kernel reduces the next step's dt
-    bool dt_next_valid = false;  // the last step reduced the next step's dt candidates
-    Scalars *sc = nullptr;
-    double *d_energy = nullptr;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev[8];
-    std::vector<double *> ls_bufs;
-    double *udf_dev[3][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};
-    mpm_wall_udf_fn udf_fn[3][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};
-    bool any_udf = false;
-    bool any_per = false;
-    bool dirty = false;  // particles moved since the last redistribute
-    int64_t steps = 0;
-    double time_host = 0.0;
-    double last_dt = 0.0;
-    double step_ms = 0.0;
-    double kernel_ms[4] = {0, 0, 0, 0};
-    int profile = 0;
-    int keep_forces = 1;
-};
+}  // namespace xmpm
+
 
 namespace {
 
@@ -139,20 +106,6 @@ int upload_udf_tables(mpm_ctx *c, double t) {
             HIPCHK(hipStreamSynchronize(c->stream));
         }
     return MPM_OK;
-}
-
-int check_launch() {
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) {
-        std::snprintf(g_err, sizeof(g_err), "kernel launch: %s", hipGetErrorString(e));
-        return MPM_E_HIP;
-    }
-    return MPM_OK;
-}
-
-int grid_flags_full(const mpm_ctx *c) {
-    return G_FINISH_MASS | G_FINISH_VEL | G_FINISH_FRC | G_BACKUP | G_UPDATE | G_BCS |
-           (c->keep_forces ? G_KEEP_FRC : 0);
 }
 
 }  // namespace
@@ -350,6 +303,12 @@ void mpm_destroy(mpm_ctx *c) {
     (void)hipFree(c->S.scan_tmp);
     (void)hipFree(c->S.tile_epoch);
     (void)hipFree(c->S.tiles);
+    (void)hipFree(c->ml_nodes);
+    (void)hipFree(c->ml_count);
+    (void)hipFree(c->ml_x);
+    (void)hipFree(c->ml_y);
+    (void)hipFree(c->ml_vc);
+    (void)hipFree(c->ml_guard);
     for (double *p : c->ls_bufs) (void)hipFree(p);
     for (int d = 0; d < 3; ++d)
         for (int h = 0; h < 2; ++h) (void)hipFree(c->udf_dev[d][h]);

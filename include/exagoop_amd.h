@@ -261,6 +261,73 @@ const char *mpm_last_error(void);
  * kernel order is p2g, grid, g2p (fused particle kernel), sort/other. */
 int mpm_last_step_timing(mpm_ctx *ctx, double *total_ms, double kernel_ms[4]);
 
+/* ---------------------------------------------------------------------------
+ * Learned coarse-step correction.  No counterpart in the reference: an
+ * addition beside the plain step, which stays the oracle.  One coarse step
+ * (nfine times the CFL dt) takes the place of nfine plain steps; after its
+ * nodal update a per-node stencil MLP corrects the nodal velocity:
+ *   inputs  27 neighbour nodes (dz, dy, dx in {-1,0,1}, dx fastest) x
+ *           (m / m_ref, m v / (m_ref v_ref)) = 108 values, zero outside the box;
+ *   network 108 -> 64 -> 64 -> 3, tanh-form GELU, BF16 MFMA operands with
+ *           FP32 accumulation; output x dv_ref = velocity correction.
+ * Parameters (FP32, MPM_MLP_NPARAMS): W1[108][64], b1[64], W2[64][64],
+ * b2[64], W3[64][3], b3[3] with W[in][out].  Callers: the learned-step
+ * driver (exagoop_amd/learned.py) and its tests.
+ * ------------------------------------------------------------------------- */
+#define MPM_MLP_NPARAMS 11331
+#define MPM_MLP_IN 128 /* feature row length (108 used, zero padded) */
+typedef struct mpm_mlp mpm_mlp;
+typedef struct mpm_dataset mpm_dataset;
+
+int mpm_mlp_create(mpm_mlp **out);
+void mpm_mlp_destroy(mpm_mlp *m);
+/* scales = {m_ref, v_ref, dv_ref}. */
+int mpm_mlp_set_params(mpm_mlp *m, const float *params, const double scales[3]);
+int mpm_mlp_get_params(mpm_mlp *m, float *params, double scales[3]);
+/* Network outputs Y[nrows][3] of host feature rows X[nrows][MPM_MLP_IN]. */
+int mpm_mlp_forward(mpm_mlp *m, int64_t nrows, const float *X, float *Y);
+/* Mean squared error (1/nrows) sum (Y - T)^2 over host rows and its gradient
+ * grad[MPM_MLP_NPARAMS]. */
+int mpm_mlp_gradient(mpm_mlp *m, int64_t nrows, const float *X, const float *T, float *grad,
+                     double *loss);
+/* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8, step number t >= 1) with
+ * the given gradient. */
+int mpm_mlp_adam(mpm_mlp *m, const float *grad, float lr, int t);
+
+/* Training rows on the device: features (BF16) and velocity-correction
+ * targets (m/s). */
+int mpm_dataset_create(int64_t capacity, mpm_dataset **out);
+void mpm_dataset_destroy(mpm_dataset *d);
+int mpm_dataset_size(mpm_dataset *d, int64_t *nrows);
+int mpm_dataset_add(mpm_dataset *d, int64_t nrows, const float *X, const float *T);
+/* Root mean square of the targets' components. */
+int mpm_dataset_target_rms(mpm_dataset *d, double *rms);
+
+/* Feature rows X[nrows][MPM_MLP_IN] (BF16 values widened to float) of the
+ * listed nodes from the context's current nodal mass and velocity. */
+int mpm_learn_features(mpm_ctx *ctx, mpm_mlp *m, int64_t nrows, const int64_t *nodes,
+                       float *X);
+/* From the context's state: one coarse step (P2G + nodal update with nfine x
+ * the CFL dt) gives the features of up to max_rows randomly chosen active
+ * nodes (scales from m); the coarse step is discarded and nfine plain steps
+ * run (the context advances as with mpm_step).  Each chosen node still
+ * active after them adds a row with target = fine nodal velocity - coarse
+ * nodal velocity.  Requires the specialised USL step (sorted, redistribution
+ * every step, no level set or UDF). */
+int mpm_learn_collect(mpm_ctx *ctx, mpm_mlp *m, mpm_dataset *d, int nfine, int max_rows,
+                      uint64_t seed);
+/* epochs of shuffled minibatches with Adam; targets are divided by dv_ref;
+ * loss_out[epoch] = mean loss of the epoch (may be NULL). */
+int mpm_mlp_train(mpm_mlp *m, mpm_dataset *d, int epochs, int batch, float lr, uint64_t seed,
+                  double *loss_out);
+/* ncoarse learned steps of nfine x dt each.  guards = {gp, ge}: the corrected
+ * nodal field must satisfy |sum m dv| <= gp sum m |v| and
+ * sum m |v + dv|^2 <= (1 + ge) sum m |v|^2 and be finite; otherwise the coarse
+ * step is dropped and nfine plain steps run instead (*fallbacks counts them).
+ * m == NULL runs the coarse steps without correction. */
+int mpm_step_learned(mpm_ctx *ctx, mpm_mlp *m, int ncoarse, int nfine, const double guards[2],
+                     int *fallbacks);
+
 #ifdef __cplusplus
 }
 #endif

@@ -103,3 +103,77 @@ def applybc(relvel, mu, normal, bc, preload=None):
     out = d3(*(relvel if preload is None else preload))
     rc = lib.orc_applybc(d3(*relvel), out, mu, d3(*normal), bc)
     return rc, np.array(out[:])
+
+
+# ---- learned correction (oracle_mlp.cpp) -------------------------------------
+
+MLP_NPARAMS = 11331
+MLP_IN_PAD = 128
+
+
+def _mlp_lib():
+    lib = load()
+    if not getattr(lib, "_mlp_bound", False):
+        F, D, I64, I = ctypes.c_void_p, ctypes.c_double, ctypes.c_int64, ctypes.c_int
+        lib.orc_bf16_round.argtypes = [ctypes.c_float]
+        lib.orc_bf16_round.restype = ctypes.c_float
+        lib.orc_mlp_features.argtypes = [F, F, F, I64, F, I64, D, D, F]
+        lib.orc_mlp_features.restype = None
+        lib.orc_mlp_forward.argtypes = [F, I64, F, I, F]
+        lib.orc_mlp_forward.restype = None
+        lib.orc_mlp_grad.argtypes = [F, I64, F, F, I, F]
+        lib.orc_mlp_grad.restype = D
+        lib.orc_mlp_adam.argtypes = [F, F, F, F, I, ctypes.c_float, ctypes.c_float,
+                                     ctypes.c_float, ctypes.c_float, I]
+        lib.orc_mlp_adam.restype = None
+        lib._mlp_bound = True
+    return lib
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def bf16_round(x):
+    lib = _mlp_lib()
+    return np.array([lib.orc_bf16_round(float(v)) for v in np.ravel(x)],
+                    dtype=np.float32).reshape(np.shape(x))
+
+
+def mlp_features(mass, vel, ncells, nodes, m_ref, v_ref):
+    """Feature rows (nrows x 128, float32) of the listed nodes."""
+    mass = np.ascontiguousarray(mass, dtype=np.float64)
+    vel = np.ascontiguousarray(vel, dtype=np.float64)
+    nodes = np.ascontiguousarray(nodes, dtype=np.int64)
+    nc = np.ascontiguousarray(ncells, dtype=np.int32)
+    X = np.zeros((len(nodes), MLP_IN_PAD), dtype=np.float32)
+    _mlp_lib().orc_mlp_features(_p(mass), _p(vel), _p(nc), mass.size, _p(nodes), len(nodes),
+                                float(m_ref), float(v_ref), _p(X))
+    return X
+
+
+def mlp_forward(params, X, exact=False):
+    params = np.ascontiguousarray(params, dtype=np.float32)
+    X = np.ascontiguousarray(X, dtype=np.float32)
+    Y = np.zeros((X.shape[0], 3), dtype=np.float32)
+    _mlp_lib().orc_mlp_forward(_p(params), X.shape[0], _p(X), int(exact), _p(Y))
+    return Y
+
+
+def mlp_grad(params, X, T, exact=False):
+    """(loss, gradient as float64) of the mean squared error."""
+    params = np.ascontiguousarray(params, dtype=np.float32)
+    X = np.ascontiguousarray(X, dtype=np.float32)
+    T = np.ascontiguousarray(T, dtype=np.float32)
+    g = np.zeros(MLP_NPARAMS, dtype=np.float64)
+    loss = _mlp_lib().orc_mlp_grad(_p(params), X.shape[0], _p(X), _p(T), int(exact), _p(g))
+    return loss, g
+
+
+def mlp_adam(params, m, v, grad, lr, t, beta1=0.9, beta2=0.999, eps=1e-8):
+    """One Adam step in place on float32 arrays."""
+    for a in (params, m, v):
+        assert a.dtype == np.float32 and a.flags.c_contiguous
+    grad = np.ascontiguousarray(grad, dtype=np.float32)
+    _mlp_lib().orc_mlp_adam(_p(params), _p(m), _p(v), _p(grad), params.size, lr, beta1, beta2,
+                            eps, t)

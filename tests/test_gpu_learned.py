@@ -1,0 +1,181 @@
+"""GPU checks of the learned coarse-step correction through the C ABI: the
+BF16 MFMA network's forward pass, gradient and Adam step against the CPU
+oracle (oracle/oracle_mlp.cpp, pinned in test_mlp_oracle.py), the nodal
+features against the oracle's gather, the learned step's plumbing (one
+uncorrected coarse step of one fine dt is the plain step; a rejected
+correction falls back to the plain steps exactly) and training."""
+import numpy as np
+import pytest
+
+import oracle
+from exagoop_amd import capi, scenes
+
+pytestmark = pytest.mark.gpu
+
+NP = capi.MLP_NPARAMS
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+def rand_params(seed, scale=0.15):
+    rng = np.random.default_rng(seed)
+    return (rng.standard_normal(NP) * scale).astype(np.float32)
+
+
+def rand_rows(seed, n):
+    rng = np.random.default_rng(seed)
+    X = np.zeros((n, capi.MLP_IN), dtype=np.float32)
+    X[:, :108] = rng.standard_normal((n, 108)).astype(np.float32)
+    T = rng.standard_normal((n, 3)).astype(np.float32)
+    return X, T
+
+
+def rel(a, b):
+    return np.linalg.norm(np.asarray(a, np.float64) - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def test_forward_matches_oracle(gpu):
+    say("[learned] forward vs oracle")
+    p = rand_params(1)
+    X, _ = rand_rows(2, 1000)  # not a multiple of the 32-row tile
+    net = capi.MLP(p)
+    y = net.forward(X)
+    ref = oracle.mlp_forward(p, X, exact=False)
+    # same BF16 rounding points; FP32 sums in a different order
+    assert rel(y, ref) <= 2e-3, rel(y, ref)
+    assert rel(y, oracle.mlp_forward(p, X, exact=True)) <= 5e-2
+
+
+def test_gradient_matches_oracle(gpu):
+    say("[learned] gradient vs oracle")
+    p = rand_params(3)
+    X, T = rand_rows(4, 777)
+    net = capi.MLP(p)
+    loss, g = net.gradient(X, T)
+    lref, gref = oracle.mlp_grad(p, X, T, exact=False)
+    assert abs(loss - lref) <= 2e-3 * abs(lref), (loss, lref)
+    assert rel(g, gref) <= 1e-2, rel(g, gref)
+    # every parameter block carries gradient
+    for lo, hi in ((0, 6912), (6912, 6976), (6976, 11072), (11072, 11136), (11136, 11328),
+                   (11328, 11331)):
+        assert rel(g[lo:hi], gref[lo:hi]) <= 2e-2, (lo, hi, rel(g[lo:hi], gref[lo:hi]))
+
+
+def test_adam_matches_oracle(gpu):
+    say("[learned] adam vs oracle")
+    p = rand_params(5)
+    net = capi.MLP(p)
+    m = np.zeros(NP, np.float32)
+    v = np.zeros(NP, np.float32)
+    pr = p.copy()
+    rng = np.random.default_rng(6)
+    for t in range(1, 4):
+        g = rng.standard_normal(NP).astype(np.float32)
+        net.adam(g, 1e-3, t)
+        oracle.mlp_adam(pr, m, v, g, 1e-3, t)
+    got, _ = net.get_params()
+    assert np.max(np.abs(got - pr)) <= 1e-6
+
+
+def small_fluid(seed=7):
+    s, pos, r, idata = scenes.dam_break(nx=32, ny=16, nz=16)
+    return s, pos, r, idata
+
+
+def test_features_match_oracle(gpu):
+    say("[learned] node features vs oracle gather")
+    s, pos, r, idata = small_fluid()
+    g = capi.Context(s)
+    g.upload(pos, r, idata)
+    g.init_internal_forces()
+    g.step(3)
+    mass = g.nodal(capi.N_MASS)
+    vel = np.stack([g.nodal(capi.N_VEL + d) for d in range(3)])
+    nodes = np.flatnonzero(mass >= s.mass_tolerance)
+    nodes = np.concatenate([nodes[:50], nodes[-50:], [0, g.nn - 1]])
+    net = capi.MLP(rand_params(8), scales=(2.0, 0.5, 1.0))
+    X = capi.learn_features(g, net, nodes)
+    ref = oracle.mlp_features(mass, vel, list(s.ncells), nodes, 2.0, 0.5)
+    assert np.array_equal(X, oracle.bf16_round(ref))
+
+
+def test_uncorrected_single_fine_coarse_step_is_the_plain_step(gpu):
+    say("[learned] coarse step of one fine dt, no network == plain step")
+    s, pos, r, idata = small_fluid()
+    a = capi.Context(s)
+    b = capi.Context(s)
+    for c in (a, b):
+        c.upload(pos, r, idata)
+        c.init_internal_forces()
+        c.step(2)
+    fb = capi.step_learned(a, None, 5, 1)
+    b.step(5)
+    assert fb == 0
+    pa, ra, _, ia = a.download()
+    pb, rb, _, ib = b.download()
+    assert np.array_equal(ia, ib)
+    assert rel(pa, pb) <= 1e-13
+    assert rel(ra, rb) <= 1e-12
+    assert a.time()[:2] == b.time()[:2] or abs(a.time()[0] - b.time()[0]) <= 1e-15
+
+
+def test_rejected_correction_falls_back_to_plain_steps(gpu):
+    say("[learned] guard rejects a wild network -> plain steps")
+    s, pos, r, idata = small_fluid()
+    a = capi.Context(s)
+    b = capi.Context(s)
+    for c in (a, b):
+        c.upload(pos, r, idata)
+        c.init_internal_forces()
+        c.step(2)
+    p = np.zeros(NP, np.float32)
+    p[-3:] = [50.0, -40.0, 30.0]  # b3: a large constant correction
+    net = capi.MLP(p, scales=(1.0, 1.0, 1.0))
+    fb = capi.step_learned(a, net, 3, 4)
+    b.step(12)
+    assert fb == 3
+    pa, ra, _, ia = a.download()
+    pb, rb, _, ib = b.download()
+    assert np.array_equal(ia, ib)
+    assert rel(pa, pb) <= 1e-13
+    assert rel(ra, rb) <= 1e-12
+
+
+def test_training_fits_a_smooth_map(gpu):
+    say("[learned] training on a synthetic smooth map")
+    rng = np.random.default_rng(9)
+    n = 8192
+    X = np.zeros((n, capi.MLP_IN), dtype=np.float32)
+    X[:, :108] = rng.uniform(-1, 1, (n, 108)).astype(np.float32)
+    A = rng.standard_normal((108, 3)) / np.sqrt(108)
+    T = (np.tanh(X[:, :108] @ A) * 0.02).astype(np.float32)  # targets in m/s, dv_ref 0.02
+    ds = capi.Dataset(n)
+    ds.add(X, T)
+    assert ds.size() == n
+    assert abs(ds.target_rms() - np.sqrt(np.mean(T.astype(np.float64) ** 2))) <= 1e-6
+    net = capi.MLP(rand_params(10, 0.05), scales=(1.0, 1.0, 0.02))
+    losses = net.train(ds, 12, 256, 2e-3, seed=1)
+    assert np.all(np.isfinite(losses))
+    assert losses[-1] <= 0.2 * losses[0], losses
+    y = net.forward(X[:512]) * 0.02
+    assert rel(y, T[:512]) <= 0.35
+
+
+def test_collect_rows_from_the_solver(gpu):
+    say("[learned] training rows from the plain solver")
+    s, pos, r, idata = small_fluid()
+    g = capi.Context(s)
+    g.upload(pos, r, idata)
+    g.init_internal_forces()
+    g.step(10)
+    net = capi.MLP(np.zeros(NP, np.float32), scales=(1.0, 1.0, 1.0))
+    ds = capi.Dataset(20000)
+    t0 = g.time()
+    capi.learn_collect(g, net, ds, 4, 3000, 11)
+    t1 = g.time()
+    assert t1[1] == t0[1] + 4  # the context advanced four plain steps
+    n = ds.size()
+    assert 1000 <= n <= 3000, n
+    assert np.isfinite(ds.target_rms()) and ds.target_rms() > 0
