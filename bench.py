@@ -10,6 +10,12 @@ timed region, inputs resident in HBM.
 The cpu_baseline leg times the CPU restatement of the reference step
 (oracle/, OpenMP, all allowed host cores) on a bounded sample of the same
 workload.
+
+The learned object compares, on the same dam break, the plain run at a fixed
+fine dt = 2.5e-5 with the learned coarse step (4 fine dt per step, the
+committed network exagoop_amd/data/learned_dam_break.npz) and the uncorrected
+coarse step over 200 fine steps: fine-step-equivalent particle-updates/s of
+each form and the relative L2 particle-velocity error against the plain run.
 """
 import argparse
 import json
@@ -37,6 +43,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--scheme", default="USL", choices=["USL", "MUSL"])
+    ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--learned-steps", type=int, default=200)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
@@ -103,6 +111,10 @@ def main():
         print(f"[bench] cpu baseline {cpu['value']:.0f} p-upd/s on {threads} threads",
               file=sys.stderr, flush=True)
 
+    learned = None
+    if not args.no_learned and args.scheme == "USL":
+        learned = learned_leg(args.learned_steps)
+
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "particle-updates/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -113,8 +125,62 @@ def main():
                    "constitutive": "Tait EOS K=2e6 gamma=7 mu=1e-3", "cfl": 0.3},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "learned": learned,
     }
     print(json.dumps(out), flush=True)
+
+
+def learned_leg(nsteps):
+    """Plain fine run vs learned and uncorrected coarse runs (same scene)."""
+    import numpy as np
+    from exagoop_amd import capi, learned, scenes
+
+    def ctx():
+        s, pos, r, idata = scenes.dam_break()
+        s.fixed_timestep = 1
+        s.timestep = learned.DT
+        g = capi.Context(s)
+        g.upload(pos, r, idata)
+        g.init_internal_forces()
+        g.time()
+        return g, pos.shape[1]
+
+    net = learned.load_net()
+    nf = learned.NFINE
+    nsteps -= nsteps % nf
+    g, npart = ctx()
+    t0 = time.perf_counter()
+    g.step(nsteps)
+    g.time()
+    t_fine = time.perf_counter() - t0
+    vf, idf = learned.velocities(g)
+    g.close()
+    res = {}
+    for name, model in (("learned", net), ("coarse", None)):
+        g, _ = ctx()
+        t0 = time.perf_counter()
+        fb = capi.step_learned(g, model, nsteps // nf, nf)
+        g.time()
+        t = time.perf_counter() - t0
+        v, ids = learned.velocities(g)
+        g.close()
+        err = learned.rel_l2(v, vf) if np.array_equal(ids, idf) else float("inf")
+        res[name] = (t, err, fb)
+    t_l, err_l, fb_l = res["learned"]
+    out = {"value": round(npart * nsteps / t_l, 1), "unit": "particle-updates/s",
+           "plain_value": round(npart * nsteps / t_fine, 1),
+           "speedup": round(t_fine / t_l, 3),
+           "rel_l2_velocity_err": float(f"{err_l:.4g}"),
+           "uncorrected_rel_l2_velocity_err": float(f"{res['coarse'][1]:.4g}"),
+           "uncorrected_value": round(npart * nsteps / res["coarse"][0], 1),
+           "fallbacks": fb_l, "fine_dt": learned.DT, "fine_steps_per_learned_step": nf,
+           "horizon_fine_steps": nsteps,
+           "network": "27x4 -> 64 -> 64 -> 3 GELU, BF16 MFMA / FP32 accumulate",
+           "weights": "exagoop_amd/data/learned_dam_break.npz"}
+    print(f"[bench] learned: {out['speedup']}x the plain fine run, rel L2 velocity error "
+          f"{out['rel_l2_velocity_err']} (uncorrected {out['uncorrected_rel_l2_velocity_err']})",
+          file=sys.stderr, flush=True)
+    return out
 
 
 if __name__ == "__main__":

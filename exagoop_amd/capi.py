@@ -454,7 +454,7 @@ def learn_collect(ctx, mlp, dataset, nfine, max_rows, seed):
                                                  seed), "mpm_learn_collect")
 
 
-def step_learned(ctx, mlp, ncoarse, nfine, guards=(0.02, 0.25)):
+def step_learned(ctx, mlp, ncoarse, nfine, guards=(0.05, 0.5)):
     """Returns the number of coarse steps that fell back to plain steps."""
     g = np.ascontiguousarray(guards, dtype=np.float64)
     fb = ctypes.c_int(0)

@@ -44,8 +44,6 @@ struct mpm_dataset {
 
 namespace {
 
-constexpr double kGuardTiny = 1e-300;
-
 // float -> bfloat16 bits, round to nearest even (the conversion the device
 // applies to the same values).
 uint16_t bf16_bits(float x) {
@@ -214,10 +212,12 @@ __global__ void k_append_rows(const int *sel, int nsel, const __bf16 *X, const d
     for (int c = 0; c < 3; ++c) d.t[3 * k + c] = (float)(N.vel[c * nn + q] - vc[3 * r + c]);
 }
 
-// Guard sums over the active nodes: [0] sum m |v|, [1..3] sum m dv,
-// [4] sum m |v|^2, [5] sum m |v + dv|^2, [6] non-finite count.
-__global__ void k_guard(const int *nodes, int n, const float *Y, double dv_ref, NState N,
-                        int64_t nn, double *sums) {
+// Guard sums over the active nodes (row count on the device): [0] sum m,
+// [1..3] sum m dv, [4] sum m |v|^2, [5] sum m |v + dv|^2, [6] non-finite
+// count.
+__global__ void k_guard_dev(const int *nodes, const int *count, const float *Y, double dv_ref,
+                            NState N, int64_t nn, double *sums) {
+    const int n = *count;
     double a[7] = {0, 0, 0, 0, 0, 0, 0};
     for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
         const int q = nodes[r];
@@ -225,13 +225,13 @@ __global__ void k_guard(const int *nodes, int n, const float *Y, double dv_ref, 
         double v2 = 0.0, w2 = 0.0;
         for (int d = 0; d < 3; ++d) {
             const double v = N.vel[d * nn + q];
-            const double dv = Y ? (double)Y[3 * r + d] * dv_ref : 0.0;
-            if (!std::isfinite(dv)) a[6] += 1.0;
+            const double dv = (double)Y[3 * r + d] * dv_ref;
+            if (!isfinite(dv)) a[6] += 1.0;
             a[1 + d] += m * dv;
             v2 += v * v;
             w2 += (v + dv) * (v + dv);
         }
-        a[0] += m * sqrt(v2);
+        a[0] += m;
         a[4] += m * v2;
         a[5] += m * w2;
     }
@@ -242,17 +242,41 @@ __global__ void k_guard(const int *nodes, int n, const float *Y, double dv_ref, 
     }
 }
 
-// Add the correction to the nodal velocity and to the increment used by FLIP.
-__global__ void k_apply(const int *nodes, int n, const float *Y, double dv_ref, NState N,
-                        int64_t nn) {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n) return;
-    const int q = nodes[r];
-    for (int d = 0; d < 3; ++d) {
-        const double dv = (double)Y[3 * r + d] * dv_ref;
-        N.vel[d * nn + q] += dv;
-        N.dvel[d * nn + q] += dv;
+// Guard decision (see mpm_step_learned): flag = 1 keeps the corrected coarse
+// step.  The sums are read back with atomics, where they were accumulated.
+__global__ void k_guard_decide(double *sums, const Scalars *sc, double gmag, double gp, double ge,
+                               int *flag) {
+    double gs[7];
+    for (int k = 0; k < 7; ++k) gs[k] = atomicAdd(&sums[k], 0.0);
+    const double dt = sc->dt;
+    const double M = gs[0];
+    const double u = M > 0 ? sqrt(gs[4] / M) : 0.0;
+    const double vs = fmax(u, gmag * dt);
+    const double dp = sqrt(gs[1] * gs[1] + gs[2] * gs[2] + gs[3] * gs[3]);
+    const double unew = M > 0 ? sqrt(gs[5] / M) : 0.0;
+    const bool ok = gs[6] == 0.0 && isfinite(unew) && dp <= gp * M * vs + 1e-300 &&
+                    unew <= (1.0 + ge) * vs + 1e-300;
+    flag[0] = ok ? 1 : 0;
+}
+
+// Add the correction to the nodal velocity and to the increment used by
+// FLIP, when the guard kept it.
+__global__ void k_apply_dev(const int *nodes, const int *count, const float *Y, double dv_ref,
+                            NState N, int64_t nn, const int *flag) {
+    if (!*flag) return;
+    const int n = *count;
+    for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
+        const int q = nodes[r];
+        for (int d = 0; d < 3; ++d) {
+            const double dv = (double)Y[3 * r + d] * dv_ref;
+            N.vel[d * nn + q] += dv;
+            N.dvel[d * nn + q] += dv;
+        }
     }
+}
+
+__global__ void k_advance_time_if(Scalars *sc, const int *flag) {
+    if (*flag) sc->time = sc->time + sc->dt;
 }
 
 __global__ void k_zero_d(double *p, int n) {
@@ -580,38 +604,41 @@ int mpm_step_learned(mpm_ctx *c, mpm_mlp *m, int ncoarse, int nfine, const doubl
     if (rc) return rc;
     if ((rc = ctx_ml_buffers(c))) return rc;
     hipStream_t st = c->stream;
+    const double gmag = std::sqrt(c->ph.grav[0] * c->ph.grav[0] + c->ph.grav[1] * c->ph.grav[1] +
+                                  c->ph.grav[2] * c->ph.grav[2]);
+    int *flag = c->ml_count + 3;
     int fb = 0;
     for (int it = 0; it < ncoarse; ++it) {
         if ((rc = coarse_begin(c, nfine))) return rc;
-        int nact = 0;
-        if ((rc = active_nodes(c, &nact))) return rc;
-        const float *Y = nullptr;
-        if (m && nact > 0) {
-            launch_mlp_features(c->g, c->N, c->ml_nodes, nact, m->sc, c->ml_x, st);
-            launch_mlp_infer(m->d, c->ml_x, nact, c->ml_y, st);
-            Y = c->ml_y;
-        }
-        bool ok = true;
-        if (m && nact > 0) {
-            hipLaunchKernelGGL(k_zero_d, dim3(1), dim3(64), 0, st, c->ml_guard, 8);
-            hipLaunchKernelGGL(k_guard, dim3(std::min<unsigned>(blocks256(nact), 1024)),
-                               dim3(256), 0, st, c->ml_nodes, nact, Y, m->sc.dv_ref, c->N,
-                               c->g.nn, c->ml_guard);
-            double gs[8];
-            HIPCHK(hipMemcpyAsync(gs, c->ml_guard, sizeof(gs), hipMemcpyDeviceToHost, st));
-            HIPCHK(hipStreamSynchronize(st));
-            if ((rc = check_launch())) return rc;
-            const double dp = std::sqrt(gs[1] * gs[1] + gs[2] * gs[2] + gs[3] * gs[3]);
-            ok = gs[6] == 0.0 && std::isfinite(gs[5]) && dp <= guards[0] * gs[0] + kGuardTiny &&
-                 gs[5] <= (1.0 + guards[1]) * gs[4] + kGuardTiny;
-            if (ok)
-                hipLaunchKernelGGL(k_apply, dim3(blocks256(nact)), dim3(256), 0, st, c->ml_nodes,
-                                   nact, Y, m->sc.dv_ref, c->N, c->g.nn);
-        }
-        if (ok) {
+        if (!m) {  // uncorrected coarse step: nothing to decide, no round trip
             if ((rc = coarse_finish(c))) return rc;
+            continue;
+        }
+        // Correction, guard and the particle kernel (into the spare buffer)
+        // are enqueued back to back; one round trip reads the decision.  A
+        // rejected step leaves the particles in P untouched.
+        launch_active_nodes(c->g, c->N, c->ph.mass_tol, c->ml_nodes, c->ml_count, nullptr, st);
+        launch_mlp_features_dev(c->g, c->N, c->ml_nodes, c->ml_count, m->sc, c->ml_x, st);
+        launch_mlp_infer_dev(m->d, c->ml_x, c->ml_count, c->ml_y, st);
+        hipLaunchKernelGGL(k_zero_d, dim3(1), dim3(64), 0, st, c->ml_guard, 8);
+        hipLaunchKernelGGL(k_guard_dev, dim3(512), dim3(256), 0, st, c->ml_nodes, c->ml_count,
+                           c->ml_y, m->sc.dv_ref, c->N, c->g.nn, c->ml_guard);
+        hipLaunchKernelGGL(k_guard_decide, dim3(1), dim3(1), 0, st, c->ml_guard, c->sc, gmag,
+                           guards[0], guards[1], flag);
+        hipLaunchKernelGGL(k_apply_dev, dim3(512), dim3(256), 0, st, c->ml_nodes, c->ml_count,
+                           c->ml_y, m->sc.dv_ref, c->N, c->g.nn, flag);
+        launch_particle_fast(c->g, c->ph, c->P, c->Q, c->S, c->N, 1, 1, c->sc, st);
+        hipLaunchKernelGGL(k_advance_time_if, dim3(1), dim3(1), 0, st, c->sc, flag);
+        if ((rc = check_launch())) return rc;
+        int ok = 0;
+        HIPCHK(hipMemcpyAsync(&ok, flag, sizeof(int), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        if (ok) {
+            std::swap(c->P, c->Q);
+            c->dt_next_valid = c->fuse_dt != 0;
+            c->dirty = false;
         } else {
-            coarse_drop(c);
+            coarse_drop(c);  // also drops the dt candidates of the rejected kernel
             ++fb;
             if ((rc = mpm_step(c, nfine))) return rc;
         }

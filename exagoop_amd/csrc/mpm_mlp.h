@@ -45,6 +45,11 @@ void launch_mlp_features(const GridP &g, const NState &N, const int *nodes, int 
                          MlpScales sc, __bf16 *X, hipStream_t s);
 // Y[r][3] = network(X[r]) for r < nrows (X padded to a multiple of 32 rows).
 void launch_mlp_infer(const MlpDev &w, const __bf16 *X, int nrows, float *Y, hipStream_t s);
+// The same with the row count read on the device (*count), no host round trip.
+void launch_mlp_features_dev(const GridP &g, const NState &N, const int *nodes, const int *count,
+                             MlpScales sc, __bf16 *X, hipStream_t s);
+void launch_mlp_infer_dev(const MlpDev &w, const __bf16 *X, const int *count, float *Y,
+                          hipStream_t s);
 // Gradient of the mean squared error over the rows idx[0..nrows) of (X, T)
 // (T scaled by inv_t) added into w.grad; the loss sum into w.loss.
 void launch_mlp_train(const MlpDev &w, const __bf16 *X, const float *T, const int *idx,

@@ -146,9 +146,10 @@ __device__ __forceinline__ void forward_tile(const MlpDev &w, const __bf16 *xrow
 // ---------------------------------------------------------------- inference
 
 __global__ __launch_bounds__(256) void k_mlp_infer(MlpDev w, const __bf16 *X, int nrows,
-                                                   float *Y) {
+                                                   const int *nrows_dev, float *Y) {
     __shared__ __bf16 h1s[4][kMRows * kMH], h2s[4][kMRows * kMH];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (nrows_dev) nrows = *nrows_dev;
     const int ntiles = (nrows + kMRows - 1) / kMRows;
     for (int t = blockIdx.x * 4 + wave; t < ntiles; t += gridDim.x * 4) {
         Fwd F;
@@ -354,11 +355,13 @@ __global__ void k_mlp_pack(MlpDev w) {
 // Row r of node nodes[r]: neighbour nb = (dz+1)*9 + (dy+1)*3 + (dx+1),
 // columns 4 nb + (0: m / m_ref, 1..3: m v / (m_ref v_ref)); zero outside
 // the nodal box and in columns 108..127.  One thread per (row, 8 columns).
-__global__ void k_mlp_features(GridP g, NState N, const int *nodes, int nrows, int nrows_pad,
-                               MlpScales sc, __bf16 *X) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void k_mlp_features(GridP g, NState N, const int *nodes, int nrows,
+                               const int *nrows_dev, MlpScales sc, __bf16 *X) {
+    if (nrows_dev) nrows = *nrows_dev;
+    const int nrows_pad = (nrows + kMRows - 1) / kMRows * kMRows;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+         t < (int64_t)nrows_pad * (kMInPad / 8); t += (int64_t)gridDim.x * blockDim.x) {
     const int r = (int)(t / (kMInPad / 8)), c8 = (int)(t % (kMInPad / 8));
-    if (r >= nrows_pad) return;
     bf16x8 out;
     const double im = 1.0 / sc.m_ref, imv = 1.0 / (sc.m_ref * sc.v_ref);
     const int nx = g.n[0] + 1, ny = g.n[1] + 1, nz = g.n[2] + 1;
@@ -385,6 +388,7 @@ __global__ void k_mlp_features(GridP g, NState N, const int *nodes, int nrows, i
         out[e] = (__bf16)v;
     }
     *reinterpret_cast<bf16x8 *>(X + (int64_t)r * kMInPad + c8 * 8) = out;
+    }
 }
 
 // Active nodes (mass >= tol): one atomic per workgroup reserves its run.
@@ -422,7 +426,12 @@ void launch_mlp_features(const GridP &g, const NState &N, const int *nodes, int 
     if (pad == 0) return;
     const int64_t nt = (int64_t)pad * (kMInPad / 8);
     hipLaunchKernelGGL(k_mlp_features, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s, g, N,
-                       nodes, nrows, pad, sc, X);
+                       nodes, nrows, (const int *)nullptr, sc, X);
+}
+
+void launch_mlp_features_dev(const GridP &g, const NState &N, const int *nodes, const int *count,
+                             MlpScales sc, __bf16 *X, hipStream_t s) {
+    hipLaunchKernelGGL(k_mlp_features, dim3(2048), dim3(256), 0, s, g, N, nodes, 0, count, sc, X);
 }
 
 void launch_mlp_infer(const MlpDev &w, const __bf16 *X, int nrows, float *Y, hipStream_t s) {
@@ -430,7 +439,13 @@ void launch_mlp_infer(const MlpDev &w, const __bf16 *X, int nrows, float *Y, hip
     const int ntiles = (nrows + kMRows - 1) / kMRows;
     int blocks = (ntiles + 3) / 4;
     if (blocks > 1024) blocks = 1024;
-    hipLaunchKernelGGL(k_mlp_infer, dim3(blocks), dim3(256), 0, s, w, X, nrows, Y);
+    hipLaunchKernelGGL(k_mlp_infer, dim3(blocks), dim3(256), 0, s, w, X, nrows,
+                       (const int *)nullptr, Y);
+}
+
+void launch_mlp_infer_dev(const MlpDev &w, const __bf16 *X, const int *count, float *Y,
+                          hipStream_t s) {
+    hipLaunchKernelGGL(k_mlp_infer, dim3(1024), dim3(256), 0, s, w, X, 0, count, Y);
 }
 
 void launch_mlp_train(const MlpDev &w, const __bf16 *X, const float *T, const int *idx,

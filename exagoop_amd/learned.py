@@ -1,0 +1,202 @@
+"""Learned coarse-step correction for the dam break: training-set generation
+from the plain solver, training on the GPU, and evaluation against the plain
+fine-step run.
+
+Scenes: the dam break of scenes.dam_break at 128 x 64 x 64 cells with a
+water block [0, w] x [0, 1] x [0, h], 2 x 2 x 2 particles per cell.  The
+fine (plain) run uses a fixed dt = 2.5e-5, a quarter of the CFL 0.3 step at
+rest; the learned coarse step covers four of them (dt = 1e-4, CFL 0.3).  A
+coarse step of 4e-4 (CFL 1.2) is unstable for this fluid with or without a
+nodal correction.  Training seeds 0..63 and
+held-out seeds 1000..1015 draw w ~ U(0.3, 0.7) and h ~ U(0.5, 1.0) from
+numpy's default_rng(seed).  One learned step covers NFINE = 4 fine steps.
+
+Feature scales: m_ref = rho dx^3 (the mass of a full cell), v_ref = 1 m/s;
+dv_ref = rms of the training targets.
+
+    python -m exagoop_amd.learned --out DIR [--train-seeds 64] [--eval-seeds 16]
+
+writes DIR/learned_dam_break.npz (parameters + scales) and DIR/learned.json
+(training losses; per held-out seed: relative L2 velocity error of the
+learned and of the uncorrected coarse run against the fine run, fallbacks,
+wall times and the speedup).
+"""
+import argparse
+import json
+import os
+import time
+
+import numpy as np
+
+from . import capi, scenes
+
+NFINE = 4
+DT = 2.5e-5
+RHO = 1000.0
+DX = 1.0 / 64.0
+M_REF = RHO * DX ** 3
+V_REF = 1.0
+TRAIN_SEEDS = list(range(64))
+EVAL_SEEDS = list(range(1000, 1016))
+DATA_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data",
+                         "learned_dam_break.npz")
+
+
+def geometry(seed):
+    rng = np.random.default_rng(seed)
+    return float(rng.uniform(0.3, 0.7)), float(rng.uniform(0.5, 1.0))
+
+
+def scene(seed, nx=128, ny=64, nz=64):
+    w, h = geometry(seed)
+    s, pos, r, idata = scenes.dam_break(nx=nx, ny=ny, nz=nz, width=w, height=h)
+    s.fixed_timestep = 1
+    s.timestep = DT
+    return s, pos, r, idata
+
+
+def context(seed, **kw):
+    s, pos, r, idata = scene(seed, **kw)
+    g = capi.Context(s)
+    g.upload(pos, r, idata)
+    g.init_internal_forces()
+    return g
+
+
+def collect(seeds, samples=10, spacing=40, rows=8000, capacity=6_000_000, log=print, **kw):
+    """Training rows from the plain trajectories of the given seeds: at
+    `samples` points `spacing` fine steps apart, one collection of up to
+    `rows` nodes each (the collection itself advances NFINE steps)."""
+    net = capi.MLP(np.zeros(capi.MLP_NPARAMS, np.float32), scales=(M_REF, V_REF, 1.0))
+    ds = capi.Dataset(capacity)
+    for k, seed in enumerate(seeds):
+        g = context(seed, **kw)
+        for j in range(samples):
+            if spacing:
+                g.step(spacing)
+            capi.learn_collect(g, net, ds, NFINE, rows, seed * 1000 + j)
+        g.close()
+        if log and (k % 8 == 7 or k == len(seeds) - 1):
+            log(f"[learned] collected seeds {k + 1}/{len(seeds)}: {ds.size()} rows")
+    return ds
+
+
+def train(ds, epochs=24, batch=4096, lr=2e-3, seed=0, log=print):
+    rms = ds.target_rms()
+    rng = np.random.default_rng(seed)
+    # He-style initialisation, zero biases
+    p = np.zeros(capi.MLP_NPARAMS, np.float32)
+    o = 0
+    for fan_in, fan_out in ((108, 64), (64, 64), (64, 3)):
+        n = fan_in * fan_out
+        p[o:o + n] = (rng.standard_normal(n) * np.sqrt(1.0 / fan_in)).astype(np.float32)
+        o += n + fan_out
+    net = capi.MLP(p, scales=(M_REF, V_REF, rms))
+    losses = []
+    for phase_lr, phase_ep in ((lr, epochs * 2 // 3), (lr * 0.2, epochs - epochs * 2 // 3)):
+        l = net.train(ds, phase_ep, batch, phase_lr, seed=seed + len(losses))
+        losses += [float(x) for x in l]
+        if log:
+            log(f"[learned] lr {phase_lr:g}: loss {l[0]:.4f} -> {l[-1]:.4f}")
+    return net, losses
+
+
+def velocities(g):
+    pos, r, idata, ids = g.download()
+    order = np.argsort(ids)
+    return r[capi.R_VEL:capi.R_VEL + 3, order], ids[order]
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def evaluate(net, seeds, nsteps=400, guards=(0.05, 0.5), log=print, **kw):
+    """Fine plain run of nsteps vs nsteps/NFINE learned (and uncorrected)
+    coarse steps from the same initial state."""
+    out = []
+    for seed in seeds:
+        g = context(seed, **kw)
+        capi_sync(g)
+        t0 = time.perf_counter()
+        g.step(nsteps)
+        capi_sync(g)
+        t_fine = time.perf_counter() - t0
+        vf, idf = velocities(g)
+        g.close()
+        res = {"seed": seed, "geometry": geometry(seed), "fine_s": t_fine}
+        for name, model in (("learned", net), ("coarse", None)):
+            g = context(seed, **kw)
+            capi_sync(g)
+            t0 = time.perf_counter()
+            fb = capi.step_learned(g, model, nsteps // NFINE, NFINE, guards)
+            capi_sync(g)
+            res[name + "_s"] = time.perf_counter() - t0
+            v, ids = velocities(g)
+            g.close()
+            if ids.shape == idf.shape and np.array_equal(ids, idf):
+                res[name + "_err"] = rel_l2(v, vf)
+            else:
+                res[name + "_err"] = float("inf")
+                res[name + "_lost"] = int(idf.size - ids.size)
+            res[name + "_fallbacks"] = fb
+        res["speedup"] = res["fine_s"] / res["learned_s"]
+        out.append(res)
+        if log:
+            log(f"[learned] seed {seed}: err learned {res['learned_err']:.3e} coarse "
+                f"{res['coarse_err']:.3e} fallbacks {res['learned_fallbacks']} speedup "
+                f"{res['speedup']:.2f}")
+    return out
+
+
+def capi_sync(g):
+    g.time()  # synchronises the context's stream
+
+
+def save(net, path):
+    p, sc = net.get_params()
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    np.savez(path, params=p, scales=sc)
+
+
+def load_net(path=DATA_FILE):
+    d = np.load(path, allow_pickle=False)
+    return capi.MLP(d["params"], scales=d["scales"])
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", required=True)
+    ap.add_argument("--train-seeds", type=int, default=64)
+    ap.add_argument("--eval-seeds", type=int, default=16)
+    ap.add_argument("--epochs", type=int, default=24)
+    ap.add_argument("--eval-steps", type=int, default=400)
+    ap.add_argument("--samples", type=int, default=10)
+    ap.add_argument("--spacing", type=int, default=40)
+    ap.add_argument("--rows", type=int, default=8000)
+    a = ap.parse_args(argv)
+    os.makedirs(a.out, exist_ok=True)
+    t0 = time.perf_counter()
+    ds = collect(TRAIN_SEEDS[:a.train_seeds], samples=a.samples, spacing=a.spacing, rows=a.rows)
+    t1 = time.perf_counter()
+    net, losses = train(ds, epochs=a.epochs)
+    t2 = time.perf_counter()
+    save(net, os.path.join(a.out, "learned_dam_break.npz"))
+    ev = evaluate(net, EVAL_SEEDS[:a.eval_seeds], nsteps=a.eval_steps)
+    summary = {
+        "rows": ds.size(), "dv_ref": float(net.get_params()[1][2]),
+        "collect_s": t1 - t0, "train_s": t2 - t1, "losses": losses,
+        "eval": ev,
+        "mean_learned_err": float(np.mean([e["learned_err"] for e in ev])),
+        "mean_coarse_err": float(np.mean([e["coarse_err"] for e in ev])),
+        "mean_speedup": float(np.mean([e["speedup"] for e in ev])),
+        "fallbacks": int(sum(e["learned_fallbacks"] for e in ev)),
+    }
+    with open(os.path.join(a.out, "learned.json"), "w") as f:
+        json.dump(summary, f, indent=1)
+    print(json.dumps({k: v for k, v in summary.items() if k not in ("eval", "losses")}),
+          flush=True)
+
+
+if __name__ == "__main__":
+    main()

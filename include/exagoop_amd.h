@@ -320,11 +320,13 @@ int mpm_learn_collect(mpm_ctx *ctx, mpm_mlp *m, mpm_dataset *d, int nfine, int m
  * loss_out[epoch] = mean loss of the epoch (may be NULL). */
 int mpm_mlp_train(mpm_mlp *m, mpm_dataset *d, int epochs, int batch, float lr, uint64_t seed,
                   double *loss_out);
-/* ncoarse learned steps of nfine x dt each.  guards = {gp, ge}: the corrected
- * nodal field must satisfy |sum m dv| <= gp sum m |v| and
- * sum m |v + dv|^2 <= (1 + ge) sum m |v|^2 and be finite; otherwise the coarse
- * step is dropped and nfine plain steps run instead (*fallbacks counts them).
- * m == NULL runs the coarse steps without correction. */
+/* ncoarse learned steps of nfine x dt each.  guards = {gp, ge}: with M the
+ * active nodal mass, u the mass-weighted rms nodal velocity of the coarse
+ * step and s = max(u, |g| dt_coarse), the correction dv must be finite,
+ * |sum m dv| <= gp M s (momentum) and the corrected rms velocity
+ * <= (1 + ge) s (energy); otherwise the coarse step is dropped and nfine
+ * plain steps run instead (*fallbacks counts them).  m == NULL runs the
+ * coarse steps without correction. */
 int mpm_step_learned(mpm_ctx *ctx, mpm_mlp *m, int ncoarse, int nfine, const double guards[2],
                      int *fallbacks);
 

@@ -179,3 +179,29 @@ def test_collect_rows_from_the_solver(gpu):
     n = ds.size()
     assert 1000 <= n <= 3000, n
     assert np.isfinite(ds.target_rms()) and ds.target_rms() > 0
+
+
+def test_trained_network_gains_over_the_uncorrected_coarse_step(gpu):
+    """The committed network (exagoop_amd/data/learned_dam_break.npz, trained
+    by exagoop_amd.learned on seeds 0..63) on a held-out dam-break geometry:
+    100 fine steps of dt = 2.5e-5 against 25 coarse steps of 1e-4 with and
+    without the correction."""
+    from exagoop_amd import learned
+    say("[learned] trained network vs uncorrected coarse step, held-out seed 1003")
+    net = learned.load_net()
+    seed, nsteps = 1003, 100
+    g = learned.context(seed)
+    g.step(nsteps)
+    vf, idf = learned.velocities(g)
+    g.close()
+    errs = {}
+    for name, model in (("learned", net), ("coarse", None)):
+        g = learned.context(seed)
+        fb = capi.step_learned(g, model, nsteps // learned.NFINE, learned.NFINE)
+        v, ids = learned.velocities(g)
+        g.close()
+        assert np.array_equal(ids, idf)
+        assert fb == 0
+        errs[name] = learned.rel_l2(v, vf)
+    say(f"    errors {errs}")
+    assert errs["learned"] <= 0.5 * errs["coarse"], errs
