@@ -552,7 +552,14 @@ int mpm_init_internal_forces(mpm_ctx *c) {
     c->dt_next_valid = false;
     // Initialise_Internal_Forces (mpm_init.cpp:368-401): P2G(1,1,1), backup,
     // G2P(update_vel=0, update_strainrate=1, dt=0), stress(dt=0, applied rate).
-    launch_p2g(c->g, c->ph, c->P, c->N, 1, 1, 1, nullptr, c->stream);
+    // The sorted deposit (particles outside their cell go through the global
+    // path) replaces the atomic one-thread-per-particle deposit when enabled.
+    if (c->sorted && c->P.np > 0) {
+        launch_sort(c->g, c->ph, c->P, c->S, nullptr, 1, c->stream);
+        launch_p2g_tiles(c->g, c->ph, c->P, c->S, c->N, (int)c->steps, 0, 1, c->sc, c->stream);
+    } else {
+        launch_p2g(c->g, c->ph, c->P, c->N, 1, 1, 1, nullptr, c->stream);
+    }
     launch_grid(c->g, c->ph, c->ls, c->udf, c->N,
                 G_FINISH_MASS | G_FINISH_VEL | G_FINISH_FRC | G_KEEP_FRC | G_BACKUP, 0.0, nullptr,
                 c->stream);

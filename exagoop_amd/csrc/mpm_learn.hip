@@ -235,10 +235,19 @@ __global__ void k_guard_dev(const int *nodes, const int *count, const float *Y, 
         a[4] += m * v2;
         a[5] += m * w2;
     }
+    // wavefront sums, then one atomic per workgroup and sum
+    __shared__ double red[7][4];
+    const int w = threadIdx.x >> 6;
     for (int k = 0; k < 7; ++k) {
         double x = a[k];
         for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
-        if ((threadIdx.x & 63) == 0 && x != 0.0) atomicAdd(&sums[k], x);
+        if ((threadIdx.x & 63) == 0) red[k][w] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x < 7) {
+        const double x = red[threadIdx.x][0] + red[threadIdx.x][1] + red[threadIdx.x][2] +
+                         red[threadIdx.x][3];
+        if (x != 0.0) atomicAdd(&sums[threadIdx.x], x);
     }
 }
 
@@ -621,7 +630,7 @@ int mpm_step_learned(mpm_ctx *c, mpm_mlp *m, int ncoarse, int nfine, const doubl
         launch_mlp_features_dev(c->g, c->N, c->ml_nodes, c->ml_count, m->sc, c->ml_x, st);
         launch_mlp_infer_dev(m->d, c->ml_x, c->ml_count, c->ml_y, st);
         hipLaunchKernelGGL(k_zero_d, dim3(1), dim3(64), 0, st, c->ml_guard, 8);
-        hipLaunchKernelGGL(k_guard_dev, dim3(512), dim3(256), 0, st, c->ml_nodes, c->ml_count,
+        hipLaunchKernelGGL(k_guard_dev, dim3(256), dim3(256), 0, st, c->ml_nodes, c->ml_count,
                            c->ml_y, m->sc.dv_ref, c->N, c->g.nn, c->ml_guard);
         hipLaunchKernelGGL(k_guard_decide, dim3(1), dim3(1), 0, st, c->ml_guard, c->sc, gmag,
                            guards[0], guards[1], flag);
