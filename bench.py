@@ -45,6 +45,9 @@ def main():
     ap.add_argument("--scheme", default="USL", choices=["USL", "MUSL"])
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--learned-steps", type=int, default=200)
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="also run one device-to-device copy of this size (HBM counter "
+                         "calibration for rocprof FETCH_SIZE / WRITE_SIZE passes)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
@@ -86,8 +89,15 @@ def main():
     dom = max(kern, key=lambda k: kern[k][0])
     dms, dbytes = kern[dom]
     achieved = dbytes / (dms * 1e-3) / 1e9
+    traffic = None
+    tfile = os.path.join(ROOT, "exagoop_amd", "data", "hbm_traffic_dam_break.json")
+    kname = {"p2g": "k_p2g_cells", "grid": "k_grid", "g2p_move_stress": "k_particle_tile"}[dom]
+    if args.scheme == "USL" and os.path.exists(tfile):
+        # rocprof FETCH_SIZE / WRITE_SIZE of this kernel on this workload, scaled by a
+        # plain copy of known size (profiles/dam_break_hbm_traffic.md)
+        traffic = json.load(open(tfile)).get(kname, {}).get("traffic_bytes")
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                 "kernel": dom,
                 "kernels_ms": {k: round(v[0], 5) for k, v in kern.items()},
                 "kernels_gbs": {k: round(v[1] / (v[0] * 1e-3) / 1e9, 2) for k, v in kern.items()},
@@ -111,6 +121,9 @@ def main():
         print(f"[bench] cpu baseline {cpu['value']:.0f} p-upd/s on {threads} threads",
               file=sys.stderr, flush=True)
 
+    if args.calib_copy_mib:
+        calib_copy(args.calib_copy_mib)
+
     learned = None
     if not args.no_learned and args.scheme == "USL":
         learned = learned_leg(args.learned_steps)
@@ -128,6 +141,28 @@ def main():
         "learned": learned,
     }
     print(json.dumps(out), flush=True)
+
+
+def calib_copy(mib):
+    """One device-to-device copy of `mib` MiB through the HIP runtime (a plain
+    copy kernel of known bytes for scaling rocprof's HBM counters)."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = mib * (1 << 20)
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    if hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(n)) or \
+            hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(n)):
+        raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 1, ctypes.c_size_t(n))
+    hip.hipDeviceSynchronize()
+    rc = hip.hipMemcpy(b, a, ctypes.c_size_t(n), 3)  # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    if rc:
+        raise RuntimeError(f"hipMemcpy failed ({rc})")
+    print(f"[bench] calibration copy {mib} MiB read + {mib} MiB written", file=sys.stderr,
+          flush=True)
 
 
 def learned_leg(nsteps):

@@ -164,9 +164,25 @@ def load_net(path=DATA_FILE):
     return capi.MLP(d["params"], scales=d["scales"])
 
 
+def demo():
+    """A short run of every learned-correction kernel for profiling: 20
+    learned steps of the committed network on held-out seed 1000, then row
+    collection on two training seeds and three epochs of training."""
+    net = load_net()
+    g = context(1000)
+    fb = capi.step_learned(g, net, 20, NFINE)
+    g.close()
+    print(f"[learned demo] 20 learned steps, {fb} fallbacks", flush=True)
+    ds = collect([1, 2], samples=2, spacing=10, rows=8000, capacity=100_000)
+    _, losses = train(ds, epochs=3, batch=4096)
+    print(f"[learned demo] training losses {losses}", flush=True)
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("--out", required=True)
+    ap.add_argument("--demo", action="store_true",
+                    help="short run of every kernel (for profiling), no outputs")
+    ap.add_argument("--out")
     ap.add_argument("--train-seeds", type=int, default=64)
     ap.add_argument("--eval-seeds", type=int, default=16)
     ap.add_argument("--epochs", type=int, default=24)
@@ -175,6 +191,11 @@ def main(argv=None):
     ap.add_argument("--spacing", type=int, default=40)
     ap.add_argument("--rows", type=int, default=8000)
     a = ap.parse_args(argv)
+    if a.demo:
+        demo()
+        return
+    if not a.out:
+        ap.error("--out is required")
     os.makedirs(a.out, exist_ok=True)
     t0 = time.perf_counter()
     ds = collect(TRAIN_SEEDS[:a.train_seeds], samples=a.samples, spacing=a.spacing, rows=a.rows)
