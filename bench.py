@@ -11,6 +11,13 @@ The cpu_baseline leg times the CPU restatement of the reference step
 (oracle/, OpenMP, all allowed host cores) on a bounded sample of the same
 workload.
 
+The batch object runs the batched ensemble (exagoop_amd/ensemble.py): 256
+independent scenes (seeds 0..255: elastic disks or dam break, 2k-64k
+particles, 16^3-48^3 cells, quadratic B-splines, FP64), one persistent kernel
+launch per step replayed from a HIP graph, decomposition chosen by the
+library's autotuner, 500 timed steps: particle-updates/s over all scenes and
+the step's algorithmic HBM bytes per second.
+
 The learned object compares, on the same dam break, the plain run at a fixed
 fine dt = 2.5e-5 with the learned coarse step (4 fine dt per step, the
 committed network exagoop_amd/data/learned_dam_break.npz) and the uncorrected
@@ -45,6 +52,9 @@ def main():
     ap.add_argument("--scheme", default="USL", choices=["USL", "MUSL"])
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--learned-steps", type=int, default=200)
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-scenes", type=int, default=256)
+    ap.add_argument("--batch-steps", type=int, default=500)
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run one device-to-device copy of this size (HBM counter "
                          "calibration for rocprof FETCH_SIZE / WRITE_SIZE passes)")
@@ -128,6 +138,10 @@ def main():
     if not args.no_learned and args.scheme == "USL":
         learned = learned_leg(args.learned_steps)
 
+    batch = None
+    if not args.no_batch:
+        batch = batch_leg(args.batch_scenes, args.batch_steps)
+
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "particle-updates/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -139,6 +153,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "learned": learned,
+        "batch": batch,
     }
     print(json.dumps(out), flush=True)
 
@@ -163,6 +178,47 @@ def calib_copy(mib):
         raise RuntimeError(f"hipMemcpy failed ({rc})")
     print(f"[bench] calibration copy {mib} MiB read + {mib} MiB written", file=sys.stderr,
           flush=True)
+
+
+def batch_leg(nscenes, nsteps):
+    """The batched ensemble: autotune, warm up (graph capture), time nsteps."""
+    import numpy as np
+    from exagoop_amd import capi, ensemble
+
+    t0 = time.perf_counter()
+    b, sc = ensemble.make_batch(range(nscenes))
+    t_build = time.perf_counter() - t0
+    best, tune_ms = b.autotune(trial_steps=3)
+    b.step(10)
+    t0 = time.perf_counter()
+    b.step(nsteps)
+    wall = time.perf_counter() - t0
+    st = b.stats()
+    npart = st["particles"]
+    secs = st["step_ms"] * nsteps / 1e3
+    # Nodes holding mass at the end of the run stand for the nodes a step touches.
+    active = sum(int(np.count_nonzero(b.nodal(k, capi.N_MASS))) for k in range(nscenes))
+    bytes_step = ((P2G_B_PER_P + PART_B_PER_P) * npart +
+                  (P2G_B_PER_N + GRID_B_PER_N + PART_B_PER_N) * active)
+    achieved = bytes_step / (st["step_ms"] * 1e-3) / 1e9
+    kinds = [x[4]["kind"] for x in sc]
+    out = {"value": round(npart * nsteps / secs, 1), "unit": "particle-updates/s",
+           "ms_per_step": round(st["step_ms"], 5), "steps": nsteps,
+           "wall_value": round(npart * nsteps / wall, 1),
+           "scenes": nscenes, "particles": npart, "active_nodes": active,
+           "disks": kinds.count("disks"), "dam_breaks": kinds.count("dam_break"),
+           "tasks_per_step": st["tasks"], "tuning": best,
+           "autotune_best_ms": round(tune_ms, 5), "build_s": round(t_build, 2),
+           "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                        "traffic": None, "kernel": "k_batch_step",
+                        "algorithmic_bytes_per_step": bytes_step},
+           "config": "seeds 0..%d of exagoop_amd.ensemble: quadratic, USL, CFL 0.3, FP64, one "
+                     "persistent launch per step, hipGraph replay" % (nscenes - 1)}
+    print(f"[bench] batch: {nscenes} scenes, {npart} particles, {st['step_ms']:.3f} ms/step, "
+          f"{out['value'] / 1e9:.3f} G p-upd/s, tuning {best}", file=sys.stderr, flush=True)
+    b.close()
+    return out
 
 
 def learned_leg(nsteps):

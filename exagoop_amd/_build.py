@@ -10,7 +10,8 @@ import subprocess
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libexagoop_amd.so")
-SOURCES = ["mpm_kernels.hip", "mpm_sort.hip", "mpm_ctx.hip", "mpm_mlp.hip", "mpm_learn.hip"]
+SOURCES = ["mpm_kernels.hip", "mpm_sort.hip", "mpm_ctx.hip", "mpm_mlp.hip", "mpm_learn.hip",
+           "mpm_batch.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",
@@ -32,17 +33,41 @@ def hipcc():
     return "hipcc"
 
 
-def build(verbose=False):
-    """Compile libexagoop_amd.so in place; raises on failure."""
+def build(verbose=False, jobs=None):
+    """Compile libexagoop_amd.so in place; raises on failure.  Each source is
+    compiled to an object in parallel (the kernels of one source call no
+    device code of another), then the objects are linked."""
+    import concurrent.futures
+    import shutil
+    import tempfile
+
     srcs = [os.path.join(CSRC, s) for s in SOURCES]
-    tmp = LIB + ".tmp"
-    cmd = [hipcc()] + HIPCC_FLAGS + srcs + ["-o", tmp]
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError("hipcc failed:\n" + r.stdout + r.stderr)
-    if verbose and r.stderr:
-        print(r.stderr)
-    os.replace(tmp, LIB)
+    tmpdir = tempfile.mkdtemp(prefix="exagoop_amd_build_")
+    try:
+        objs = [os.path.join(tmpdir, os.path.basename(s) + ".o") for s in srcs]
+        cflags = [f for f in HIPCC_FLAGS if f != "-shared"]
+
+        def compile_one(pair):
+            src, obj = pair
+            r = subprocess.run([hipcc()] + cflags + ["-c", src, "-o", obj],
+                               capture_output=True, text=True)
+            return src, r
+
+        n = jobs or min(len(srcs), os.cpu_count() or 1)
+        with concurrent.futures.ThreadPoolExecutor(max_workers=n) as ex:
+            for src, r in ex.map(compile_one, zip(srcs, objs)):
+                if r.returncode != 0:
+                    raise RuntimeError("hipcc failed on %s:\n%s%s" % (src, r.stdout, r.stderr))
+                if verbose and r.stderr:
+                    print(r.stderr)
+        tmp = LIB + ".tmp"
+        r = subprocess.run([hipcc()] + HIPCC_FLAGS + objs + ["-o", tmp], capture_output=True,
+                           text=True)
+        if r.returncode != 0:
+            raise RuntimeError("hipcc link failed:\n" + r.stdout + r.stderr)
+        os.replace(tmp, LIB)
+    finally:
+        shutil.rmtree(tmpdir, ignore_errors=True)
     return LIB
 
 

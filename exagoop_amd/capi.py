@@ -138,6 +138,20 @@ SIGNATURES = {
     "mpm_mlp_train": (ctypes.c_int, [_P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_float,
                                      ctypes.c_uint64, _DP]),
     "mpm_step_learned": (ctypes.c_int, [_P, _P, ctypes.c_int, ctypes.c_int, _DP, _IP]),
+    "mpm_batch_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(Specs), _LP,
+                                        ctypes.POINTER(_P)]),
+    "mpm_batch_destroy": (None, [_P]),
+    "mpm_batch_upload": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int64, _DP, _DP, _IP]),
+    "mpm_batch_init": (ctypes.c_int, [_P]),
+    "mpm_batch_step": (ctypes.c_int, [_P, ctypes.c_int]),
+    "mpm_batch_num_particles": (ctypes.c_int, [_P, ctypes.c_int, _LP]),
+    "mpm_batch_download": (ctypes.c_int, [_P, ctypes.c_int, _DP, _DP, _IP, _LP]),
+    "mpm_batch_get_nodal": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, _DP]),
+    "mpm_batch_get_time": (ctypes.c_int, [_P, ctypes.c_int, _DP, _LP, _DP]),
+    "mpm_batch_get_tuning": (ctypes.c_int, [_P, ctypes.c_void_p]),
+    "mpm_batch_set_tuning": (ctypes.c_int, [_P, ctypes.c_void_p]),
+    "mpm_batch_autotune": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_void_p, _DP]),
+    "mpm_batch_stats": (ctypes.c_int, [_P, _DP, _LP, _IP]),
 }
 
 
@@ -462,3 +476,114 @@ def step_learned(ctx, mlp, ncoarse, nfine, guards=(0.05, 0.5)):
                                                 ncoarse, nfine, dptr(g), ctypes.byref(fb)),
            "mpm_step_learned")
     return fb.value
+
+
+class BatchTuning(ctypes.Structure):
+    """mpm_batch_tuning: work decomposition of the persistent batched step."""
+    _fields_ = [("workgroups", ctypes.c_int), ("sort_chunk", ctypes.c_int),
+                ("tile_chunk", ctypes.c_int), ("node_chunk", ctypes.c_int),
+                ("graph_steps", ctypes.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Batch:
+    """A batched ensemble of independent scenes (mpm_batch_*)."""
+
+    def __init__(self, specs_list, counts):
+        self._lib = load()
+        self._b = None
+        n = len(specs_list)
+        arr = (Specs * n)()
+        for k, sp in enumerate(specs_list):
+            arr[k] = sp
+        self.counts = np.ascontiguousarray(counts, dtype=np.int64)
+        assert self.counts.size == n
+        self.nn = [(sp.ncells[0] + 1) * (sp.ncells[1] + 1) * (sp.ncells[2] + 1)
+                   for sp in specs_list]
+        b = _P()
+        _check(self._lib, self._lib.mpm_batch_create(n, arr, lptr(self.counts), ctypes.byref(b)),
+               "mpm_batch_create")
+        self._b = b
+        self.nscenes = n
+
+    def close(self):
+        if getattr(self, "_b", None):
+            self._lib.mpm_batch_destroy(self._b)
+            self._b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _call(self, name, *args):
+        _check(self._lib, getattr(self._lib, "mpm_batch_" + name)(self._b, *args),
+               "mpm_batch_" + name)
+
+    def upload(self, scene, pos, rdata, idata):
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        rdata = np.ascontiguousarray(rdata, dtype=np.float64)
+        idata = np.ascontiguousarray(idata, dtype=np.int32)
+        n = pos.shape[1]
+        assert pos.shape == (3, n) and rdata.shape == (NREAL, n) and idata.shape == (NINT, n)
+        self._call("upload", scene, n, dptr(pos), dptr(rdata), iptr(idata))
+
+    def init(self):
+        self._call("init")
+
+    def step(self, n=1):
+        self._call("step", n)
+
+    def num_particles(self, scene):
+        v = ctypes.c_int64(0)
+        self._call("num_particles", scene, ctypes.byref(v))
+        return v.value
+
+    def download(self, scene):
+        n = self.num_particles(scene)
+        pos = np.zeros((3, n))
+        rdata = np.zeros((NREAL, n))
+        idata = np.zeros((NINT, n), dtype=np.int32)
+        ids = np.zeros(n, dtype=np.int64)
+        self._call("download", scene, dptr(pos), dptr(rdata), iptr(idata), lptr(ids))
+        return pos, rdata, idata, ids
+
+    def nodal(self, scene, comp):
+        out = np.zeros(self.nn[scene])
+        self._call("get_nodal", scene, comp, dptr(out))
+        return out
+
+    def time(self, scene):
+        t = ctypes.c_double(0)
+        s = ctypes.c_int64(0)
+        dt = ctypes.c_double(0)
+        self._call("get_time", scene, ctypes.byref(t), ctypes.byref(s), ctypes.byref(dt))
+        return t.value, s.value, dt.value
+
+    def tuning(self):
+        t = BatchTuning()
+        self._call("get_tuning", ctypes.byref(t))
+        return t.as_dict()
+
+    def set_tuning(self, **kw):
+        t = BatchTuning()
+        self._call("get_tuning", ctypes.byref(t))
+        for k, v in kw.items():
+            setattr(t, k, int(v))
+        self._call("set_tuning", ctypes.byref(t))
+
+    def autotune(self, trial_steps=4):
+        t = BatchTuning()
+        ms = ctypes.c_double(0)
+        self._call("autotune", trial_steps, ctypes.byref(t), ctypes.byref(ms))
+        return t.as_dict(), ms.value
+
+    def stats(self):
+        ms = ctypes.c_double(0)
+        n = ctypes.c_int64(0)
+        k = ctypes.c_int(0)
+        self._call("stats", ctypes.byref(ms), ctypes.byref(n), ctypes.byref(k))
+        return {"step_ms": ms.value, "particles": n.value, "tasks": k.value}

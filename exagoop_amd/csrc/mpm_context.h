@@ -68,6 +68,11 @@ int set_err(int code, const char *what);
 int hip_err(hipError_t e, const char *expr);
 // Status of the last kernel launch.
 int check_launch();
+// Validated grid and step parameters of a problem specification.
+int specs_params(const mpm_specs *s, GridP &g, PhysP &ph);
+// Every wall velocity of the specification is zero (idle node tiles then
+// hold the all-zero massless state).
+bool walls_at_rest(const mpm_specs *s);
 
 inline int grid_flags_full(const mpm_ctx *c) {
     return G_FINISH_MASS | G_FINISH_VEL | G_FINISH_FRC | G_BACKUP | G_UPDATE | G_BCS |

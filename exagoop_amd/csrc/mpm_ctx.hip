@@ -42,6 +42,80 @@ int check_launch() {
     return MPM_OK;
 }
 
+// Validated grid and step parameters of a problem specification
+// (Initialise_Domain, mpm_init.cpp:108-299, and the MPMspecs fields the step
+// reads).
+int specs_params(const mpm_specs *s, GridP &g, PhysP &ph) {
+    int od[3];
+    int rc = mpm_order_directional(s, od);
+    if (rc) return rc;
+    for (int d = 0; d < 3; ++d) {
+        if (s->ncells[d] < 1 || !(s->phi[d] > s->plo[d]))
+            return set_err(MPM_E_ARG, "bad domain");
+        if (s->bclo[d] < 0 || s->bclo[d] > MPM_BC_OUTFLOW || s->bchi[d] < 0 ||
+            s->bchi[d] > MPM_BC_OUTFLOW)
+            return set_err(MPM_E_BC, "unknown boundary condition");
+        // applybc aborts on BC_OUTFLOW at the first nodal-BC pass
+        // (mpm_kernels.H:107-110).
+        if (s->bclo[d] == MPM_BC_OUTFLOW || s->bchi[d] == MPM_BC_OUTFLOW)
+            return set_err(MPM_E_BC, "outflow BC is not supported by applybc");
+    }
+    if (s->stress_update_scheme != MPM_USL && s->stress_update_scheme != MPM_MUSL)
+        return set_err(MPM_E_ARG, "stress_update_scheme must be USL or MUSL");
+    std::memset(&g, 0, sizeof(g));
+    g.uniform = (od[0] == od[1] && od[1] == od[2]);
+    for (int d = 0; d < 3; ++d) {
+        g.plo[d] = s->plo[d];
+        g.phi[d] = s->phi[d];
+        g.n[d] = s->ncells[d];
+        g.hi[d] = s->ncells[d] - 1;
+        g.dx[d] = (s->phi[d] - s->plo[d]) / (double)s->ncells[d];
+        g.dxi[d] = 1.0 / g.dx[d];
+        g.per[d] = s->periodic[d] ? 1 : 0;
+        g.od[d] = od[d];
+        double rhi = s->phi[d];
+        while ((int)std::floor((rhi - s->plo[d]) * g.dxi[d]) >= g.n[d])
+            rhi = std::nextafter(rhi, -INFINITY);
+        g.rhi[d] = rhi;
+    }
+    g.nn = (int64_t)(g.n[0] + 1) * (g.n[1] + 1) * (g.n[2] + 1);
+    for (int d = 0; d < 3; ++d) {
+        g.ntc[d] = (g.n[d] + kTileCells - 1) / kTileCells;
+        g.ntn[d] = g.n[d] / kTileCells + 1;
+    }
+    std::memset(&ph, 0, sizeof(ph));
+    for (int d = 0; d < 3; ++d) {
+        ph.grav[d] = s->gravity[d];
+        ph.slab_lo[d] = s->force_slab_lo[d];
+        ph.slab_hi[d] = s->force_slab_hi[d];
+        ph.extforce[d] = s->extforce[d];
+        ph.bclo[d] = s->bclo[d];
+        ph.bchi[d] = s->bchi[d];
+        ph.mu_lo[d] = s->wall_mu_lo[d];
+        ph.mu_hi[d] = s->wall_mu_hi[d];
+    }
+    for (int q = 0; q < 9; ++q) {
+        ph.wv_lo[q] = s->wall_vel_lo[q];
+        ph.wv_hi[q] = s->wall_vel_hi[q];
+    }
+    ph.extloads = s->external_loads;
+    ph.mass_tol = s->mass_tolerance;
+    ph.alpha = s->alpha_pic_flip;
+    ph.applied_rate = s->applied_strainrate;
+    ph.applied_time = s->applied_strainrate_time;
+    ph.delta = s->calculate_strain_based_on_delta == 1;
+    ph.cfl = s->CFL;
+    ph.dt_min = s->dt_min_limit;
+    ph.dt_max = s->dt_max_limit;
+    return MPM_OK;
+}
+
+bool walls_at_rest(const mpm_specs *s) {
+    for (int q = 0; q < 9; ++q)
+        if (s->wall_vel_lo[q] != 0.0 || s->wall_vel_hi[q] != 0.0) return false;
+    return true;
+}
+
 }  // namespace xmpm
 
 
@@ -157,82 +231,22 @@ int mpm_create(const mpm_specs *s, mpm_ctx **out) {
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
         return set_err(MPM_E_NODEVICE, "no HIP device");
-    int od[3];
-    int rc = mpm_order_directional(s, od);
+    GridP g;
+    PhysP ph;
+    int rc = specs_params(s, g, ph);
     if (rc) return rc;
-    for (int d = 0; d < 3; ++d) {
-        if (s->ncells[d] < 1 || !(s->phi[d] > s->plo[d]))
-            return set_err(MPM_E_ARG, "bad domain");
-        if (s->bclo[d] < 0 || s->bclo[d] > MPM_BC_OUTFLOW || s->bchi[d] < 0 ||
-            s->bchi[d] > MPM_BC_OUTFLOW)
-            return set_err(MPM_E_BC, "unknown boundary condition");
-        // applybc aborts on BC_OUTFLOW at the first nodal-BC pass
-        // (mpm_kernels.H:107-110).
-        if (s->bclo[d] == MPM_BC_OUTFLOW || s->bchi[d] == MPM_BC_OUTFLOW)
-            return set_err(MPM_E_BC, "outflow BC is not supported by applybc");
-    }
-    if (s->stress_update_scheme != MPM_USL && s->stress_update_scheme != MPM_MUSL)
-        return set_err(MPM_E_ARG, "stress_update_scheme must be USL or MUSL");
     mpm_ctx *c = new mpm_ctx();
     c->s = *s;
-    GridP &g = c->g;
-    std::memset(&g, 0, sizeof(g));
-    g.uniform = (od[0] == od[1] && od[1] == od[2]);
-    for (int d = 0; d < 3; ++d) {
-        g.plo[d] = s->plo[d];
-        g.phi[d] = s->phi[d];
-        g.n[d] = s->ncells[d];
-        g.hi[d] = s->ncells[d] - 1;
-        g.dx[d] = (s->phi[d] - s->plo[d]) / (double)s->ncells[d];
-        g.dxi[d] = 1.0 / g.dx[d];
-        g.per[d] = s->periodic[d] ? 1 : 0;
-        g.od[d] = od[d];
-        double rhi = s->phi[d];
-        while ((int)std::floor((rhi - s->plo[d]) * g.dxi[d]) >= g.n[d])
-            rhi = std::nextafter(rhi, -INFINITY);
-        g.rhi[d] = rhi;
-        c->any_per |= (g.per[d] != 0);
-    }
-    g.nn = (int64_t)(g.n[0] + 1) * (g.n[1] + 1) * (g.n[2] + 1);
-    for (int d = 0; d < 3; ++d) {
-        g.ntc[d] = (g.n[d] + kTileCells - 1) / kTileCells;
-        g.ntn[d] = g.n[d] / kTileCells + 1;
-    }
-    PhysP &ph = c->ph;
-    std::memset(&ph, 0, sizeof(ph));
-    for (int d = 0; d < 3; ++d) {
-        ph.grav[d] = s->gravity[d];
-        ph.slab_lo[d] = s->force_slab_lo[d];
-        ph.slab_hi[d] = s->force_slab_hi[d];
-        ph.extforce[d] = s->extforce[d];
-        ph.bclo[d] = s->bclo[d];
-        ph.bchi[d] = s->bchi[d];
-        ph.mu_lo[d] = s->wall_mu_lo[d];
-        ph.mu_hi[d] = s->wall_mu_hi[d];
-    }
-    for (int q = 0; q < 9; ++q) {
-        ph.wv_lo[q] = s->wall_vel_lo[q];
-        ph.wv_hi[q] = s->wall_vel_hi[q];
-    }
-    ph.extloads = s->external_loads;
-    ph.mass_tol = s->mass_tolerance;
-    ph.alpha = s->alpha_pic_flip;
-    ph.applied_rate = s->applied_strainrate;
-    ph.applied_time = s->applied_strainrate_time;
-    ph.delta = s->calculate_strain_based_on_delta == 1;
-    ph.cfl = s->CFL;
-    ph.dt_min = s->dt_min_limit;
-    ph.dt_max = s->dt_max_limit;
+    c->g = g;
+    c->ph = ph;
+    for (int d = 0; d < 3; ++d) c->any_per |= (g.per[d] != 0);
     std::memset(&c->ls, 0, sizeof(c->ls));
     std::memset(&c->udf, 0, sizeof(c->udf));
     std::memset(&c->P, 0, sizeof(c->P));
     std::memset(&c->Q, 0, sizeof(c->Q));
     std::memset(&c->N, 0, sizeof(c->N));
     std::memset(&c->S, 0, sizeof(c->S));
-    for (int d = 0; d < 3; ++d)
-        c->skip_ok &= (s->wall_vel_lo[3 * d] == 0.0 && s->wall_vel_lo[3 * d + 1] == 0.0 &&
-                       s->wall_vel_lo[3 * d + 2] == 0.0 && s->wall_vel_hi[3 * d] == 0.0 &&
-                       s->wall_vel_hi[3 * d + 1] == 0.0 && s->wall_vel_hi[3 * d + 2] == 0.0);
+    c->skip_ok = walls_at_rest(s);
 
     auto fail = [&](int code) {
         mpm_destroy(c);

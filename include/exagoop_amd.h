@@ -330,6 +330,64 @@ int mpm_mlp_train(mpm_mlp *m, mpm_dataset *d, int epochs, int batch, float lr, u
 int mpm_step_learned(mpm_ctx *ctx, mpm_mlp *m, int ncoarse, int nfine, const double guards[2],
                      int *fallbacks);
 
+/* ---------------------------------------------------------------------------
+ * Batched ensembles.  No counterpart in the reference (one scene per run):
+ * many independent scenes of uneven size advance together, one persistent
+ * kernel launch per step, the steps replayed from a captured HIP graph.
+ * Each scene follows the specialised sorted USL step of mpm_step (the time
+ * loop body, main.cpp:83-187) and gives the same results as a context of its
+ * own, up to the order of the FP64 atomic sums.  Requirements: one order for
+ * every direction of every scene, USL, redistribution every step
+ * (num_redist = 1), walls at rest, no level set or wall UDF, and one
+ * calculate_strain_based_on_delta setting for the whole batch.  The nodal
+ * force components are not kept (mpm_batch_get_nodal returns zeros for them).
+ * Callers: the ensemble driver (exagoop_amd/ensemble.py), its tests and the
+ * benchmark.
+ * ------------------------------------------------------------------------- */
+typedef struct mpm_batch mpm_batch;
+
+/* Work decomposition of the persistent step (the autotuner's search space):
+ * persistent workgroups (multiple of 8; 0 = as many as fit on the device at
+ * once), particles per key/scatter task, particles per P2G/particle task
+ * (each such task takes a share of the scene's non-empty 4^3-cell tiles),
+ * nodes per grid task, steps captured per graph replay. */
+typedef struct mpm_batch_tuning {
+    int workgroups;
+    int sort_chunk;
+    int tile_chunk;
+    int node_chunk;
+    int graph_steps;
+} mpm_batch_tuning;
+
+/* nscenes scenes with specs[s] and np[s] >= 1 particles each; allocates all
+ * device state on the current device. */
+int mpm_batch_create(int nscenes, const mpm_specs *specs, const int64_t *np, mpm_batch **out);
+void mpm_batch_destroy(mpm_batch *b);
+/* Particles of one scene, as mpm_upload_particles (np must match). */
+int mpm_batch_upload(mpm_batch *b, int scene, int64_t np, const double *pos,
+                     const double *rdata, const int *idata);
+/* Initialise_Internal_Forces and the first Calculate_time_step of every
+ * scene (all scenes uploaded). */
+int mpm_batch_init(mpm_batch *b);
+/* nsteps steps of every scene. */
+int mpm_batch_step(mpm_batch *b, int nsteps);
+int mpm_batch_num_particles(mpm_batch *b, int scene, int64_t *np);
+/* As mpm_download_particles / mpm_get_nodal / mpm_get_time for one scene. */
+int mpm_batch_download(mpm_batch *b, int scene, double *pos, double *rdata, int *idata,
+                       int64_t *id);
+int mpm_batch_get_nodal(mpm_batch *b, int scene, int comp, double *out);
+int mpm_batch_get_time(mpm_batch *b, int scene, double *time, int64_t *steps, double *last_dt);
+int mpm_batch_get_tuning(mpm_batch *b, mpm_batch_tuning *t);
+int mpm_batch_set_tuning(mpm_batch *b, const mpm_batch_tuning *t);
+/* Autotuner: times trial_steps steps per candidate decomposition (coordinate
+ * search over the mpm_batch_tuning fields) from a snapshot of the state,
+ * restores the snapshot after every trial and keeps the fastest: the
+ * simulation state is unchanged.  best / best_ms may be NULL. */
+int mpm_batch_autotune(mpm_batch *b, int trial_steps, mpm_batch_tuning *best, double *best_ms);
+/* Milliseconds per step of the last mpm_batch_step call (HIP events), the
+ * particle slots of all scenes and the tasks of one step. */
+int mpm_batch_stats(mpm_batch *b, double *step_ms, int64_t *particles, int *tasks);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,241 @@
+"""GPU checks of the batched ensemble (mpm_batch_*) through the C ABI.
+
+Each scene of a batch must give what a context of its own gives (the
+single-scene sorted step, pinned to the CPU oracle in test_gpu_parity.py):
+nodal mass / velocity / delta-velocity after one step within 1e-12 relative
+L2, particle state after 25 steps within 1e-10 (the only difference is the
+order of the FP64 atomic sums; a field that is zero up to rounding is held
+to 10 x the run-to-run spread of the single-scene step instead).  Two small scenes are also checked straight
+against the oracle.  Any work decomposition (tuning), the autotuner's trials
+and graph replays leave the results unchanged; the full 256-scene ensemble
+keeps every particle, its masses and a dt equal to the CFL bound recomputed
+on the host from the state the step left.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from exagoop_amd import capi, ensemble
+from exagoop_amd.capi import (N_DVEL, N_MASS, N_MASS_OLD, N_VEL, R_BULK, R_DENSITY, R_E, R_MASS,
+                              R_NU, R_VEL, MUSL)
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def rel(a, b):
+    a = np.asarray(a, dtype=float).ravel()
+    b = np.asarray(b, dtype=float).ravel()
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a) if nb == 0.0 else np.linalg.norm(a - b) / nb
+
+
+def small_seeds(k=6, max_target=6000, max_n=24):
+    out = []
+    for s in range(10_000):
+        p = ensemble.draw(s)
+        if p["target"] <= max_target and p["n"] <= max_n:
+            out.append(s)
+        if len(out) == k:
+            return out
+    raise RuntimeError("not enough small scenes")
+
+
+SEEDS = small_seeds()
+
+
+def context_of(sc):
+    s, pos, r, idata, _ = sc
+    g = capi.Context(s)
+    g.upload(pos, r, idata)
+    g.init_internal_forces()
+    return g
+
+
+def compare_particles(a, b, tol, floor=None):
+    """Particle state a against b: ids and integer fields equal, positions and
+    every real field within tol relative L2, or, where `floor` (per-field
+    relative differences between two runs of the same single-scene step) is
+    given, within 10 x that run-to-run spread if larger.  A field that is
+    zero up to rounding (the strain rate of a rigid motion) carries only
+    the rounding noise of the FP64 atomic sums, so its relative difference is
+    set by that noise, not by tol."""
+    pa, ra, ia, ida = a
+    pb, rb, ib, idb = b
+    assert np.array_equal(ida, idb)
+    assert np.array_equal(ia, ib)
+    assert rel(pa, pb) <= tol, rel(pa, pb)
+    bad = {}
+    for f in range(ra.shape[0]):
+        t = tol if floor is None else max(tol, 10.0 * floor[f])
+        e = rel(ra[f], rb[f])
+        if not e <= t:
+            bad[f] = (e, t)
+    assert not bad, bad
+
+
+def spread(a, b):
+    """Per-field relative L2 difference of two downloads."""
+    return [rel(a[1][f], b[1][f]) for f in range(a[1].shape[0])]
+
+
+def test_batch_rejects_what_it_does_not_run(gpu):
+    say("[batch] unsupported specs are errors")
+    s, pos, r, idata, _ = ensemble.scene(SEEDS[0])
+    for field, value in (("stress_update_scheme", MUSL), ("num_redist", 2)):
+        t = s.copy()
+        setattr(t, field, value)
+        with pytest.raises(capi.MPMError):
+            capi.Batch([s, t], [pos.shape[1]] * 2)
+    t = s.copy()
+    t.wall_vel_lo[1] = 0.1
+    with pytest.raises(capi.MPMError):
+        capi.Batch([t], [pos.shape[1]])
+    t = s.copy()
+    t.order_scheme = 3
+    with pytest.raises(capi.MPMError):
+        capi.Batch([s, t], [pos.shape[1]] * 2)
+    b = capi.Batch([s], [pos.shape[1]])
+    with pytest.raises(capi.MPMError):
+        b.init()  # nothing uploaded
+    with pytest.raises(capi.MPMError):
+        b.upload(0, pos[:, :-1], r[:, :-1], idata[:, :-1])  # count differs from create
+    with pytest.raises(capi.MPMError):
+        b.set_tuning(workgroups=12)
+
+
+def test_batch_one_step_nodal_matches_contexts(gpu):
+    say(f"[batch] one step, nodal parity, seeds {SEEDS}")
+    b, sc = ensemble.make_batch(SEEDS)
+    b.step(1)
+    for k, x in enumerate(sc):
+        g = context_of(x)
+        g.step(1)
+        for c in (N_MASS, N_VEL, N_VEL + 1, N_VEL + 2, N_DVEL, N_DVEL + 1, N_DVEL + 2, N_MASS_OLD):
+            e = rel(b.nodal(k, c), g.nodal(c))
+            assert e <= 1e-12, (k, c, e)
+        assert abs(b.time(k)[0] - g.time()[0]) <= 1e-15 * abs(g.time()[0])
+        assert b.time(k)[2] == g.time()[2]  # the step's dt, from the same particles
+        g.close()
+
+
+def test_batch_25_steps_match_contexts(gpu):
+    say(f"[batch] 25 steps, particle parity, seeds {SEEDS}")
+    b, sc = ensemble.make_batch(SEEDS)
+    b.step(25)
+    for k, x in enumerate(sc):
+        g = context_of(x)
+        g.step(25)
+        g2 = context_of(x)
+        g2.step(25)
+        ref = g.download()
+        compare_particles(b.download(k), ref, 1e-10, floor=spread(g2.download(), ref))
+        tb, tg = b.time(k), g.time()
+        assert tb[1] == tg[1] == 25
+        assert abs(tb[0] - tg[0]) <= 1e-12 * tg[0]
+        g.close()
+        g2.close()
+
+
+def test_batch_matches_the_oracle(gpu):
+    seeds = small_seeds(2, max_target=3000, max_n=20)
+    say(f"[batch] 10 steps against the CPU oracle, seeds {seeds}")
+    b, sc = ensemble.make_batch(seeds)
+    b.step(10)
+    for k, (s, pos, r, idata, _) in enumerate(sc):
+        o = oracle.Oracle(s)
+        o.upload(pos, r, idata)
+        o.init_internal_forces()
+        o.step(10)
+        g = context_of(sc[k])
+        g.step(10)
+        g2 = context_of(sc[k])
+        g2.step(10)
+        compare_particles(b.download(k), o.download(), 1e-10,
+                          floor=spread(g2.download(), g.download()))
+        assert abs(b.time(k)[0] - o.time()[0]) <= 1e-12 * o.time()[0]
+        mass = r[R_MASS].sum()
+        assert b.download(k)[1][R_MASS].sum() == mass  # particle masses never change
+
+
+@pytest.mark.parametrize("tuning", [
+    dict(workgroups=8, sort_chunk=256, tile_chunk=1, node_chunk=256, graph_steps=3),
+    dict(workgroups=16, sort_chunk=1 << 20, tile_chunk=1 << 20, node_chunk=1 << 20,
+         graph_steps=1),
+])
+def test_any_decomposition_gives_the_same_steps(gpu, tuning):
+    say(f"[batch] decomposition {tuning} vs default")
+    a, _ = ensemble.make_batch(SEEDS)
+    a2, _ = ensemble.make_batch(SEEDS)
+    b, _ = ensemble.make_batch(SEEDS, tuning=tuning)
+    assert b.tuning() == tuning
+    for x in (a, a2, b):
+        x.step(10)
+    for k in range(len(SEEDS)):
+        ref = a.download(k)
+        compare_particles(b.download(k), ref, 1e-12, floor=spread(a2.download(k), ref))
+
+
+def test_autotune_leaves_the_state_unchanged(gpu):
+    say("[batch] autotune then step vs step")
+    a, _ = ensemble.make_batch(SEEDS)
+    a2, _ = ensemble.make_batch(SEEDS)
+    b, _ = ensemble.make_batch(SEEDS)
+    best, ms = b.autotune(trial_steps=2)
+    say(f"    best {best} {ms:.3f} ms/step")
+    assert ms > 0 and b.tuning() == best
+    assert b.time(0)[1] == 0
+    for x in (a, a2, b):
+        x.step(6)
+    for k in range(len(SEEDS)):
+        ref = a.download(k)
+        compare_particles(b.download(k), ref, 1e-12, floor=spread(a2.download(k), ref))
+
+
+def cfl_dt(s, pos, r):
+    """Calculate_time_step on the host: CFL x min dx / (c + |v|), clamped."""
+    dx = min((s.phi[d] - s.plo[d]) / s.ncells[d] for d in range(3))
+    rho = r[R_DENSITY]
+    E, nu = r[R_E], r[R_NU]
+    lam = E * nu / ((1 + nu) * (1 - 2.0 * nu))
+    mu = E / (2.0 * (1 + nu))
+    c_el = np.sqrt((lam + 2.0 * mu) / rho)
+    c_fl = np.sqrt(r[R_BULK] / rho)
+    fluid = r[R_BULK] > 0
+    c = np.where(fluid, c_fl, c_el)
+    v = np.sqrt((r[R_VEL:R_VEL + 3] ** 2).sum(axis=0))
+    return min(max(s.CFL * np.min(dx / (c + v)), s.dt_min_limit), s.dt_max_limit)
+
+
+def test_full_ensemble_properties(gpu):
+    say("[batch] 256-scene ensemble, 40 steps")
+    seeds = list(range(256))
+    b, sc = ensemble.make_batch(seeds)
+    b.autotune(trial_steps=2)
+    b.step(40)
+    st = b.stats()
+    say(f"    {st}")
+    assert st["particles"] == sum(x[1].shape[1] for x in sc)
+    picks = list(range(0, 256, 17))
+    states = {}
+    for k in picks:
+        s, pos0, r0, idata0, prm = sc[k]
+        pos, r, idata, ids = b.download(k)
+        assert np.array_equal(ids, np.arange(pos0.shape[1]))  # nobody left the box
+        assert np.all(np.isfinite(pos)) and np.all(np.isfinite(r))
+        assert np.all(r[R_MASS] == r0[R_MASS])
+        t, steps, dt = b.time(k)
+        assert steps == 40 and t > 0 and dt > 0
+        states[k] = (t, cfl_dt(s, pos, r))
+    b.step(1)  # its dt is the CFL bound of the state above
+    for k in picks:
+        t, steps, dt = b.time(k)
+        t0, want = states[k]
+        assert abs(dt - want) <= 1e-14 * want, (k, dt, want)
+        assert t == t0 + dt
