@@ -152,6 +152,7 @@ SIGNATURES = {
     "mpm_batch_set_tuning": (ctypes.c_int, [_P, ctypes.c_void_p]),
     "mpm_batch_autotune": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_void_p, _DP]),
     "mpm_batch_stats": (ctypes.c_int, [_P, _DP, _LP, _IP]),
+    "mpm_batch_profile": (ctypes.c_int, [_P, ctypes.c_int, _DP, _DP, _LP, _DP, _DP]),
 }
 
 
@@ -580,6 +581,19 @@ class Batch:
         ms = ctypes.c_double(0)
         self._call("autotune", trial_steps, ctypes.byref(t), ctypes.byref(ms))
         return t.as_dict(), ms.value
+
+    STAGES = ("keys", "scan", "scatter", "p2g", "grid", "particle")
+
+    def profile(self, nsteps=5):
+        """Per-stage workgroup-ms per step (busy, wait), tasks, other, step ms."""
+        busy, wait = np.zeros(6), np.zeros(6)
+        tasks = np.zeros(6, dtype=np.int64)
+        other, ms = ctypes.c_double(0), ctypes.c_double(0)
+        self._call("profile", nsteps, dptr(busy), dptr(wait), lptr(tasks), ctypes.byref(other),
+                   ctypes.byref(ms))
+        return {"stages": {k: {"busy_wg_ms": float(busy[i]), "wait_wg_ms": float(wait[i]),
+                               "tasks": int(tasks[i])} for i, k in enumerate(self.STAGES)},
+                "other_wg_ms": other.value, "step_ms": ms.value}
 
     def stats(self):
         ms = ctypes.c_double(0)

@@ -9,7 +9,7 @@
 //   SCAN     exclusive scan of the cell counts, list of non-empty tiles
 //   SCATTER  sorted permutation + P2G packets          (particle chunks)
 //   P2G      tile deposits                              (tile shares)
-//   GRID     nodal pipeline, idle node tiles skipped    (node chunks)
+//   GRID     nodal pipeline over the active node tiles   (node-tile chunks)
 //   PART     fused particle pipeline, sorted write, dt candidates (tile shares)
 // The tasks sit in eight queues, stage-major: every task of stage k of a
 // scene comes before every task of stage k + 1 of that scene, and a scene's
@@ -27,6 +27,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstddef>
 #include <cstring>
 #include <numeric>
 #include <vector>
@@ -80,6 +81,10 @@ struct BCtl {
     unsigned exited;
     int err;  // a bounded wait ran out
     int pad[14];
+    // Instrumented launches: per stage, wall-clock ticks of task work and of
+    // the wait for the previous stage, summed over tasks; tasks run; ticks
+    // spent outside tasks (taking tasks, leaving).
+    unsigned long long busy[B_NSTAGE], wait[B_NSTAGE], ntasks[B_NSTAGE], other;
 };
 
 struct BQueues {
@@ -227,12 +232,14 @@ __device__ __forceinline__ int64_t share(int64_t n, int part, int nparts) {
 template <int O, bool DELTA>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) void k_batch_step(const BScene *__restrict__ scenes,
                                                        BSync *sync, BQueues Q, BCtl *ctl,
-                                                       unsigned spin_limit) {
+                                                       unsigned spin_limit, int prof) {
     __shared__ BLds lds;
     __shared__ int4 task_s;
     const int tid = threadIdx.x;
     const int home = blockIdx.x & (kQueues - 1);
     unsigned dry = 0;  // queues this workgroup found empty (thread 0)
+    unsigned long long t_end = prof ? wall_clock64() : 0;  // end of the last task (thread 0)
+    unsigned long long other = 0;
     for (;;) {
         if (tid == 0) {
             int4 t = make_int4(-1, 0, 0, 0);
@@ -256,10 +263,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
         if (t.x < 0) break;  // every queue drained (uniform)
         const int s = t.x, stage = t.y, part = t.z, nparts = t.w;
         const BScene &sd = scenes[s];
+        unsigned long long t_start = 0, t_work = 0;
+        if (prof && tid == 0) t_start = wall_clock64();
         if (stage > 0) {
             if (tid == 0) wait_count(&sync[s].done[stage - 1], sd.ntask[stage - 1], ctl, spin_limit);
             __syncthreads();
         }
+        if (prof && tid == 0) t_work = wall_clock64();
         const int step = __hip_atomic_load(&sync[s].step, __ATOMIC_RELAXED,
                                            __HIP_MEMORY_SCOPE_AGENT);
         const int epoch = step + 1;  // number of the step being taken
@@ -285,10 +295,30 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
             ls.nb = 0;
             UdfP udf;
             for (int d = 0; d < 3; ++d) udf.tab[d][0] = udf.tab[d][1] = nullptr;
+            // Node tiles (4^3 nodes), one per wavefront: a tile idle since
+            // before the previous step already holds the zero massless state
+            // and is passed over with one read.
             const double dt = sd.sc->dt;
-            const int64_t a = share(sd.g.nn, part, nparts), b = share(sd.g.nn, part + 1, nparts);
-            for (int64_t node = a + tid; node < b; node += kBlock)
-                grid_one(sd.g, sd.ph, ls, udf, sd.N, kGridFlags, dt, node, S.tile_epoch, epoch);
+            const GridP &g = sd.g;
+            const int ntx = g.ntn[0], nty = g.ntn[1];
+            const int ntn = ntx * nty * g.ntn[2];
+            const int a = (int)share(ntn, part, nparts), b = (int)share(ntn, part + 1, nparts);
+            const int lane = tid & 63;
+            for (int t0 = a; t0 < b; t0 += kBlock / 64) {
+                const int t = t0 + (tid >> 6);
+                if (t >= b) continue;
+                // (written by this launch's P2G tasks: a vector load, never a
+                // scalar-cache hit)
+                const int e = __hip_atomic_load(&S.tile_epoch[t], __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_AGENT);
+                if (e != epoch && e != epoch - 1) continue;
+                const int I = (t % ntx) * kTileCells + (lane & 3);
+                const int J = ((t / ntx) % nty) * kTileCells + ((lane >> 2) & 3);
+                const int K = (t / (ntx * nty)) * kTileCells + (lane >> 4);
+                if (I > g.n[0] || J > g.n[1] || K > g.n[2]) continue;
+                grid_at(g, sd.ph, ls, udf, sd.N, kGridFlags, dt, nidx(g, I, J, K), I, J, K,
+                        S.tile_epoch, epoch);
+            }
         } else {
             const int nt = S.tiles[0];
             const int a = (int)share(nt, part, nparts), b = (int)share(nt, part + 1, nparts);
@@ -301,6 +331,14 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
         }
         __syncthreads();  // the task's writes are done (and task_s is free)
         if (tid == 0) {
+            if (prof) {
+                const unsigned long long t1 = wall_clock64();
+                atomicAdd(&ctl->wait[stage], t_work - t_start);
+                atomicAdd(&ctl->busy[stage], t1 - t_work);
+                atomicAdd(&ctl->ntasks[stage], 1ull);
+                other += t_start - t_end;
+                t_end = t1;
+            }
             const int prev = __hip_atomic_fetch_add(&sync[s].done[stage], 1, __ATOMIC_ACQ_REL,
                                                     __HIP_MEMORY_SCOPE_AGENT);
             if (stage == B_PART && prev + 1 == sd.ntask[B_PART]) finish_scene(sd, sync[s]);
@@ -308,6 +346,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
     }
     // Every task is dequeued; the last workgroup out has seen all finish.
     if (tid == 0) {
+        if (prof) atomicAdd(&ctl->other, other + (wall_clock64() - t_end));
         const unsigned e = __hip_atomic_fetch_add(&ctl->exited, 1u, __ATOMIC_ACQ_REL,
                                                   __HIP_MEMORY_SCOPE_AGENT);
         if (e + 1 == gridDim.x) {
@@ -323,7 +362,7 @@ __global__ void k_fill_int(int *p, int64_t n, int v) {
     if (i < n) p[i] = v;
 }
 
-typedef void (*BatchKernel)(const BScene *, BSync *, BQueues, BCtl *, unsigned);
+typedef void (*BatchKernel)(const BScene *, BSync *, BQueues, BCtl *, unsigned, int);
 
 BatchKernel batch_kernel(int O, int delta) {
     if (delta) {
@@ -444,11 +483,11 @@ int build_queues(mpm_batch *b) {
     return MPM_OK;
 }
 
-int launch_steps(mpm_batch *b, int n) {
+int launch_steps(mpm_batch *b, int n, int prof = 0) {
     BatchKernel k = batch_kernel(b->O, b->delta);
     for (int i = 0; i < n; ++i)
         hipLaunchKernelGGL(k, dim3(b->tune.workgroups), dim3(kBlock), 0, b->st, b->d_scn, b->sync,
-                           b->q, b->ctl, b->spin_limit);
+                           b->q, b->ctl, b->spin_limit, prof);
     return check_launch();
 }
 
@@ -940,6 +979,36 @@ int mpm_batch_autotune(mpm_batch *b, int trial_steps, mpm_batch_tuning *best_out
     if (best_out) *best_out = best;
     if (best_ms) *best_ms = bms;
     return done(MPM_OK);
+}
+
+int mpm_batch_profile(mpm_batch *b, int nsteps, double busy_ms[6], double wait_ms[6],
+                      int64_t tasks[6], double *other_ms, double *step_ms) {
+    if (!b || nsteps < 1) return set_err(MPM_E_ARG, "bad argument");
+    if (!b->inited) return set_err(MPM_E_ARG, "batch: mpm_batch_init first");
+    int khz = 0;
+    HIPCHK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, 0));
+    if (khz <= 0) return set_err(MPM_E_HIP, "no wall clock rate");
+    const size_t off = offsetof(BCtl, busy);
+    HIPCHK(hipMemsetAsync(reinterpret_cast<char *>(b->ctl) + off, 0, sizeof(BCtl) - off, b->st));
+    HIPCHK(hipEventRecord(b->ev[0], b->st));
+    int rc = launch_steps(b, nsteps, 1);
+    if (rc) return rc;
+    HIPCHK(hipEventRecord(b->ev[1], b->st));
+    HIPCHK(hipEventSynchronize(b->ev[1]));
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, b->ev[0], b->ev[1]);
+    b->steps += nsteps;
+    BCtl h;
+    HIPCHK(hipMemcpy(&h, b->ctl, sizeof(BCtl), hipMemcpyDeviceToHost));
+    const double tick_ms = 1.0 / khz;
+    for (int k = 0; k < B_NSTAGE; ++k) {
+        if (busy_ms) busy_ms[k] = h.busy[k] * tick_ms / nsteps;
+        if (wait_ms) wait_ms[k] = h.wait[k] * tick_ms / nsteps;
+        if (tasks) tasks[k] = (int64_t)(h.ntasks[k] / nsteps);
+    }
+    if (other_ms) *other_ms = h.other * tick_ms / nsteps;
+    if (step_ms) *step_ms = ms / nsteps;
+    return check_flags(b);
 }
 
 int mpm_batch_stats(mpm_batch *b, double *step_ms, int64_t *particles, int *tasks) {

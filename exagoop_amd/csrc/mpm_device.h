@@ -47,6 +47,20 @@ __device__ __forceinline__ int targets(int I, int n, int per, int t[2], bool img
     return c;
 }
 
+// targets() without the arrays: the first target in t0, the second (the
+// periodic image, when there are two) in t1; returns the count.
+__device__ __forceinline__ int targets2(int I, int n, int per, int &t0, int &t1) {
+    const bool in = I >= 0 && I <= n;
+    int im = -1;
+    if (per) {
+        if (I <= 0 && I + n >= 0) im = I + n;
+        else if (I >= n && I - n <= n) im = I - n;
+    }
+    t0 = in ? I : im;
+    t1 = im;
+    return (in ? 1 : 0) + (im >= 0 ? 1 : 0);
+}
+
 // Node read by G2P for an unwrapped index: periodic ghost nodes hold the
 // value one period away (FillBoundary); non-periodic ghost nodes hold zero.
 __device__ __forceinline__ int wrap_read(int I, int n, int per, bool &ok) {

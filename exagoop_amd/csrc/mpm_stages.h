@@ -119,14 +119,27 @@ __device__ __forceinline__ void p2g_one(const GridP &g, const PhysP &ph, const P
 // Nodal_Time_Update_Momentum (:451-482), nodal_bcs (:663-766),
 // apply_udf_nodal_bcs (:1061-1175), nodal_levelset_bcs (:129-220),
 // store_delta_velocity (:368-392).
-// One node of the pipeline.
+// One node of the pipeline: grid_one for a node index, grid_at for a node
+// whose (I, J, K) the caller knows.
+__device__ __forceinline__ void grid_at(const GridP &g, const PhysP &ph, const LsP &ls,
+                                        const UdfP &udf, const NState &N, int flags, double dt,
+                                        int64_t node, int I, int J, int K, const int *tile_epoch,
+                                        int epoch);
+
 __device__ __forceinline__ void grid_one(const GridP &g, const PhysP &ph, const LsP &ls,
                                          const UdfP &udf, const NState &N, int flags, double dt,
                                          int64_t node, const int *tile_epoch, int epoch) {
-    const int64_t nn = g.nn;
     const int I = (int)(node % (g.n[0] + 1));
     const int J = (int)((node / (g.n[0] + 1)) % (g.n[1] + 1));
     const int K = (int)(node / ((int64_t)(g.n[0] + 1) * (g.n[1] + 1)));
+    grid_at(g, ph, ls, udf, N, flags, dt, node, I, J, K, tile_epoch, epoch);
+}
+
+__device__ __forceinline__ void grid_at(const GridP &g, const PhysP &ph, const LsP &ls,
+                                        const UdfP &udf, const NState &N, int flags, double dt,
+                                        int64_t node, int I, int J, int K, const int *tile_epoch,
+                                        int epoch) {
+    const int64_t nn = g.nn;
     if (tile_epoch) {
         // Node tiles no particle deposited on this step hold the massless
         // state, which is all-zero when walls are at rest and no UDF is set:

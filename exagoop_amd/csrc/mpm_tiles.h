@@ -63,6 +63,12 @@ __device__ __forceinline__ int active_tile(const SortState &S, int b) {
     return p < na ? S.tiles[1 + p] : -1;
 }
 
+// a[d] for a per-lane d in 0..2 (d = 3 gives a[2]).
+template <typename T>
+__device__ __forceinline__ T pick3(const T (&a)[3], int d) {
+    return d == 0 ? a[0] : (d == 1 ? a[1] : a[2]);
+}
+
 __device__ __forceinline__ int cell_key(const GridP &g, const int c[3]) {
     const int t[3] = {c[0] / kTileCells, c[1] / kTileCells, c[2] / kTileCells};
     const int tile = (t[2] * g.ntc[1] + t[1]) * g.ntc[0] + t[0];
@@ -220,9 +226,11 @@ __device__ __attribute__((noinline)) void deposit_global(const GridP &g, const d
 template <int O>
 __device__ __forceinline__ void p2g_factors4(const SplineTables &T, const GridP &g, int d, int i,
                                              double x, double f[8]) {
-    const int od = O ? O : g.od[d];
-    const double dx = g.dx[d], dxi = g.dxi[d], plo = g.plo[d];
-    const int hi = g.hi[d], per = g.per[d];
+    // d differs between lanes: select among the three uniform values rather
+    // than index the parameter block per lane.
+    const int od = O ? O : pick3(g.od, d);
+    const double dx = pick3(g.dx, d), dxi = pick3(g.dxi, d), plo = pick3(g.plo, d);
+    const int hi = pick3(g.hi, d), per = pick3(g.per, d);
     if (od == 1) {
         const double r = (x - (plo + i * dx)) * dxi;
         f[0] = f[1] = f[6] = f[7] = 0.0;
@@ -443,7 +451,7 @@ __device__ __forceinline__ void p2g_tile(const GridP &g, const PhysP &ph, const 
                                          : ((s_part == 1) ? T0[1] + ry : T0[2] + cz);
             const bool out =
                 s_part < 3 && cur.act &&
-                (int)floor((cur.xd - g.plo[s_part]) * g.dxi[s_part]) != ic;
+                (int)floor((cur.xd - pick3(g.plo, s_part)) * pick3(g.dxi, s_part)) != ic;
             const unsigned long long ob = __ballot(out);
             const bool slot_out = ((ob >> s_slot) | (ob >> (s_slot + 16)) |
                                    (ob >> (s_slot + 32))) & 1ull;
@@ -568,19 +576,23 @@ __device__ __forceinline__ void p2g_tile(const GridP &g, const PhysP &ph, const 
         bool any = false;
         for (int c = 0; c < NC; ++c) any |= (v[c] != 0.0);
         if (!any) continue;
-        int tx_[2], ty_[2], tz_[2];
-        bool ix[2], iy[2], iz[2];
-        const int cx = targets(T0[0] + ax - 1, g.n[0], g.per[0], tx_, ix);
-        const int cy = targets(T0[1] + ay - 1, g.n[1], g.per[1], ty_, iy);
-        const int cz2 = targets(T0[2] + az - 1, g.n[2], g.per[2], tz_, iz);
-        for (int qz = 0; qz < cz2; ++qz)
-            for (int qy = 0; qy < cy; ++qy)
-                for (int qx = 0; qx < cx; ++qx) {
-                    const int64_t node = nidx(g, tx_[qx], ty_[qy], tz_[qz]);
+        int tx0, tx1, ty0, ty1, tz0, tz1;
+        const int cx = targets2(T0[0] + ax - 1, g.n[0], g.per[0], tx0, tx1);
+        const int cy = targets2(T0[1] + ay - 1, g.n[1], g.per[1], ty0, ty1);
+        const int cz2 = targets2(T0[2] + az - 1, g.n[2], g.per[2], tz0, tz1);
+#pragma unroll
+        for (int qz = 0; qz < 2; ++qz)
+#pragma unroll
+            for (int qy = 0; qy < 2; ++qy)
+#pragma unroll
+                for (int qx = 0; qx < 2; ++qx) {
+                    if (qz >= cz2 || qy >= cy || qx >= cx) continue;
+                    const int I = qx ? tx1 : tx0, J = qy ? ty1 : ty0, K = qz ? tz1 : tz0;
+                    const int64_t node = nidx(g, I, J, K);
                     for (int c = 0; c < NC; ++c)
                         if (v[c] != 0.0)
                             unsafeAtomicAdd(&acc[(FORCES ? c : c + 1) * nn + node], v[c]);
-                    S.tile_epoch[node_tile(g, tx_[qx], ty_[qy], tz_[qz])] = epoch;
+                    S.tile_epoch[node_tile(g, I, J, K)] = epoch;
                 }
     }
 }

@@ -139,3 +139,39 @@ def make_batch(seeds, tuning=None):
         b.set_tuning(**tuning)
     b.init()
     return b, sc
+
+
+def main(argv=None):
+    """Run an ensemble: build, autotune, time, and optionally profile the
+    persistent step's stages; prints one JSON object."""
+    import argparse
+    import json
+    import time
+
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--scenes", type=int, default=256)
+    ap.add_argument("--seed0", type=int, default=0)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--profile-steps", type=int, default=0,
+                    help="then run this many instrumented steps and report per-stage times")
+    a = ap.parse_args(argv)
+    t0 = time.perf_counter()
+    b, sc = make_batch(range(a.seed0, a.seed0 + a.scenes))
+    out = {"scenes": a.scenes, "build_s": round(time.perf_counter() - t0, 2)}
+    if not a.no_autotune:
+        best, ms = b.autotune(trial_steps=3)
+        out["autotune"] = {"tuning": best, "ms_per_step": ms}
+    b.step(10)
+    b.step(a.steps)
+    st = b.stats()
+    out.update({"steps": a.steps, "ms_per_step": st["step_ms"], "particles": st["particles"],
+                "tasks_per_step": st["tasks"], "tuning": b.tuning(),
+                "particle_updates_per_s": st["particles"] / (st["step_ms"] * 1e-3)})
+    if a.profile_steps:
+        out["profile"] = b.profile(a.profile_steps)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

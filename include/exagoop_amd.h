@@ -384,6 +384,14 @@ int mpm_batch_set_tuning(mpm_batch *b, const mpm_batch_tuning *t);
  * restores the snapshot after every trial and keeps the fastest: the
  * simulation state is unchanged.  best / best_ms may be NULL. */
 int mpm_batch_autotune(mpm_batch *b, int trial_steps, mpm_batch_tuning *best, double *best_ms);
+/* nsteps steps with an instrumented launch (no graph; the scenes advance):
+ * per stage (keys, scan, scatter, p2g, grid, particle), workgroup-milliseconds
+ * per step spent in its tasks (busy_ms) and waiting for the previous stage
+ * (wait_ms), and its tasks per step; workgroup-milliseconds per step outside
+ * tasks (taking tasks, leaving); milliseconds per step.  Any output may be
+ * NULL. */
+int mpm_batch_profile(mpm_batch *b, int nsteps, double busy_ms[6], double wait_ms[6],
+                      int64_t tasks[6], double *other_ms, double *step_ms);
 /* Milliseconds per step of the last mpm_batch_step call (HIP events), the
  * particle slots of all scenes and the tasks of one step. */
 int mpm_batch_stats(mpm_batch *b, double *step_ms, int64_t *particles, int *tasks);
