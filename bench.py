@@ -18,6 +18,11 @@ launch per step replayed from a HIP graph, decomposition chosen by the
 library's autotuner, 500 timed steps: particle-updates/s over all scenes and
 the step's algorithmic HBM bytes per second.
 
+The column object runs the twisted column at full size (256 x 256 x 896
+cells, 100 663 296 particles, cubic, exagoop_amd/column.py): 20 timed steps
+after 3 warm-up steps, particle-updates/s, per-kernel times, and the
+size-independent checks of its final state.
+
 The learned object compares, on the same dam break, the plain run at a fixed
 fine dt = 2.5e-5 with the learned coarse step (4 fine dt per step, the
 committed network exagoop_amd/data/learned_dam_break.npz) and the uncorrected
@@ -53,6 +58,8 @@ def main():
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--learned-steps", type=int, default=200)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-column", action="store_true")
+    ap.add_argument("--column-steps", type=int, default=20)
     ap.add_argument("--batch-scenes", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=500)
     ap.add_argument("--calib-copy-mib", type=int, default=0,
@@ -142,6 +149,12 @@ def main():
     if not args.no_batch:
         batch = batch_leg(args.batch_scenes, args.batch_steps)
 
+    column = None
+    if not args.no_column:
+        from exagoop_amd import column as column_mod
+        column = column_mod.run(1.0, args.column_steps, 3,
+                                log=lambda m: print(m, file=sys.stderr, flush=True))
+
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "particle-updates/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -154,6 +167,7 @@ def main():
         "cpu_baseline": cpu,
         "learned": learned,
         "batch": batch,
+        "column": column,
     }
     print(json.dumps(out), flush=True)
 

@@ -20,15 +20,20 @@ FOURBYTHREE = 1.3333333333  # constants.H:10
 PI_REF = 3.141592654        # constants.H:12
 
 
-def lattice(specs, ppc, inside):
-    """Cell-offset lattice of all cells, filtered by inside(x, y, z)."""
+def lattice(specs, ppc, inside, box=None):
+    """Cell-offset lattice of all cells (or of the cells overlapping
+    box = ((lo, hi),) * 3), filtered by inside(x, y, z)."""
     n = specs.ncells
     plo, phi = specs.plo, specs.phi
     dx = [(phi[d] - plo[d]) / n[d] for d in range(3)]
     offs = [(2 * np.arange(1, ppc[d] + 1) - 1) / (2 * ppc[d]) for d in range(3)]
     coords = []
     for d in range(3):
-        c = (np.arange(n[d])[:, None] + offs[d][None, :]).ravel() * dx[d] + plo[d]
+        c0, c1 = 0, n[d]
+        if box is not None:
+            c0 = max(0, int(np.floor((box[d][0] - plo[d]) / dx[d])) - 1)
+            c1 = min(n[d], int(np.ceil((box[d][1] - plo[d]) / dx[d])) + 1)
+        c = (np.arange(c0, c1)[:, None] + offs[d][None, :]).ravel() * dx[d] + plo[d]
         coords.append(c)
     X, Y, Z = np.meshgrid(coords[0], coords[1], coords[2], indexing="ij")
     X, Y, Z = X.ravel(), Y.ravel(), Z.ravel()
@@ -127,15 +132,19 @@ def dam_break(nx=128, ny=64, nz=64, width=0.5, height=0.5, ppc=2, order=3, schem
     return s, pos, r, idata
 
 
-def twisted_column(nx=256, ny=256, nz=896, ppc=2, omega0=10.0):
-    """Linear-elastic column 1x1x6 in a 2x2x7 domain, clamped base (no-slip
-    z-lo, slip elsewhere), initial twist v = w(z) z x (x-1, y-1, 0) with
-    w(z) = omega0 sin(pi z / 12), E = 1e6, nu = 0.3, rho = 1000, cubic
-    B-splines, CFL 0.3."""
+def twisted_column(res=1.0, ppc=2, omega0=10.0):
+    """Linear-elastic column [0.5, 1.5]^2 x [0, 6] in the domain [0, 2]^2 x
+    [0, 7], clamped base (no-slip z-lo, slip elsewhere), initial twist
+    v = w(z) z x (x-1, y-1, 0) with w(z) = omega0 sin(pi z / 12), E = 1e6,
+    nu = 0.3, rho = 1000, cubic B-splines, CFL 0.3, USL.  res = 1 is the full
+    size: 256 x 256 x 896 cells (dx = 1/128), 2 x 2 x 2 particles per cell,
+    100 663 296 particles; res scales the cell count per direction (the
+    geometry stays)."""
+    n = [int(round(256 * res)), int(round(256 * res)), int(round(896 * res))]
     s = default_specs()
     s.plo[:] = [0.0, 0.0, 0.0]
-    s.phi[:] = [2.0 * nx / 256, 2.0 * ny / 256, 7.0 * nz / 896]
-    s.ncells[:] = [nx, ny, nz]
+    s.phi[:] = [2.0, 2.0, 7.0]
+    s.ncells[:] = n
     for d in range(3):
         s.bclo[d] = s.bchi[d] = BC_SLIP
     s.bclo[2] = BC_NOSLIP
@@ -143,14 +152,16 @@ def twisted_column(nx=256, ny=256, nz=896, ppc=2, omega0=10.0):
     s.num_redist = 1
     s.gravity[:] = [0.0, 0.0, 0.0]
     s.CFL = 0.3
-    cx = 0.5 * (s.plo[0] + s.phi[0])
-    cy = 0.5 * (s.plo[1] + s.phi[1])
-    ztop = 6.0 * nz / 896
+    s.stress_update_scheme = USL
+    cx, cy, ztop = 1.0, 1.0, 6.0
 
     def inside(X, Y, Z):
         return (np.abs(X - cx) <= 0.5) & (np.abs(Y - cy) <= 0.5) & (Z <= ztop)
 
-    pos, vp = lattice(s, (ppc, ppc, ppc), inside)
+    # lattice of the cells the column overlaps only (the full size has 100.7 M
+    # particles in a 470 M-point domain lattice)
+    box = ((cx - 0.5, cx + 0.5), (cy - 0.5, cy + 0.5), (0.0, ztop))
+    pos, vp = lattice(s, (ppc, ppc, ppc), inside, box=box)
     w = omega0 * np.sin(np.pi * pos[2] / 12.0)
     vel = np.zeros_like(pos)
     vel[0] = -w * (pos[1] - cy)

@@ -290,3 +290,61 @@ def test_dam_break_full_size_properties(gpu):
     e1 = g.energies()
     # gravity impulse bounds the z-momentum change from rest
     assert e1[4] <= 0.0 and abs(e1[4]) <= e0[5] * 9.81 * t * 1.01
+
+
+def rotation_partner(p0):
+    """Index j of the particle at each particle's initial position rotated by
+    90 degrees about the twisted column's axis x = y = 1."""
+    key = lambda P: np.round(P * 2 ** 20).astype(np.int64)
+    k0 = key(p0)
+    idx = {tuple(k0[:, i]): i for i in range(k0.shape[1])}
+    kr = key(np.stack([2.0 - p0[1], p0[0], p0[2]]))
+    return np.array([idx[tuple(kr[:, i])] for i in range(kr.shape[1])])
+
+
+def rotation_asymmetry(pos0, ids, rdata):
+    """Relative L2 distance between the velocity field and its image under the
+    90-degree rotation that maps the column's lattice onto itself."""
+    perm = rotation_partner(pos0[:, ids])
+    v = rdata[R_VEL:R_VEL + 3]
+    vr = np.stack([-v[1], v[0], v[2]])
+    return rel(v[:, perm], vr)
+
+
+def test_twisted_column_small_parity(gpu):
+    """Twisted column at 1/16 resolution (16 x 16 x 56 cells, 24 576
+    particles, cubic): GPU against the oracle after 20 steps, and the
+    90-degree rotation symmetry of the geometry kept by both (the oracle's
+    own asymmetry is at rounding level)."""
+    say("[parity] twisted column 16x16x56, 20 steps")
+    s, pos, r, idata = scenes.twisted_column(1 / 16)
+    g, o = pair(s, pos, r, idata)
+    o.set_threads(16)
+    for ctx in (g, o):
+        ctx.init_internal_forces()
+        ctx.step(20)
+    compare_particles(g, o, 1e-10)
+    pg, rg, _, idg = g.download()
+    assert rotation_asymmetry(pos, idg, rg) <= 1e-12
+
+
+def test_twisted_column_quarter_size_properties(gpu):
+    """Twisted column at 1/4 resolution (64 x 64 x 224 cells, 1 572 864
+    particles): size-independent properties after 40 steps -- every particle
+    kept, masses unchanged (the total bit-identical), the velocity field
+    symmetric under the 90-degree rotation about the column axis, zero net
+    horizontal momentum."""
+    say("[properties] twisted column 64x64x224, 40 steps")
+    s, pos, r, idata = scenes.twisted_column(1 / 4)
+    g = capi.Context(s)
+    g.upload(pos, r, idata)
+    g.init_internal_forces()
+    g.step(40)
+    p1, r1, _, ids = g.download()
+    assert np.array_equal(ids, np.arange(pos.shape[1]))
+    assert np.isfinite(r1).all() and np.isfinite(p1).all()
+    assert r1[R_MASS].sum() == r[R_MASS].sum()
+    assert rotation_asymmetry(pos, ids, r1) <= 1e-10
+    e = g.energies()
+    scale = np.sqrt(2.0 * e[0] * e[5])
+    assert abs(e[2]) <= 1e-10 * scale and abs(e[3]) <= 1e-10 * scale
