@@ -101,6 +101,7 @@ SIGNATURES = {
     "mpm_num_particles": (ctypes.c_int, [_P, _LP]),
     "mpm_download_particles": (ctypes.c_int, [_P, _DP, _DP, _IP, _LP]),
     "mpm_get_nodal": (ctypes.c_int, [_P, ctypes.c_int, _DP]),
+    "mpm_get_particle_field": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int64, _DP]),
     "mpm_reset_nodal": (ctypes.c_int, [_P]),
     "mpm_p2g": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "mpm_backup_velocity": (ctypes.c_int, [_P]),
@@ -335,6 +336,12 @@ class Context(_Base):
 
     def set_option(self, opt, value):
         self._call("set_option", self._ctx, opt, value)
+
+    def particle_field(self, field, n_upload):
+        """Real field `field` of every particle by upload index (NaN: removed)."""
+        out = np.empty(n_upload)
+        self._call("get_particle_field", self._ctx, field, n_upload, dptr(out))
+        return out
 
     def last_step_timing(self):
         tot = ctypes.c_double(0)

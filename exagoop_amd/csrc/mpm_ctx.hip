@@ -442,6 +442,30 @@ int mpm_download_particles(mpm_ctx *c, double *pos, double *rdata, int *idata, i
     return MPM_OK;
 }
 
+int mpm_get_particle_field(mpm_ctx *c, int field, int64_t n_upload, double *out) {
+    if (!c || !out || field < 0 || field >= MPM_NREAL) return set_err(MPM_E_ARG, "bad argument");
+    if (n_upload != (int64_t)c->cold_radius.size())
+        return set_err(MPM_E_ARG, "n_upload differs from the uploaded particle count");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const int64_t np = c->P.np, cap = c->P.cap;
+    for (int64_t k = 0; k < n_upload; ++k) out[k] = std::numeric_limits<double>::quiet_NaN();
+    if (np == 0) return MPM_OK;
+    std::vector<int64_t> ids(np);
+    HIPCHK(hipMemcpy(ids.data(), c->P.id, np * sizeof(int64_t), hipMemcpyDeviceToHost));
+    if (field == MPM_R_RADIUS || field == MPM_R_YACC) {
+        const std::vector<double> &cold = field == MPM_R_RADIUS ? c->cold_radius : c->cold_yacc;
+        for (int64_t i = 0; i < np; ++i)
+            if (ids[i] >= 0) out[ids[i]] = cold[ids[i]];
+        return MPM_OK;
+    }
+    std::vector<double> h(np);
+    HIPCHK(hipMemcpy(h.data(), c->P.r + (int64_t)field * cap, np * sizeof(double),
+                     hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < np; ++i)
+        if (ids[i] >= 0) out[ids[i]] = h[i];
+    return MPM_OK;
+}
+
 int mpm_get_nodal(mpm_ctx *c, int comp, double *out) {
     if (!c || !out || comp < 0 || comp >= MPM_N_NCOMP) return set_err(MPM_E_ARG, "bad comp");
     HIPCHK(hipStreamSynchronize(c->stream));

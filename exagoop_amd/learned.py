@@ -16,10 +16,23 @@ dv_ref = rms of the training targets.
 
     python -m exagoop_amd.learned --out DIR [--train-seeds 64] [--eval-seeds 16]
 
-writes DIR/learned_dam_break.npz (parameters + scales) and DIR/learned.json
-(training losses; per held-out seed: relative L2 velocity error of the
-learned and of the uncorrected coarse run against the fine run, fallbacks,
-wall times and the speedup).
+writes DIR/learned_dam_break.npz (parameters + scales) and
+DIR/learned_dam_break.json (training losses; per held-out seed: relative L2
+velocity error of the learned and of the uncorrected coarse run against the
+fine run, fallbacks, wall times and the speedup).
+
+The twisted column (scenes.twisted_column, full size 256 x 256 x 896 cells,
+100.7 M particles, cubic) is the second problem:
+
+    python -m exagoop_amd.learned --problem column --out DIR [--res 1.0]
+
+Training seeds 0..31 and held-out seeds 1000..1003 draw the twist amplitude
+omega0 ~ U(5, 15) rad/s from numpy's default_rng(seed).  The fine run uses a
+fixed dt = 1.6e-3 dx (CFL ~ 0.075 for E = 1e6 and the fastest initial
+particle); one learned step covers NFINE = 4 of them (CFL ~ 0.3).  m_ref =
+rho dx^3, v_ref = 1 m/s.  Velocities are read back one field at a time by
+upload index (mpm_get_particle_field), so evaluating 100 M particles costs no
+full download.
 """
 import argparse
 import json
@@ -40,6 +53,74 @@ TRAIN_SEEDS = list(range(64))
 EVAL_SEEDS = list(range(1000, 1016))
 DATA_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data",
                          "learned_dam_break.npz")
+
+
+class DamBreak:
+    """The dam-break problem (module constants above)."""
+    name = "dam_break"
+
+    def __init__(self):
+        self.dt, self.m_ref, self.v_ref = DT, M_REF, V_REF
+        self.train_seeds, self.eval_seeds = TRAIN_SEEDS, EVAL_SEEDS
+        self.data_file = DATA_FILE
+
+    def scene(self, seed):
+        return scene(seed)
+
+    def describe(self, seed):
+        return {"geometry": geometry(seed)}
+
+
+class Column:
+    """The twisted column at resolution res (1 = full size)."""
+    name = "column"
+
+    def __init__(self, res=1.0):
+        self.res = res
+        self.dx = 1.0 / (128.0 * res)
+        self.dt = 1.6e-3 * self.dx
+        self.m_ref = RHO * self.dx ** 3
+        self.v_ref = 1.0
+        self.train_seeds = list(range(32))
+        self.eval_seeds = list(range(1000, 1004))
+        self.data_file = os.path.join(os.path.dirname(DATA_FILE), "learned_column.npz")
+
+    @staticmethod
+    def omega(seed):
+        return float(np.random.default_rng(seed).uniform(5.0, 15.0))
+
+    def scene(self, seed):
+        s, pos, r, idata = scenes.twisted_column(self.res, omega0=self.omega(seed))
+        s.fixed_timestep = 1
+        s.timestep = self.dt
+        return s, pos, r, idata
+
+    def describe(self, seed):
+        return {"omega0": self.omega(seed), "res": self.res}
+
+
+DAM_BREAK = DamBreak()
+
+
+def problem(name, res=1.0):
+    return DAM_BREAK if name == "dam_break" else Column(res)
+
+
+def prob_context(prob, seed):
+    """Context of problem prob's scene `seed`, uploaded and initialised; also
+    the uploaded particle count."""
+    s, pos, r, idata = prob.scene(seed)
+    n = pos.shape[1]
+    g = capi.Context(s)
+    g.upload(pos, r, idata)
+    del pos, r, idata
+    g.init_internal_forces()
+    return g, n
+
+
+def velocities_by_id(g, n):
+    """(3, n) particle velocities by upload index (NaN for removed particles)."""
+    return np.stack([g.particle_field(capi.R_VEL + d, n) for d in range(3)])
 
 
 def geometry(seed):
@@ -63,14 +144,16 @@ def context(seed, **kw):
     return g
 
 
-def collect(seeds, samples=10, spacing=40, rows=8000, capacity=6_000_000, log=print, **kw):
+def collect(seeds, samples=10, spacing=40, rows=8000, capacity=6_000_000, log=print,
+            prob=None, **kw):
     """Training rows from the plain trajectories of the given seeds: at
     `samples` points `spacing` fine steps apart, one collection of up to
     `rows` nodes each (the collection itself advances NFINE steps)."""
-    net = capi.MLP(np.zeros(capi.MLP_NPARAMS, np.float32), scales=(M_REF, V_REF, 1.0))
+    prob = prob or DAM_BREAK
+    net = capi.MLP(np.zeros(capi.MLP_NPARAMS, np.float32), scales=(prob.m_ref, prob.v_ref, 1.0))
     ds = capi.Dataset(capacity)
     for k, seed in enumerate(seeds):
-        g = context(seed, **kw)
+        g = context(seed, **kw) if prob is DAM_BREAK else prob_context(prob, seed)[0]
         for j in range(samples):
             if spacing:
                 g.step(spacing)
@@ -81,7 +164,8 @@ def collect(seeds, samples=10, spacing=40, rows=8000, capacity=6_000_000, log=pr
     return ds
 
 
-def train(ds, epochs=24, batch=4096, lr=2e-3, seed=0, log=print):
+def train(ds, epochs=24, batch=4096, lr=2e-3, seed=0, log=print, prob=None):
+    prob = prob or DAM_BREAK
     rms = ds.target_rms()
     rng = np.random.default_rng(seed)
     # He-style initialisation, zero biases
@@ -91,7 +175,7 @@ def train(ds, epochs=24, batch=4096, lr=2e-3, seed=0, log=print):
         n = fan_in * fan_out
         p[o:o + n] = (rng.standard_normal(n) * np.sqrt(1.0 / fan_in)).astype(np.float32)
         o += n + fan_out
-    net = capi.MLP(p, scales=(M_REF, V_REF, rms))
+    net = capi.MLP(p, scales=(prob.m_ref, prob.v_ref, rms))
     losses = []
     for phase_lr, phase_ep in ((lr, epochs * 2 // 3), (lr * 0.2, epochs - epochs * 2 // 3)):
         l = net.train(ds, phase_ep, batch, phase_lr, seed=seed + len(losses))
@@ -111,34 +195,49 @@ def rel_l2(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
-def evaluate(net, seeds, nsteps=400, guards=(0.05, 0.5), log=print, **kw):
+def evaluate(net, seeds, nsteps=400, guards=(0.05, 0.5), log=print, prob=None, **kw):
     """Fine plain run of nsteps vs nsteps/NFINE learned (and uncorrected)
     coarse steps from the same initial state."""
+    prob = prob or DAM_BREAK
     out = []
     for seed in seeds:
-        g = context(seed, **kw)
+        if prob is DAM_BREAK and kw:
+            g, n = context(seed, **kw), None
+        else:
+            g, n = prob_context(prob, seed)
         capi_sync(g)
         t0 = time.perf_counter()
         g.step(nsteps)
         capi_sync(g)
         t_fine = time.perf_counter() - t0
-        vf, idf = velocities(g)
+        if n is None:
+            vf, idf = velocities(g)
+        else:
+            vf = velocities_by_id(g, n)
         g.close()
-        res = {"seed": seed, "geometry": geometry(seed), "fine_s": t_fine}
+        res = {"seed": seed, "fine_s": t_fine, **prob.describe(seed)}
         for name, model in (("learned", net), ("coarse", None)):
-            g = context(seed, **kw)
+            if n is None:
+                g = context(seed, **kw)
+            else:
+                g, _ = prob_context(prob, seed)
             capi_sync(g)
             t0 = time.perf_counter()
             fb = capi.step_learned(g, model, nsteps // NFINE, NFINE, guards)
             capi_sync(g)
             res[name + "_s"] = time.perf_counter() - t0
-            v, ids = velocities(g)
+            if n is None:
+                v, ids = velocities(g)
+                same = ids.shape == idf.shape and np.array_equal(ids, idf)
+            else:
+                v = velocities_by_id(g, n)
+                same = np.array_equal(np.isnan(v), np.isnan(vf))
+                v, vf_ = np.nan_to_num(v), np.nan_to_num(vf)
             g.close()
-            if ids.shape == idf.shape and np.array_equal(ids, idf):
-                res[name + "_err"] = rel_l2(v, vf)
+            if same:
+                res[name + "_err"] = rel_l2(v, vf if n is None else vf_)
             else:
                 res[name + "_err"] = float("inf")
-                res[name + "_lost"] = int(idf.size - ids.size)
             res[name + "_fallbacks"] = fb
         res["speedup"] = res["fine_s"] / res["learned_s"]
         out.append(res)
@@ -183,8 +282,10 @@ def main(argv=None):
     ap.add_argument("--demo", action="store_true",
                     help="short run of every kernel (for profiling), no outputs")
     ap.add_argument("--out")
-    ap.add_argument("--train-seeds", type=int, default=64)
-    ap.add_argument("--eval-seeds", type=int, default=16)
+    ap.add_argument("--problem", default="dam_break", choices=["dam_break", "column"])
+    ap.add_argument("--res", type=float, default=1.0, help="column resolution (1 = full size)")
+    ap.add_argument("--train-seeds", type=int, default=None)
+    ap.add_argument("--eval-seeds", type=int, default=None)
     ap.add_argument("--epochs", type=int, default=24)
     ap.add_argument("--eval-steps", type=int, default=400)
     ap.add_argument("--samples", type=int, default=10)
@@ -197,14 +298,18 @@ def main(argv=None):
     if not a.out:
         ap.error("--out is required")
     os.makedirs(a.out, exist_ok=True)
+    prob = problem(a.problem, a.res)
+    train_seeds = prob.train_seeds[:a.train_seeds] if a.train_seeds else prob.train_seeds
+    eval_seeds = prob.eval_seeds[:a.eval_seeds] if a.eval_seeds else prob.eval_seeds
     t0 = time.perf_counter()
-    ds = collect(TRAIN_SEEDS[:a.train_seeds], samples=a.samples, spacing=a.spacing, rows=a.rows)
+    ds = collect(train_seeds, samples=a.samples, spacing=a.spacing, rows=a.rows, prob=prob)
     t1 = time.perf_counter()
-    net, losses = train(ds, epochs=a.epochs)
+    net, losses = train(ds, epochs=a.epochs, prob=prob)
     t2 = time.perf_counter()
-    save(net, os.path.join(a.out, "learned_dam_break.npz"))
-    ev = evaluate(net, EVAL_SEEDS[:a.eval_seeds], nsteps=a.eval_steps)
+    save(net, os.path.join(a.out, "learned_%s.npz" % prob.name))
+    ev = evaluate(net, eval_seeds, nsteps=a.eval_steps, prob=prob)
     summary = {
+        "problem": prob.name, "train_seeds": len(train_seeds), "fine_dt": prob.dt,
         "rows": ds.size(), "dv_ref": float(net.get_params()[1][2]),
         "collect_s": t1 - t0, "train_s": t2 - t1, "losses": losses,
         "eval": ev,
@@ -213,7 +318,7 @@ def main(argv=None):
         "mean_speedup": float(np.mean([e["speedup"] for e in ev])),
         "fallbacks": int(sum(e["learned_fallbacks"] for e in ev)),
     }
-    with open(os.path.join(a.out, "learned.json"), "w") as f:
+    with open(os.path.join(a.out, "learned_%s.json" % prob.name), "w") as f:
         json.dump(summary, f, indent=1)
     print(json.dumps({k: v for k, v in summary.items() if k not in ("eval", "losses")}),
           flush=True)

@@ -157,6 +157,11 @@ int mpm_num_particles(mpm_ctx *ctx, int64_t *np);
  * pos/rdata/idata as in upload, id[np] the upload index of each particle. */
 int mpm_download_particles(mpm_ctx *ctx, double *pos, double *rdata,
                            int *idata, int64_t *id);
+/* One real field (mpm_real_field) of every particle by upload index:
+ * out[id] for id < n_upload (the count of the last upload), NaN for removed
+ * particles.  A light read-back of the same data mpm_download_particles
+ * returns (the particle output of mpm_particle_outputs.cpp:188-334). */
+int mpm_get_particle_field(mpm_ctx *ctx, int field, int64_t n_upload, double *out);
 /* Copies nodal component comp (mpm_nodal_comp) of the valid nodal box,
  * (n0+1)(n1+1)(n2+1) values, i fastest. */
 int mpm_get_nodal(mpm_ctx *ctx, int comp, double *out);
