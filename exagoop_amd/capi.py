@@ -101,6 +101,16 @@ SIGNATURES = {
     "mpm_num_particles": (ctypes.c_int, [_P, _LP]),
     "mpm_download_particles": (ctypes.c_int, [_P, _DP, _DP, _IP, _LP]),
     "mpm_get_nodal": (ctypes.c_int, [_P, ctypes.c_int, _DP]),
+    "mpm_read_particle_file": (ctypes.c_int, [ctypes.c_char_p, _LP, _DP, _DP, _IP, _DP]),
+    "mpm_load_particle_file": (ctypes.c_int, [_P, ctypes.c_char_p, _DP]),
+    "mpm_autogen_particles": (ctypes.c_int, [ctypes.POINTER(Specs), _DP, _DP, _DP, _IP,
+                                             ctypes.c_double, ctypes.c_int, ctypes.c_double,
+                                             ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                             ctypes.c_double, ctypes.c_int, _LP, _DP, _DP, _IP,
+                                             _DP]),
+    "mpm_write_particle_file": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int64, _DP, _DP, _IP]),
+    "mpm_write_checkpoint": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.c_int]),
+    "mpm_read_checkpoint": (ctypes.c_int, [_P, ctypes.c_char_p, _IP]),
     "mpm_get_particle_field": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int64, _DP]),
     "mpm_reset_nodal": (ctypes.c_int, [_P]),
     "mpm_p2g": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
@@ -336,6 +346,21 @@ class Context(_Base):
 
     def set_option(self, opt, value):
         self._call("set_option", self._ctx, opt, value)
+
+    def write_checkpoint(self, path, output_it=0):
+        self._call("write_checkpoint", self._ctx, os.fsencode(path), output_it)
+
+    def read_checkpoint(self, path):
+        """Returns the output index stored in the checkpoint."""
+        oit = ctypes.c_int(0)
+        self._call("read_checkpoint", self._ctx, os.fsencode(path), ctypes.byref(oit))
+        return oit.value
+
+    def load_particle_file(self, path):
+        """mpm_load_particle_file; returns (total mass, total volume)."""
+        tot = np.zeros(2)
+        self._call("load_particle_file", self._ctx, os.fsencode(path), dptr(tot))
+        return float(tot[0]), float(tot[1])
 
     def particle_field(self, field, n_upload):
         """Real field `field` of every particle by upload index (NaN: removed)."""
@@ -608,3 +633,51 @@ class Batch:
         k = ctypes.c_int(0)
         self._call("stats", ctypes.byref(ms), ctypes.byref(n), ctypes.byref(k))
         return {"step_ms": ms.value, "particles": n.value, "tasks": k.value}
+
+
+def read_particle_file(path):
+    """mpm_read_particle_file: (pos, rdata, idata, (total mass, total volume))."""
+    lib = load()
+    n = ctypes.c_int64(0)
+    _check(lib, lib.mpm_read_particle_file(os.fsencode(path), ctypes.byref(n), None, None, None,
+                                           None), "mpm_read_particle_file")
+    pos = np.zeros((3, n.value))
+    rdata = np.zeros((NREAL, n.value))
+    idata = np.zeros((NINT, n.value), dtype=np.int32)
+    tot = np.zeros(2)
+    _check(lib, lib.mpm_read_particle_file(os.fsencode(path), ctypes.byref(n), dptr(pos),
+                                           dptr(rdata), iptr(idata), dptr(tot)),
+           "mpm_read_particle_file")
+    return pos, rdata, idata, (float(tot[0]), float(tot[1]))
+
+
+def autogen_particles(specs, mincoords, maxcoords, vel, ppc, dens, model, E=0.0, nu=0.0, K=0.0,
+                      gamma=0.0, visc=0.0, multi_per_cell=1):
+    """mpm_autogen_particles: (pos, rdata, idata, (total mass, total volume))."""
+    lib = load()
+    mn = np.ascontiguousarray(mincoords, dtype=np.float64)
+    mx = np.ascontiguousarray(maxcoords, dtype=np.float64)
+    v = np.ascontiguousarray(vel, dtype=np.float64)
+    pc = np.ascontiguousarray(ppc, dtype=np.int32)
+    n = ctypes.c_int64(0)
+    args = (ctypes.byref(specs), dptr(mn), dptr(mx), dptr(v), iptr(pc), dens, model, E, nu, K,
+            gamma, visc, multi_per_cell)
+    _check(lib, lib.mpm_autogen_particles(*args, ctypes.byref(n), None, None, None, None),
+           "mpm_autogen_particles")
+    pos = np.zeros((3, n.value))
+    rdata = np.zeros((NREAL, n.value))
+    idata = np.zeros((NINT, n.value), dtype=np.int32)
+    tot = np.zeros(2)
+    if n.value:
+        _check(lib, lib.mpm_autogen_particles(*args, ctypes.byref(n), dptr(pos), dptr(rdata),
+                                              iptr(idata), dptr(tot)), "mpm_autogen_particles")
+    return pos, rdata, idata, (float(tot[0]), float(tot[1]))
+
+
+def write_particle_file(path, pos, rdata, idata):
+    lib = load()
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    rdata = np.ascontiguousarray(rdata, dtype=np.float64)
+    idata = np.ascontiguousarray(idata, dtype=np.int32)
+    _check(lib, lib.mpm_write_particle_file(os.fsencode(path), pos.shape[1], dptr(pos),
+                                            dptr(rdata), iptr(idata)), "mpm_write_particle_file")

@@ -13,6 +13,10 @@
 #include <cstring>
 #include <limits>
 #include <numeric>
+#include <string>
+#include <sys/stat.h>
+#include <cerrno>
+#include <fstream>
 #include <vector>
 
 #include "mpm_context.h"
@@ -332,8 +336,16 @@ void mpm_destroy(mpm_ctx *c) {
     delete c;
 }
 
-int mpm_upload_particles(mpm_ctx *c, int64_t np, const double *pos, const double *rdata,
-                         const int *idata) {
+}  // extern "C"
+
+namespace {
+
+// Particles into a context: ids[i] the upload index of particle i (NULL:
+// 0..np-1); the cold fields by upload index, n_upload of them (NULL: taken
+// from rdata / idata, n_upload = np).
+int upload_impl(mpm_ctx *c, int64_t np, const double *pos, const double *rdata, const int *idata,
+                const int64_t *ids_in, int64_t n_upload, const double *cold_rad,
+                const double *cold_yacc, const int *cold_rigid) {
     if (!c || np < 0 || (np > 0 && (!pos || !rdata || !idata)))
         return set_err(MPM_E_ARG, "bad particle arrays");
     for (int64_t i = 0; i < np; ++i) {
@@ -361,10 +373,17 @@ int mpm_upload_particles(mpm_ctx *c, int64_t np, const double *pos, const double
     c->Q.vsnap = P.vsnap;
     P.np = np;
     c->Q.np = np;
-    c->cold_radius.assign(rdata + (size_t)MPM_R_RADIUS * np, rdata + (size_t)MPM_R_RADIUS * np + np);
-    c->cold_yacc.assign(rdata + (size_t)MPM_R_YACC * np, rdata + (size_t)MPM_R_YACC * np + np);
-    c->cold_rigid.assign(idata + (size_t)MPM_I_RIGID_ID * np,
-                         idata + (size_t)MPM_I_RIGID_ID * np + np);
+    if (cold_rad) {
+        c->cold_radius.assign(cold_rad, cold_rad + n_upload);
+        c->cold_yacc.assign(cold_yacc, cold_yacc + n_upload);
+        c->cold_rigid.assign(cold_rigid, cold_rigid + n_upload);
+    } else {
+        c->cold_radius.assign(rdata + (size_t)MPM_R_RADIUS * np,
+                              rdata + (size_t)MPM_R_RADIUS * np + np);
+        c->cold_yacc.assign(rdata + (size_t)MPM_R_YACC * np, rdata + (size_t)MPM_R_YACC * np + np);
+        c->cold_rigid.assign(idata + (size_t)MPM_I_RIGID_ID * np,
+                             idata + (size_t)MPM_I_RIGID_ID * np + np);
+    }
     c->all_dirty = true;
     c->dt_next_valid = false;
     if (np > 0) {
@@ -374,7 +393,8 @@ int mpm_upload_particles(mpm_ctx *c, int64_t np, const double *pos, const double
         HIPCHK(hipMemcpy(P.idata, idata, (int64_t)MPM_NINT * np * sizeof(int),
                          hipMemcpyHostToDevice));
         std::vector<int64_t> ids(np);
-        std::iota(ids.begin(), ids.end(), (int64_t)0);
+        if (ids_in) ids.assign(ids_in, ids_in + np);
+        else std::iota(ids.begin(), ids.end(), (int64_t)0);
         HIPCHK(hipMemcpy(P.id, ids.data(), np * sizeof(int64_t), hipMemcpyHostToDevice));
         // InitParticles ends with Redistribute() (mpm_init.cpp:1166).
         launch_redistribute(c->g, P, c->stream);
@@ -383,6 +403,121 @@ int mpm_upload_particles(mpm_ctx *c, int64_t np, const double *pos, const double
     }
     c->dirty = false;
     HIPCHK(hipStreamSynchronize(c->stream));
+    return MPM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mpm_upload_particles(mpm_ctx *c, int64_t np, const double *pos, const double *rdata,
+                         const int *idata) {
+    return upload_impl(c, np, pos, rdata, idata, nullptr, np, nullptr, nullptr, nullptr);
+}
+
+// writeCheckpointFile (mpm_particle_outputs.cpp:355-499) and WriteHeader
+// (:311-353): dir/Header as the reference writes it; the particles (the
+// reference's AMReX particle checkpoint) as dir/particles.bin:
+//   "XMPMCHK1", int64 n_live, n_upload, MPM_NREAL, MPM_NINT, then pos[3][n],
+//   rdata[MPM_NREAL][n], int32 idata[MPM_NINT][n], int64 id[n] of the live
+//   particles by upload index, then the cold fields by upload index:
+//   radius[n_upload], yacceleration[n_upload], int32 rigid_body_id[n_upload].
+int mpm_write_checkpoint(mpm_ctx *c, const char *dir, int output_it) {
+    if (!c || !dir) return set_err(MPM_E_ARG, "null argument");
+    if (mkdir(dir, 0755) != 0 && errno != EEXIST)
+        return set_err(MPM_E_ARG, "cannot create checkpoint directory");
+    double time = 0.0, last_dt = 0.0;
+    int64_t steps = 0;
+    int rc = mpm_get_time(c, &time, &steps, &last_dt);
+    if (rc) return rc;
+    {
+        std::ofstream h(std::string(dir) + "/Header", std::ios::out | std::ios::trunc);
+        if (!h.good()) return set_err(MPM_E_ARG, "cannot write checkpoint Header");
+        h.precision(17);
+        h << "Checkpoint version: 1\n" << steps << "\n" << output_it << "\n" << time << "\n";
+        if (!h.good()) return set_err(MPM_E_ARG, "cannot write checkpoint Header");
+    }
+    int64_t n = 0;
+    rc = mpm_num_particles(c, &n);
+    if (rc) return rc;
+    std::vector<double> pos(3 * n), r((size_t)MPM_NREAL * n);
+    std::vector<int> idata((size_t)MPM_NINT * n);
+    std::vector<int64_t> id(n);
+    if (n) {
+        rc = mpm_download_particles(c, pos.data(), r.data(), idata.data(), id.data());
+        if (rc) return rc;
+    }
+    std::ofstream f(std::string(dir) + "/particles.bin",
+                    std::ios::out | std::ios::trunc | std::ios::binary);
+    if (!f.good()) return set_err(MPM_E_ARG, "cannot write checkpoint particles");
+    const int64_t hdr[4] = {n, (int64_t)c->cold_radius.size(), MPM_NREAL, MPM_NINT};
+    f.write("XMPMCHK1", 8);
+    f.write(reinterpret_cast<const char *>(hdr), sizeof(hdr));
+    f.write(reinterpret_cast<const char *>(pos.data()), pos.size() * sizeof(double));
+    f.write(reinterpret_cast<const char *>(r.data()), r.size() * sizeof(double));
+    f.write(reinterpret_cast<const char *>(idata.data()), idata.size() * sizeof(int));
+    f.write(reinterpret_cast<const char *>(id.data()), id.size() * sizeof(int64_t));
+    f.write(reinterpret_cast<const char *>(c->cold_radius.data()),
+            c->cold_radius.size() * sizeof(double));
+    f.write(reinterpret_cast<const char *>(c->cold_yacc.data()),
+            c->cold_yacc.size() * sizeof(double));
+    f.write(reinterpret_cast<const char *>(c->cold_rigid.data()),
+            c->cold_rigid.size() * sizeof(int));
+    if (!f.good()) return set_err(MPM_E_ARG, "cannot write checkpoint particles");
+    return MPM_OK;
+}
+
+// readCheckpointFile (mpm_particle_outputs.cpp:537-589): step, output index
+// and time from dir/Header, the particles from dir/particles.bin.  As in the
+// reference, the caller runs Initialise_Internal_Forces next.
+int mpm_read_checkpoint(mpm_ctx *c, const char *dir, int *output_it) {
+    if (!c || !dir) return set_err(MPM_E_ARG, "null argument");
+    std::ifstream h(std::string(dir) + "/Header");
+    if (!h.good()) return set_err(MPM_E_ARG, "cannot open checkpoint Header");
+    std::string line;
+    std::getline(h, line);
+    if (line != "Checkpoint version: 1") return set_err(MPM_E_ARG, "not a checkpoint Header");
+    int64_t steps = 0;
+    int oit = 0;
+    double time = 0.0;
+    if (!(h >> steps >> oit >> time)) return set_err(MPM_E_ARG, "bad checkpoint Header");
+    std::ifstream f(std::string(dir) + "/particles.bin", std::ios::binary);
+    char magic[8];
+    int64_t hdr[4];
+    if (!f.read(magic, 8) || std::memcmp(magic, "XMPMCHK1", 8) != 0 ||
+        !f.read(reinterpret_cast<char *>(hdr), sizeof(hdr)))
+        return set_err(MPM_E_ARG, "not a checkpoint particle file");
+    const int64_t n = hdr[0], nu = hdr[1];
+    if (n < 0 || nu < n || hdr[2] != MPM_NREAL || hdr[3] != MPM_NINT)
+        return set_err(MPM_E_ARG, "checkpoint particle layout differs from this build");
+    std::vector<double> pos(3 * n), r((size_t)MPM_NREAL * n), crad(nu), cyacc(nu);
+    std::vector<int> idata((size_t)MPM_NINT * n), crig(nu);
+    std::vector<int64_t> id(n);
+    f.read(reinterpret_cast<char *>(pos.data()), pos.size() * sizeof(double));
+    f.read(reinterpret_cast<char *>(r.data()), r.size() * sizeof(double));
+    f.read(reinterpret_cast<char *>(idata.data()), idata.size() * sizeof(int));
+    f.read(reinterpret_cast<char *>(id.data()), id.size() * sizeof(int64_t));
+    f.read(reinterpret_cast<char *>(crad.data()), crad.size() * sizeof(double));
+    f.read(reinterpret_cast<char *>(cyacc.data()), cyacc.size() * sizeof(double));
+    f.read(reinterpret_cast<char *>(crig.data()), crig.size() * sizeof(int));
+    if (!f.good()) return set_err(MPM_E_ARG, "truncated checkpoint particle file");
+    for (int64_t i = 0; i < n; ++i)
+        if (id[i] < 0 || id[i] >= nu) return set_err(MPM_E_ARG, "bad particle id in checkpoint");
+    int rc = upload_impl(c, n, pos.data(), r.data(), idata.data(), id.data(), nu, crad.data(),
+                         cyacc.data(), crig.data());
+    if (rc) return rc;
+    c->steps = steps;
+    Scalars sc;
+    std::memset(&sc, 0, sizeof(sc));
+    sc.time = time;
+    sc.dtmin_bits = sc.dtmin_next_bits =
+        (unsigned long long)0x7ff0000000000000ull;  // +inf: no candidates yet
+    HIPCHK(hipMemcpy(c->sc, &sc, sizeof(Scalars), hipMemcpyHostToDevice));
+    c->time_host = time;
+    c->last_dt = 0.0;
+    c->dt_next_valid = false;
+    c->all_dirty = true;
+    if (output_it) *output_it = oit;
     return MPM_OK;
 }
 

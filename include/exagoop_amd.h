@@ -151,6 +151,51 @@ void mpm_destroy(mpm_ctx *ctx);
  * [MPM_NREAL*np], idata [MPM_NINT*np], all component-major. */
 int mpm_upload_particles(mpm_ctx *ctx, int64_t np, const double *pos,
                          const double *rdata, const int *idata);
+/* Particle input formats.  pos, rdata, idata as in mpm_upload_particles; call
+ * once with pos = NULL to get *np, then with arrays of that size.  totals
+ * (may be NULL) receives {total mass, total volume} of the phase-0 particles.
+ *
+ * InitParticles(filename) (mpm_init.cpp:942-1167): the ASCII particle file
+ * ("dim: 3", "number_of_material_points: N", a '#' header line, then per
+ * particle: phase x y z radius density vx vy vz cm_id, then E nu for models
+ * 0 and 2 or Bulk_modulus Gamma_pressure Dynamic_viscosity for model 1);
+ * volume = 4/3 pi r^3 with the reference's truncated constants, mass =
+ * density volume.  Every condition the reference aborts on is an error. */
+int mpm_read_particle_file(const char *path, int64_t *np, double *pos, double *rdata,
+                           int *idata, double totals[2]);
+/* The same, uploaded into the context (InitParticles + its Redistribute). */
+int mpm_load_particle_file(mpm_ctx *ctx, const char *path, double totals[2]);
+/* InitParticles(mincoords, maxcoords, vel, ppc, ...) with generate_particle
+ * (mpm_init.cpp:1199-1473): particles of the cells of the specs' domain (x
+ * fastest) inside [mincoords, maxcoords]; one at each cell centre, or with
+ * multi_per_cell a ppc lattice per cell, where, as in the reference's 3-D
+ * build, ppc[2] is ignored and one layer is placed along z (volume = cell
+ * volume / (ppc[0] ppc[1])).  Radius (3 V / (4 pi))^(1/3). */
+int mpm_autogen_particles(const mpm_specs *s, const double mincoords[3],
+                          const double maxcoords[3], const double vel[3], const int ppc[3],
+                          double dens, int model, double E, double nu, double K, double gamma,
+                          double visc, int multi_per_cell, int64_t *np, double *pos,
+                          double *rdata, int *idata, double totals[2]);
+/* Writes the ASCII particle file in the layout of the reference's generator
+ * (Generate_MPs_Inputfile_Generic.py:1020-1127): tab-separated, integers as
+ * such, reals as %.6e; the header's model columns are those of particle 0's
+ * model, and each row carries its own model's fields. */
+int mpm_write_particle_file(const char *path, int64_t np, const double *pos, const double *rdata,
+                            const int *idata);
+
+/* Checkpoint / restart.  mpm_write_checkpoint replaces writeCheckpointFile
+ * (mpm_particle_outputs.cpp:355-499): dir/Header in the reference's layout
+ * ("Checkpoint version: 1", step, output index, time to 17 digits; WriteHeader
+ * :311-353) and dir/particles.bin holding every particle field, the ids and
+ * the cold fields (the reference stores the particles in AMReX's checkpoint
+ * format instead).  mpm_read_checkpoint replaces readCheckpointFile
+ * (:537-589): the context takes the particles (ids kept), step count and
+ * time; as the reference's driver does (main.cpp:65-69), the caller then runs
+ * mpm_init_internal_forces.  No nodal data is stored (the reference stores
+ * none). */
+int mpm_write_checkpoint(mpm_ctx *ctx, const char *dir, int output_it);
+int mpm_read_checkpoint(mpm_ctx *ctx, const char *dir, int *output_it);
+
 /* Number of live particles (removed ones excluded). */
 int mpm_num_particles(mpm_ctx *ctx, int64_t *np);
 /* Copies live particles back in the upload's order (ids preserved):
