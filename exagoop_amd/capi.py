@@ -28,6 +28,7 @@ BC_PERIODIC, BC_NOSLIP, BC_SLIP, BC_PARTIALSLIP, BC_OUTFLOW = 0, 1, 2, 3, 4
 USL, MUSL = 0, 1
 
 OPT_PROFILE, OPT_KEEP_FORCES, OPT_SORTED, OPT_FAST, OPT_FUSED_DT = 1, 2, 3, 4, 5
+LS_SPHERE, LS_PLANE, LS_CYLINDER, LS_BOX = 0, 1, 2, 3
 MLP_NPARAMS = 11331
 MLP_IN = 128
 
@@ -110,6 +111,9 @@ SIGNATURES = {
                                              _DP]),
     "mpm_write_particle_file": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int64, _DP, _DP, _IP]),
     "mpm_write_checkpoint": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.c_int]),
+    "mpm_levelset_analytic": (ctypes.c_int, [ctypes.POINTER(Specs), ctypes.c_int, _DP,
+                                             ctypes.c_int, _DP]),
+    "mpm_remove_particles_inside_levelset": (ctypes.c_int, [_P, _LP]),
     "mpm_read_checkpoint": (ctypes.c_int, [_P, ctypes.c_char_p, _IP]),
     "mpm_get_particle_field": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int64, _DP]),
     "mpm_reset_nodal": (ctypes.c_int, [_P]),
@@ -355,6 +359,11 @@ class Context(_Base):
         oit = ctypes.c_int(0)
         self._call("read_checkpoint", self._ctx, os.fsencode(path), ctypes.byref(oit))
         return oit.value
+
+    def remove_particles_inside_levelset(self):
+        n = ctypes.c_int64(0)
+        self._call("remove_particles_inside_levelset", self._ctx, ctypes.byref(n))
+        return n.value
 
     def load_particle_file(self, path):
         """mpm_load_particle_file; returns (total mass, total volume)."""
@@ -681,3 +690,16 @@ def write_particle_file(path, pos, rdata, idata):
     idata = np.ascontiguousarray(idata, dtype=np.int32)
     _check(lib, lib.mpm_write_particle_file(os.fsencode(path), pos.shape[1], dptr(pos),
                                             dptr(rdata), iptr(idata)), "mpm_write_particle_file")
+
+
+def levelset_analytic(specs, kind, params, fluid_inside=False):
+    """mpm_levelset_analytic: signed distance on the nodal box, shape
+    (n2+1, n1+1, n0+1) (i fastest in memory)."""
+    lib = load()
+    n = specs.ncells
+    out = np.zeros((n[2] + 1, n[1] + 1, n[0] + 1))
+    prm = np.ascontiguousarray(params, dtype=np.float64)
+    _check(lib, lib.mpm_levelset_analytic(ctypes.byref(specs), kind, dptr(prm),
+                                          1 if fluid_inside else 0, dptr(out)),
+           "mpm_levelset_analytic")
+    return out

@@ -905,6 +905,27 @@ int mpm_add_levelset_body(mpm_ctx *c, const double *phi, int bc, double mu,
     return MPM_OK;
 }
 
+int mpm_remove_particles_inside_levelset(mpm_ctx *c, int64_t *removed) {
+    if (!c) return set_err(MPM_E_ARG, "null context");
+    int *d = nullptr;
+    HIPCHK(hipMalloc(&d, sizeof(int)));
+    int h = 0;
+    hipError_t e = hipMemcpy(d, &h, sizeof(int), hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        launch_remove_inside_ls(c->g, c->ls, c->P, d, c->stream);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) e = hipMemcpy(&h, d, sizeof(int), hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    if (e != hipSuccess) return hip_err(e, "remove particles inside level set");
+    // Redistribute() drops them (mpm_init.cpp:1595): every kernel skips a
+    // removed particle and the next sorted step leaves it out.
+    c->dt_next_valid = false;
+    if (removed) *removed = h;
+    return MPM_OK;
+}
+
 int mpm_set_wall_udf(mpm_ctx *c, int dir, int hi, mpm_wall_udf_fn fn) {
     if (!c || dir < 0 || dir > 2 || hi < 0 || hi > 1) return set_err(MPM_E_ARG, "bad face");
     c->udf_fn[dir][hi] = fn;

@@ -126,6 +126,9 @@ void launch_advance_time(Scalars *sc, hipStream_t s);
 void launch_energies(const PState &P, double *out6, hipStream_t s);
 void launch_snapshot_vel(const PState &P, hipStream_t s);
 void launch_redistribute(const GridP &g, const PState &P, hipStream_t s);
+// Marks particles inside any level-set body removed; adds their count to *removed.
+void launch_remove_inside_ls(const GridP &g, const LsP &ls, const PState &P, int *removed,
+                             hipStream_t s);
 
 // Sorted step (mpm_sort.hip).
 struct SortState {

@@ -172,6 +172,23 @@ __global__ void k_redistribute(GridP g, PState P) {
     P.id[i] = id;
 }
 
+// removeParticlesInsideEB (mpm_init.cpp:1520-1596): a particle where any
+// body's level set, interpolated as the level-set BC reads it, is below
+// TINYVAL (constants.H:24) is marked removed.
+__global__ __launch_bounds__(kBlock) void k_remove_inside_ls(GridP g, LsP ls, PState P,
+                                                             int *removed) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.np || P.id[i] < 0) return;
+    const double x[3] = {P.pos[i], P.pos[P.cap + i], P.pos[2 * P.cap + i]};
+    for (int b = 0; b < ls.nb; ++b) {
+        if (ls_value(g, ls.phi[b], x) < 1e-20) {
+            P.id[i] = -1;
+            atomicAdd(removed, 1);
+            break;
+        }
+    }
+}
+
 inline unsigned blocks_for(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
 }  // namespace
@@ -232,6 +249,13 @@ void launch_energies(const PState &P, double *out6, hipStream_t s) {
 void launch_snapshot_vel(const PState &P, hipStream_t s) {
     if (P.np <= 0) return;
     hipLaunchKernelGGL(k_snapshot, dim3(blocks_for(P.np)), dim3(kBlock), 0, s, P);
+}
+
+void launch_remove_inside_ls(const GridP &g, const LsP &ls, const PState &P, int *removed,
+                             hipStream_t s) {
+    if (P.np <= 0 || ls.nb <= 0) return;
+    hipLaunchKernelGGL(k_remove_inside_ls, dim3(blocks_for(P.np)), dim3(kBlock), 0, s, g, ls, P,
+                       removed);
 }
 
 void launch_redistribute(const GridP &g, const PState &P, hipStream_t s) {

@@ -274,6 +274,29 @@ int mpm_get_time(mpm_ctx *ctx, double *time, int64_t *steps, double *last_dt);
 int mpm_add_levelset_body(mpm_ctx *ctx, const double *phi, int bc, double mu,
                           const double wall_vel[3]);
 
+/* Analytic level-set bodies: the signed distance, negative inside the solid,
+ * on the valid nodal box in the layout mpm_add_levelset_body takes, of the
+ * shapes the reference builds with AMReX EB2 implicit functions
+ * (mpm_eb.cpp:212-388).  params:
+ *   MPM_LS_SPHERE   {cx, cy, cz, radius}
+ *   MPM_LS_PLANE    {px, py, pz, nx, ny, nz}  (solid behind the plane, against n)
+ *   MPM_LS_CYLINDER {cx, cy, cz, radius, height, direction}  (height <= 0: unbounded)
+ *   MPM_LS_BOX      {lo_x, lo_y, lo_z, hi_x, hi_y, hi_z}
+ * fluid_inside (the *_has_fluid_inside inputs) flips the sign: the solid is
+ * then the outside of the shape.  EB2's own signed-distance fill is AMReX
+ * code absent from the reference tree; these are the shapes' exact
+ * distances. */
+#define MPM_LS_SPHERE 0
+#define MPM_LS_PLANE 1
+#define MPM_LS_CYLINDER 2
+#define MPM_LS_BOX 3
+int mpm_levelset_analytic(const mpm_specs *s, int type, const double *params, int fluid_inside,
+                          double *phi);
+/* removeParticlesInsideEB (mpm_init.cpp:1520-1596): particles where any
+ * body's level set (interpolated as the level-set BC reads it) is below
+ * TINYVAL = 1e-20 are removed; *removed (may be NULL) is their number. */
+int mpm_remove_particles_inside_levelset(mpm_ctx *ctx, int64_t *removed);
+
 /* Wall-velocity UDF for face (dir, hi) (mpm_specs.H:452-485,
  * nodal_data_ops.cpp:929-1175, mpm_particle_timestep.cpp:417-475).  The
  * context evaluates it on the host at each BC pass as the reference does. */

@@ -104,3 +104,70 @@ def test_loaded_particle_file_steps_like_the_uploaded_arrays(gpu, tmp_path):
     pb, rb, ib, idb = b.download()
     assert np.array_equal(ida, idb) and np.array_equal(ia, ib)
     assert rel(pa, pb) <= 1e-12 and rel(ra[R_MASS], rb[R_MASS]) == 0.0
+
+
+def trilinear(phi, s, x):
+    """Nodal phi at points x (3, n) by trilinear interpolation on the nodes."""
+    n = np.array(s.ncells)
+    lo = np.array(s.plo)
+    dx = (np.array(s.phi) - lo) / n
+    f = (x - lo[:, None]) / dx[:, None]
+    i = np.clip(np.floor(f).astype(int), 0, n[:, None] - 1)
+    t = f - i
+    out = np.zeros(x.shape[1])
+    for cz in (0, 1):
+        for cy in (0, 1):
+            for cx in (0, 1):
+                w = ((t[0] if cx else 1 - t[0]) * (t[1] if cy else 1 - t[1]) *
+                     (t[2] if cz else 1 - t[2]))
+                out += w * phi[i[2] + cz, i[1] + cy, i[0] + cx]
+    return out
+
+
+def test_particles_inside_an_analytic_body_are_removed(gpu):
+    """removeParticlesInsideEB with a sphere body: the removed particles are
+    those where the interpolated level set is negative; none of the kept
+    ones lies inside the sphere by more than the interpolation error."""
+    say("[io] remove particles inside a level-set sphere")
+    s, pos, r, idata = scenes.random_block(47, n=(16, 16, 16), order=2, model=0, frac=0.8)
+    c, rad = (0.5, 0.5, 0.5), 0.22
+    phi = capi.levelset_analytic(s, capi.LS_SPHERE, [*c, rad])
+    g = capi.Context(s)
+    g.upload(pos, r, idata)
+    g.add_levelset_body(phi, capi.BC_NOSLIP, 0.0)
+    n0 = g.num_particles()
+    removed = g.remove_particles_inside_levelset()
+    expect = trilinear(phi, s, pos) < 1e-20
+    assert removed == int(expect.sum()) > 0
+    assert g.num_particles() == n0 - removed
+    p1, _, _, ids = g.download()
+    assert np.array_equal(ids, np.flatnonzero(~expect))
+    dist = np.sqrt(((p1 - np.array(c)[:, None]) ** 2).sum(axis=0))
+    assert dist.min() >= rad - 0.5 / 16  # interpolation error near the surface
+    g.init_internal_forces()
+    g.step(3)
+    assert g.num_particles() == n0 - removed
+
+
+def test_analytic_body_bc_matches_the_oracle(gpu):
+    """The level-set BC with an analytic cylinder body: GPU against the
+    oracle after 10 steps."""
+    import oracle
+    say("[io] level-set BC with an analytic cylinder, vs oracle")
+    s, pos, r, idata = scenes.random_block(53, n=(16, 12, 12), order=3, model=0, speed=1.0)
+    phi = capi.levelset_analytic(s, capi.LS_CYLINDER, [0.5, 0.375, 0.375, 0.15, 0.0, 2])
+    keep = trilinear(phi, s, pos) > 0.0
+    pos, r, idata = pos[:, keep], r[:, keep], idata[:, keep]
+    g = capi.Context(s)
+    o = oracle.Oracle(s)
+    for ctx in (g, o):
+        ctx.upload(pos, r, idata)
+        ctx.add_levelset_body(phi, capi.BC_SLIP, 0.0)
+        ctx.init_internal_forces()
+        ctx.step(10)
+    pg, rg, ig, idg = g.download()
+    po, ro, io_, ido = o.download()
+    assert np.array_equal(idg, ido)
+    assert rel(pg, po) <= 1e-10
+    for f in range(rg.shape[0]):
+        assert rel(rg[f], ro[f]) <= 1e-10, f
