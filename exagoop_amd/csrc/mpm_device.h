@@ -488,7 +488,7 @@ __device__ inline void stage_stress(const PhysP &ph, Rec &R, double dt, double t
         linear_elastic(R.eps, R.sig, R.E, R.nu);
     } else if (R.model == 1) {
         const double pinf = 0.0;
-        R.p = R.K * (pow(1.0 / R.J, R.gam) - 1.0) + pinf;
+        R.p = R.K * (ool_pow(1.0 / R.J, R.gam) - 1.0) + pinf;
         newtonian(R.D, R.sig, R.mu, R.p);
     } else {
         neo_hookean(R.F, R.sig, R.E, R.nu);

@@ -271,6 +271,12 @@ __device__ __forceinline__ void newtonian(const double D[6], double s[6], double
     s[YZ] = 2.0 * mu * D[YZ];
 }
 
+// pow and log of the constitutive updates, called out of line: inlined,
+// their expansions set the register count (and so the occupancy) of the
+// kernels that contain them; once per particle the call costs little.
+__device__ __attribute__((noinline)) double ool_pow(double x, double y) { return pow(x, y); }
+__device__ __attribute__((noinline)) double ool_log(double x) { return log(x); }
+
 // neo_hookean (constitutive_models.H:200-250), 3-D branch.
 __device__ __forceinline__ void neo_hookean(const double F[9], double s[6], double E,
                                             double v) {
@@ -290,7 +296,7 @@ __device__ __forceinline__ void neo_hookean(const double F[9], double s[6], doub
     B[YZ] = rd(1, 2);
     double det = det3(F);
     det = (det < 1.e-12) ? 1.e-12 : det;  // std::max(det, 1e-12)
-    const double vol = lam * log(det) / det;
+    const double vol = lam * ool_log(det) / det;
     const double muJ = mu / det;
     s[XX] = muJ * (B[XX] - 1.0) + vol;
     s[YY] = muJ * (B[YY] - 1.0) + vol;

@@ -915,14 +915,13 @@ __device__ __forceinline__ void particle_tile(const GridP &g, const PhysP &ph, c
                 linear_elastic(de, ds, E, nu);
                 for (int q = 0; q < 6; ++q) sig[q] = Pin.r[(MPM_R_STRESS + q) * ci + j] + ds[q];
             } else if (model == 2) {
-                // (the Tait branch goes last: its pow expansion is the widest
-                // point of the kernel and F, eps, E, nu are dead there)
+                // (the Tait branch goes last: F, eps, E, nu are dead there)
                 neo_hookean(F, sig, E, nu);
             } else if (model == 0) {
                 linear_elastic(eps, sig, E, nu);
             } else {
                 const double pinf = 0.0;
-                p = K * (pow(1.0 / J, gam) - 1.0) + pinf;
+                p = K * (ool_pow(1.0 / J, gam) - 1.0) + pinf;
                 newtonian(D, sig, mu, p);
             }
         } else {
