@@ -115,6 +115,8 @@ SIGNATURES = {
                                              ctypes.c_int, _DP]),
     "mpm_remove_particles_inside_levelset": (ctypes.c_int, [_P, _LP]),
     "mpm_read_checkpoint": (ctypes.c_int, [_P, ctypes.c_char_p, _IP]),
+    "mpm_write_ascii_particles": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.c_double,
+                                                 ctypes.c_char_p, ctypes.c_int64]),
     "mpm_get_particle_field": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int64, _DP]),
     "mpm_reset_nodal": (ctypes.c_int, [_P]),
     "mpm_p2g": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
@@ -353,6 +355,12 @@ class Context(_Base):
 
     def write_checkpoint(self, path, output_it=0):
         self._call("write_checkpoint", self._ctx, os.fsencode(path), output_it)
+
+    def write_ascii_particles(self, prefix, time):
+        """Returns the path written."""
+        buf = ctypes.create_string_buffer(4096)
+        self._call("write_ascii_particles", self._ctx, os.fsencode(prefix), time, buf, 4096)
+        return buf.value.decode()
 
     def read_checkpoint(self, path):
         """Returns the output index stored in the checkpoint."""

@@ -13,6 +13,8 @@
 #include <cstring>
 #include <limits>
 #include <numeric>
+#include <iomanip>
+#include <sstream>
 #include <string>
 #include <sys/stat.h>
 #include <cerrno>
@@ -465,6 +467,47 @@ int mpm_write_checkpoint(mpm_ctx *c, const char *dir, int output_it) {
             c->cold_rigid.size() * sizeof(int));
     if (!f.good()) return set_err(MPM_E_ARG, "cannot write checkpoint particles");
     return MPM_OK;
+}
+
+// writeAsciiFiles (mpm_particle_outputs.cpp:188-196, called with 6 digits by
+// utilities.cpp:94-99): file "<prefix>_t<time, fixed, 6 decimals>".  The
+// body stands in for AMReX's WriteAsciiFile, absent from the reference tree:
+// the particle count, then one line per live particle (upload-index order):
+// x y z id cpu, the MPM_NREAL real fields in record order, the MPM_NINT
+// integer fields; reals to 17 significant digits.
+int mpm_write_ascii_particles(mpm_ctx *c, const char *prefix, double time, char *path_out,
+                              int64_t path_len) {
+    if (!c || !prefix) return set_err(MPM_E_ARG, "null argument");
+    std::ostringstream name;
+    name << prefix << "_t" << std::fixed << std::setprecision(6) << time;
+    const std::string path = name.str();
+    if (path_out) {
+        if (path_len < (int64_t)path.size() + 1) return set_err(MPM_E_ARG, "path buffer too small");
+        std::memcpy(path_out, path.c_str(), path.size() + 1);
+    }
+    int64_t n = 0;
+    int rc = mpm_num_particles(c, &n);
+    if (rc) return rc;
+    std::vector<double> pos(3 * n), r((size_t)MPM_NREAL * n);
+    std::vector<int> idata((size_t)MPM_NINT * n);
+    std::vector<int64_t> id(n);
+    if (n) {
+        rc = mpm_download_particles(c, pos.data(), r.data(), idata.data(), id.data());
+        if (rc) return rc;
+    }
+    FILE *f = std::fopen(path.c_str(), "w");
+    if (!f) return set_err(MPM_E_ARG, "cannot open the ASCII particle file");
+    std::fprintf(f, "%lld\n", (long long)n);
+    for (int64_t i = 0; i < n; ++i) {
+        for (int d = 0; d < 3; ++d) std::fprintf(f, "%.17g ", pos[d * n + i]);
+        std::fprintf(f, "%lld 0", (long long)id[i]);
+        for (int k = 0; k < MPM_NREAL; ++k) std::fprintf(f, " %.17g", r[(size_t)k * n + i]);
+        for (int k = 0; k < MPM_NINT; ++k) std::fprintf(f, " %d", idata[(size_t)k * n + i]);
+        std::fprintf(f, "\n");
+    }
+    const bool ok = std::ferror(f) == 0;
+    std::fclose(f);
+    return ok ? MPM_OK : set_err(MPM_E_ARG, "write failed");
 }
 
 // readCheckpointFile (mpm_particle_outputs.cpp:537-589): step, output index

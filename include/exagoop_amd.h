@@ -194,6 +194,13 @@ int mpm_write_particle_file(const char *path, int64_t np, const double *pos, con
  * mpm_init_internal_forces.  No nodal data is stored (the reference stores
  * none). */
 int mpm_write_checkpoint(mpm_ctx *ctx, const char *dir, int output_it);
+/* writeAsciiFiles (mpm_particle_outputs.cpp:188-196): the live particles as
+ * text in "<prefix>_t<time with 6 decimals>" (the reference's name rule; the
+ * body's layout, AMReX's WriteAsciiFile in the reference, is the count, then
+ * per particle: x y z id cpu, the real fields, the integer fields).  The path
+ * is copied to path_out when given. */
+int mpm_write_ascii_particles(mpm_ctx *ctx, const char *prefix, double time, char *path_out,
+                              int64_t path_len);
 int mpm_read_checkpoint(mpm_ctx *ctx, const char *dir, int *output_it);
 
 /* Number of live particles (removed ones excluded). */

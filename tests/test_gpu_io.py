@@ -171,3 +171,25 @@ def test_analytic_body_bc_matches_the_oracle(gpu):
     assert rel(pg, po) <= 1e-10
     for f in range(rg.shape[0]):
         assert rel(rg[f], ro[f]) <= 1e-10, f
+
+
+def test_ascii_particle_output(gpu, tmp_path):
+    say("[io] ASCII particle output")
+    s, pos, r, idata = scenes.random_block(59, order=2, model=0)
+    g = capi.Context(s)
+    g.upload(pos, r, idata)
+    g.init_internal_forces()
+    g.step(3)
+    t = g.time()[0]
+    path = g.write_ascii_particles(str(tmp_path / "matpnt"), t)
+    assert path == str(tmp_path / "matpnt") + "_t%.6f" % t
+    lines = open(path).read().split("\n")
+    n = int(lines[0])
+    p, rr, ii, ids = g.download()
+    assert n == p.shape[1]
+    rows = np.array([[float(v) for v in l.split()] for l in lines[1:1 + n]])
+    assert rows.shape == (n, 3 + 2 + capi.NREAL + capi.NINT)
+    assert np.array_equal(rows[:, :3].T, p)  # 17 digits round-trip exactly
+    assert np.array_equal(rows[:, 3].astype(np.int64), ids)
+    assert np.array_equal(rows[:, 5:5 + capi.NREAL].T, rr)
+    assert np.array_equal(rows[:, 5 + capi.NREAL:].T.astype(np.int32), ii)
