@@ -25,6 +25,16 @@ I_PHASE, I_RIGID_ID, I_MODEL = 0, 1, 2
 N_MASS, N_VEL, N_FRC, N_DVEL, N_MASS_OLD = 0, 1, 4, 7, 10
 
 BC_PERIODIC, BC_NOSLIP, BC_SLIP, BC_PARTIALSLIP, BC_OUTFLOW = 0, 1, 2, 3, 4
+
+# Thermal module (the USE_TEMP build): particle fields (realData 46..52,
+# mpm_specs.H:52-58), nodal components (constants.H:58-63), wall conditions
+# (constants.H:95-99).
+NTHERMAL = 7
+TH_TEMPERATURE, TH_SPECIFIC_HEAT, TH_CONDUCTIVITY, TH_HEAT_FLUX, TH_HEAT_SOURCE = 0, 1, 2, 3, 6
+N_MASS_SPHEAT, N_MASS_SPHEAT_TEMP, N_TEMPERATURE = 20, 21, 22
+N_SOURCE_TEMP, N_DELTA_TEMPERATURE, N_MASS_CONDUCTIVITY = 23, 24, 25
+BC_TEMP_PERIODIC, BC_TEMP_ISOTHERMAL, BC_TEMP_ADIABATIC = 0, 1, 2
+BC_TEMP_USERFLUX, BC_TEMP_CONVECTION = 3, 4
 USL, MUSL = 0, 1
 
 OPT_PROFILE, OPT_KEEP_FORCES, OPT_SORTED, OPT_FAST, OPT_FUSED_DT = 1, 2, 3, 4, 5
@@ -61,6 +71,26 @@ class Specs(ctypes.Structure):
         s = Specs()
         ctypes.pointer(s)[0] = self
         return s
+
+
+class ThermalSpecs(ctypes.Structure):
+    """mpm_thermal_specs: per-face temperature conditions and parameters."""
+    _fields_ = [
+        ("bclo", _i3), ("bchi", _i3), ("T_wall_lo", _d3), ("T_wall_hi", _d3),
+        ("flux_lo", _d3), ("flux_hi", _d3), ("h_lo", _d3), ("h_hi", _d3),
+        ("T_inf_lo", _d3), ("T_inf_hi", _d3),
+    ]
+
+
+def default_thermal_specs(specs):
+    """Reference defaults (mpm_specs.H:227-234): periodic faces periodic,
+    the others adiabatic, all parameters zero."""
+    t = ThermalSpecs()
+    for d in range(3):
+        code = BC_TEMP_PERIODIC if specs.periodic[d] else BC_TEMP_ADIABATIC
+        t.bclo[d] = code
+        t.bchi[d] = code
+    return t
 
 
 def default_specs():
@@ -170,6 +200,10 @@ SIGNATURES = {
     "mpm_batch_autotune": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_void_p, _DP]),
     "mpm_batch_stats": (ctypes.c_int, [_P, _DP, _LP, _IP]),
     "mpm_batch_profile": (ctypes.c_int, [_P, ctypes.c_int, _DP, _DP, _LP, _DP, _DP]),
+    "mpm_thermal_specs_default": (None, [ctypes.POINTER(Specs), ctypes.POINTER(ThermalSpecs)]),
+    "mpm_enable_thermal": (ctypes.c_int, [_P, ctypes.POINTER(ThermalSpecs)]),
+    "mpm_upload_thermal": (ctypes.c_int, [_P, ctypes.c_int64, _DP]),
+    "mpm_download_thermal": (ctypes.c_int, [_P, ctypes.c_int64, _DP]),
 }
 
 
@@ -329,6 +363,27 @@ class _Base:
     def energies(self):
         out = np.zeros(6)
         self._call("energies", self._ctx, dptr(out))
+        return out
+
+    def enable_thermal(self, tspecs=None):
+        """Turn on the thermal stages; tspecs defaults to the reference's."""
+        t = default_thermal_specs(self.specs) if tspecs is None else tspecs
+        self._call("enable_thermal", self._ctx, ctypes.byref(t))
+        self._n_thermal = None
+
+    def upload_thermal(self, th):
+        """th: (NTHERMAL, n_upload) by upload index."""
+        th = np.ascontiguousarray(th, dtype=np.float64)
+        assert th.ndim == 2 and th.shape[0] == NTHERMAL
+        self._call("upload_thermal", self._ctx, th.shape[1], dptr(th))
+        self._n_thermal = th.shape[1]
+
+    def download_thermal(self, n_upload=None):
+        """(NTHERMAL, n_upload) by upload index, NaN for removed particles."""
+        n = self._n_thermal if n_upload is None else n_upload
+        assert n is not None, "pass n_upload (no thermal upload yet)"
+        out = np.zeros((NTHERMAL, n))
+        self._call("download_thermal", self._ctx, n, dptr(out))
         return out
 
 

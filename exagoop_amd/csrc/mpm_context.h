@@ -59,6 +59,16 @@ struct mpm_ctx {
     double *ml_vc = nullptr;      // coarse nodal velocity of selected rows [row][3]
     double *ml_guard = nullptr;   // guard sums
     int64_t ml_cap = 0;
+    // Thermal module (mpm_enable_thermal): particle fields in slot order,
+    // MPM_NTHERMAL * th_cap each (th_alt: the sorted step's output), nodal
+    // components 20..25 (6 * nn) and the wall-value scratch (nn).
+    int thermal = 0;
+    mpm_thermal_specs ts{};
+    double *th = nullptr;
+    double *th_alt = nullptr;
+    int64_t th_cap = 0;
+    double *tn = nullptr;
+    double *tscr = nullptr;
 };
 
 namespace xmpm {
@@ -73,6 +83,20 @@ int specs_params(const mpm_specs *s, GridP &g, PhysP &ph);
 // Every wall velocity of the specification is zero (idle node tiles then
 // hold the all-zero massless state).
 bool walls_at_rest(const mpm_specs *s);
+
+// Thermal stages (mpm_thermal.hip).  thermal_on_upload re-creates the
+// particle thermal buffers (zeroed) for new particle buffers; thermal_init is
+// the initial pass's thermal block; thermal_grid the step's nodal half (after
+// the momentum grid pass); thermal_particles the step's G2P half (before the
+// particle kernel; perm / count: the sorted order, or null); thermal_swap
+// follows the swap of the particle buffers.
+int thermal_on_upload(mpm_ctx *c);
+int thermal_init(mpm_ctx *c);
+int thermal_grid(mpm_ctx *c, hipStream_t st);
+int thermal_particles(mpm_ctx *c, bool musl, const int *perm, const int *count,
+                      hipStream_t st);
+void thermal_swap(mpm_ctx *c);
+void thermal_free(mpm_ctx *c);
 
 inline int grid_flags_full(const mpm_ctx *c) {
     return G_FINISH_MASS | G_FINISH_VEL | G_FINISH_FRC | G_BACKUP | G_UPDATE | G_BCS |

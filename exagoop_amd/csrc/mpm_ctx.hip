@@ -310,6 +310,7 @@ void mpm_destroy(mpm_ctx *c) {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     free_particles(c);
+    thermal_free(c);
     (void)hipFree(c->N.acc);
     (void)hipFree(c->N.mass);
     (void)hipFree(c->N.vel);
@@ -405,7 +406,7 @@ int upload_impl(mpm_ctx *c, int64_t np, const double *pos, const double *rdata, 
     }
     c->dirty = false;
     HIPCHK(hipStreamSynchronize(c->stream));
-    return MPM_OK;
+    return thermal_on_upload(c);
 }
 
 }  // namespace
@@ -645,11 +646,13 @@ int mpm_get_particle_field(mpm_ctx *c, int field, int64_t n_upload, double *out)
 }
 
 int mpm_get_nodal(mpm_ctx *c, int comp, double *out) {
-    if (!c || !out || comp < 0 || comp >= MPM_N_NCOMP) return set_err(MPM_E_ARG, "bad comp");
+    const int ncomp = c && c->thermal ? (int)MPM_N_NCOMP_TEMP : (int)MPM_N_NCOMP;
+    if (!c || !out || comp < 0 || comp >= ncomp) return set_err(MPM_E_ARG, "bad comp");
     HIPCHK(hipStreamSynchronize(c->stream));
     const int64_t nn = c->g.nn;
     const double *src = nullptr;
-    if (comp == MPM_N_MASS) src = c->N.mass;
+    if (comp >= MPM_N_MASS_SPHEAT) src = c->tn + (comp - MPM_N_MASS_SPHEAT) * nn;
+    else if (comp == MPM_N_MASS) src = c->N.mass;
     else if (comp >= MPM_N_VEL && comp < MPM_N_VEL + 3) src = c->N.vel + (comp - MPM_N_VEL) * nn;
     else if (comp >= MPM_N_FRC && comp < MPM_N_FRC + 3) src = c->N.frc + (comp - MPM_N_FRC) * nn;
     else if (comp >= MPM_N_DVEL && comp < MPM_N_DVEL + 3)
@@ -781,7 +784,9 @@ int mpm_init_internal_forces(mpm_ctx *c) {
                 c->stream);
     launch_particle(c->g, c->ph, c->ls, c->udf, c->P, c->N, P_G2P_GRAD | P_STRESS_INIT, 0.0, 0.0,
                     nullptr, c->stream);
-    return check_launch();
+    int rc = check_launch();
+    if (rc) return rc;
+    return thermal_init(c);
 }
 
 int mpm_step(mpm_ctx *c, int nsteps) {
@@ -828,6 +833,10 @@ int mpm_step(mpm_ctx *c, int nsteps) {
         launch_grid(c->g, c->ph, c->ls, c->udf, c->N, grid_flags_full(c), 0.0, c->sc, st, te,
                     step);
         c->all_dirty = false;
+        if (c->thermal) {
+            int rc = thermal_grid(c, st);
+            if (rc) return rc;
+        }
         if (prof) HIPCHK(hipEventRecord(c->ev[4], st));
         const int tail = P_MOVE | (redist ? P_REDIST : 0) | P_VOLUME | P_STRESS;
         int last = P_G2P_VEL | P_G2P_GRAD | tail;
@@ -841,6 +850,12 @@ int mpm_step(mpm_ctx *c, int nsteps) {
             launch_grid(c->g, c->ph, c->ls, c->udf, c->N, G_FINISH_VEL | G_BCS, 0.0, c->sc, st, te,
                         step);
             last = P_G2P_GRAD | tail;
+        }
+        if (c->thermal) {
+            int rc = thermal_particles(c, musl, sorted ? c->S.perm : nullptr,
+                                       sorted ? c->S.cell_start + c->S.nkeys : nullptr, st);
+            if (rc) return rc;
+            if (sorted) thermal_swap(c);
         }
         // The tile kernel needs every particle inside its tile: particles
         // redistributed every step.

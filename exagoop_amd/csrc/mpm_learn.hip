@@ -111,6 +111,8 @@ int learned_ok(const mpm_ctx *c) {
         return set_err(MPM_E_UNSUPPORTED,
                        "the learned step needs the specialised sorted step (redistribution "
                        "every step, uniform order, no level set or wall UDF)");
+    if (c->thermal)
+        return set_err(MPM_E_UNSUPPORTED, "the learned step has no thermal stages");
     if (c->P.np <= 0) return set_err(MPM_E_ARG, "no particles");
     return MPM_OK;
 }

@@ -1,0 +1,561 @@
+// mpm_thermal.hip — the thermal module (the USE_TEMP build's heat
+// conduction) on the device: thermal P2G, the nodal temperature update, the
+// temperature wall conditions in the reference's serial order, and thermal
+// G2P, run by mpm_step / mpm_init_internal_forces when enabled.
+//
+// Layout: the particle thermal fields (realData 46..52 of the USE_TEMP
+// build) in a component-major record of their own, stride P.cap, in the same
+// slot order as the particle state (the sorted step's G2P writes them by
+// sorted position into the second buffer, as the particle kernel does the
+// mechanical record); the nodal components 20..25 (constants.H:58-63) in one
+// array of 6 * nn, component-major.  One thread per particle or node: the
+// module is a few small HBM-bound passes next to the mechanical step.
+
+#include <cmath>
+#include <limits>
+#include <vector>
+
+#include "mpm_context.h"
+#include "mpm_device.h"
+
+using namespace xmpm;
+
+namespace {
+
+constexpr int kThreads = 256;
+// Nodal thermal components, offsets in units of nn (comps 20..25).
+enum { TN_MSP = 0, TN_MSPT = 1, TN_T = 2, TN_SRC = 3, TN_DT = 4, TN_MC = 5, TN_COMPS = 6 };
+
+inline int blocks(int64_t n) { return (int)((n + kThreads - 1) / kThreads); }
+
+// deposit_onto_grid_temperature (mpm_particle_grid_ops.cpp:656-884), one
+// thread per particle slot.  Deposits land on the valid nodes and, in
+// periodic directions, on the node one period away (the reference's ghost-
+// particle images), as the momentum deposit does.
+template <bool MT, bool SRC>
+__global__ __launch_bounds__(kThreads) void k_th_p2g(GridP g, PState P, const double *th,
+                                                     double *tn) {
+    const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (i >= P.np) return;
+    if (P.id[i] < 0) return;
+    const int64_t cap = P.cap;
+    if (P.idata[MPM_I_PHASE * cap + i] != 0) return;
+    double x[3];
+    int iv[3], mn[3], mx[3];
+    for (int d = 0; d < 3; ++d) {
+        x[d] = P.pos[d * cap + i];
+        iv[d] = (int)floor((x[d] - g.plo[d]) * g.dxi[d]);
+        bounds(iv[d], 0, g.hi[d], g.od[d], g.per[d], mn[d], mx[d]);
+    }
+    double bv[3][4], bd[3][4];
+    for (int d = 0; d < 3; ++d)
+        for (int l = mn[d]; l < mx[d]; ++l)
+            basis1d(g.od[d], g.per[d], iv[d], l, 0, g.hi[d], x[d], g.plo[d], g.dx[d],
+                    bv[d][l - mn[d]], bd[d][l - mn[d]]);
+    const double m = P.r[MPM_R_MASS * cap + i], V = P.r[MPM_R_VOLUME * cap + i];
+    const double T = th[MPM_TH_TEMPERATURE * cap + i];
+    const double cp = th[MPM_TH_SPECIFIC_HEAT * cap + i];
+    const double kc = th[MPM_TH_CONDUCTIVITY * cap + i];
+    const double Q = th[MPM_TH_HEAT_SOURCE * cap + i];
+    const double q[3] = {th[(MPM_TH_HEAT_FLUX + 0) * cap + i], th[(MPM_TH_HEAT_FLUX + 1) * cap + i],
+                         th[(MPM_TH_HEAT_FLUX + 2) * cap + i]};
+    const int64_t nn = g.nn;
+    for (int n = mn[2]; n < mx[2]; ++n) {
+        int tz[2];
+        bool iz[2];
+        const int cz = targets(iv[2] + n, g.n[2], g.per[2], tz, iz);
+        for (int mm = mn[1]; mm < mx[1]; ++mm) {
+            int ty[2];
+            bool iy[2];
+            const int cy = targets(iv[1] + mm, g.n[1], g.per[1], ty, iy);
+            for (int l = mn[0]; l < mx[0]; ++l) {
+                int tx[2];
+                bool ix[2];
+                const int cx = targets(iv[0] + l, g.n[0], g.per[0], tx, ix);
+                if (!cx || !cy || !cz) continue;
+                const double vx = bv[0][l - mn[0]], vy = bv[1][mm - mn[1]], vz = bv[2][n - mn[2]];
+                const double N = vx * vy * vz;
+                double a0 = 0.0, a1 = 0.0, a2 = 0.0, s = 0.0;
+                if (MT) {
+                    a0 = m * cp * N;
+                    a1 = m * cp * T * N;
+                    a2 = m * kc * N;
+                }
+                if (SRC) {
+                    const double gr[3] = {bd[0][l - mn[0]] * vy * vz, vx * bd[1][mm - mn[1]] * vz,
+                                          vx * vy * bd[2][n - mn[2]]};
+                    double net = 0.0;
+                    for (int d = 0; d < 3; ++d) net += q[d] * gr[d];
+                    s = net * V + V * Q * N;
+                }
+                for (int qz = 0; qz < cz; ++qz)
+                    for (int qy = 0; qy < cy; ++qy)
+                        for (int qx = 0; qx < cx; ++qx) {
+                            const int64_t node = nidx(g, tx[qx], ty[qy], tz[qz]);
+                            if (MT) {
+                                unsafeAtomicAdd(&tn[TN_MSP * nn + node], a0);
+                                unsafeAtomicAdd(&tn[TN_MSPT * nn + node], a1);
+                                unsafeAtomicAdd(&tn[TN_MC * nn + node], a2);
+                            }
+                            if (SRC) unsafeAtomicAdd(&tn[TN_SRC * nn + node], s);
+                        }
+            }
+        }
+    }
+}
+
+// FabArray::SumBoundary over periodic direction d (AMReX, called at
+// mpm_particle_grid_ops.cpp:848-853): the two face nodes of each periodic
+// pair both take their sum.  One thread per pair; directions in turn.
+__global__ __launch_bounds__(kThreads) void k_th_sum_boundary(GridP g, double *tn, int d,
+                                                              int mass_temp, int source) {
+    const int a = d == 0 ? 1 : 0, b = d == 2 ? 1 : 2;
+    const int64_t na = g.n[a] + 1, nb = g.n[b] + 1;
+    const int64_t t = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (t >= na * nb) return;
+    int X[3];
+    X[a] = (int)(t % na);
+    X[b] = (int)(t / na);
+    X[d] = 0;
+    const int64_t lo = nidx(g, X[0], X[1], X[2]);
+    X[d] = g.n[d];
+    const int64_t hi = nidx(g, X[0], X[1], X[2]);
+    const int64_t nn = g.nn;
+    const int comps[4] = {TN_MSP, TN_MSPT, TN_MC, TN_SRC};
+    for (int c = 0; c < 4; ++c) {
+        if (c < 3 ? !mass_temp : !source) continue;
+        double *v = tn + comps[c] * nn;
+        const double s = v[lo] + v[hi];
+        v[lo] = s;
+        v[hi] = s;
+    }
+}
+
+// The nodal passes between the deposit and the wall conditions, fused per
+// node: the deposit's normalisation (mpm_particle_grid_ops.cpp:857-881; a
+// node without thermal mass keeps its temperature), backup_current_temperature
+// (nodal_data_ops.cpp:95-114) and, in a time step,
+// Nodal_Time_Update_Temperature (nodal_data_ops.cpp:500-527).
+template <bool UPDATE>
+__global__ __launch_bounds__(kThreads) void k_th_nodal(GridP g, double tol, double *tn,
+                                                       const Scalars *sc) {
+    const int64_t q = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    const int64_t nn = g.nn;
+    if (q >= nn) return;
+    const double msp = tn[TN_MSP * nn + q];
+    double T = tn[TN_T * nn + q];
+    if (msp > 0.0) {
+        T = (msp >= tol) ? tn[TN_MSPT * nn + q] / msp : 0.0;
+        tn[TN_DT * nn + q] = T;
+    }
+    if (UPDATE) {
+        if (msp >= tol) T += tn[TN_SRC * nn + q] / msp * sc->dt;
+        else T = 0.0;
+    }
+    tn[TN_T * nn + q] = T;
+}
+
+__device__ __forceinline__ bool on_face(const GridP &g, int I, int J, int K) {
+    return I == 0 || I == g.n[0] || J == 0 || J == g.n[1] || K == 0 || K == g.n[2];
+}
+
+// nodal_bcs_temperature (nodal_data_ops.cpp:786-910): the value a wall node
+// ends with when a serial ParallelFor sweeps the nodal box (k outermost, i
+// fastest) updating the temperature in place.  A node reads its inward
+// neighbour's value: the updated one when the neighbour comes earlier in the
+// sweep (a hi-face neighbour that is itself a wall node), else the value
+// before the pass.  An updated neighbour lies on fewer hi faces than the node
+// (its index in that direction is n - 1), so the recursion is at most three
+// deep.  T holds the values before the pass.
+template <int D>
+__device__ double th_wall(const GridP &g, const mpm_thermal_specs &t, const double *T,
+                          const double *MC, const double *M, int I, int J, int K) {
+    const int X[3] = {I, J, K};
+    const int64_t q = nidx(g, I, J, K);
+    double v = T[q];
+    bool applied = false;
+    for (int d = 0; d < 3; ++d) {
+        const bool lo = X[d] == 0, hi = X[d] == g.n[d];
+        if (!lo && !hi) continue;
+        const int bc = lo ? t.bclo[d] : t.bchi[d];
+        if (bc == MPM_BC_TEMP_PERIODIC) {
+            applied = true;
+            continue;
+        }
+        if (bc == MPM_BC_TEMP_ISOTHERMAL) {
+            v = lo ? t.T_wall_lo[d] : t.T_wall_hi[d];
+            applied = true;
+            continue;
+        }
+        if (bc < MPM_BC_TEMP_ADIABATIC || bc > MPM_BC_TEMP_CONVECTION) continue;
+        if (bc == MPM_BC_TEMP_ADIABATIC && applied) continue;
+        int Y[3] = {I, J, K};
+        Y[d] += lo ? 1 : -1;
+        double tnb = T[nidx(g, Y[0], Y[1], Y[2])];
+        if constexpr (D > 0) {
+            if (hi && on_face(g, Y[0], Y[1], Y[2]))
+                tnb = th_wall<D - 1>(g, t, T, MC, M, Y[0], Y[1], Y[2]);
+        }
+        if (bc == MPM_BC_TEMP_ADIABATIC) {
+            v = tnb;
+        } else {
+            const double mk = MC[q], m = M[q];
+            const double k_node = (m > 0.0) ? mk / m : 1.0;
+            if (bc == MPM_BC_TEMP_USERFLUX) {
+                const double fl = lo ? t.flux_lo[d] : t.flux_hi[d];
+                v = tnb + fl * g.dx[d] / k_node;
+            } else {
+                const double hc = lo ? t.h_lo[d] : t.h_hi[d];
+                const double tinf = lo ? t.T_inf_lo[d] : t.T_inf_hi[d];
+                const double Bi = hc * g.dx[d] / k_node;
+                v = (tnb + Bi * tinf) / (1.0 + Bi);
+            }
+        }
+        applied = true;
+    }
+    return v;
+}
+
+__device__ __forceinline__ void node_ijk(const GridP &g, int64_t q, int &I, int &J, int &K) {
+    const int64_t n0 = g.n[0] + 1, n1 = g.n[1] + 1;
+    I = (int)(q % n0);
+    J = (int)((q / n0) % n1);
+    K = (int)(q / (n0 * n1));
+}
+
+// Wall values into scr (wall nodes only); the temperatures stay untouched
+// until every wall value is known.
+__global__ __launch_bounds__(kThreads) void k_th_walls(GridP g, mpm_thermal_specs t,
+                                                       const double *tn, const double *mass,
+                                                       double *scr) {
+    const int64_t q = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (q >= g.nn) return;
+    int I, J, K;
+    node_ijk(g, q, I, J, K);
+    if (!on_face(g, I, J, K)) return;
+    const int64_t nn = g.nn;
+    scr[q] = th_wall<3>(g, t, tn + TN_T * nn, tn + TN_MC * nn, mass, I, J, K);
+}
+
+// The wall values into the temperature and, when delta is set,
+// store_delta_temperature (nodal_data_ops.cpp:409-430).
+__global__ __launch_bounds__(kThreads) void k_th_apply_walls(GridP g, double *tn,
+                                                             const double *scr, int delta) {
+    const int64_t q = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    const int64_t nn = g.nn;
+    if (q >= nn) return;
+    int I, J, K;
+    node_ijk(g, q, I, J, K);
+    double T = tn[TN_T * nn + q];
+    if (on_face(g, I, J, K)) {
+        T = scr[q];
+        tn[TN_T * nn + q] = T;
+    }
+    if (delta && tn[TN_MSP * nn + q] > 0.0) tn[TN_DT * nn + q] = T - tn[TN_DT * nn + q];
+}
+
+// interpolate_from_grid_temperature (mpm_particle_grid_ops.cpp:1129-1270):
+// temperature += the nodal change by bilin_interp / quadratic_interp /
+// cubic_interp (interpolants.H:513-565, 680-862, 1029-1210, chosen by the
+// x-direction order for all directions); heat flux = -k grad T.  Thread i
+// reads slot perm[i] (the sorted step) or i, and writes out slot i.
+template <bool UT, bool UQ>
+__global__ __launch_bounds__(kThreads) void k_th_g2p(GridP g, PState P, const int *perm,
+                                                     const int *count, const double *thin,
+                                                     double *thout, const double *tn) {
+    const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    const int64_t n = count ? (int64_t)*count : P.np;
+    if (i >= n) return;
+    const int64_t j = perm ? (int64_t)perm[i] : i;
+    const int64_t cap = P.cap, nn = g.nn;
+    double f[MPM_NTHERMAL];
+    for (int c = 0; c < MPM_NTHERMAL; ++c) f[c] = thin[c * cap + j];
+    if (P.id[j] >= 0 && P.idata[MPM_I_PHASE * cap + j] == 0) {
+        double x[3];
+        int iv[3], mn[3], mx[3];
+        for (int d = 0; d < 3; ++d) {
+            x[d] = P.pos[d * cap + j];
+            iv[d] = (int)floor((x[d] - g.plo[d]) * g.dxi[d]);
+            bounds(iv[d], 0, g.hi[d], g.od[d], g.per[d], mn[d], mx[d]);
+        }
+        if (UT) {
+            const double *DT = tn + TN_DT * nn;
+            const int O = g.od[0];
+            double value;
+            if (O == 1) {
+                const int ii = iv[0], jj = iv[1], kk = iv[2];
+                const double a = (x[0] - (g.plo[0] + ii * g.dx[0])) / g.dx[0];
+                const double b = (x[1] - (g.plo[1] + jj * g.dx[1])) / g.dx[1];
+                const double c = (x[2] - (g.plo[2] + kk * g.dx[2])) / g.dx[2];
+                value = (1.0 - a) * (1.0 - b) * (1.0 - c) * nodal_read(g, DT, ii, jj, kk);
+                value += a * (1.0 - b) * (1.0 - c) * nodal_read(g, DT, ii + 1, jj, kk);
+                value += (1.0 - a) * b * (1.0 - c) * nodal_read(g, DT, ii, jj + 1, kk);
+                value += a * b * (1.0 - c) * nodal_read(g, DT, ii + 1, jj + 1, kk);
+                value += (1.0 - a) * (1.0 - b) * c * nodal_read(g, DT, ii, jj, kk + 1);
+                value += a * (1.0 - b) * c * nodal_read(g, DT, ii + 1, jj, kk + 1);
+                value += (1.0 - a) * b * c * nodal_read(g, DT, ii, jj + 1, kk + 1);
+                value += a * b * c * nodal_read(g, DT, ii + 1, jj + 1, kk + 1);
+            } else {
+                double wv[3][4];
+                for (int d = 0; d < 3; ++d)
+                    for (int l = mn[d]; l < mx[d]; ++l)
+                        wv[d][l - mn[d]] =
+                            interp1d(O, g.per[d], iv[d] + l, 0, g.hi[d], x[d], g.plo[d], g.dx[d]);
+                value = 0.0;
+                for (int nz = mn[2]; nz < mx[2]; ++nz)
+                    for (int ny = mn[1]; ny < mx[1]; ++ny)
+                        for (int nx = mn[0]; nx < mx[0]; ++nx)
+                            value += wv[0][nx - mn[0]] * wv[1][ny - mn[1]] * wv[2][nz - mn[2]] *
+                                     nodal_read(g, DT, iv[0] + nx, iv[1] + ny, iv[2] + nz);
+            }
+            f[MPM_TH_TEMPERATURE] += value;
+        }
+        if (UQ) {
+            const double *T = tn + TN_T * nn;
+            double bv[3][4], bd[3][4];
+            for (int d = 0; d < 3; ++d)
+                for (int l = mn[d]; l < mx[d]; ++l)
+                    basis1d(g.od[d], g.per[d], iv[d], l, 0, g.hi[d], x[d], g.plo[d], g.dx[d],
+                            bv[d][l - mn[d]], bd[d][l - mn[d]]);
+            double gT[3] = {0.0, 0.0, 0.0};
+            for (int nz = mn[2]; nz < mx[2]; ++nz)
+                for (int ny = mn[1]; ny < mx[1]; ++ny)
+                    for (int nx = mn[0]; nx < mx[0]; ++nx) {
+                        const double vx = bv[0][nx - mn[0]], vy = bv[1][ny - mn[1]],
+                                     vz = bv[2][nz - mn[2]];
+                        const double gr[3] = {bd[0][nx - mn[0]] * vy * vz,
+                                              vx * bd[1][ny - mn[1]] * vz,
+                                              vx * vy * bd[2][nz - mn[2]]};
+                        const double Tn = nodal_read(g, T, iv[0] + nx, iv[1] + ny, iv[2] + nz);
+                        for (int d = 0; d < 3; ++d) gT[d] += Tn * gr[d];
+                    }
+            for (int d = 0; d < 3; ++d)
+                f[MPM_TH_HEAT_FLUX + d] = -f[MPM_TH_CONDUCTIVITY] * gT[d];
+        }
+    }
+    for (int c = 0; c < MPM_NTHERMAL; ++c) thout[c * cap + i] = f[c];
+}
+
+void launch_th_p2g(mpm_ctx *c, bool mass_temp, bool source, hipStream_t st) {
+    const GridP &g = c->g;
+    const int64_t np = c->P.np;
+    if (np > 0) {
+        if (mass_temp && source)
+            k_th_p2g<true, true><<<blocks(np), kThreads, 0, st>>>(g, c->P, c->th, c->tn);
+        else if (mass_temp)
+            k_th_p2g<true, false><<<blocks(np), kThreads, 0, st>>>(g, c->P, c->th, c->tn);
+        else if (source)
+            k_th_p2g<false, true><<<blocks(np), kThreads, 0, st>>>(g, c->P, c->th, c->tn);
+    }
+    for (int d = 0; d < 3; ++d) {
+        if (!g.per[d]) continue;
+        const int a = d == 0 ? 1 : 0, b = d == 2 ? 1 : 2;
+        const int64_t pairs = (int64_t)(g.n[a] + 1) * (g.n[b] + 1);
+        k_th_sum_boundary<<<blocks(pairs), kThreads, 0, st>>>(g, c->tn, d, mass_temp ? 1 : 0,
+                                                              source ? 1 : 0);
+    }
+}
+
+void launch_th_walls(mpm_ctx *c, bool delta, hipStream_t st) {
+    const int64_t nn = c->g.nn;
+    k_th_walls<<<blocks(nn), kThreads, 0, st>>>(c->g, c->ts, c->tn, c->N.mass, c->tscr);
+    k_th_apply_walls<<<blocks(nn), kThreads, 0, st>>>(c->g, c->tn, c->tscr, delta ? 1 : 0);
+}
+
+void launch_th_g2p(mpm_ctx *c, bool ut, bool uq, const int *perm, const int *count,
+                   double *out, hipStream_t st) {
+    const int64_t np = c->P.np;
+    if (np <= 0) return;
+    if (ut && uq)
+        k_th_g2p<true, true><<<blocks(np), kThreads, 0, st>>>(c->g, c->P, perm, count, c->th, out,
+                                                              c->tn);
+    else if (ut)
+        k_th_g2p<true, false><<<blocks(np), kThreads, 0, st>>>(c->g, c->P, perm, count, c->th,
+                                                               out, c->tn);
+    else
+        k_th_g2p<false, true><<<blocks(np), kThreads, 0, st>>>(c->g, c->P, perm, count, c->th,
+                                                               out, c->tn);
+}
+
+int check_specs(const mpm_thermal_specs *t) {
+    for (int d = 0; d < 3; ++d)
+        if (t->bclo[d] < MPM_BC_TEMP_PERIODIC || t->bclo[d] > MPM_BC_TEMP_CONVECTION ||
+            t->bchi[d] < MPM_BC_TEMP_PERIODIC || t->bchi[d] > MPM_BC_TEMP_CONVECTION)
+            return set_err(MPM_E_BC, "temperature BC code must be 0..4");
+    return MPM_OK;
+}
+
+// Particle thermal buffers for the current capacity, zeroed.
+int alloc_particle_thermal(mpm_ctx *c) {
+    (void)hipFree(c->th);
+    (void)hipFree(c->th_alt);
+    c->th = c->th_alt = nullptr;
+    const int64_t cap = c->P.cap > 0 ? c->P.cap : 1;
+    const size_t bytes = (size_t)MPM_NTHERMAL * cap * sizeof(double);
+    if (hipMalloc(&c->th, bytes) != hipSuccess || hipMalloc(&c->th_alt, bytes) != hipSuccess)
+        return set_err(MPM_E_OOM, "thermal particle buffers");
+    launch_zero(c->th, (int64_t)MPM_NTHERMAL * cap, c->stream);
+    launch_zero(c->th_alt, (int64_t)MPM_NTHERMAL * cap, c->stream);
+    c->th_cap = cap;
+    return check_launch();
+}
+
+// Upload index of every particle slot (host copy).
+int slot_ids(mpm_ctx *c, std::vector<int64_t> &ids) {
+    ids.resize(c->P.np);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (c->P.np > 0)
+        HIPCHK(hipMemcpy(ids.data(), c->P.id, c->P.np * sizeof(int64_t), hipMemcpyDeviceToHost));
+    return MPM_OK;
+}
+
+}  // namespace
+
+namespace xmpm {
+
+int thermal_on_upload(mpm_ctx *c) {
+    if (!c->thermal) return MPM_OK;
+    int rc = alloc_particle_thermal(c);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return MPM_OK;
+}
+
+// The thermal block of the initial pass (mpm_init.cpp:417-441) on freshly
+// zeroed thermal nodal data (mpm_init.cpp:280-281): deposit (mass terms, no
+// source), backup, walls, delta, G2P of temperature and heat flux.
+int thermal_init(mpm_ctx *c) {
+    if (!c->thermal) return MPM_OK;
+    hipStream_t st = c->stream;
+    const int64_t nn = c->g.nn;
+    launch_zero(c->tn, TN_COMPS * nn, st);
+    launch_th_p2g(c, true, false, st);
+    k_th_nodal<false><<<blocks(nn), kThreads, 0, st>>>(c->g, c->s.mass_tolerance, c->tn, nullptr);
+    launch_th_walls(c, true, st);
+    launch_th_g2p(c, true, true, nullptr, nullptr, c->th, st);
+    return check_launch();
+}
+
+// The grid half of a step's thermal stages (main.cpp:99-105), after the
+// momentum grid pass has finished the nodal mass: Reset_Nodaldata_to_Zero's
+// share of the thermal components, deposit, normalise + backup + update,
+// walls, delta.
+int thermal_grid(mpm_ctx *c, hipStream_t st) {
+    const int64_t nn = c->g.nn;
+    launch_zero(c->tn, TN_COMPS * nn, st);
+    launch_th_p2g(c, true, true, st);
+    k_th_nodal<true><<<blocks(nn), kThreads, 0, st>>>(c->g, c->s.mass_tolerance, c->tn, c->sc);
+    launch_th_walls(c, true, st);
+    return check_launch();
+}
+
+// The particle half, before the particle kernel moves the particles:
+// USL G2P of temperature and heat flux (main.cpp:112-114); MUSL G2P of
+// temperature, the source-only re-deposit, the walls again and G2P of the
+// heat flux (main.cpp:130-135).  With perm (the sorted step) the last G2P
+// writes the second buffer by sorted position; thermal_swap follows the
+// particle buffers' swap.
+int thermal_particles(mpm_ctx *c, bool musl, const int *perm, const int *count,
+                      hipStream_t st) {
+    double *out = perm ? c->th_alt : c->th;
+    if (!musl) {
+        launch_th_g2p(c, true, true, perm, count, out, st);
+        return check_launch();
+    }
+    const int64_t nn = c->g.nn;
+    launch_th_g2p(c, true, false, nullptr, nullptr, c->th, st);
+    // P2G_Temperature(reset = 1, mass_temp = 0, source = 1): the reset
+    // clears the thermal mass, its temperature sum and the source.
+    launch_zero(c->tn + TN_MSP * nn, 2 * nn, st);
+    launch_zero(c->tn + TN_SRC * nn, nn, st);
+    launch_th_p2g(c, false, true, st);
+    launch_th_walls(c, false, st);
+    launch_th_g2p(c, false, true, perm, count, out, st);
+    return check_launch();
+}
+
+void thermal_swap(mpm_ctx *c) { std::swap(c->th, c->th_alt); }
+
+void thermal_free(mpm_ctx *c) {
+    (void)hipFree(c->th);
+    (void)hipFree(c->th_alt);
+    (void)hipFree(c->tn);
+    (void)hipFree(c->tscr);
+    c->th = c->th_alt = c->tn = c->tscr = nullptr;
+}
+
+}  // namespace xmpm
+
+extern "C" {
+
+void mpm_thermal_specs_default(const mpm_specs *s, mpm_thermal_specs *t) {
+    if (!t) return;
+    *t = mpm_thermal_specs{};
+    for (int d = 0; d < 3; ++d) {
+        const int code = (s && s->periodic[d]) ? MPM_BC_TEMP_PERIODIC : MPM_BC_TEMP_ADIABATIC;
+        t->bclo[d] = code;
+        t->bchi[d] = code;
+    }
+}
+
+int mpm_enable_thermal(mpm_ctx *c, const mpm_thermal_specs *t) {
+    if (!c || !t) return set_err(MPM_E_ARG, "null argument");
+    int rc = check_specs(t);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const int64_t nn = c->g.nn;
+    if (!c->tn) {
+        if (hipMalloc(&c->tn, TN_COMPS * nn * sizeof(double)) != hipSuccess ||
+            hipMalloc(&c->tscr, nn * sizeof(double)) != hipSuccess)
+            return set_err(MPM_E_OOM, "thermal nodal buffers");
+    }
+    launch_zero(c->tn, TN_COMPS * nn, c->stream);
+    launch_zero(c->tscr, nn, c->stream);
+    c->ts = *t;
+    c->thermal = 1;
+    rc = alloc_particle_thermal(c);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return check_launch();
+}
+
+int mpm_upload_thermal(mpm_ctx *c, int64_t n, const double *th) {
+    if (!c || n < 0 || (n > 0 && !th)) return set_err(MPM_E_ARG, "bad argument");
+    if (!c->thermal) return set_err(MPM_E_ARG, "thermal module not enabled");
+    if (n != (int64_t)c->cold_radius.size())
+        return set_err(MPM_E_ARG, "n_upload must be the count of the last particle upload");
+    std::vector<int64_t> ids;
+    int rc = slot_ids(c, ids);
+    if (rc) return rc;
+    const int64_t cap = c->th_cap, np = c->P.np;
+    std::vector<double> h((size_t)MPM_NTHERMAL * cap, 0.0);
+    for (int64_t s = 0; s < np; ++s) {
+        const int64_t id = ids[s];
+        if (id < 0) continue;
+        for (int f = 0; f < MPM_NTHERMAL; ++f) h[(size_t)f * cap + s] = th[(size_t)f * n + id];
+    }
+    HIPCHK(hipMemcpy(c->th, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+    return MPM_OK;
+}
+
+int mpm_download_thermal(mpm_ctx *c, int64_t n, double *th) {
+    if (!c || n < 0 || (n > 0 && !th)) return set_err(MPM_E_ARG, "bad argument");
+    if (!c->thermal) return set_err(MPM_E_ARG, "thermal module not enabled");
+    if (n != (int64_t)c->cold_radius.size())
+        return set_err(MPM_E_ARG, "n_upload must be the count of the last particle upload");
+    std::vector<int64_t> ids;
+    int rc = slot_ids(c, ids);
+    if (rc) return rc;
+    const int64_t cap = c->th_cap, np = c->P.np;
+    std::vector<double> h((size_t)MPM_NTHERMAL * cap);
+    HIPCHK(hipMemcpy(h.data(), c->th, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+    std::fill(th, th + (size_t)MPM_NTHERMAL * n, std::numeric_limits<double>::quiet_NaN());
+    for (int64_t s = 0; s < np; ++s) {
+        const int64_t id = ids[s];
+        if (id < 0 || id >= n) continue;
+        for (int f = 0; f < MPM_NTHERMAL; ++f) th[(size_t)f * n + id] = h[(size_t)f * cap + s];
+    }
+    return MPM_OK;
+}
+
+}  // extern "C"

@@ -214,8 +214,9 @@ int mpm_download_particles(mpm_ctx *ctx, double *pos, double *rdata,
  * particles.  A light read-back of the same data mpm_download_particles
  * returns (the particle output of mpm_particle_outputs.cpp:188-334). */
 int mpm_get_particle_field(mpm_ctx *ctx, int field, int64_t n_upload, double *out);
-/* Copies nodal component comp (mpm_nodal_comp) of the valid nodal box,
- * (n0+1)(n1+1)(n2+1) values, i fastest. */
+/* Copies nodal component comp (mpm_nodal_comp; mpm_nodal_thermal_comp with
+ * the thermal module enabled) of the valid nodal box, (n0+1)(n1+1)(n2+1)
+ * values, i fastest. */
 int mpm_get_nodal(mpm_ctx *ctx, int comp, double *out);
 
 /* ------------------------------------------------------------------------ */
@@ -475,6 +476,69 @@ int mpm_batch_profile(mpm_batch *b, int nsteps, double busy_ms[6], double wait_m
 /* Milliseconds per step of the last mpm_batch_step call (HIP events), the
  * particle slots of all scenes and the tasks of one step. */
 int mpm_batch_stats(mpm_batch *b, double *step_ms, int64_t *particles, int *tasks);
+
+/* ------------------------------------------------------------------------ */
+/* Thermal module: the USE_TEMP build's heat conduction (a context option).  */
+/* ------------------------------------------------------------------------ */
+/* Particle thermal fields, realData 46..52 of the USE_TEMP build
+ * (mpm_specs.H:52-58).  They travel in a record of their own (component-
+ * major, by upload index) so the 46-field mechanical record is unchanged. */
+#define MPM_NTHERMAL 7
+enum mpm_thermal_field {
+    MPM_TH_TEMPERATURE = 0,
+    MPM_TH_SPECIFIC_HEAT = 1,
+    MPM_TH_CONDUCTIVITY = 2,
+    MPM_TH_HEAT_FLUX = 3, /* 3..5 */
+    MPM_TH_HEAT_SOURCE = 6
+};
+
+/* Nodal components of the USE_TEMP build (constants.H:31, 58-63), readable
+ * with mpm_get_nodal once the module is enabled. */
+enum mpm_nodal_thermal_comp {
+    MPM_N_MASS_SPHEAT = 20,      /* sum m c_p N */
+    MPM_N_MASS_SPHEAT_TEMP = 21, /* sum m c_p T N */
+    MPM_N_TEMPERATURE = 22,
+    MPM_N_SOURCE_TEMP = 23,
+    MPM_N_DELTA_TEMPERATURE = 24,
+    MPM_N_MASS_CONDUCTIVITY = 25, /* sum m k N */
+    MPM_N_NCOMP_TEMP = 26
+};
+
+/* Temperature wall conditions (constants.H:95-99). */
+enum mpm_bc_temp {
+    MPM_BC_TEMP_PERIODIC = 0,
+    MPM_BC_TEMP_ISOTHERMAL = 1, /* T = T_wall */
+    MPM_BC_TEMP_ADIABATIC = 2,  /* T = T of the inward neighbour */
+    MPM_BC_TEMP_USERFLUX = 3,   /* T = T_nb + flux dx / k_node */
+    MPM_BC_TEMP_CONVECTION = 4  /* T = (T_nb + Bi T_inf) / (1 + Bi), Bi = h dx / k_node */
+};
+
+/* The temperature-BC subset of MPMspecs (mpm_specs.H:166-176, 227-234,
+ * 488-560): per face type and parameters. */
+typedef struct mpm_thermal_specs {
+    int bclo[3];
+    int bchi[3];
+    double T_wall_lo[3], T_wall_hi[3];
+    double flux_lo[3], flux_hi[3];
+    double h_lo[3], h_hi[3];
+    double T_inf_lo[3], T_inf_hi[3];
+} mpm_thermal_specs;
+
+/* Reference defaults (mpm_specs.H:227-234, 166-176): periodic faces
+ * MPM_BC_TEMP_PERIODIC, all others MPM_BC_TEMP_ADIABATIC, parameters 0. */
+void mpm_thermal_specs_default(const mpm_specs *s, mpm_thermal_specs *t);
+
+/* Turns on the thermal stages of mpm_init_internal_forces and mpm_step
+ * (main.cpp:99-105, 112-116, 130-135; mpm_init.cpp:417-441) with these wall
+ * conditions; the thermal fields of all particles start at zero.  Level-set
+ * bodies carry no temperature condition (the reference's default). */
+int mpm_enable_thermal(mpm_ctx *ctx, const mpm_thermal_specs *t);
+/* Thermal fields by upload index: th[f * n_upload + i] for field f of the
+ * particle uploaded at index i (n_upload = count of the last upload; the
+ * reader's T, spheat, thermcond, heatsrc columns, mpm_init.cpp:1097-1108). */
+int mpm_upload_thermal(mpm_ctx *ctx, int64_t n_upload, const double *th);
+/* Same layout; NaN for particles removed since the upload. */
+int mpm_download_thermal(mpm_ctx *ctx, int64_t n_upload, double *th);
 
 #ifdef __cplusplus
 }

@@ -302,6 +302,7 @@ struct Part {
     double r[MPM_NREAL];
     int idata[MPM_NINT];
     int64_t id;  // upload index; < 0 means removed
+    double th[MPM_NTHERMAL];  // realData 46..52 of the USE_TEMP build
 };
 
 struct LsBody {
@@ -341,6 +342,8 @@ struct orc_ctx {
     double last_dt = 0.0;
     int64_t steps = 0;
     int nthreads = 1;
+    bool thermal = false;
+    mpm_thermal_specs ts;
 
     inline double &A(int i, int j, int k, int c) {
         size_t idx = (size_t)c * N[0] * N[1] * N[2] +
@@ -491,7 +494,7 @@ void fill_boundary(orc_ctx *c) {
                     }
                 }
                 if (!ghost || !ok) continue;
-                for (int comp = 0; comp < MPM_N_NCOMP; ++comp)
+                for (int comp = 0; comp < MPM_N_NCOMP_TEMP; ++comp)
                     c->A(i, j, k, comp) = c->A(S[0], S[1], S[2], comp);
             }
 }
@@ -564,7 +567,9 @@ int orc_create(const mpm_specs *specs, orc_ctx **out) {
     if (specs->order_scheme == 2) c->ng_cells = 3;
     c->ng_nodal = (specs->order_scheme == 1) ? 1 : 3;
     for (int d = 0; d < 3; ++d) c->N[d] = c->n[d] + 1 + 2 * c->ng_nodal;
-    c->nd.assign((size_t)MPM_N_NCOMP * c->N[0] * c->N[1] * c->N[2], 0.0);
+    // NUM_STATES of the USE_TEMP build (constants.H:31): the thermal
+    // components follow the 20 mechanical ones.
+    c->nd.assign((size_t)MPM_N_NCOMP_TEMP * c->N[0] * c->N[1] * c->N[2], 0.0);
     *out = c;
     return MPM_OK;
 }
@@ -582,6 +587,7 @@ int orc_upload_particles(orc_ctx *c, int64_t np, const double *pos,
         for (int d = 0; d < 3; ++d) p.pos[d] = pos[d * np + i];
         for (int f = 0; f < MPM_NREAL; ++f) p.r[f] = rdata[f * np + i];
         for (int f = 0; f < MPM_NINT; ++f) p.idata[f] = idata[f * np + i];
+        for (int f = 0; f < MPM_NTHERMAL; ++f) p.th[f] = 0.0;
         p.id = i;
         int m = p.idata[MPM_I_MODEL];
         if (m < 0 || m > 2) return MPM_E_MODEL;
@@ -617,7 +623,8 @@ int orc_download_particles(orc_ctx *c, double *pos, double *rdata, int *idata,
 }
 
 int orc_get_nodal(orc_ctx *c, int comp, double *out) {
-    if (comp < 0 || comp >= MPM_N_NCOMP || !out) return MPM_E_ARG;
+    if (comp < 0 || comp >= (c->thermal ? (int)MPM_N_NCOMP_TEMP : (int)MPM_N_NCOMP) || !out)
+        return MPM_E_ARG;
     size_t q = 0;
     for (int k = 0; k <= c->n[2]; ++k)
         for (int j = 0; j <= c->n[1]; ++j)
@@ -1238,6 +1245,342 @@ int orc_calculate_dt(orc_ctx *c, double *dt_out) {
     return MPM_OK;
 }
 
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Thermal module (the USE_TEMP build): heat conduction on the same
+// particles and grid, as the reference's time loop and initial pass run it.
+// ---------------------------------------------------------------------------
+namespace {
+
+enum {
+    kMsp = MPM_N_MASS_SPHEAT,
+    kMspt = MPM_N_MASS_SPHEAT_TEMP,
+    kTemp = MPM_N_TEMPERATURE,
+    kSrc = MPM_N_SOURCE_TEMP,
+    kDtemp = MPM_N_DELTA_TEMPERATURE,
+    kMcond = MPM_N_MASS_CONDUCTIVITY
+};
+
+// FabArray::SumBoundary(scomp, ncomp, periodicity) on the one-box nodal data
+// (AMReX; called at mpm_particle_grid_ops.cpp:848-853): every valid node
+// becomes the sum of the values at all of its periodic images inside the
+// valid box, itself first, so a node on a periodic face adds the value of the
+// opposite face node (which the ghost-particle images have already filled
+// with the same deposits).
+void sum_boundary(orc_ctx *c, int comp0, int ncomp) {
+    const int *per = c->s.periodic;
+    if (!per[0] && !per[1] && !per[2]) return;
+    const int n0 = c->n[0] + 1, n1 = c->n[1] + 1, n2 = c->n[2] + 1;
+    std::vector<double> tmp((size_t)n0 * n1 * n2);
+    auto at = [&](int i, int j, int k) { return ((size_t)k * n1 + j) * n0 + i; };
+    for (int comp = comp0; comp < comp0 + ncomp; ++comp) {
+        for (int k = 0; k < n2; ++k)
+            for (int j = 0; j < n1; ++j)
+                for (int i = 0; i < n0; ++i) tmp[at(i, j, k)] = c->A(i, j, k, comp);
+        for (int k = 0; k < n2; ++k)
+            for (int j = 0; j < n1; ++j)
+                for (int i = 0; i < n0; ++i) {
+                    const int I[3] = {i, j, k};
+                    int im[3][2], ni[3];
+                    for (int d = 0; d < 3; ++d) {
+                        ni[d] = 0;
+                        im[d][ni[d]++] = I[d];
+                        if (per[d] && I[d] == 0) im[d][ni[d]++] = c->n[d];
+                        else if (per[d] && I[d] == c->n[d]) im[d][ni[d]++] = 0;
+                    }
+                    double sum = 0.0;
+                    for (int a = 0; a < ni[2]; ++a)
+                        for (int b = 0; b < ni[1]; ++b)
+                            for (int e = 0; e < ni[0]; ++e)
+                                sum += tmp[at(im[0][e], im[1][b], im[2][a])];
+                    c->A(i, j, k, comp) = sum;
+                }
+    }
+}
+
+// deposit_onto_grid_temperature (mpm_particle_grid_ops.cpp:656-884).
+void p2g_thermal(orc_ctx *c, bool reset, bool mass_temp, bool source) {
+    const mpm_specs &s = c->s;
+    if (reset)  // :681-700
+        for (int k = 0; k <= c->n[2]; ++k)
+            for (int j = 0; j <= c->n[1]; ++j)
+                for (int i = 0; i <= c->n[0]; ++i) {
+                    c->A(i, j, k, kMsp) = 0.0;
+                    c->A(i, j, k, kMspt) = 0.0;
+                    c->A(i, j, k, kSrc) = 0.0;
+                    if (mass_temp) c->A(i, j, k, kMcond) = 0.0;
+                }
+    const int64_t nreal = (int64_t)c->parts.size();
+    const int64_t nt = nreal + (int64_t)c->ghosts.size();
+    const bool par = c->par();
+#pragma omp parallel for schedule(static) num_threads(c->nthreads) if (par)
+    for (int64_t ip = 0; ip < nt; ++ip) {
+        const Part &p = ip < nreal ? c->parts[ip] : c->ghosts[ip - nreal];
+        if (p.id < 0) continue;
+        if (p.idata[MPM_I_PHASE] != 0) continue;
+        double xp[3] = {p.pos[0], p.pos[1], p.pos[2]};
+        int iv[3];
+        c->cell(xp, iv);
+        int mn[3], mx[3];
+        for (int d = 0; d < 3; ++d) {
+            int b0, b1;
+            compute_bounds(iv[d], c->lo[d], c->hi[d], c->od[d], s.periodic[d], b0, b1);
+            mn[d] = std::max(b0, 0 - iv[d]);
+            mx[d] = std::min(b1, c->n[d] - iv[d] + 1);
+        }
+        const double mass = p.r[MPM_R_MASS], vol = p.r[MPM_R_VOLUME];
+        const double cp = p.th[MPM_TH_SPECIFIC_HEAT], T = p.th[MPM_TH_TEMPERATURE];
+        const double kc = p.th[MPM_TH_CONDUCTIVITY], Q = p.th[MPM_TH_HEAT_SOURCE];
+        for (int n = mn[2]; n < mx[2]; n++)
+            for (int m = mn[1]; m < mx[1]; m++)
+                for (int l = mn[0]; l < mx[0]; l++) {
+                    const int I = iv[0] + l, J = iv[1] + m, K = iv[2] + n;
+                    const int st[3] = {l, m, n};
+                    const double N = basisval(st, iv, xp, s.plo, c->dx, c->od, s.periodic,
+                                              c->lo, c->hi);
+                    if (mass_temp) {
+                        atomic_add(&c->A(I, J, K, kMsp), mass * cp * N, par);
+                        atomic_add(&c->A(I, J, K, kMspt), mass * cp * T * N, par);
+                        atomic_add(&c->A(I, J, K, kMcond), mass * kc * N, par);
+                    }
+                    if (source) {
+                        // basisvalder per direction: the gradient of
+                        // basisval_and_grad (the same products).
+                        double g[3];
+                        basisval_and_grad(st, iv, xp, s.plo, c->dx, c->od, s.periodic, c->lo,
+                                          c->hi, g);
+                        double net = 0.0;
+                        for (int d = 0; d < 3; ++d) net += p.th[MPM_TH_HEAT_FLUX + d] * g[d];
+                        const double int_src = net * vol;
+                        const double ext_src = vol * Q * N;
+                        atomic_add(&c->A(I, J, K, kSrc), int_src + ext_src, par);
+                    }
+                }
+    }
+    if (mass_temp) {
+        sum_boundary(c, kMsp, 2);
+        sum_boundary(c, kMcond, 1);
+    }
+    if (source) sum_boundary(c, kSrc, 1);
+    // :857-881 — a node without thermal mass keeps its temperature.
+    if (mass_temp)
+        for (int k = 0; k <= c->n[2]; ++k)
+            for (int j = 0; j <= c->n[1]; ++j)
+                for (int i = 0; i <= c->n[0]; ++i)
+                    if (c->A(i, j, k, kMsp) > 0.0) {
+                        if (c->A(i, j, k, kMsp) >= s.mass_tolerance)
+                            c->A(i, j, k, kTemp) = c->A(i, j, k, kMspt) / c->A(i, j, k, kMsp);
+                        else
+                            c->A(i, j, k, kTemp) = 0.0;
+                    }
+}
+
+// backup_current_temperature (nodal_data_ops.cpp:95-114).
+void backup_temperature(orc_ctx *c) {
+    for (int k = 0; k <= c->n[2]; ++k)
+        for (int j = 0; j <= c->n[1]; ++j)
+            for (int i = 0; i <= c->n[0]; ++i)
+                if (c->A(i, j, k, kMsp) > 0.0) c->A(i, j, k, kDtemp) = c->A(i, j, k, kTemp);
+}
+
+// Nodal_Time_Update_Temperature (nodal_data_ops.cpp:500-527).
+void nodal_update_temperature(orc_ctx *c, double dt) {
+    for (int k = 0; k <= c->n[2]; ++k)
+        for (int j = 0; j <= c->n[1]; ++j)
+            for (int i = 0; i <= c->n[0]; ++i) {
+                if (c->A(i, j, k, kMsp) >= c->s.mass_tolerance)
+                    c->A(i, j, k, kTemp) += c->A(i, j, k, kSrc) / c->A(i, j, k, kMsp) * dt;
+                else
+                    c->A(i, j, k, kTemp) = 0.0;
+            }
+}
+
+// nodal_bcs_temperature (nodal_data_ops.cpp:786-910) in the order a serial
+// ParallelFor visits the nodal box (k outermost, i fastest): a wall node
+// reads its inward neighbour after that neighbour's own update when the
+// neighbour comes earlier in the sweep.  Unknown codes change nothing.
+void bcs_temperature(orc_ctx *c) {
+    const mpm_thermal_specs &t = c->ts;
+    for (int k = 0; k <= c->n[2]; ++k)
+        for (int j = 0; j <= c->n[1]; ++j)
+            for (int i = 0; i <= c->n[0]; ++i) {
+                const int I[3] = {i, j, k};
+                bool applied = false;
+                for (int d = 0; d < 3; ++d) {
+                    const bool is_lo = I[d] == c->lo[d];
+                    const bool is_hi = I[d] == c->hi[d] + 1;
+                    if (!is_lo && !is_hi) continue;
+                    const int bc = is_lo ? t.bclo[d] : t.bchi[d];
+                    int nb[3] = {i, j, k};
+                    nb[d] += is_lo ? 1 : -1;
+                    double &T = c->A(i, j, k, kTemp);
+                    const double Tnb = c->A(nb[0], nb[1], nb[2], kTemp);
+                    if (bc == MPM_BC_TEMP_PERIODIC) {
+                        applied = true;
+                    } else if (bc == MPM_BC_TEMP_ISOTHERMAL) {
+                        T = is_lo ? t.T_wall_lo[d] : t.T_wall_hi[d];
+                        applied = true;
+                    } else if (bc == MPM_BC_TEMP_ADIABATIC) {
+                        if (!applied) {
+                            T = Tnb;
+                            applied = true;
+                        }
+                    } else if (bc == MPM_BC_TEMP_USERFLUX) {
+                        const double q = is_lo ? t.flux_lo[d] : t.flux_hi[d];
+                        const double mk = c->A(i, j, k, kMcond), m = c->A(i, j, k, MPM_N_MASS);
+                        const double k_node = (m > 0.0) ? mk / m : 1.0;
+                        T = Tnb + q * c->dx[d] / k_node;
+                        applied = true;
+                    } else if (bc == MPM_BC_TEMP_CONVECTION) {
+                        const double hc = is_lo ? t.h_lo[d] : t.h_hi[d];
+                        const double Tinf = is_lo ? t.T_inf_lo[d] : t.T_inf_hi[d];
+                        const double mk = c->A(i, j, k, kMcond), m = c->A(i, j, k, MPM_N_MASS);
+                        const double k_node = (m > 0.0) ? mk / m : 1.0;
+                        const double Bi = hc * c->dx[d] / k_node;
+                        T = (Tnb + Bi * Tinf) / (1.0 + Bi);
+                        applied = true;
+                    }
+                }
+            }
+}
+
+// store_delta_temperature (nodal_data_ops.cpp:409-430).
+void store_delta_temperature(orc_ctx *c) {
+    for (int k = 0; k <= c->n[2]; ++k)
+        for (int j = 0; j <= c->n[1]; ++j)
+            for (int i = 0; i <= c->n[0]; ++i)
+                if (c->A(i, j, k, kMsp) > 0.0)
+                    c->A(i, j, k, kDtemp) = c->A(i, j, k, kTemp) - c->A(i, j, k, kDtemp);
+}
+
+// interpolate_from_grid_temperature (mpm_particle_grid_ops.cpp:1129-1270)
+// with bilin_interp / quadratic_interp / cubic_interp (interpolants.H:513-565,
+// 680-862, 1029-1210), chosen by the x-direction order for all directions.
+void g2p_thermal(orc_ctx *c, bool update_T, bool update_q) {
+    const mpm_specs &s = c->s;
+    fill_boundary(c);
+    const int64_t np = (int64_t)c->parts.size();
+#pragma omp parallel for schedule(static) num_threads(c->nthreads) if (c->par())
+    for (int64_t ip = 0; ip < np; ++ip) {
+        Part &p = c->parts[ip];
+        if (p.id < 0) continue;
+        if (p.idata[MPM_I_PHASE] != 0) continue;
+        double xp[3] = {p.pos[0], p.pos[1], p.pos[2]};
+        int iv[3];
+        c->cell(xp, iv);
+        int mn[3], mx[3];
+        for (int d = 0; d < 3; ++d)
+            compute_bounds(iv[d], c->lo[d], c->hi[d], c->od[d], s.periodic[d], mn[d], mx[d]);
+        if (update_T) {
+            const int o = c->od[0];
+            double value = 0.0;
+            if (o == 1) {
+                const int i = iv[0], j = iv[1], k = iv[2];
+                const double a = (xp[0] - (s.plo[0] + i * c->dx[0])) / c->dx[0];
+                const double b = (xp[1] - (s.plo[1] + j * c->dx[1])) / c->dx[1];
+                const double g = (xp[2] - (s.plo[2] + k * c->dx[2])) / c->dx[2];
+                value = (1.0 - a) * (1.0 - b) * (1.0 - g) * c->A(i, j, k, kDtemp);
+                value += a * (1.0 - b) * (1.0 - g) * c->A(i + 1, j, k, kDtemp);
+                value += (1.0 - a) * b * (1.0 - g) * c->A(i, j + 1, k, kDtemp);
+                value += a * b * (1.0 - g) * c->A(i + 1, j + 1, k, kDtemp);
+                value += (1.0 - a) * (1.0 - b) * g * c->A(i, j, k + 1, kDtemp);
+                value += a * (1.0 - b) * g * c->A(i + 1, j, k + 1, kDtemp);
+                value += (1.0 - a) * b * g * c->A(i, j + 1, k + 1, kDtemp);
+                value += a * b * g * c->A(i + 1, j + 1, k + 1, kDtemp);
+            } else {
+                const double inv[3] = {1.0 / c->dx[0], 1.0 / c->dx[1], 1.0 / c->dx[2]};
+                auto w1 = [&](int d, int idx) {
+                    const int t = s.periodic[d] ? 3 : node_type_chain(idx, c->lo[d], c->hi[d]);
+                    const double x0 = s.plo[d] + idx * c->dx[d];
+                    return (o == 2) ? quad_val(t, (xp[d] - x0) / c->dx[d])
+                                    : cubic_val(t, (xp[d] - x0) * inv[d]);
+                };
+                for (int n = mn[2]; n < mx[2]; ++n)
+                    for (int m = mn[1]; m < mx[1]; ++m)
+                        for (int l = mn[0]; l < mx[0]; ++l) {
+                            const double lv = w1(0, iv[0] + l), mv = w1(1, iv[1] + m),
+                                         nv = w1(2, iv[2] + n);
+                            value += lv * mv * nv * c->A(iv[0] + l, iv[1] + m, iv[2] + n, kDtemp);
+                        }
+            }
+            p.th[MPM_TH_TEMPERATURE] += value;
+        }
+        if (update_q) {
+            double gT[3] = {0.0, 0.0, 0.0};
+            for (int n = mn[2]; n < mx[2]; ++n)
+                for (int m = mn[1]; m < mx[1]; ++m)
+                    for (int l = mn[0]; l < mx[0]; ++l) {
+                        const int st[3] = {l, m, n};
+                        double g[3];
+                        basisval_and_grad(st, iv, xp, s.plo, c->dx, c->od, s.periodic, c->lo,
+                                          c->hi, g);
+                        const double Tn = c->A(iv[0] + l, iv[1] + m, iv[2] + n, kTemp);
+                        for (int d = 0; d < 3; ++d) gT[d] += Tn * g[d];
+                    }
+            for (int d = 0; d < 3; ++d)
+                p.th[MPM_TH_HEAT_FLUX + d] = -p.th[MPM_TH_CONDUCTIVITY] * gT[d];
+        }
+    }
+}
+
+// The thermal block of the initial pass (mpm_init.cpp:417-441), on nodal
+// data that the reference has just zeroed (mpm_init.cpp:280-281).
+void init_thermal(orc_ctx *c) {
+    const size_t nodes = (size_t)c->N[0] * c->N[1] * c->N[2];
+    std::fill(c->nd.begin() + (size_t)MPM_N_MASS_SPHEAT * nodes,
+              c->nd.begin() + (size_t)MPM_N_NCOMP_TEMP * nodes, 0.0);
+    p2g_thermal(c, true, true, false);
+    backup_temperature(c);
+    bcs_temperature(c);
+    store_delta_temperature(c);
+    g2p_thermal(c, true, true);
+}
+
+int check_thermal_specs(const mpm_thermal_specs *t) {
+    for (int d = 0; d < 3; ++d)
+        if (t->bclo[d] < MPM_BC_TEMP_PERIODIC || t->bclo[d] > MPM_BC_TEMP_CONVECTION ||
+            t->bchi[d] < MPM_BC_TEMP_PERIODIC || t->bchi[d] > MPM_BC_TEMP_CONVECTION)
+            return MPM_E_BC;
+    return MPM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int orc_enable_thermal(orc_ctx *c, const mpm_thermal_specs *t) {
+    if (!c || !t) return MPM_E_ARG;
+    int rc = check_thermal_specs(t);
+    if (rc) return rc;
+    c->thermal = true;
+    c->ts = *t;
+    for (auto &p : c->parts)
+        for (int f = 0; f < MPM_NTHERMAL; ++f) p.th[f] = 0.0;
+    return MPM_OK;
+}
+
+int orc_upload_thermal(orc_ctx *c, int64_t n, const double *th) {
+    if (!c || !c->thermal || n < 0 || (n > 0 && !th)) return MPM_E_ARG;
+    for (auto &p : c->parts) {
+        if (p.id < 0) continue;
+        if (p.id >= n) return MPM_E_ARG;
+        for (int f = 0; f < MPM_NTHERMAL; ++f) p.th[f] = th[(size_t)f * n + p.id];
+    }
+    return MPM_OK;
+}
+
+int orc_download_thermal(orc_ctx *c, int64_t n, double *th) {
+    if (!c || !c->thermal || n < 0 || (n > 0 && !th)) return MPM_E_ARG;
+    std::fill(th, th + (size_t)MPM_NTHERMAL * n, std::numeric_limits<double>::quiet_NaN());
+    for (const auto &p : c->parts) {
+        if (p.id < 0) continue;
+        if (p.id >= n) return MPM_E_ARG;
+        for (int f = 0; f < MPM_NTHERMAL; ++f) th[(size_t)f * n + p.id] = p.th[f];
+    }
+    return MPM_OK;
+}
+
 // Initialise_Internal_Forces (mpm_init.cpp:368-401).
 int orc_init_internal_forces(orc_ctx *c) {
     fill_neighbors(c);
@@ -1248,7 +1591,10 @@ int orc_init_internal_forces(orc_ctx *c) {
     if (rc) return rc;
     // apply_constitutive_model(dt = 0, applied_strainrate): always the
     // total-strain form, with the applied rate regardless of time.
-    return stress_impl(c, 0.0, c->s.applied_strainrate, false);
+    rc = stress_impl(c, 0.0, c->s.applied_strainrate, false);
+    if (rc) return rc;
+    if (c->thermal) init_thermal(c);
+    return MPM_OK;
 }
 
 int orc_step(orc_ctx *c, int nsteps) {
@@ -1268,8 +1614,16 @@ int orc_step(orc_ctx *c, int nsteps) {
         orc_backup_velocity(c);
         orc_nodal_update(c, dt);
         if ((rc = orc_apply_nodal_bcs(c, dt, c->time))) return rc;
+        if (c->thermal) {  // main.cpp:99-105
+            p2g_thermal(c, true, true, true);
+            backup_temperature(c);
+            nodal_update_temperature(c, dt);
+            bcs_temperature(c);
+            store_delta_temperature(c);
+        }
         if (s.stress_update_scheme == MPM_USL) {
             orc_g2p(c, 1, 1, dt);
+            if (c->thermal) g2p_thermal(c, true, true);  // main.cpp:112-114
             if ((rc = orc_move(c, dt, step))) return rc;
         } else {
             orc_g2p(c, 1, 0, dt);
@@ -1277,6 +1631,12 @@ int orc_step(orc_ctx *c, int nsteps) {
             if ((rc = orc_p2g(c, 0, 1, 0))) return rc;
             if ((rc = orc_apply_nodal_bcs(c, dt, c->time))) return rc;
             orc_g2p(c, 0, 1, dt);
+            if (c->thermal) {  // main.cpp:130-135
+                g2p_thermal(c, true, false);
+                p2g_thermal(c, true, false, true);
+                bcs_temperature(c);
+                g2p_thermal(c, false, true);
+            }
             if ((rc = orc_move(c, dt, step))) return rc;
         }
         orc_update_volume(c);

@@ -54,6 +54,10 @@ int orc_add_levelset_body(orc_ctx *ctx, const double *phi, int bc, double mu,
                           const double wall_vel[3]);
 int orc_set_wall_udf(orc_ctx *ctx, int dir, int hi, mpm_wall_udf_fn fn);
 int orc_energies(orc_ctx *ctx, double out[6]);
+/* Thermal module (the USE_TEMP build). */
+int orc_enable_thermal(orc_ctx *ctx, const mpm_thermal_specs *t);
+int orc_upload_thermal(orc_ctx *ctx, int64_t n_upload, const double *th);
+int orc_download_thermal(orc_ctx *ctx, int64_t n_upload, double *th);
 
 /* Pieces exposed for closed-form pinning in tests. */
 double orc_quadraticspline_1d(int type, double z);

@@ -1,0 +1,191 @@
+"""Pins of the oracle's thermal module (the USE_TEMP build's heat conduction)
+against what the mathematics fixes.
+
+- A sine mode in a periodic slab decays at alpha (2 pi / L)^2 (alpha =
+  k / (rho c_p)), with second-order convergence under refinement, for
+  linear, quadratic and cubic shape functions; the total heat sum m c_p T is
+  conserved to round-off.  A dropped volume factor, a wrong flux sign or a
+  wrong gradient would change the rate by orders of magnitude or its sign.
+- Isothermal walls: the first Dirichlet mode sin(pi x / L) decays at
+  alpha (pi / L)^2.
+- Adiabatic walls: the heat sum is conserved; the first Neumann mode's
+  decay rate converges to alpha (pi / L)^2 (first order: the wall copies its
+  neighbour's temperature).
+- Special cases of the wall formulas: convection with h = 0 and a user flux
+  of 0 are the adiabatic wall; convection with a huge h is the isothermal
+  wall at T_inf.
+"""
+import numpy as np
+import pytest
+
+from exagoop_amd import capi, scenes
+
+oracle = pytest.importorskip("oracle")
+
+
+def slab(order, nx, dt, periodic_x, init, tspec=None, steps=0, threads=8):
+    s = capi.default_specs()
+    s.plo[:] = [0.0, 0.0, 0.0]
+    s.phi[:] = [1.0, 4.0 / nx, 4.0 / nx]
+    s.ncells[:] = [nx, 4, 4]
+    s.periodic[:] = [periodic_x, 1, 1]
+    s.bclo[0] = s.bchi[0] = capi.BC_PERIODIC if periodic_x else capi.BC_SLIP
+    for d in (1, 2):
+        s.bclo[d] = s.bchi[d] = capi.BC_PERIODIC
+    s.order_scheme = order
+    s.gravity[:] = [0.0, 0.0, 0.0]
+    s.fixed_timestep = 1
+    s.timestep = dt
+    pos, vp = scenes.lattice(s, (2, 2, 2), lambda X, Y, Z: np.ones_like(X, bool))
+    r, idata = scenes.make_records(pos, vp, 1.0, 0, E=1e3, nu=0.3)  # rho = 1
+    th = np.zeros((capi.NTHERMAL, pos.shape[1]))
+    th[capi.TH_TEMPERATURE] = init(pos[0])
+    th[capi.TH_SPECIFIC_HEAT] = 1.0
+    th[capi.TH_CONDUCTIVITY] = 1.0  # alpha = 1
+    o = oracle.Oracle(s, threads=threads)
+    o.upload(pos, r, idata)
+    ts = capi.default_thermal_specs(s)
+    if tspec:
+        tspec(ts)
+    o.enable_thermal(ts)
+    o.upload_thermal(th)
+    o.init_internal_forces()
+    if steps:
+        o.step(steps)
+    return o, pos, r
+
+
+def amplitude(pos, T, mode, offset=True):
+    cols = [np.ones_like(T)] if offset else []
+    A = np.stack(cols + [mode(pos[0])]).T
+    return np.linalg.lstsq(A, T, rcond=None)[0][-1]
+
+
+def decay_rate(order, nx, periodic_x, mode, tspec=None, steps=120, dt=2e-5):
+    o, pos, r = slab(order, nx, dt, periodic_x, lambda x: 1.0 + 0.1 * mode(x), tspec)
+    th0 = o.download_thermal()
+    a0 = amplitude(pos, th0[capi.TH_TEMPERATURE], mode)
+    o.step(steps)
+    th1 = o.download_thermal()
+    a1 = amplitude(pos, th1[capi.TH_TEMPERATURE], mode)
+    m = r[capi.R_MASS]
+    heat = [(m * t[capi.TH_SPECIFIC_HEAT] * t[capi.TH_TEMPERATURE]).sum() for t in (th0, th1)]
+    return -np.log(a1 / a0) / (steps * dt), heat
+
+
+@pytest.mark.parametrize("order", [1, 2, 3])
+def test_periodic_sine_mode_decays_at_the_analytic_rate(order):
+    exact = (2 * np.pi) ** 2
+    mode = lambda x: np.sin(2 * np.pi * x)  # noqa: E731
+    errs = []
+    for nx in (16, 32):
+        rate, heat = decay_rate(order, nx, 1, mode)
+        errs.append(abs(rate / exact - 1.0))
+        assert abs(heat[1] - heat[0]) <= 1e-12 * abs(heat[0])
+    assert errs[1] < 0.03
+    assert 3.0 < errs[0] / errs[1] < 5.0  # second order
+
+
+def test_isothermal_walls_give_the_dirichlet_rate():
+    def iso(ts):
+        ts.bclo[0] = ts.bchi[0] = capi.BC_TEMP_ISOTHERMAL  # T_wall = 0
+
+    for order in (1, 2, 3):
+        o, pos, _ = slab(order, 32, 2e-5, 0, lambda x: np.sin(np.pi * x), iso)
+        mode = lambda x: np.sin(np.pi * x)  # noqa: E731
+        a0 = amplitude(pos, o.download_thermal()[capi.TH_TEMPERATURE], mode, False)
+        o.step(120)
+        a1 = amplitude(pos, o.download_thermal()[capi.TH_TEMPERATURE], mode, False)
+        rate = -np.log(a1 / a0) / (120 * 2e-5)
+        assert abs(rate / np.pi ** 2 - 1.0) < 0.02, (order, rate)
+
+
+def test_adiabatic_walls_conserve_heat_and_converge_to_the_neumann_rate():
+    mode = lambda x: np.cos(np.pi * x)  # noqa: E731
+    errs = []
+    for nx in (16, 32):
+        rate, heat = decay_rate(2, nx, 0, mode)
+        assert abs(heat[1] - heat[0]) <= 1e-12 * abs(heat[0])
+        errs.append(abs(rate / np.pi ** 2 - 1.0))
+    assert errs[1] < 0.1 and errs[1] < 0.6 * errs[0]
+
+
+def test_uniform_temperature_stays_uniform_behind_adiabatic_walls():
+    o, pos, _ = slab(3, 16, 2e-5, 0, lambda x: np.full_like(x, 3.5), steps=20)
+    th = o.download_thermal()
+    assert np.abs(th[capi.TH_TEMPERATURE] - 3.5).max() < 1e-12
+    assert np.abs(th[capi.TH_HEAT_FLUX:capi.TH_HEAT_FLUX + 3]).max() < 1e-9
+
+
+def _final_T(tspec, order=2):
+    # one thread: a deterministic deposit order, so equal inputs give equal bits
+    o, _, _ = slab(order, 16, 2e-5, 0, lambda x: 1.0 + x * x, tspec, steps=30, threads=1)
+    return o.download_thermal()[capi.TH_TEMPERATURE]
+
+
+def test_wall_formulas_reduce_to_their_special_cases():
+    adiabatic = _final_T(None)
+
+    def conv0(ts):
+        ts.bclo[0] = ts.bchi[0] = capi.BC_TEMP_CONVECTION  # h = 0: Bi = 0
+        ts.T_inf_lo[0] = ts.T_inf_hi[0] = 5.0
+
+    def flux0(ts):
+        ts.bclo[0] = ts.bchi[0] = capi.BC_TEMP_USERFLUX  # q = 0
+
+    assert np.array_equal(_final_T(conv0), adiabatic)
+    assert np.array_equal(_final_T(flux0), adiabatic)
+
+    def conv_inf(ts):
+        ts.bclo[0] = ts.bchi[0] = capi.BC_TEMP_CONVECTION
+        ts.h_lo[0] = ts.h_hi[0] = 1e14
+        ts.T_inf_lo[0], ts.T_inf_hi[0] = 0.25, 0.75
+
+    def iso(ts):
+        ts.bclo[0] = ts.bchi[0] = capi.BC_TEMP_ISOTHERMAL
+        ts.T_wall_lo[0], ts.T_wall_hi[0] = 0.25, 0.75
+
+    a, b = _final_T(conv_inf), _final_T(iso)
+    assert np.abs(a - b).max() < 1e-10
+    assert np.abs(a - adiabatic).max() > 1e-3
+
+
+def test_user_flux_feeds_heat_at_the_prescribed_rate():
+    """A flux q through the lo wall (T_wall = T_nb + q dx / k) with the hi
+    wall adiabatic: the heat sum grows at about q times the wall area."""
+    q = 2.0
+
+    def flux(ts):
+        ts.bclo[0] = capi.BC_TEMP_USERFLUX
+        ts.flux_lo[0] = q
+
+    dt, steps = 2e-5, 200
+    o, pos, r = slab(1, 32, dt, 0, lambda x: np.ones_like(x), flux)
+    th0 = o.download_thermal()
+    o.step(steps)
+    th1 = o.download_thermal()
+    m = r[capi.R_MASS]
+    heat = [(m * t[capi.TH_TEMPERATURE]).sum() for t in (th0, th1)]
+    area = (4.0 / 32) ** 2
+    rate = (heat[1] - heat[0]) / (steps * dt)
+    assert rate > 0 and abs(rate / (q * area) - 1.0) < 0.25, rate / (q * area)
+
+
+def test_thermal_fields_follow_the_upload_index():
+    s = capi.default_specs()
+    s.phi[:] = [1.0, 1.0, 1.0]
+    s.ncells[:] = [8, 8, 8]
+    o = oracle.Oracle(s)
+    pos, vp = scenes.lattice(s, (1, 1, 1), lambda X, Y, Z: X < 0.5)
+    r, idata = scenes.make_records(pos, vp, 1.0, 0, E=1e3, nu=0.3)
+    o.upload(pos, r, idata)
+    with pytest.raises(capi.MPMError):
+        o.upload_thermal(np.zeros((capi.NTHERMAL, pos.shape[1])))  # not enabled
+    o.enable_thermal()
+    th = np.arange(capi.NTHERMAL * pos.shape[1], dtype=float).reshape(capi.NTHERMAL, -1)
+    o.upload_thermal(th)
+    assert np.array_equal(o.download_thermal(), th)
+    bad = capi.default_thermal_specs(s)
+    bad.bchi[2] = 7
+    with pytest.raises(capi.MPMError):
+        o.enable_thermal(bad)
