@@ -204,6 +204,7 @@ SIGNATURES = {
     "mpm_enable_thermal": (ctypes.c_int, [_P, ctypes.POINTER(ThermalSpecs)]),
     "mpm_upload_thermal": (ctypes.c_int, [_P, ctypes.c_int64, _DP]),
     "mpm_download_thermal": (ctypes.c_int, [_P, ctypes.c_int64, _DP]),
+    "mpm_set_wall_temp_udf": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, UDF_FN]),
 }
 
 
@@ -359,6 +360,16 @@ class _Base:
         self._udf_refs = getattr(self, "_udf_refs", [])
         self._udf_refs.append(cb)
         self._call("set_wall_udf", self._ctx, d, hi, cb)
+
+    def set_wall_temp_udf(self, d, hi, fn):
+        """fn(x, y, z, t) -> (v0, v1): T_wall, flux or h; T_inf (convection)."""
+        def _cb(x, y, z, t, out):
+            v = fn(x, y, z, t)
+            out[0], out[1] = v[0], v[1]
+        cb = UDF_FN(_cb)
+        self._udf_refs = getattr(self, "_udf_refs", [])
+        self._udf_refs.append(cb)
+        self._call("set_wall_temp_udf", self._ctx, d, hi, cb)
 
     def energies(self):
         out = np.zeros(6)

@@ -69,6 +69,11 @@ struct mpm_ctx {
     int64_t th_cap = 0;
     double *tn = nullptr;
     double *tscr = nullptr;
+    // Temperature wall functions and their per-step face tables (2 values per
+    // face node).
+    mpm_wall_temp_udf_fn tudf_fn[3][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};
+    double *tudf_dev[3][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};
+    bool any_tudf = false;
 };
 
 namespace xmpm {
@@ -96,6 +101,8 @@ int thermal_grid(mpm_ctx *c, hipStream_t st);
 int thermal_particles(mpm_ctx *c, bool musl, const int *perm, const int *count,
                       hipStream_t st);
 void thermal_swap(mpm_ctx *c);
+// The temperature wall functions evaluated at time t into the face tables.
+int thermal_udf_tables(mpm_ctx *c, double t);
 void thermal_free(mpm_ctx *c);
 
 inline int grid_flags_full(const mpm_ctx *c) {

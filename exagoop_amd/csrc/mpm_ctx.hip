@@ -812,11 +812,13 @@ int mpm_step(mpm_ctx *c, int nsteps) {
         // inside the dt launch; the last step's advance follows the loop.
         launch_dt(c->g, c->ph, c->P, c->sc, c->s.fixed_timestep == 1, c->s.timestep, st,
                   fused_dt ? 1 : 0, it > 0 ? 1 : 0);
-        if (c->any_udf) {
+        const bool temp_udf = c->thermal && c->any_tudf;
+        if (c->any_udf || temp_udf) {
             Scalars h;
             HIPCHK(hipMemcpyAsync(&h, c->sc, sizeof(Scalars), hipMemcpyDeviceToHost, st));
             HIPCHK(hipStreamSynchronize(st));
-            int rc = upload_udf_tables(c, h.time);
+            int rc = c->any_udf ? upload_udf_tables(c, h.time) : MPM_OK;
+            if (!rc && temp_udf) rc = thermal_udf_tables(c, h.time);
             if (rc) return rc;
         }
         const bool sorted = c->sorted != 0;

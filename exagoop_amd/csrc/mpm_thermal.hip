@@ -216,6 +216,71 @@ __device__ double th_wall(const GridP &g, const mpm_thermal_specs &t, const doub
     return v;
 }
 
+// Device face tables of the temperature wall functions (null: no function).
+struct ThUdf {
+    const double *tab[3][2];
+};
+
+__device__ __forceinline__ bool on_udf_face(const GridP &g, const ThUdf &u, int I, int J, int K) {
+    const int X[3] = {I, J, K};
+    for (int d = 0; d < 3; ++d)
+        if ((X[d] == 0 && u.tab[d][0]) || (X[d] == g.n[d] && u.tab[d][1])) return true;
+    return false;
+}
+
+// apply_udf_nodal_bcs_temperature (nodal_data_ops.cpp:1308-1446): the value a
+// node on a face with a function ends with after the second serial sweep,
+// which applies the face's temperature BC type with the function's values
+// (no precedence between faces: the last direction wins).  Neighbours as in
+// th_wall; T holds the values after the wall conditions.
+template <int D>
+__device__ double th_udf(const GridP &g, const mpm_thermal_specs &t, const ThUdf &u,
+                         const double *T, const double *MC, const double *M, int I, int J,
+                         int K) {
+    const int X[3] = {I, J, K};
+    const int64_t q = nidx(g, I, J, K);
+    double v = T[q];
+    for (int dir = 0; dir < 3; ++dir) {
+        const double *tab = nullptr;
+        int bc = -1;
+        bool lo = false;
+        if (X[dir] == 0 && u.tab[dir][0]) {
+            tab = u.tab[dir][0];
+            bc = t.bclo[dir];
+            lo = true;
+        } else if (X[dir] == g.n[dir] && u.tab[dir][1]) {
+            tab = u.tab[dir][1];
+            bc = t.bchi[dir];
+        }
+        if (!tab) continue;
+        const int p0 = (dir == 0) ? 1 : 0;
+        const int p1 = (dir == 2) ? 1 : 2;
+        const int64_t flat = ((int64_t)X[p0] * (g.n[p1] + 1) + X[p1]) * 2;
+        const double v0 = tab[flat], v1 = tab[flat + 1];
+        if (bc == MPM_BC_TEMP_ISOTHERMAL) {
+            v = v0;
+            continue;
+        }
+        if (bc != MPM_BC_TEMP_USERFLUX && bc != MPM_BC_TEMP_CONVECTION) continue;
+        int Y[3] = {I, J, K};
+        Y[dir] += lo ? 1 : -1;
+        double tnb = T[nidx(g, Y[0], Y[1], Y[2])];
+        if constexpr (D > 0) {
+            if (!lo && on_udf_face(g, u, Y[0], Y[1], Y[2]))
+                tnb = th_udf<D - 1>(g, t, u, T, MC, M, Y[0], Y[1], Y[2]);
+        }
+        const double mk = MC[q], m = M[q];
+        const double k_node = (m > 0.0) ? mk / m : 1.0;
+        if (bc == MPM_BC_TEMP_USERFLUX) {
+            v = tnb + v0 * g.dx[dir] / k_node;
+        } else {
+            const double Bi = v0 * g.dx[dir] / k_node;
+            v = (tnb + Bi * v1) / (1.0 + Bi);
+        }
+    }
+    return v;
+}
+
 __device__ __forceinline__ void node_ijk(const GridP &g, int64_t q, int &I, int &J, int &K) {
     const int64_t n0 = g.n[0] + 1, n1 = g.n[1] + 1;
     I = (int)(q % n0);
@@ -237,17 +302,32 @@ __global__ __launch_bounds__(kThreads) void k_th_walls(GridP g, mpm_thermal_spec
     scr[q] = th_wall<3>(g, t, tn + TN_T * nn, tn + TN_MC * nn, mass, I, J, K);
 }
 
-// The wall values into the temperature and, when delta is set,
+// Function values of the nodes on faces with a temperature function into scr.
+__global__ __launch_bounds__(kThreads) void k_th_udf_walls(GridP g, mpm_thermal_specs t, ThUdf u,
+                                                           const double *tn, const double *mass,
+                                                           double *scr) {
+    const int64_t q = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (q >= g.nn) return;
+    int I, J, K;
+    node_ijk(g, q, I, J, K);
+    if (!on_udf_face(g, u, I, J, K)) return;
+    const int64_t nn = g.nn;
+    scr[q] = th_udf<3>(g, t, u, tn + TN_T * nn, tn + TN_MC * nn, mass, I, J, K);
+}
+
+// The wall values into the temperature (the nodes on faces with a function
+// when udf is set, else all wall nodes) and, when delta is set,
 // store_delta_temperature (nodal_data_ops.cpp:409-430).
-__global__ __launch_bounds__(kThreads) void k_th_apply_walls(GridP g, double *tn,
-                                                             const double *scr, int delta) {
+__global__ __launch_bounds__(kThreads) void k_th_apply_walls(GridP g, ThUdf u, int udf,
+                                                             double *tn, const double *scr,
+                                                             int delta) {
     const int64_t q = (int64_t)blockIdx.x * kThreads + threadIdx.x;
     const int64_t nn = g.nn;
     if (q >= nn) return;
     int I, J, K;
     node_ijk(g, q, I, J, K);
     double T = tn[TN_T * nn + q];
-    if (on_face(g, I, J, K)) {
+    if (udf ? on_udf_face(g, u, I, J, K) : on_face(g, I, J, K)) {
         T = scr[q];
         tn[TN_T * nn + q] = T;
     }
@@ -356,10 +436,21 @@ void launch_th_p2g(mpm_ctx *c, bool mass_temp, bool source, hipStream_t st) {
     }
 }
 
-void launch_th_walls(mpm_ctx *c, bool delta, hipStream_t st) {
+// Apply_Nodal_BCs_Temperature (utilities.cpp:221-240) without level-set
+// bodies: the wall conditions, then (udf) the face functions; then, when
+// delta is set, store_delta_temperature.
+void launch_th_walls(mpm_ctx *c, bool delta, bool udf, hipStream_t st) {
     const int64_t nn = c->g.nn;
+    ThUdf u{};
+    const bool any = udf && c->any_tudf;
+    for (int d = 0; d < 3; ++d)
+        for (int h = 0; h < 2; ++h) u.tab[d][h] = (any && c->tudf_fn[d][h]) ? c->tudf_dev[d][h] : nullptr;
     k_th_walls<<<blocks(nn), kThreads, 0, st>>>(c->g, c->ts, c->tn, c->N.mass, c->tscr);
-    k_th_apply_walls<<<blocks(nn), kThreads, 0, st>>>(c->g, c->tn, c->tscr, delta ? 1 : 0);
+    k_th_apply_walls<<<blocks(nn), kThreads, 0, st>>>(c->g, u, 0, c->tn, c->tscr,
+                                                      (delta && !any) ? 1 : 0);
+    if (!any) return;
+    k_th_udf_walls<<<blocks(nn), kThreads, 0, st>>>(c->g, c->ts, u, c->tn, c->N.mass, c->tscr);
+    k_th_apply_walls<<<blocks(nn), kThreads, 0, st>>>(c->g, u, 1, c->tn, c->tscr, delta ? 1 : 0);
 }
 
 void launch_th_g2p(mpm_ctx *c, bool ut, bool uq, const int *perm, const int *count,
@@ -431,7 +522,7 @@ int thermal_init(mpm_ctx *c) {
     launch_zero(c->tn, TN_COMPS * nn, st);
     launch_th_p2g(c, true, false, st);
     k_th_nodal<false><<<blocks(nn), kThreads, 0, st>>>(c->g, c->s.mass_tolerance, c->tn, nullptr);
-    launch_th_walls(c, true, st);
+    launch_th_walls(c, true, false, st);
     launch_th_g2p(c, true, true, nullptr, nullptr, c->th, st);
     return check_launch();
 }
@@ -445,7 +536,7 @@ int thermal_grid(mpm_ctx *c, hipStream_t st) {
     launch_zero(c->tn, TN_COMPS * nn, st);
     launch_th_p2g(c, true, true, st);
     k_th_nodal<true><<<blocks(nn), kThreads, 0, st>>>(c->g, c->s.mass_tolerance, c->tn, c->sc);
-    launch_th_walls(c, true, st);
+    launch_th_walls(c, true, true, st);
     return check_launch();
 }
 
@@ -469,14 +560,46 @@ int thermal_particles(mpm_ctx *c, bool musl, const int *perm, const int *count,
     launch_zero(c->tn + TN_MSP * nn, 2 * nn, st);
     launch_zero(c->tn + TN_SRC * nn, nn, st);
     launch_th_p2g(c, false, true, st);
-    launch_th_walls(c, false, st);
+    launch_th_walls(c, false, true, st);
     launch_th_g2p(c, false, true, perm, count, out, st);
     return check_launch();
 }
 
 void thermal_swap(mpm_ctx *c) { std::swap(c->th, c->th_alt); }
 
+// compute_udf_temp_at_nodes (nodal_data_ops.cpp:1178-1306), 3-D.
+int thermal_udf_tables(mpm_ctx *c, double t) {
+    for (int d = 0; d < 3; ++d)
+        for (int h = 0; h < 2; ++h) {
+            if (!c->tudf_fn[d][h]) continue;
+            const int p0 = (d == 0) ? 1 : 0;
+            const int p1 = (d == 2) ? 1 : 2;
+            std::vector<double> tab((size_t)(c->g.n[p0] + 1) * (c->g.n[p1] + 1) * 2, 0.0);
+            for (int j = 0; j <= c->g.n[p0]; ++j)
+                for (int k = 0; k <= c->g.n[p1]; ++k) {
+                    double co[3] = {0, 0, 0};
+                    co[d] = h ? c->s.phi[d] : c->s.plo[d];
+                    co[p0] = c->s.plo[p0] + j * c->g.dx[p0];
+                    co[p1] = c->s.plo[p1] + k * c->g.dx[p1];
+                    double out[2] = {0.0, 0.0};
+                    c->tudf_fn[d][h](co[0], co[1], co[2], t, out);
+                    const size_t f = ((size_t)j * (c->g.n[p1] + 1) + k) * 2;
+                    tab[f] = out[0];
+                    tab[f + 1] = out[1];
+                }
+            HIPCHK(hipMemcpyAsync(c->tudf_dev[d][h], tab.data(), tab.size() * sizeof(double),
+                                  hipMemcpyHostToDevice, c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+        }
+    return MPM_OK;
+}
+
 void thermal_free(mpm_ctx *c) {
+    for (int d = 0; d < 3; ++d)
+        for (int h = 0; h < 2; ++h) {
+            (void)hipFree(c->tudf_dev[d][h]);
+            c->tudf_dev[d][h] = nullptr;
+        }
     (void)hipFree(c->th);
     (void)hipFree(c->th_alt);
     (void)hipFree(c->tn);
@@ -516,6 +639,24 @@ int mpm_enable_thermal(mpm_ctx *c, const mpm_thermal_specs *t) {
     rc = alloc_particle_thermal(c);
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(c->stream));
+    return check_launch();
+}
+
+int mpm_set_wall_temp_udf(mpm_ctx *c, int dir, int hi, mpm_wall_temp_udf_fn fn) {
+    if (!c || dir < 0 || dir > 2 || hi < 0 || hi > 1) return set_err(MPM_E_ARG, "bad face");
+    if (fn && !c->tudf_dev[dir][hi]) {
+        const int p0 = (dir == 0) ? 1 : 0;
+        const int p1 = (dir == 2) ? 1 : 2;
+        const int64_t n = (int64_t)(c->g.n[p0] + 1) * (c->g.n[p1] + 1) * 2;
+        if (hipMalloc(&c->tudf_dev[dir][hi], n * sizeof(double)) != hipSuccess)
+            return set_err(MPM_E_OOM, "temperature function table");
+        launch_zero(c->tudf_dev[dir][hi], n, c->stream);
+        HIPCHK(hipStreamSynchronize(c->stream));
+    }
+    c->tudf_fn[dir][hi] = fn;
+    c->any_tudf = false;
+    for (int d = 0; d < 3; ++d)
+        for (int h = 0; h < 2; ++h) c->any_tudf |= (c->tudf_fn[d][h] != nullptr);
     return check_launch();
 }
 

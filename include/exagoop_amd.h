@@ -533,6 +533,18 @@ void mpm_thermal_specs_default(const mpm_specs *s, mpm_thermal_specs *t);
  * conditions; the thermal fields of all particles start at zero.  Level-set
  * bodies carry no temperature condition (the reference's default). */
 int mpm_enable_thermal(mpm_ctx *ctx, const mpm_thermal_specs *t);
+/* Temperature wall user function, the signature of TempUDFFuncPtr
+ * (mpm_specs.H:13): f(x, y, z, t, out[2]).  out[0] is the wall temperature
+ * (isothermal face), the flux (user-flux face) or h (convection face); out[1]
+ * is T_inf (convection face). */
+typedef void (*mpm_wall_temp_udf_fn)(double, double, double, double, double *);
+/* Sets (fn) or clears (NULL) the temperature function of one face.  It is
+ * evaluated at the face's nodes at the start of every step
+ * (compute_udf_temp_at_nodes, nodal_data_ops.cpp:1178-1306) and applied after
+ * the wall conditions of both temperature BC passes of the step, by the face's
+ * temperature BC type (apply_udf_nodal_bcs_temperature, :1308-1446).  The
+ * initial pass applies the wall conditions only, as the reference's does. */
+int mpm_set_wall_temp_udf(mpm_ctx *ctx, int dir, int hi, mpm_wall_temp_udf_fn fn);
 /* Thermal fields by upload index: th[f * n_upload + i] for field f of the
  * particle uploaded at index i (n_upload = count of the last upload; the
  * reader's T, spheat, thermcond, heatsrc columns, mpm_init.cpp:1097-1108). */

@@ -344,6 +344,8 @@ struct orc_ctx {
     int nthreads = 1;
     bool thermal = false;
     mpm_thermal_specs ts;
+    mpm_wall_temp_udf_fn tudf[3][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};
+    std::vector<double> tudf_tab[3][2];
 
     inline double &A(int i, int j, int k, int c) {
         size_t idx = (size_t)c * N[0] * N[1] * N[2] +
@@ -1445,6 +1447,86 @@ void bcs_temperature(orc_ctx *c) {
             }
 }
 
+bool any_temp_udf(const orc_ctx *c) {
+    for (int d = 0; d < 3; ++d)
+        for (int h = 0; h < 2; ++h)
+            if (c->tudf[d][h]) return true;
+    return false;
+}
+
+// compute_udf_temp_at_nodes (nodal_data_ops.cpp:1178-1306), 3-D: the face
+// functions at the face nodes, two values per node.
+void eval_temp_udf_tables(orc_ctx *c, double t) {
+    for (int d = 0; d < 3; ++d)
+        for (int h = 0; h < 2; ++h) {
+            if (!c->tudf[d][h]) continue;
+            const int p0 = (d == 0) ? 1 : 0;
+            const int p1 = (d == 2) ? 1 : 2;
+            auto &tab = c->tudf_tab[d][h];
+            tab.assign((size_t)(c->n[p0] + 1) * (c->n[p1] + 1) * 2, 0.0);
+            for (int j = 0; j <= c->n[p0]; ++j)
+                for (int k = 0; k <= c->n[p1]; ++k) {
+                    double co[3] = {0, 0, 0};
+                    co[d] = h ? c->s.phi[d] : c->s.plo[d];
+                    co[p0] = c->s.plo[p0] + j * c->dx[p0];
+                    co[p1] = c->s.plo[p1] + k * c->dx[p1];
+                    double out[2] = {0.0, 0.0};
+                    c->tudf[d][h](co[0], co[1], co[2], t, out);
+                    const size_t f = ((size_t)j * (c->n[p1] + 1) + k) * 2;
+                    tab[f] = out[0];
+                    tab[f + 1] = out[1];
+                }
+        }
+}
+
+// apply_udf_nodal_bcs_temperature (nodal_data_ops.cpp:1308-1446): a second
+// serial sweep over the nodal box; on a face with a function, the face's
+// temperature BC type applied with the function's values (other types
+// change nothing).
+void udf_bcs_temperature(orc_ctx *c) {
+    if (!any_temp_udf(c)) return;
+    const mpm_thermal_specs &t = c->ts;
+    for (int k = 0; k <= c->n[2]; ++k)
+        for (int j = 0; j <= c->n[1]; ++j)
+            for (int i = 0; i <= c->n[0]; ++i) {
+                const int I[3] = {i, j, k};
+                for (int dir = 0; dir < 3; ++dir) {
+                    const double *tab = nullptr;
+                    int bc = -1, sign = 0;
+                    if (I[dir] == c->lo[dir] && c->tudf[dir][0]) {
+                        tab = c->tudf_tab[dir][0].data();
+                        bc = t.bclo[dir];
+                        sign = 1;
+                    } else if (I[dir] == c->hi[dir] + 1 && c->tudf[dir][1]) {
+                        tab = c->tudf_tab[dir][1].data();
+                        bc = t.bchi[dir];
+                        sign = -1;
+                    }
+                    if (!tab) continue;
+                    const int p0 = (dir == 0) ? 1 : 0;
+                    const int p1 = (dir == 2) ? 1 : 2;
+                    const size_t flat = ((size_t)I[p0] * (c->n[p1] + 1) + I[p1]) * 2;
+                    const double v0 = tab[flat], v1 = tab[flat + 1];
+                    double &T = c->A(i, j, k, kTemp);
+                    int nb[3] = {i, j, k};
+                    nb[dir] += sign;
+                    if (bc == MPM_BC_TEMP_ISOTHERMAL) {
+                        T = v0;
+                    } else if (bc == MPM_BC_TEMP_USERFLUX || bc == MPM_BC_TEMP_CONVECTION) {
+                        const double Tnb = c->A(nb[0], nb[1], nb[2], kTemp);
+                        const double mk = c->A(i, j, k, kMcond), m = c->A(i, j, k, MPM_N_MASS);
+                        const double k_node = (m > 0.0) ? mk / m : 1.0;
+                        if (bc == MPM_BC_TEMP_USERFLUX) {
+                            T = Tnb + v0 * c->dx[dir] / k_node;
+                        } else {
+                            const double Bi = v0 * c->dx[dir] / k_node;
+                            T = (Tnb + Bi * v1) / (1.0 + Bi);
+                        }
+                    }
+                }
+            }
+}
+
 // store_delta_temperature (nodal_data_ops.cpp:409-430).
 void store_delta_temperature(orc_ctx *c) {
     for (int k = 0; k <= c->n[2]; ++k)
@@ -1570,6 +1652,13 @@ int orc_upload_thermal(orc_ctx *c, int64_t n, const double *th) {
     return MPM_OK;
 }
 
+int orc_set_wall_temp_udf(orc_ctx *c, int dir, int hi, mpm_wall_temp_udf_fn fn) {
+    if (!c || dir < 0 || dir > 2 || hi < 0 || hi > 1) return MPM_E_ARG;
+    c->tudf[dir][hi] = fn;
+    if (!fn) c->tudf_tab[dir][hi].clear();
+    return MPM_OK;
+}
+
 int orc_download_thermal(orc_ctx *c, int64_t n, double *th) {
     if (!c || !c->thermal || n < 0 || (n > 0 && !th)) return MPM_E_ARG;
     std::fill(th, th + (size_t)MPM_NTHERMAL * n, std::numeric_limits<double>::quiet_NaN());
@@ -1618,7 +1707,12 @@ int orc_step(orc_ctx *c, int nsteps) {
             p2g_thermal(c, true, true, true);
             backup_temperature(c);
             nodal_update_temperature(c, dt);
+            // Apply_Nodal_BCs_Temperature (utilities.cpp:221-240)
             bcs_temperature(c);
+            if (any_temp_udf(c)) {
+                eval_temp_udf_tables(c, c->time);
+                udf_bcs_temperature(c);
+            }
             store_delta_temperature(c);
         }
         if (s.stress_update_scheme == MPM_USL) {
@@ -1635,6 +1729,7 @@ int orc_step(orc_ctx *c, int nsteps) {
                 g2p_thermal(c, true, false);
                 p2g_thermal(c, true, false, true);
                 bcs_temperature(c);
+                udf_bcs_temperature(c);
                 g2p_thermal(c, false, true);
             }
             if ((rc = orc_move(c, dt, step))) return rc;

@@ -120,6 +120,34 @@ def test_thermal_step_matches_the_oracle(gpu, order, scheme, sorted_, periodic):
     assert rel(pg, po) <= 1e-10
 
 
+@pytest.mark.parametrize("scheme", [USL, MUSL])
+def test_face_functions_match_the_oracle(gpu, scheme):
+    """Temperature functions on five faces (isothermal, user flux, two
+    convection faces and an adiabatic one, where a function changes
+    nothing), varying in space and time, after the wall conditions of every
+    BC pass of the step."""
+    say(f"[thermal] face functions scheme={scheme}")
+    s, pos, r, idata = scenes.random_block(101, n=(10, 9, 8), order=2, model=0, scheme=scheme,
+                                           frac=1.0, speed=0.1)
+    th = thermal_fields(pos.shape[1], 17)
+    g, o = pair(s, pos, r, idata, th, mixed_walls(s))
+    fns = {
+        (0, 0): lambda x, y, z, t: (1.0 + y + 2.0 * z + 50.0 * t, 0.0),
+        (1, 0): lambda x, y, z, t: (5.0 * np.sin(8.0 * x) + t, 0.0),
+        (0, 1): lambda x, y, z, t: (20.0 + 10.0 * z, y - t),
+        (2, 1): lambda x, y, z, t: (15.0 + 30.0 * x * y, 2.0),
+        (2, 0): lambda x, y, z, t: (99.0, 99.0),
+    }
+    for c in (g, o):
+        for (d, hi), fn in fns.items():
+            c.set_wall_temp_udf(d, hi, fn)
+        c.step(1)
+    compare(g, o, 1e-12)
+    for c in (g, o):
+        c.step(9)
+    compare(g, o, 1e-10)
+
+
 def test_thermal_fields_follow_the_particles_through_removals(gpu):
     """Particles leaving a non-periodic box are removed (Redistribute); the
     thermal record follows the survivors by upload index on both sides."""

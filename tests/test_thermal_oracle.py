@@ -189,3 +189,55 @@ def test_thermal_fields_follow_the_upload_index():
     bad.bchi[2] = 7
     with pytest.raises(capi.MPMError):
         o.enable_thermal(bad)
+
+
+def test_constant_face_functions_equal_the_constant_walls():
+    """On the slab (y and z periodic), a face function that returns the
+    wall's own parameters reproduces the wall exactly, for every type."""
+    cases = [
+        (capi.BC_TEMP_ISOTHERMAL, lambda ts: setattr_pair(ts, "T_wall", 0.3, 0.9), (0.3, 0.0),
+         (0.9, 0.0)),
+        (capi.BC_TEMP_USERFLUX, lambda ts: setattr_pair(ts, "flux", 2.0, -1.0), (2.0, 0.0),
+         (-1.0, 0.0)),
+        (capi.BC_TEMP_CONVECTION, lambda ts: (setattr_pair(ts, "h", 30.0, 10.0),
+                                              setattr_pair(ts, "T_inf", -1.0, 4.0)),
+         (30.0, -1.0), (10.0, 4.0)),
+    ]
+    for bc, params, vlo, vhi in cases:
+        def walls(ts, bc=bc, params=params):
+            ts.bclo[0] = ts.bchi[0] = bc
+            params(ts)
+
+        o1, _, _ = slab(2, 16, 2e-5, 0, lambda x: 1.0 + x * x, walls, steps=0, threads=1)
+        o2, _, _ = slab(2, 16, 2e-5, 0, lambda x: 1.0 + x * x, walls, steps=0, threads=1)
+        o2.set_wall_temp_udf(0, 0, lambda x, y, z, t, v=vlo: v)
+        o2.set_wall_temp_udf(0, 1, lambda x, y, z, t, v=vhi: v)
+        for o in (o1, o2):
+            o.step(10)
+        assert np.array_equal(o1.download_thermal(), o2.download_thermal()), bc
+
+
+def setattr_pair(ts, name, lo, hi):
+    getattr(ts, name + "_lo")[0] = lo
+    getattr(ts, name + "_hi")[0] = hi
+
+
+def test_face_function_sees_node_coordinates_and_step_time():
+    """An isothermal face function T(x, y, z, t) = 10 y + 100 z + 1000 t:
+    after a step the face nodes hold it at their coordinates and the step's
+    start time."""
+    def iso(ts):
+        ts.bclo[0] = capi.BC_TEMP_ISOTHERMAL
+
+    o, _, _ = slab(1, 16, 2e-5, 0, lambda x: np.ones_like(x), iso, steps=3, threads=1)
+    o.set_wall_temp_udf(0, 0, lambda x, y, z, t: (10 * y + 100 * z + 1000 * t, 0.0))
+    t0 = o.time()[0]
+    o.step(1)
+    s = o.specs
+    n = list(s.ncells)
+    T = o.nodal(capi.N_TEMPERATURE).reshape(n[2] + 1, n[1] + 1, n[0] + 1)
+    dy = (s.phi[1] - s.plo[1]) / n[1]
+    dz = (s.phi[2] - s.plo[2]) / n[2]
+    K, J = np.meshgrid(np.arange(n[2] + 1), np.arange(n[1] + 1), indexing="ij")
+    expect = 10 * (J * dy) + 100 * (K * dz) + 1000 * t0
+    assert np.abs(T[:, :, 0] - expect).max() < 1e-12
