@@ -205,6 +205,11 @@ SIGNATURES = {
     "mpm_upload_thermal": (ctypes.c_int, [_P, ctypes.c_int64, _DP]),
     "mpm_download_thermal": (ctypes.c_int, [_P, ctypes.c_int64, _DP]),
     "mpm_set_wall_temp_udf": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, UDF_FN]),
+    "mpm_read_particle_file_thermal": (ctypes.c_int, [ctypes.c_char_p, _LP, _DP, _DP, _IP, _DP,
+                                                      _DP]),
+    "mpm_load_particle_file_thermal": (ctypes.c_int, [_P, ctypes.c_char_p, _DP]),
+    "mpm_write_particle_file_thermal": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int64, _DP, _DP,
+                                                       _IP, _DP]),
 }
 
 
@@ -439,10 +444,14 @@ class Context(_Base):
         self._call("remove_particles_inside_levelset", self._ctx, ctypes.byref(n))
         return n.value
 
-    def load_particle_file(self, path):
-        """mpm_load_particle_file; returns (total mass, total volume)."""
+    def load_particle_file(self, path, thermal=False):
+        """mpm_load_particle_file (thermal: the USE_TEMP layout, also
+        uploading the thermal fields); returns (total mass, total volume)."""
         tot = np.zeros(2)
-        self._call("load_particle_file", self._ctx, os.fsencode(path), dptr(tot))
+        name = "load_particle_file_thermal" if thermal else "load_particle_file"
+        self._call(name, self._ctx, os.fsencode(path), dptr(tot))
+        if thermal:
+            self._n_thermal = self.num_particles()
         return float(tot[0]), float(tot[1])
 
     def particle_field(self, field, n_upload):
@@ -718,8 +727,9 @@ class Batch:
         return {"step_ms": ms.value, "particles": n.value, "tasks": k.value}
 
 
-def read_particle_file(path):
-    """mpm_read_particle_file: (pos, rdata, idata, (total mass, total volume))."""
+def read_particle_file(path, thermal=False):
+    """mpm_read_particle_file: (pos, rdata, idata, (total mass, total volume));
+    thermal: mpm_read_particle_file_thermal, (pos, rdata, idata, th, totals)."""
     lib = load()
     n = ctypes.c_int64(0)
     _check(lib, lib.mpm_read_particle_file(os.fsencode(path), ctypes.byref(n), None, None, None,
@@ -728,6 +738,13 @@ def read_particle_file(path):
     rdata = np.zeros((NREAL, n.value))
     idata = np.zeros((NINT, n.value), dtype=np.int32)
     tot = np.zeros(2)
+    if thermal:
+        th = np.zeros((NTHERMAL, n.value))
+        _check(lib, lib.mpm_read_particle_file_thermal(os.fsencode(path), ctypes.byref(n),
+                                                       dptr(pos), dptr(rdata), iptr(idata),
+                                                       dptr(th), dptr(tot)),
+               "mpm_read_particle_file_thermal")
+        return pos, rdata, idata, th, (float(tot[0]), float(tot[1]))
     _check(lib, lib.mpm_read_particle_file(os.fsencode(path), ctypes.byref(n), dptr(pos),
                                            dptr(rdata), iptr(idata), dptr(tot)),
            "mpm_read_particle_file")
@@ -757,11 +774,19 @@ def autogen_particles(specs, mincoords, maxcoords, vel, ppc, dens, model, E=0.0,
     return pos, rdata, idata, (float(tot[0]), float(tot[1]))
 
 
-def write_particle_file(path, pos, rdata, idata):
+def write_particle_file(path, pos, rdata, idata, th=None):
+    """mpm_write_particle_file; with th, mpm_write_particle_file_thermal."""
     lib = load()
     pos = np.ascontiguousarray(pos, dtype=np.float64)
     rdata = np.ascontiguousarray(rdata, dtype=np.float64)
     idata = np.ascontiguousarray(idata, dtype=np.int32)
+    if th is not None:
+        th = np.ascontiguousarray(th, dtype=np.float64)
+        _check(lib, lib.mpm_write_particle_file_thermal(os.fsencode(path), pos.shape[1],
+                                                        dptr(pos), dptr(rdata), iptr(idata),
+                                                        dptr(th)),
+               "mpm_write_particle_file_thermal")
+        return
     _check(lib, lib.mpm_write_particle_file(os.fsencode(path), pos.shape[1], dptr(pos),
                                             dptr(rdata), iptr(idata)), "mpm_write_particle_file")
 

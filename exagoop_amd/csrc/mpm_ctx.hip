@@ -467,6 +467,22 @@ int mpm_write_checkpoint(mpm_ctx *c, const char *dir, int output_it) {
     f.write(reinterpret_cast<const char *>(c->cold_rigid.data()),
             c->cold_rigid.size() * sizeof(int));
     if (!f.good()) return set_err(MPM_E_ARG, "cannot write checkpoint particles");
+    if (!c->thermal) return MPM_OK;
+    // The USE_TEMP build's checkpoint carries realData 46..52 with the rest
+    // of the record: here a file of their own, by upload index.
+    const int64_t nu = (int64_t)c->cold_radius.size();
+    std::vector<double> th((size_t)MPM_NTHERMAL * nu);
+    if (nu) {
+        rc = mpm_download_thermal(c, nu, th.data());
+        if (rc) return rc;
+    }
+    std::ofstream t(std::string(dir) + "/thermal.bin",
+                    std::ios::out | std::ios::trunc | std::ios::binary);
+    const int64_t thdr[2] = {nu, MPM_NTHERMAL};
+    t.write("XMPMTHM1", 8);
+    t.write(reinterpret_cast<const char *>(thdr), sizeof(thdr));
+    t.write(reinterpret_cast<const char *>(th.data()), th.size() * sizeof(double));
+    if (!t.good()) return set_err(MPM_E_ARG, "cannot write checkpoint thermal fields");
     return MPM_OK;
 }
 
@@ -550,6 +566,19 @@ int mpm_read_checkpoint(mpm_ctx *c, const char *dir, int *output_it) {
     int rc = upload_impl(c, n, pos.data(), r.data(), idata.data(), id.data(), nu, crad.data(),
                          cyacc.data(), crig.data());
     if (rc) return rc;
+    std::ifstream t(std::string(dir) + "/thermal.bin", std::ios::binary);
+    if (c->thermal && t.good()) {
+        int64_t thdr[2];
+        if (!t.read(magic, 8) || std::memcmp(magic, "XMPMTHM1", 8) != 0 ||
+            !t.read(reinterpret_cast<char *>(thdr), sizeof(thdr)) || thdr[0] != nu ||
+            thdr[1] != MPM_NTHERMAL)
+            return set_err(MPM_E_ARG, "checkpoint thermal file does not match its particles");
+        std::vector<double> th((size_t)MPM_NTHERMAL * nu);
+        if (!t.read(reinterpret_cast<char *>(th.data()), th.size() * sizeof(double)))
+            return set_err(MPM_E_ARG, "truncated checkpoint thermal file");
+        rc = mpm_upload_thermal(c, nu, th.data());
+        if (rc) return rc;
+    }
     c->steps = steps;
     Scalars sc;
     std::memset(&sc, 0, sizeof(sc));

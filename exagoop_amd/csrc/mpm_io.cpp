@@ -76,12 +76,10 @@ int read_header(std::ifstream &ifs, int64_t &np) {
     return MPM_OK;
 }
 
-}  // namespace
-
-extern "C" {
-
-int mpm_read_particle_file(const char *path, int64_t *np, double *pos, double *rdata, int *idata,
-                           double totals[2]) {
+// InitParticles(filename); th non-null: the USE_TEMP build's layout, four
+// more columns per particle after the model's (mpm_init.cpp:1097-1108).
+int read_file(const char *path, int64_t *np, double *pos, double *rdata, int *idata, double *th,
+              double totals[2]) {
     if (!path || !np) return set_err(MPM_E_ARG, "null argument");
     std::ifstream ifs(path);
     if (!ifs.good()) return set_err(MPM_E_ARG, "cannot open particle file");
@@ -121,6 +119,18 @@ int mpm_read_particle_file(const char *path, int64_t *np, double *pos, double *r
         } else {
             return set_err(MPM_E_MODEL, "Incorrect constitutive model");
         }
+        if (th) {
+            double T, cp, kc, Q;
+            if (!read_tok(ifs, T, "Error reading temperature")) return MPM_E_ARG;
+            if (!read_tok(ifs, cp, "Error reading specific_heat")) return MPM_E_ARG;
+            if (!read_tok(ifs, kc, "Error reading thermal_conductivity")) return MPM_E_ARG;
+            if (!read_tok(ifs, Q, "Error reading heat_source")) return MPM_E_ARG;
+            th[MPM_TH_TEMPERATURE * n + i] = T;
+            th[MPM_TH_SPECIFIC_HEAT * n + i] = cp;
+            th[MPM_TH_CONDUCTIVITY * n + i] = kc;
+            for (int d = 0; d < 3; ++d) th[(MPM_TH_HEAT_FLUX + d) * n + i] = 0.0;
+            th[MPM_TH_HEAT_SOURCE * n + i] = Q;
+        }
         for (int d = 0; d < 3; ++d) {
             pos[d * n + i] = x[d];
             rdata[(MPM_R_VEL + d) * n + i] = v[d];
@@ -150,16 +160,77 @@ int mpm_read_particle_file(const char *path, int64_t *np, double *pos, double *r
     return MPM_OK;
 }
 
-int mpm_load_particle_file(mpm_ctx *c, const char *path, double totals[2]) {
+int load_file(mpm_ctx *c, const char *path, bool thermal, double totals[2]) {
     if (!c || !path) return set_err(MPM_E_ARG, "null argument");
+    if (thermal && !c->thermal) return set_err(MPM_E_ARG, "thermal module not enabled");
     int64_t n = 0;
-    int rc = mpm_read_particle_file(path, &n, nullptr, nullptr, nullptr, nullptr);
+    int rc = read_file(path, &n, nullptr, nullptr, nullptr, nullptr, nullptr);
     if (rc) return rc;
-    std::vector<double> pos(3 * n), r((size_t)MPM_NREAL * n);
+    std::vector<double> pos(3 * n), r((size_t)MPM_NREAL * n), th;
     std::vector<int> idata((size_t)MPM_NINT * n);
-    rc = mpm_read_particle_file(path, &n, pos.data(), r.data(), idata.data(), totals);
+    if (thermal) th.resize((size_t)MPM_NTHERMAL * n);
+    rc = read_file(path, &n, pos.data(), r.data(), idata.data(), thermal ? th.data() : nullptr,
+                   totals);
     if (rc) return rc;
-    return mpm_upload_particles(c, n, pos.data(), r.data(), idata.data());
+    rc = mpm_upload_particles(c, n, pos.data(), r.data(), idata.data());
+    if (rc || !thermal) return rc;
+    return mpm_upload_thermal(c, n, th.data());
+}
+
+int write_file(const char *path, int64_t np, const double *pos, const double *rdata,
+               const int *idata, const double *th) {
+    if (!path || np < 1 || !pos || !rdata || !idata) return set_err(MPM_E_ARG, "bad argument");
+    FILE *f = std::fopen(path, "w");
+    if (!f) return set_err(MPM_E_ARG, "cannot open particle file for writing");
+    const int m0 = idata[MPM_I_MODEL * np];
+    std::fprintf(f, "dim: 3\nnumber_of_material_points: %lld\n", (long long)np);
+    std::fprintf(f, "# phase x y z radius density vx vy vz cm_id%s%s\n",
+                 m0 == 1 ? " Bulk_modulus Gamma_pressure Dynamic_viscosity" : " E nu",
+                 th ? " T spheat thermcond heatsrc" : "");
+    for (int64_t i = 0; i < np; ++i) {
+        const int model = idata[MPM_I_MODEL * np + i];
+        std::fprintf(f, "%d", idata[MPM_I_PHASE * np + i]);
+        for (int d = 0; d < 3; ++d) std::fprintf(f, "\t%.6e", pos[d * np + i]);
+        std::fprintf(f, "\t%.6e\t%.6e", rdata[MPM_R_RADIUS * np + i], rdata[MPM_R_DENSITY * np + i]);
+        for (int d = 0; d < 3; ++d) std::fprintf(f, "\t%.6e", rdata[(MPM_R_VEL + d) * np + i]);
+        std::fprintf(f, "\t%d", model);
+        if (model == 1)
+            std::fprintf(f, "\t%.6e\t%.6e\t%.6e", rdata[MPM_R_BULK * np + i],
+                         rdata[MPM_R_GAMMA * np + i], rdata[MPM_R_VISC * np + i]);
+        else
+            std::fprintf(f, "\t%.6e\t%.6e", rdata[MPM_R_E * np + i], rdata[MPM_R_NU * np + i]);
+        if (th)
+            std::fprintf(f, "\t%.6e\t%.6e\t%.6e\t%.6e", th[MPM_TH_TEMPERATURE * np + i],
+                         th[MPM_TH_SPECIFIC_HEAT * np + i], th[MPM_TH_CONDUCTIVITY * np + i],
+                         th[MPM_TH_HEAT_SOURCE * np + i]);
+        std::fprintf(f, "\n");
+    }
+    const bool ok = std::ferror(f) == 0;
+    std::fclose(f);
+    return ok ? MPM_OK : set_err(MPM_E_ARG, "write failed");
+}
+
+}  // namespace
+
+extern "C" {
+
+int mpm_read_particle_file(const char *path, int64_t *np, double *pos, double *rdata, int *idata,
+                           double totals[2]) {
+    return read_file(path, np, pos, rdata, idata, nullptr, totals);
+}
+
+int mpm_read_particle_file_thermal(const char *path, int64_t *np, double *pos, double *rdata,
+                                   int *idata, double *th, double totals[2]) {
+    if (pos && !th) return set_err(MPM_E_ARG, "null argument");
+    return read_file(path, np, pos, rdata, idata, pos ? th : nullptr, totals);
+}
+
+int mpm_load_particle_file(mpm_ctx *c, const char *path, double totals[2]) {
+    return load_file(c, path, false, totals);
+}
+
+int mpm_load_particle_file_thermal(mpm_ctx *c, const char *path, double totals[2]) {
+    return load_file(c, path, true, totals);
 }
 
 int mpm_autogen_particles(const mpm_specs *s, const double mincoords[3],
@@ -260,30 +331,13 @@ int mpm_autogen_particles(const mpm_specs *s, const double mincoords[3],
 
 int mpm_write_particle_file(const char *path, int64_t np, const double *pos, const double *rdata,
                             const int *idata) {
-    if (!path || np < 1 || !pos || !rdata || !idata) return set_err(MPM_E_ARG, "bad argument");
-    FILE *f = std::fopen(path, "w");
-    if (!f) return set_err(MPM_E_ARG, "cannot open particle file for writing");
-    const int m0 = idata[MPM_I_MODEL * np];
-    std::fprintf(f, "dim: 3\nnumber_of_material_points: %lld\n", (long long)np);
-    std::fprintf(f, "# phase x y z radius density vx vy vz cm_id%s\n",
-                 m0 == 1 ? " Bulk_modulus Gamma_pressure Dynamic_viscosity" : " E nu");
-    for (int64_t i = 0; i < np; ++i) {
-        const int model = idata[MPM_I_MODEL * np + i];
-        std::fprintf(f, "%d", idata[MPM_I_PHASE * np + i]);
-        for (int d = 0; d < 3; ++d) std::fprintf(f, "\t%.6e", pos[d * np + i]);
-        std::fprintf(f, "\t%.6e\t%.6e", rdata[MPM_R_RADIUS * np + i], rdata[MPM_R_DENSITY * np + i]);
-        for (int d = 0; d < 3; ++d) std::fprintf(f, "\t%.6e", rdata[(MPM_R_VEL + d) * np + i]);
-        std::fprintf(f, "\t%d", model);
-        if (model == 1)
-            std::fprintf(f, "\t%.6e\t%.6e\t%.6e", rdata[MPM_R_BULK * np + i],
-                         rdata[MPM_R_GAMMA * np + i], rdata[MPM_R_VISC * np + i]);
-        else
-            std::fprintf(f, "\t%.6e\t%.6e", rdata[MPM_R_E * np + i], rdata[MPM_R_NU * np + i]);
-        std::fprintf(f, "\n");
-    }
-    const bool ok = std::ferror(f) == 0;
-    std::fclose(f);
-    return ok ? MPM_OK : set_err(MPM_E_ARG, "write failed");
+    return write_file(path, np, pos, rdata, idata, nullptr);
+}
+
+int mpm_write_particle_file_thermal(const char *path, int64_t np, const double *pos,
+                                    const double *rdata, const int *idata, const double *th) {
+    if (!th) return set_err(MPM_E_ARG, "null argument");
+    return write_file(path, np, pos, rdata, idata, th);
 }
 
 }  // extern "C"

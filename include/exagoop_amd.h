@@ -165,6 +165,13 @@ int mpm_read_particle_file(const char *path, int64_t *np, double *pos, double *r
                            int *idata, double totals[2]);
 /* The same, uploaded into the context (InitParticles + its Redistribute). */
 int mpm_load_particle_file(mpm_ctx *ctx, const char *path, double totals[2]);
+/* The USE_TEMP build's file (mpm_init.cpp:1097-1108): four more columns per
+ * particle after the model's, temperature specific_heat thermal_conductivity
+ * heat_source; th [MPM_NTHERMAL * np] receives them (heat flux zero).  The
+ * loader needs the thermal module enabled and uploads both records. */
+int mpm_read_particle_file_thermal(const char *path, int64_t *np, double *pos, double *rdata,
+                                   int *idata, double *th, double totals[2]);
+int mpm_load_particle_file_thermal(mpm_ctx *ctx, const char *path, double totals[2]);
 /* InitParticles(mincoords, maxcoords, vel, ppc, ...) with generate_particle
  * (mpm_init.cpp:1199-1473): particles of the cells of the specs' domain (x
  * fastest) inside [mincoords, maxcoords]; one at each cell centre, or with
@@ -182,6 +189,10 @@ int mpm_autogen_particles(const mpm_specs *s, const double mincoords[3],
  * model, and each row carries its own model's fields. */
 int mpm_write_particle_file(const char *path, int64_t np, const double *pos, const double *rdata,
                             const int *idata);
+/* The same with the generator's temperature columns (T spheat thermcond
+ * heatsrc, Generate_MPs_Inputfile_Generic.py:1083-1090) from th. */
+int mpm_write_particle_file_thermal(const char *path, int64_t np, const double *pos,
+                                    const double *rdata, const int *idata, const double *th);
 
 /* Checkpoint / restart.  mpm_write_checkpoint replaces writeCheckpointFile
  * (mpm_particle_outputs.cpp:355-499): dir/Header in the reference's layout
@@ -192,7 +203,9 @@ int mpm_write_particle_file(const char *path, int64_t np, const double *pos, con
  * (:537-589): the context takes the particles (ids kept), step count and
  * time; as the reference's driver does (main.cpp:65-69), the caller then runs
  * mpm_init_internal_forces.  No nodal data is stored (the reference stores
- * none). */
+ * none).  With the thermal module enabled, dir/thermal.bin holds the thermal
+ * fields by upload index (the USE_TEMP build's record carries them), and a
+ * context with the module enabled reads them back. */
 int mpm_write_checkpoint(mpm_ctx *ctx, const char *dir, int output_it);
 /* writeAsciiFiles (mpm_particle_outputs.cpp:188-196): the live particles as
  * text in "<prefix>_t<time with 6 decimals>" (the reference's name rule; the

@@ -48,10 +48,12 @@ def _blank(n, dim):
     return pos, r, idata
 
 
-def read_particle_file(path, dim=3, max_records=None):
+def read_particle_file(path, dim=3, max_records=None, thermal=False):
     """(pos, rdata, idata, (total mass, total volume)) as InitParticles fills
     the particle tile; raises FileError where it aborts.  max_records reads
-    only the first records (for a file cut after its head)."""
+    only the first records (for a file cut after its head).  thermal: the
+    USE_TEMP build's layout (mpm_init.cpp:1097-1108), returning
+    (pos, rdata, idata, th, totals) with th rows T, c_p, k, q (3, zero), Q."""
     with open(path) as f:
         text = f.read()
     lines = text.split("\n")
@@ -74,6 +76,7 @@ def read_particle_file(path, dim=3, max_records=None):
     if max_records is not None:
         n = min(n, max_records)
     pos, r, idata = _blank(n, dim)
+    th = np.zeros((7, n))
     k = 0
     at = 0  # characters consumed (the stream position)
     tmass = tvol = 0.0
@@ -113,6 +116,11 @@ def read_particle_file(path, dim=3, max_records=None):
             r[R_VISC, i] = take(float, "Dynamic_viscosity")
         else:
             raise FileError("Incorrect constitutive model")
+        if thermal:
+            th[0, i] = take(float, "temperature")
+            th[1, i] = take(float, "specific_heat")
+            th[2, i] = take(float, "thermal_conductivity")
+            th[6, i] = take(float, "heat_source")
         vol = FOURBYTHREE * PI * math.pow(rad, THREE)
         r[R_VOLUME, i] = vol
         r[R_MASS, i] = dens * vol
@@ -120,6 +128,8 @@ def read_particle_file(path, dim=3, max_records=None):
         if phase == 0:
             tmass += dens * vol
             tvol += vol
+    if thermal:
+        return pos, r, idata, th, (tmass, tvol)
     return pos, r, idata, (tmass, tvol)
 
 
@@ -171,14 +181,17 @@ def autogen_particles(plo, phi, ncells, mincoords, maxcoords, vel, ppc, dens, mo
     return pos, r, idata, (tm, tv)
 
 
-def write_particle_file(path, pos, r, idata):
+def write_particle_file(path, pos, r, idata, th=None):
     """The generator's ASCII layout for a 3-D file: header, '#' column line
-    (core fields, then the model fields of the first particle's model), rows
+    (core fields, then the model fields of the first particle's model, then
+    with th the temperature fields T spheat thermcond heatsrc), rows
     tab-separated with integers as str() and reals as f"{v:.6e}"."""
     n = pos.shape[1]
     m0 = int(idata[I_MODEL, 0])
     cols = ["phase", "x", "y", "z", "radius", "density", "vx", "vy", "vz", "cm_id"]
     cols += ["Bulk_modulus", "Gamma_pressure", "Dynamic_viscosity"] if m0 == 1 else ["E", "nu"]
+    if th is not None:
+        cols += ["T", "spheat", "thermcond", "heatsrc"]
     out = ["dim: 3", f"number_of_material_points: {n}", "# " + " ".join(cols)]
     for i in range(n):
         model = int(idata[I_MODEL, i])
@@ -191,6 +204,8 @@ def write_particle_file(path, pos, r, idata):
             row += [f"{r[f, i]:.6e}" for f in (R_BULK, R_GAMMA, R_VISC)]
         else:
             row += [f"{r[f, i]:.6e}" for f in (R_E, R_NU)]
+        if th is not None:
+            row += [f"{th[f, i]:.6e}" for f in (0, 1, 2, 6)]
         out.append("\t".join(row))
     with open(path, "w") as f:
         f.write("\n".join(out) + "\n")

@@ -193,3 +193,65 @@ def test_ascii_particle_output(gpu, tmp_path):
     assert np.array_equal(rows[:, 3].astype(np.int64), ids)
     assert np.array_equal(rows[:, 5:5 + capi.NREAL].T, rr)
     assert np.array_equal(rows[:, 5 + capi.NREAL:].T.astype(np.int32), ii)
+
+
+def test_checkpoint_carries_the_thermal_fields(gpu, tmp_path):
+    """With the thermal module on, the checkpoint carries the thermal record
+    by upload index; a context with the module enabled reads it back, and
+    one without it reads the mechanical record only."""
+    say("[io] thermal checkpoint")
+    s, pos, r, idata = elastic_scene()
+    n = pos.shape[1]
+    rng = np.random.default_rng(11)
+    th = np.zeros((capi.NTHERMAL, n))
+    th[capi.TH_TEMPERATURE] = rng.uniform(0, 1, n)
+    th[capi.TH_SPECIFIC_HEAT] = 1.0
+    th[capi.TH_CONDUCTIVITY] = 30.0
+    a = capi.Context(s)
+    a.upload(pos, r, idata)
+    a.enable_thermal()
+    a.upload_thermal(th)
+    a.init_internal_forces()
+    a.step(5)
+    ck = str(tmp_path / "chk_thermal")
+    a.write_checkpoint(ck)
+    b = capi.Context(s)
+    b.enable_thermal()
+    b.read_checkpoint(ck)
+    tha, thb = a.download_thermal(), b.download_thermal(n)
+    assert np.array_equal(np.isnan(tha), np.isnan(thb))
+    live = ~np.isnan(tha)
+    assert np.array_equal(tha[live], thb[live])
+    c = capi.Context(s)
+    c.read_checkpoint(ck)
+    pc, _, _, _ = c.download()
+    pa, _, _, _ = a.download()
+    assert np.array_equal(pc, pa)
+
+
+def test_thermal_particle_file_loads_both_records(gpu, tmp_path):
+    say("[io] thermal particle file -> context")
+    s, pos, r, idata = scenes.random_block(43, order=2, model=1)
+    n = pos.shape[1]
+    rng = np.random.default_rng(12)
+    th = np.zeros((capi.NTHERMAL, n))
+    th[capi.TH_TEMPERATURE] = rng.uniform(0, 1, n)
+    th[capi.TH_SPECIFIC_HEAT] = rng.uniform(1, 2, n)
+    th[capi.TH_CONDUCTIVITY] = rng.uniform(20, 40, n)
+    th[capi.TH_HEAT_SOURCE] = rng.uniform(-10, 10, n)
+    path = str(tmp_path / "mpm_particles_thermal.dat")
+    capi.write_particle_file(path, pos, r, idata, th)
+    fpos, fr, fid, fth, tot = capi.read_particle_file(path, thermal=True)
+    a = capi.Context(s)
+    a.enable_thermal()
+    assert a.load_particle_file(path, thermal=True) == tot
+    b = capi.Context(s)
+    b.upload(fpos, fr, fid)
+    b.enable_thermal()
+    b.upload_thermal(fth)
+    for g in (a, b):
+        g.init_internal_forces()
+        g.step(5)
+    ta, tb = a.download_thermal(), b.download_thermal()
+    for f in range(capi.NTHERMAL):
+        assert rel(ta[f], tb[f]) <= 1e-12, f

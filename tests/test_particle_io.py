@@ -143,3 +143,49 @@ def test_autogen_matches_the_oracle(multi, ppc):
         assert np.allclose(rl[oio.R_VOLUME], cell / (ppc[0] * ppc[1]))
     rad = (3.0 * 0.25 * rl[oio.R_VOLUME][0] / 3.141592654) ** (1.0 / 3.0)
     assert rl[oio.R_RADIUS][0] == rad
+
+
+def test_thermal_columns_write_and_read_as_the_use_temp_build(tmp_path):
+    """The generator's temperature columns (T spheat thermcond heatsrc after
+    the model's) written byte for byte as the restated generator writes them,
+    and read back as the USE_TEMP reader does (heat flux zero)."""
+    pos, r, idata = random_particles(5)
+    n = pos.shape[1]
+    rng = np.random.default_rng(6)
+    th = np.zeros((capi.NTHERMAL, n))
+    th[capi.TH_TEMPERATURE] = rng.uniform(250, 400, n)
+    th[capi.TH_SPECIFIC_HEAT] = rng.uniform(100, 4000, n)
+    th[capi.TH_CONDUCTIVITY] = rng.uniform(0.1, 400, n)
+    th[capi.TH_HEAT_FLUX:capi.TH_HEAT_FLUX + 3] = 9.0  # not written
+    th[capi.TH_HEAT_SOURCE] = rng.uniform(-1e3, 1e3, n)
+    a, b = str(tmp_path / "lib.dat"), str(tmp_path / "orc.dat")
+    capi.write_particle_file(a, pos, r, idata, th)
+    oio.write_particle_file(b, pos, r, idata, th)
+    assert open(a).read() == open(b).read()
+    assert open(a).read().split("\n")[2].endswith("Dynamic_viscosity T spheat thermcond heatsrc")
+    pl, rl, il, thl, tl = capi.read_particle_file(a, thermal=True)
+    po, ro, io_, tho, to = oio.read_particle_file(a, thermal=True)
+    assert np.array_equal(pl, po) and np.array_equal(rl, ro) and np.array_equal(il, io_)
+    assert np.array_equal(thl, tho) and tl == to
+    six = np.vectorize(lambda v: float(f"{v:.6e}"))
+    for f in (capi.TH_TEMPERATURE, capi.TH_SPECIFIC_HEAT, capi.TH_CONDUCTIVITY,
+              capi.TH_HEAT_SOURCE):
+        assert np.array_equal(thl[f], six(th[f]))
+    assert np.all(thl[capi.TH_HEAT_FLUX:capi.TH_HEAT_FLUX + 3] == 0.0)
+    # the mechanical reader stops short of the extra columns: the next
+    # record's phase is read from the temperature column and the record
+    # after fails
+    with pytest.raises(capi.MPMError):
+        capi.read_particle_file(a)
+
+
+@pytest.mark.parametrize("cut,msg", [(1, "heat_source"), (2, "thermal_conductivity"),
+                                     (3, "specific_heat"), (4, "temperature")])
+def test_truncated_thermal_columns_are_errors(tmp_path, cut, msg):
+    row = "0 0.1 0.1 0.1 0.01 1000 0 0 0 0 1e4 0.3 300 1000 2.5 7".split()
+    text = "dim: 3\nnumber_of_material_points: 1\n# h\n" + " ".join(row[:-cut]) + "\n"
+    path = write_text(tmp_path, "t.dat", text)
+    with pytest.raises(oio.FileError, match="Error reading " + msg):
+        oio.read_particle_file(path, thermal=True)
+    with pytest.raises(capi.MPMError, match="Error reading " + msg):
+        capi.read_particle_file(path, thermal=True)

@@ -241,3 +241,15 @@ def test_face_function_sees_node_coordinates_and_step_time():
     K, J = np.meshgrid(np.arange(n[2] + 1), np.arange(n[1] + 1), indexing="ij")
     expect = 10 * (J * dy) + 100 * (K * dz) + 1000 * t0
     assert np.abs(T[:, :, 0] - expect).max() < 1e-12
+
+
+def test_reference_1d_heat_conduction_validator():
+    """The reference's own validation case (Tests/1D_Heat_Conduction): walls
+    at 0 and 1, 100 cells, two particles per cell, linear, MUSL, dt 1e-5;
+    at t = 0.05 the particle temperatures are within the validator's RMS
+    tolerance of the series solution."""
+    import heat_cases
+
+    rms, o = heat_cases.run(lambda s: oracle.Oracle(s, threads=1), "1D_Heat_Conduction")
+    assert o.time()[1] == 5000
+    assert rms < heat_cases.CASES["1D_Heat_Conduction"]["tol"] / 5, rms
