@@ -103,6 +103,21 @@ class MPMParticleContainer {
     void SetWallUDF(int dir, int hi, mpm_wall_udf_fn fn) {
         check(mpm_set_wall_udf(ctx_, dir, hi, fn), "mpm_set_wall_udf");
     }
+    // Thermal module (the USE_TEMP build): the temperature wall conditions of
+    // MPMspecs (bclo_temp / bchi_temp and their parameters), then the thermal
+    // fields by upload index.
+    void EnableThermal(const mpm_thermal_specs &t) {
+        check(mpm_enable_thermal(ctx_, &t), "mpm_enable_thermal");
+    }
+    void SetThermalFields(int64_t n_upload, const double *th) {
+        check(mpm_upload_thermal(ctx_, n_upload, th), "mpm_upload_thermal");
+    }
+    void GetThermalFields(int64_t n_upload, double *th) const {
+        check(mpm_download_thermal(ctx_, n_upload, th), "mpm_download_thermal");
+    }
+    void SetWallTempUDF(int dir, int hi, mpm_wall_temp_udf_fn fn) {
+        check(mpm_set_wall_temp_udf(ctx_, dir, hi, fn), "mpm_set_wall_temp_udf");
+    }
     mpm_ctx *raw() { return ctx_; }
 
   private:
