@@ -35,6 +35,7 @@ N_MASS_SPHEAT, N_MASS_SPHEAT_TEMP, N_TEMPERATURE = 20, 21, 22
 N_SOURCE_TEMP, N_DELTA_TEMPERATURE, N_MASS_CONDUCTIVITY = 23, 24, 25
 BC_TEMP_PERIODIC, BC_TEMP_ISOTHERMAL, BC_TEMP_ADIABATIC = 0, 1, 2
 BC_TEMP_USERFLUX, BC_TEMP_CONVECTION = 3, 4
+LS_TEMP_ADIABATIC, LS_TEMP_ISOTHERMAL, LS_TEMP_HEATFLUX, LS_TEMP_CONVECTIVE = 0, 1, 3, 4
 USL, MUSL = 0, 1
 
 OPT_PROFILE, OPT_KEEP_FORCES, OPT_SORTED, OPT_FAST, OPT_FUSED_DT = 1, 2, 3, 4, 5
@@ -205,6 +206,9 @@ SIGNATURES = {
     "mpm_upload_thermal": (ctypes.c_int, [_P, ctypes.c_int64, _DP]),
     "mpm_download_thermal": (ctypes.c_int, [_P, ctypes.c_int64, _DP]),
     "mpm_set_wall_temp_udf": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, UDF_FN]),
+    "mpm_set_levelset_body_temp": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                                  ctypes.c_double, ctypes.c_double,
+                                                  ctypes.c_double]),
     "mpm_read_particle_file_thermal": (ctypes.c_int, [ctypes.c_char_p, _LP, _DP, _DP, _IP, _DP,
                                                       _DP]),
     "mpm_load_particle_file_thermal": (ctypes.c_int, [_P, ctypes.c_char_p, _DP]),
@@ -375,6 +379,11 @@ class _Base:
         self._udf_refs = getattr(self, "_udf_refs", [])
         self._udf_refs.append(cb)
         self._call("set_wall_temp_udf", self._ctx, d, hi, cb)
+
+    def set_levelset_body_temp(self, body, temp_bc, T_wall=0.0, heat_flux=0.0, h_conv=0.0,
+                               T_inf=0.0):
+        self._call("set_levelset_body_temp", self._ctx, body, temp_bc, T_wall, heat_flux, h_conv,
+                   T_inf)
 
     def energies(self):
         out = np.zeros(6)

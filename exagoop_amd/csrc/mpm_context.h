@@ -74,6 +74,10 @@ struct mpm_ctx {
     mpm_wall_temp_udf_fn tudf_fn[3][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};
     double *tudf_dev[3][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};
     bool any_tudf = false;
+    // Temperature conditions of the level-set bodies (MPM_LS_TEMP_*) and their
+    // T_wall, heat flux, h, T_inf.
+    int ls_tbc[MPM_MAX_LS_BODIES] = {};
+    double ls_tpar[MPM_MAX_LS_BODIES][4] = {};
 };
 
 namespace xmpm {

@@ -940,6 +940,9 @@ int mpm_step(mpm_ctx *c, int nsteps) {
     if (h.err) {
         int zero = 0;
         HIPCHK(hipMemcpy(&c->sc->err, &zero, sizeof(int), hipMemcpyHostToDevice));
+        if (h.err & 2)
+            return set_err(MPM_E_UNSUPPORTED,
+                           "a level-set temperature condition chained more than 4 nodes");
         return set_err(MPM_E_UNSUPPORTED, "a particle lay outside its sorted tile");
     }
     return MPM_OK;

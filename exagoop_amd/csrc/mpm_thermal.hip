@@ -281,6 +281,72 @@ __device__ double th_udf(const GridP &g, const mpm_thermal_specs &t, const ThUdf
     return v;
 }
 
+// One level-set body's temperature condition.
+struct ThBody {
+    const double *phi;
+    int bc;
+    double T_wall, q, h, T_inf;
+};
+
+// The dominant-direction neighbour of a node that the body's condition
+// changes, or false when it changes nothing there.
+__device__ inline bool th_ls_active(const GridP &g, const ThBody &b, const double *MSP, int I,
+                                    int J, int K, int Y[3], int &dom) {
+    const int64_t q = nidx(g, I, J, K);
+    const double x[3] = {g.plo[0] + I * g.dx[0], g.plo[1] + J * g.dx[1], g.plo[2] + K * g.dx[2]};
+    if (ls_value(g, b.phi, x) >= 0.0 || MSP[q] <= 0.0) return false;
+    double nrm[3] = {1.0, 0.0, 0.0};
+    ls_grad(g, b.phi, x, nrm);
+    const double gm = sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2]);
+    if (gm < 1.0e-10) return false;
+    for (int d = 0; d < 3; ++d) nrm[d] /= gm;
+    dom = 0;
+    for (int d = 1; d < 3; ++d)
+        if (fabs(nrm[d]) > fabs(nrm[dom])) dom = d;
+    Y[0] = I;
+    Y[1] = J;
+    Y[2] = K;
+    Y[dom] += (nrm[dom] > 0.0) ? 1 : -1;
+    if (b.bc == MPM_LS_TEMP_ISOTHERMAL) return true;
+    return Y[dom] >= -1 && Y[dom] <= g.n[dom] + 1;
+}
+
+// nodal_levelset_bcs_temperature (nodal_data_ops.cpp:235-350): the value a
+// node ends with after one body's serial in-place sweep.  The neighbour read
+// is the updated one when it comes earlier in the sweep (the normal's
+// dominant component is negative); a ghost neighbour holds zero in this
+// pass (the reset of the step, no fill since).  Chains deeper than D set
+// bit 2 of *err.
+template <int D>
+__device__ double th_ls(const GridP &g, const ThBody &b, const double *T, const double *MSP,
+                        const double *MC, const double *M, int I, int J, int K, int *err) {
+    const int64_t q = nidx(g, I, J, K);
+    int Y[3], dom = 0;
+    if (!th_ls_active(g, b, MSP, I, J, K, Y, dom)) return T[q];
+    if (b.bc == MPM_LS_TEMP_ISOTHERMAL) return b.T_wall;
+    double tnb = 0.0;
+    if (Y[dom] >= 0 && Y[dom] <= g.n[dom]) {
+        tnb = T[nidx(g, Y[0], Y[1], Y[2])];
+        const bool earlier = Y[dom] < (dom == 0 ? I : (dom == 1 ? J : K));
+        if (earlier) {
+            if constexpr (D > 0) {
+                tnb = th_ls<D - 1>(g, b, T, MSP, MC, M, Y[0], Y[1], Y[2], err);
+            } else {
+                int Z[3], dz = 0;
+                if (th_ls_active(g, b, MSP, Y[0], Y[1], Y[2], Z, dz)) atomicOr(err, 2);
+            }
+        }
+    }
+    const double mk = MC[q], m = M[q];
+    const double k_node = (m > 0.0) ? mk / m : 1.0;
+    if (b.bc == MPM_LS_TEMP_HEATFLUX) return tnb + b.q * g.dx[dom] / k_node;
+    if (b.bc == MPM_LS_TEMP_CONVECTIVE) {
+        const double Bi = b.h * g.dx[dom] / k_node;
+        return (tnb + Bi * b.T_inf) / (1.0 + Bi);
+    }
+    return T[q];
+}
+
 __device__ __forceinline__ void node_ijk(const GridP &g, int64_t q, int &I, int &J, int &K) {
     const int64_t n0 = g.n[0] + 1, n1 = g.n[1] + 1;
     I = (int)(q % n0);
@@ -315,23 +381,38 @@ __global__ __launch_bounds__(kThreads) void k_th_udf_walls(GridP g, mpm_thermal_
     scr[q] = th_udf<3>(g, t, u, tn + TN_T * nn, tn + TN_MC * nn, mass, I, J, K);
 }
 
-// The wall values into the temperature (the nodes on faces with a function
-// when udf is set, else all wall nodes) and, when delta is set,
-// store_delta_temperature (nodal_data_ops.cpp:409-430).
-__global__ __launch_bounds__(kThreads) void k_th_apply_walls(GridP g, ThUdf u, int udf,
-                                                             double *tn, const double *scr,
-                                                             int delta) {
+// One body's values into scr, every node.
+__global__ __launch_bounds__(kThreads) void k_th_ls_walls(GridP g, ThBody b, const double *tn,
+                                                          const double *mass, double *scr,
+                                                          int *err) {
+    const int64_t q = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (q >= g.nn) return;
+    int I, J, K;
+    node_ijk(g, q, I, J, K);
+    const int64_t nn = g.nn;
+    scr[q] = th_ls<4>(g, b, tn + TN_T * nn, tn + TN_MSP * nn, tn + TN_MC * nn, mass, I, J, K,
+                      err);
+}
+
+// The values of scr into the temperature: mode 0 the wall nodes, 1 the nodes
+// on faces with a function, 2 every node.
+__global__ __launch_bounds__(kThreads) void k_th_apply_walls(GridP g, ThUdf u, int mode,
+                                                             double *tn, const double *scr) {
     const int64_t q = (int64_t)blockIdx.x * kThreads + threadIdx.x;
     const int64_t nn = g.nn;
     if (q >= nn) return;
     int I, J, K;
     node_ijk(g, q, I, J, K);
-    double T = tn[TN_T * nn + q];
-    if (udf ? on_udf_face(g, u, I, J, K) : on_face(g, I, J, K)) {
-        T = scr[q];
-        tn[TN_T * nn + q] = T;
-    }
-    if (delta && tn[TN_MSP * nn + q] > 0.0) tn[TN_DT * nn + q] = T - tn[TN_DT * nn + q];
+    const bool take = mode == 2 || (mode == 1 ? on_udf_face(g, u, I, J, K) : on_face(g, I, J, K));
+    if (take) tn[TN_T * nn + q] = scr[q];
+}
+
+// store_delta_temperature (nodal_data_ops.cpp:409-430).
+__global__ __launch_bounds__(kThreads) void k_th_delta(GridP g, double *tn) {
+    const int64_t q = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    const int64_t nn = g.nn;
+    if (q >= nn) return;
+    if (tn[TN_MSP * nn + q] > 0.0) tn[TN_DT * nn + q] = tn[TN_T * nn + q] - tn[TN_DT * nn + q];
 }
 
 // interpolate_from_grid_temperature (mpm_particle_grid_ops.cpp:1129-1270):
@@ -436,21 +517,36 @@ void launch_th_p2g(mpm_ctx *c, bool mass_temp, bool source, hipStream_t st) {
     }
 }
 
-// Apply_Nodal_BCs_Temperature (utilities.cpp:221-240) without level-set
-// bodies: the wall conditions, then (udf) the face functions; then, when
-// delta is set, store_delta_temperature.
-void launch_th_walls(mpm_ctx *c, bool delta, bool udf, hipStream_t st) {
+enum { LS_NONE = 0, LS_ALL = 1, LS_DIRICHLET = 2 };
+
+// Apply_Nodal_BCs_Temperature (utilities.cpp:221-240): the wall conditions,
+// then (udf) the face functions, then the level-set bodies in order (ls:
+// none, all, or the isothermal ones only); then, when delta is set,
+// store_delta_temperature.
+void launch_th_walls(mpm_ctx *c, bool delta, bool udf, int ls, hipStream_t st) {
     const int64_t nn = c->g.nn;
     ThUdf u{};
     const bool any = udf && c->any_tudf;
     for (int d = 0; d < 3; ++d)
         for (int h = 0; h < 2; ++h) u.tab[d][h] = (any && c->tudf_fn[d][h]) ? c->tudf_dev[d][h] : nullptr;
     k_th_walls<<<blocks(nn), kThreads, 0, st>>>(c->g, c->ts, c->tn, c->N.mass, c->tscr);
-    k_th_apply_walls<<<blocks(nn), kThreads, 0, st>>>(c->g, u, 0, c->tn, c->tscr,
-                                                      (delta && !any) ? 1 : 0);
-    if (!any) return;
-    k_th_udf_walls<<<blocks(nn), kThreads, 0, st>>>(c->g, c->ts, u, c->tn, c->N.mass, c->tscr);
-    k_th_apply_walls<<<blocks(nn), kThreads, 0, st>>>(c->g, u, 1, c->tn, c->tscr, delta ? 1 : 0);
+    k_th_apply_walls<<<blocks(nn), kThreads, 0, st>>>(c->g, u, 0, c->tn, c->tscr);
+    if (any) {
+        k_th_udf_walls<<<blocks(nn), kThreads, 0, st>>>(c->g, c->ts, u, c->tn, c->N.mass,
+                                                        c->tscr);
+        k_th_apply_walls<<<blocks(nn), kThreads, 0, st>>>(c->g, u, 1, c->tn, c->tscr);
+    }
+    for (int b = 0; ls != LS_NONE && b < c->ls.nb; ++b) {
+        const int bc = c->ls_tbc[b];
+        if (bc == MPM_LS_TEMP_ADIABATIC) continue;
+        if (ls == LS_DIRICHLET && bc != MPM_LS_TEMP_ISOTHERMAL) continue;
+        const ThBody body{c->ls.phi[b], bc, c->ls_tpar[b][0], c->ls_tpar[b][1], c->ls_tpar[b][2],
+                          c->ls_tpar[b][3]};
+        k_th_ls_walls<<<blocks(nn), kThreads, 0, st>>>(c->g, body, c->tn, c->N.mass, c->tscr,
+                                                       &c->sc->err);
+        k_th_apply_walls<<<blocks(nn), kThreads, 0, st>>>(c->g, u, 2, c->tn, c->tscr);
+    }
+    if (delta) k_th_delta<<<blocks(nn), kThreads, 0, st>>>(c->g, c->tn);
 }
 
 void launch_th_g2p(mpm_ctx *c, bool ut, bool uq, const int *perm, const int *count,
@@ -522,7 +618,7 @@ int thermal_init(mpm_ctx *c) {
     launch_zero(c->tn, TN_COMPS * nn, st);
     launch_th_p2g(c, true, false, st);
     k_th_nodal<false><<<blocks(nn), kThreads, 0, st>>>(c->g, c->s.mass_tolerance, c->tn, nullptr);
-    launch_th_walls(c, true, false, st);
+    launch_th_walls(c, true, false, LS_NONE, st);
     launch_th_g2p(c, true, true, nullptr, nullptr, c->th, st);
     return check_launch();
 }
@@ -536,7 +632,8 @@ int thermal_grid(mpm_ctx *c, hipStream_t st) {
     launch_zero(c->tn, TN_COMPS * nn, st);
     launch_th_p2g(c, true, true, st);
     k_th_nodal<true><<<blocks(nn), kThreads, 0, st>>>(c->g, c->s.mass_tolerance, c->tn, c->sc);
-    launch_th_walls(c, true, true, st);
+    launch_th_walls(c, true, true, c->s.stress_update_scheme == MPM_MUSL ? LS_DIRICHLET : LS_ALL,
+                    st);
     return check_launch();
 }
 
@@ -560,7 +657,7 @@ int thermal_particles(mpm_ctx *c, bool musl, const int *perm, const int *count,
     launch_zero(c->tn + TN_MSP * nn, 2 * nn, st);
     launch_zero(c->tn + TN_SRC * nn, nn, st);
     launch_th_p2g(c, false, true, st);
-    launch_th_walls(c, false, true, st);
+    launch_th_walls(c, false, true, LS_ALL, st);
     launch_th_g2p(c, false, true, perm, count, out, st);
     return check_launch();
 }
@@ -658,6 +755,20 @@ int mpm_set_wall_temp_udf(mpm_ctx *c, int dir, int hi, mpm_wall_temp_udf_fn fn) 
     for (int d = 0; d < 3; ++d)
         for (int h = 0; h < 2; ++h) c->any_tudf |= (c->tudf_fn[d][h] != nullptr);
     return check_launch();
+}
+
+int mpm_set_levelset_body_temp(mpm_ctx *c, int body, int temp_bc, double T_wall,
+                               double heat_flux, double h_conv, double T_inf) {
+    if (!c || body < 0 || body >= c->ls.nb) return set_err(MPM_E_ARG, "no such level-set body");
+    if (temp_bc != MPM_LS_TEMP_ADIABATIC && temp_bc != MPM_LS_TEMP_ISOTHERMAL &&
+        temp_bc != MPM_LS_TEMP_HEATFLUX && temp_bc != MPM_LS_TEMP_CONVECTIVE)
+        return set_err(MPM_E_BC, "level-set temperature condition must be 0, 1, 3 or 4");
+    c->ls_tbc[body] = temp_bc;
+    c->ls_tpar[body][0] = T_wall;
+    c->ls_tpar[body][1] = heat_flux;
+    c->ls_tpar[body][2] = h_conv;
+    c->ls_tpar[body][3] = T_inf;
+    return MPM_OK;
 }
 
 int mpm_upload_thermal(mpm_ctx *c, int64_t n, const double *th) {

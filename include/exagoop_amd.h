@@ -558,6 +558,24 @@ typedef void (*mpm_wall_temp_udf_fn)(double, double, double, double, double *);
  * temperature BC type (apply_udf_nodal_bcs_temperature, :1308-1446).  The
  * initial pass applies the wall conditions only, as the reference's does. */
 int mpm_set_wall_temp_udf(mpm_ctx *ctx, int dir, int hi, mpm_wall_temp_udf_fn fn);
+/* Temperature condition of a level-set body (LevelSetBody::temp_bc_int,
+ * mpm_eb.H:78-118; mpm_eb.cpp:521-545): the codes and parameters of
+ * <name>.temp_bc_type, lset_T_wall, heat_flux, h_conv, T_inf. */
+enum mpm_ls_temp_bc {
+    MPM_LS_TEMP_ADIABATIC = 0, /* no condition (the default) */
+    MPM_LS_TEMP_ISOTHERMAL = 1,
+    MPM_LS_TEMP_HEATFLUX = 3,
+    MPM_LS_TEMP_CONVECTIVE = 4
+};
+/* Sets the temperature condition of level-set body `body` (0-based, in the
+ * order of mpm_add_levelset_body).  Applied after the wall conditions and
+ * the face functions of a temperature BC pass (nodal_levelset_bcs_temperature,
+ * nodal_data_ops.cpp:235-350): at nodes inside the body with thermal mass,
+ * T = T_wall, or from the neighbour along the normal's dominant direction
+ * with the flux or convection formula; the first pass of a MUSL step applies
+ * the isothermal bodies only (utilities.cpp:221-240, main.cpp:103-104). */
+int mpm_set_levelset_body_temp(mpm_ctx *ctx, int body, int temp_bc, double T_wall,
+                               double heat_flux, double h_conv, double T_inf);
 /* Thermal fields by upload index: th[f * n_upload + i] for field f of the
  * particle uploaded at index i (n_upload = count of the last upload; the
  * reader's T, spheat, thermcond, heatsrc columns, mpm_init.cpp:1097-1108). */

@@ -310,6 +310,9 @@ struct LsBody {
     int bc;
     double mu;
     double w[3];
+    // temperature condition (LevelSetBody, mpm_eb.H:78-118)
+    int temp_bc = MPM_LS_TEMP_ADIABATIC;
+    double T_wall = 0.0, heat_flux = 0.0, h_conv = 0.0, T_inf = 0.0;
 };
 
 inline void atomic_add(double *p, double v, bool par) {
@@ -1527,6 +1530,52 @@ void udf_bcs_temperature(orc_ctx *c) {
             }
 }
 
+// nodal_levelset_bcs_temperature (nodal_data_ops.cpp:235-350): per body, in
+// order, a serial sweep over the nodal box; dirichlet_only applies the
+// isothermal bodies only.  The neighbour may be a ghost node (the level-set
+// box is the nodal box grown by one).
+void ls_bcs_temperature(orc_ctx *c, bool dirichlet_only) {
+    const mpm_specs &s = c->s;
+    for (const auto &b : c->bodies) {
+        if (b.temp_bc == MPM_LS_TEMP_ADIABATIC) continue;
+        if (dirichlet_only && b.temp_bc != MPM_LS_TEMP_ISOTHERMAL) continue;
+        for (int k = 0; k <= c->n[2]; ++k)
+            for (int j = 0; j <= c->n[1]; ++j)
+                for (int i = 0; i <= c->n[0]; ++i) {
+                    const double xp[3] = {s.plo[0] + i * c->dx[0], s.plo[1] + j * c->dx[1],
+                                          s.plo[2] + k * c->dx[2]};
+                    const double lsval = ls_value(c, b, xp);
+                    if (lsval >= 0.0 || c->A(i, j, k, kMsp) <= 0.0) continue;
+                    double nrm[3] = {1.0, 0.0, 0.0};
+                    ls_grad(c, b, xp, nrm);
+                    double gm = 0.0;
+                    for (int d = 0; d < 3; d++) gm += nrm[d] * nrm[d];
+                    gm = std::sqrt(gm);
+                    if (gm < 1.0e-10) continue;
+                    for (int d = 0; d < 3; d++) nrm[d] /= gm;
+                    if (b.temp_bc == MPM_LS_TEMP_ISOTHERMAL) {
+                        c->A(i, j, k, kTemp) = b.T_wall;
+                        continue;
+                    }
+                    int dom = 0;
+                    for (int d = 1; d < 3; d++)
+                        if (std::abs(nrm[d]) > std::abs(nrm[dom])) dom = d;
+                    int nb[3] = {i, j, k};
+                    nb[dom] += (nrm[dom] > 0.0) ? 1 : -1;
+                    if (nb[dom] < -1 || nb[dom] > c->n[dom] + 1) continue;
+                    const double mk = c->A(i, j, k, kMcond), m = c->A(i, j, k, MPM_N_MASS);
+                    const double k_node = (m > 0.0) ? mk / m : 1.0;
+                    const double T_nb = c->A(nb[0], nb[1], nb[2], kTemp);
+                    if (b.temp_bc == MPM_LS_TEMP_HEATFLUX) {
+                        c->A(i, j, k, kTemp) = T_nb + b.heat_flux * c->dx[dom] / k_node;
+                    } else if (b.temp_bc == MPM_LS_TEMP_CONVECTIVE) {
+                        const double Bi = b.h_conv * c->dx[dom] / k_node;
+                        c->A(i, j, k, kTemp) = (T_nb + Bi * b.T_inf) / (1.0 + Bi);
+                    }
+                }
+    }
+}
+
 // store_delta_temperature (nodal_data_ops.cpp:409-430).
 void store_delta_temperature(orc_ctx *c) {
     for (int k = 0; k <= c->n[2]; ++k)
@@ -1659,6 +1708,21 @@ int orc_set_wall_temp_udf(orc_ctx *c, int dir, int hi, mpm_wall_temp_udf_fn fn) 
     return MPM_OK;
 }
 
+int orc_set_levelset_body_temp(orc_ctx *c, int body, int temp_bc, double T_wall,
+                               double heat_flux, double h_conv, double T_inf) {
+    if (!c || body < 0 || body >= (int)c->bodies.size()) return MPM_E_ARG;
+    if (temp_bc != MPM_LS_TEMP_ADIABATIC && temp_bc != MPM_LS_TEMP_ISOTHERMAL &&
+        temp_bc != MPM_LS_TEMP_HEATFLUX && temp_bc != MPM_LS_TEMP_CONVECTIVE)
+        return MPM_E_BC;
+    LsBody &b = c->bodies[body];
+    b.temp_bc = temp_bc;
+    b.T_wall = T_wall;
+    b.heat_flux = heat_flux;
+    b.h_conv = h_conv;
+    b.T_inf = T_inf;
+    return MPM_OK;
+}
+
 int orc_download_thermal(orc_ctx *c, int64_t n, double *th) {
     if (!c || !c->thermal || n < 0 || (n > 0 && !th)) return MPM_E_ARG;
     std::fill(th, th + (size_t)MPM_NTHERMAL * n, std::numeric_limits<double>::quiet_NaN());
@@ -1713,6 +1777,7 @@ int orc_step(orc_ctx *c, int nsteps) {
                 eval_temp_udf_tables(c, c->time);
                 udf_bcs_temperature(c);
             }
+            ls_bcs_temperature(c, s.stress_update_scheme == MPM_MUSL);
             store_delta_temperature(c);
         }
         if (s.stress_update_scheme == MPM_USL) {
@@ -1730,6 +1795,7 @@ int orc_step(orc_ctx *c, int nsteps) {
                 p2g_thermal(c, true, false, true);
                 bcs_temperature(c);
                 udf_bcs_temperature(c);
+                ls_bcs_temperature(c, false);
                 g2p_thermal(c, false, true);
             }
             if ((rc = orc_move(c, dt, step))) return rc;

@@ -59,6 +59,8 @@ int orc_enable_thermal(orc_ctx *ctx, const mpm_thermal_specs *t);
 int orc_upload_thermal(orc_ctx *ctx, int64_t n_upload, const double *th);
 int orc_download_thermal(orc_ctx *ctx, int64_t n_upload, double *th);
 int orc_set_wall_temp_udf(orc_ctx *ctx, int dir, int hi, mpm_wall_temp_udf_fn fn);
+int orc_set_levelset_body_temp(orc_ctx *ctx, int body, int temp_bc, double T_wall,
+                               double heat_flux, double h_conv, double T_inf);
 
 /* Pieces exposed for closed-form pinning in tests. */
 double orc_quadraticspline_1d(int type, double z);

@@ -1,16 +1,20 @@
 """The reference's heat-conduction validation cases (Tests/1D_Heat_Conduction,
 Tests/1D_Heat_Conduction_Convective, Tests/1D_Heat_Conduction_HeatFlux,
-Tests/2D_Heat_Conduction: PreProcess/config.json for the set-up,
+Tests/2D_Heat_Conduction, Tests/2D_Heat_Conduction_Cylinder_Dirichlet:
+PreProcess/config.json for the set-up,
 PostProcess/validate.py for the time, the analytic solution and the RMS
 tolerance), set up for the 3-D step, and their analytic solutions.
 
 The 1-D and 2-D builds' domains become slabs one or a few cells thick along
 the missing directions, with adiabatic temperature walls and slip walls
 there: with fields uniform across the slab these walls carry no heat, so the
-3-D solution is the 1-D / 2-D one.  A slab is at least five cells thick
-where the order is quadratic, so that direction keeps the quadratic
-spline (the small-grid fallback of mpm_init.cpp:180-233 makes a thinner
-non-periodic direction linear).  Material: rho = c_p = k = 1 (alpha = 1),
+3-D solution is the 1-D / 2-D one.  The missing directions' cells are 1
+long, so particle volumes and masses are the lower-dimensional builds'
+lengths or areas (mass_tolerance then cuts at the same fraction of a
+particle).  A slab is at least five cells thick where the order is
+quadratic, so that direction keeps the quadratic spline (the small-grid
+fallback of mpm_init.cpp:180-233 makes a thinner non-periodic direction
+linear).  Material: rho = c_p = k = 1 (alpha = 1),
 elastic E = 1e6, nu = 0.3, at rest without gravity; initial T = 0.
 """
 import numpy as np
@@ -69,35 +73,74 @@ def dirichlet_2d(x, y, t, T0=0.0, T1=1.0, L=1.0, H=1.0, nterms=25):
     return T
 
 
+def laplace_cylinder(x, y, R=0.15, cx=0.5, cy=0.5, T_in=1.0, T_out=0.0, N=400):
+    """Steady T on [0, 1]^2 with T = T_in inside the circle of radius R and
+    T = T_out on the square's edges: second-order finite differences on an
+    N x N grid, bilinear interpolation to the points."""
+    from scipy.interpolate import RegularGridInterpolator
+    from scipy.sparse import lil_matrix
+    from scipy.sparse.linalg import spsolve
+
+    xs = np.linspace(0.0, 1.0, N + 1)
+    X, Y = np.meshgrid(xs, xs, indexing="ij")
+    inner = (X - cx) ** 2 + (Y - cy) ** 2 <= R * R
+    edge = np.zeros_like(inner)
+    edge[0, :] = edge[-1, :] = edge[:, 0] = edge[:, -1] = True
+    idx = np.arange((N + 1) ** 2).reshape(N + 1, N + 1)
+    A = lil_matrix(((N + 1) ** 2, (N + 1) ** 2))
+    b = np.zeros((N + 1) ** 2)
+    for i in range(N + 1):
+        for j in range(N + 1):
+            k = idx[i, j]
+            if inner[i, j] or edge[i, j]:
+                A[k, k] = 1.0
+                b[k] = T_in if inner[i, j] else T_out
+            else:
+                A[k, k] = -4.0
+                for a, c in ((i + 1, j), (i - 1, j), (i, j + 1), (i, j - 1)):
+                    A[k, idx[a, c]] = 1.0
+    T = spsolve(A.tocsr(), b).reshape(N + 1, N + 1)
+    return RegularGridInterpolator((xs, xs), T)(np.stack([x, y], axis=1))
+
+
 # name: (cells, extent, ppc, order, scheme, alpha, periodic, x/y temperature walls, steps,
-#        dt, RMS tolerance)
+#        dt, RMS tolerance; body: an isothermal cylinder along z)
 CASES = {
     "1D_Heat_Conduction": dict(
-        cells=(100, 1, 1), extent=(1.0, 0.01, 0.01), ppc=(2, 1, 1), order=1, scheme=capi.MUSL,
+        cells=(100, 1, 1), extent=(1.0, 1.0, 1.0), ppc=(2, 1, 1), order=1, scheme=capi.MUSL,
         alpha=1.0, periodic=(0, 0, 0), num_redist=10,
         walls={(0, 0): (capi.BC_TEMP_ISOTHERMAL, "T_wall", 0.0),
                (0, 1): (capi.BC_TEMP_ISOTHERMAL, "T_wall", 1.0)},
         steps=5000, dt=1e-5, tol=1e-2),
     "1D_Heat_Conduction_Convective": dict(
-        cells=(40, 4, 5), extent=(1.0, 0.1, 0.125), ppc=(2, 2, 1), order=2, scheme=capi.MUSL,
+        cells=(40, 4, 5), extent=(1.0, 0.1, 5.0), ppc=(2, 2, 1), order=2, scheme=capi.MUSL,
         alpha=1.0, periodic=(0, 0, 0), num_redist=10,
         walls={(0, 0): (capi.BC_TEMP_ISOTHERMAL, "T_wall", 1.0),
                (0, 1): (capi.BC_TEMP_CONVECTION, "h", 2.0, "T_inf", 0.0)},
         steps=50000, dt=1e-5, tol=1e-2),
     "1D_Heat_Conduction_HeatFlux": dict(
-        cells=(40, 4, 1), extent=(1.0, 0.1, 0.025), ppc=(2, 2, 1), order=1, scheme=capi.MUSL,
+        cells=(40, 4, 1), extent=(1.0, 0.1, 1.0), ppc=(2, 2, 1), order=1, scheme=capi.MUSL,
         alpha=1.0, periodic=(0, 0, 0), num_redist=10,
         walls={(0, 0): (capi.BC_TEMP_ISOTHERMAL, "T_wall", 0.0),
                (0, 1): (capi.BC_TEMP_USERFLUX, "flux", 1.0)},
         steps=50000, dt=1e-5, tol=1e-2),
     "2D_Heat_Conduction": dict(
-        cells=(32, 32, 1), extent=(1.0, 1.0, 1.0 / 32), ppc=(2, 2, 1), order=1, scheme=capi.MUSL,
+        cells=(32, 32, 1), extent=(1.0, 1.0, 1.0), ppc=(2, 2, 1), order=1, scheme=capi.MUSL,
         alpha=1.0, periodic=(0, 1, 0), num_redist=1,
         walls={(0, 0): (capi.BC_TEMP_ISOTHERMAL, "T_wall", 1.0),
                (0, 1): (capi.BC_TEMP_ISOTHERMAL, "T_wall", 1.0),
                (1, 0): (capi.BC_TEMP_ISOTHERMAL, "T_wall", 1.0),
                (1, 1): (capi.BC_TEMP_ISOTHERMAL, "T_wall", 1.0)},
         steps=5000, dt=1e-5, tol=5e-2),
+    "2D_Heat_Conduction_Cylinder_Dirichlet": dict(
+        cells=(128, 128, 1), extent=(1.0, 1.0, 1.0), ppc=(2, 2, 1), order=1,
+        scheme=capi.MUSL, alpha=1.0, periodic=(0, 0, 0), num_redist=10,
+        walls={(0, 0): (capi.BC_TEMP_ISOTHERMAL, "T_wall", 0.0),
+               (0, 1): (capi.BC_TEMP_ISOTHERMAL, "T_wall", 0.0),
+               (1, 0): (capi.BC_TEMP_ISOTHERMAL, "T_wall", 0.0),
+               (1, 1): (capi.BC_TEMP_ISOTHERMAL, "T_wall", 0.0)},
+        body=dict(center=(0.5, 0.5), radius=0.15, T_wall=1.0),
+        steps=10000, dt=1e-5, tol=1e-1),
 }
 
 
@@ -111,7 +154,8 @@ def build(name):
     s.periodic[:] = list(c["periodic"])
     for d in range(3):
         s.bclo[d] = s.bchi[d] = capi.BC_PERIODIC if c["periodic"][d] else capi.BC_SLIP
-    s.bclo[0] = s.bchi[0] = capi.BC_NOSLIP
+    if "body" not in c:
+        s.bclo[0] = s.bchi[0] = capi.BC_NOSLIP
     s.order_scheme = c["order"]
     s.stress_update_scheme = c["scheme"]
     s.alpha_pic_flip = c["alpha"]
@@ -137,6 +181,8 @@ def build(name):
 
 
 def exact(name, pos, t):
+    if name == "2D_Heat_Conduction_Cylinder_Dirichlet":
+        return laplace_cylinder(pos[0], pos[1])
     if name == "1D_Heat_Conduction":
         return dirichlet_1d(pos[0], t)
     if name == "1D_Heat_Conduction_Convective":
@@ -155,10 +201,25 @@ def run(ctx_factory, name, steps=None):
     ctx.upload(pos, r, idata)
     ctx.enable_thermal(ts)
     ctx.upload_thermal(th)
+    body = CASES[name].get("body")
+    if body:
+        # the level-set body (a "sphere" of the 2-D build: a cylinder along
+        # z here), its particles removed as removeParticlesInsideEB does
+        cx, cy = body["center"]
+        phi = capi.levelset_analytic(s, capi.LS_CYLINDER, [cx, cy, 0.0, body["radius"], 0.0, 2])
+        ctx.add_levelset_body(phi, capi.BC_SLIP, 0.0)
+        ctx.set_levelset_body_temp(0, capi.LS_TEMP_ISOTHERMAL, T_wall=body["T_wall"])
+        ctx.remove_particles_inside_levelset()
     ctx.init_internal_forces()
     ctx.step(CASES[name]["steps"] if steps is None else steps)
     T = ctx.download_thermal()[capi.TH_TEMPERATURE]
     p, _, _, ids = ctx.download()
     t = ctx.time()[0]
-    err = T[ids] - exact(name, p, t)
+    Tp = T[ids]
+    if body:
+        # the validator's ring: distance from the axis in [R, 0.5]
+        rr = np.hypot(p[0] - body["center"][0], p[1] - body["center"][1])
+        keep = (rr >= body["radius"]) & (rr <= 0.5)
+        p, Tp = p[:, keep], Tp[keep]
+    err = Tp - exact(name, p, t)
     return float(np.sqrt(np.mean(err ** 2))), ctx

@@ -148,6 +148,64 @@ def test_face_functions_match_the_oracle(gpu, scheme):
     compare(g, o, 1e-10)
 
 
+def trilinear(phi, s, x):
+    """Nodal phi at points x (3, n), trilinear on the nodes."""
+    n = np.array(s.ncells)
+    lo = np.array(s.plo)
+    dx = (np.array(s.phi) - lo) / n
+    f = (x - lo[:, None]) / dx[:, None]
+    i = np.clip(np.floor(f).astype(int), 0, n[:, None] - 1)
+    t = f - i
+    out = np.zeros(x.shape[1])
+    for cz in (0, 1):
+        for cy in (0, 1):
+            for cx in (0, 1):
+                w = ((t[0] if cx else 1 - t[0]) * (t[1] if cy else 1 - t[1]) *
+                     (t[2] if cz else 1 - t[2]))
+                out += w * phi[i[2] + cz, i[1] + cy, i[0] + cx]
+    return out
+
+
+@pytest.mark.parametrize("scheme", [USL, MUSL])
+def test_levelset_body_conditions_match_the_oracle(gpu, scheme):
+    """Three bodies with an isothermal, a heat-flux and a convective
+    condition (particles inside them removed first, as
+    removeParticlesInsideEB does): the body sweeps, whose nodes read
+    neighbours updated earlier in the same sweep, match the oracle's serial
+    sweeps."""
+    say(f"[thermal] level-set body conditions scheme={scheme}")
+    s, pos, r, idata = scenes.random_block(107, n=(14, 12, 10), order=2, model=0, scheme=scheme,
+                                           frac=1.0, speed=0.1)
+    bodies = [
+        (capi.LS_SPHERE, [0.3, 0.35, 0.3, 0.1], capi.LS_TEMP_ISOTHERMAL, dict(T_wall=2.0)),
+        (capi.LS_BOX, [0.6, 0.2, 0.1, 0.75, 0.4, 0.3], capi.LS_TEMP_HEATFLUX,
+         dict(heat_flux=15.0)),
+        (capi.LS_CYLINDER, [0.5, 0.0, 0.45, 0.08, 0.0, 1], capi.LS_TEMP_CONVECTIVE,
+         dict(h_conv=60.0, T_inf=-1.0)),
+    ]
+    phis = [capi.levelset_analytic(s, kind, prm) for kind, prm, _, _ in bodies]
+    keep = np.ones(pos.shape[1], bool)
+    for phi in phis:
+        keep &= trilinear(phi, s, pos) >= 1e-20
+    pos, r, idata = pos[:, keep], r[:, keep], idata[:, keep]
+    th = thermal_fields(pos.shape[1], 19)
+    g = capi.Context(s)
+    o = oracle.Oracle(s)
+    for c in (g, o):
+        c.upload(pos, r, idata)
+        for b, (phi, (_, _, tbc, kw)) in enumerate(zip(phis, bodies)):
+            c.add_levelset_body(phi, capi.BC_SLIP, 0.0)
+            c.set_levelset_body_temp(b, tbc, **kw)
+        c.enable_thermal(mixed_walls(s))
+        c.upload_thermal(th)
+        c.init_internal_forces()
+        c.step(1)
+    compare(g, o, 1e-12)
+    for c in (g, o):
+        c.step(9)
+    compare(g, o, 1e-10)
+
+
 def test_thermal_fields_follow_the_particles_through_removals(gpu):
     """Particles leaving a non-periodic box are removed (Redistribute); the
     thermal record follows the survivors by upload index on both sides."""
