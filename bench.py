@@ -23,6 +23,10 @@ cells, 100 663 296 particles, cubic, exagoop_amd/column.py): 20 timed steps
 after 3 warm-up steps, particle-updates/s, per-kernel times, and the
 size-independent checks of its final state.
 
+The thermal object runs the same dam break with the thermal module on
+(uniform temperature, adiabatic walls): milliseconds per step and the
+thermal stages' share of them.
+
 The learned object compares, on the same dam break, the plain run at a fixed
 fine dt = 2.5e-5 with the learned coarse step (4 fine dt per step, the
 committed network exagoop_amd/data/learned_dam_break.npz) and the uncorrected
@@ -60,6 +64,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-column", action="store_true")
     ap.add_argument("--column-steps", type=int, default=20)
+    ap.add_argument("--no-thermal", action="store_true")
+    ap.add_argument("--thermal-steps", type=int, default=50)
     ap.add_argument("--batch-scenes", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=500)
     ap.add_argument("--calib-copy-mib", type=int, default=0,
@@ -149,6 +155,10 @@ def main():
     if not args.no_batch:
         batch = batch_leg(args.batch_scenes, args.batch_steps)
 
+    thermal = None
+    if not args.no_thermal:
+        thermal = thermal_leg(s, pos, r, idata, args.thermal_steps, ev_ms / args.steps)
+
     column = None
     if not args.no_column:
         from exagoop_amd import column as column_mod
@@ -168,8 +178,36 @@ def main():
         "learned": learned,
         "batch": batch,
         "column": column,
+        "thermal": thermal,
     }
     print(json.dumps(out), flush=True)
+
+
+def thermal_leg(s, pos, r, idata, nsteps, plain_ms):
+    """The same dam break with the thermal module on (c_p, k, T uniform and
+    adiabatic walls): milliseconds per step and the thermal stages' share."""
+    import numpy as np
+    from exagoop_amd import capi
+
+    n = pos.shape[1]
+    th = np.zeros((capi.NTHERMAL, n))
+    th[capi.TH_TEMPERATURE] = 300.0
+    th[capi.TH_SPECIFIC_HEAT] = 4186.0
+    th[capi.TH_CONDUCTIVITY] = 0.6
+    g = capi.Context(s)
+    g.upload(pos, r, idata)
+    g.enable_thermal()
+    g.upload_thermal(th)
+    g.init_internal_forces()
+    g.step(3)
+    g.step(nsteps)
+    ms, _ = g.last_step_timing()
+    per = ms / nsteps
+    print(f"[bench] thermal dam break: {per:.4f} ms/step", file=sys.stderr, flush=True)
+    return {"value": round(n * nsteps / (ms * 1e-3), 1), "unit": "particle-updates/s",
+            "ms_per_step": round(per, 5), "plain_ms_per_step": round(plain_ms, 5),
+            "thermal_ms_per_step": round(per - plain_ms, 5), "steps": nsteps,
+            "workload": "dam break with the thermal module (USE_TEMP stages), uniform T"}
 
 
 def calib_copy(mib):
