@@ -69,6 +69,7 @@ struct mpm_ctx {
     int64_t th_cap = 0;
     double *tn = nullptr;
     double *tscr = nullptr;
+    double *tpkt = nullptr;  // thermal P2G packets by sorted position (kPktComps * S.pcap)
     // Temperature wall functions and their per-step face tables (2 values per
     // face node).
     mpm_wall_temp_udf_fn tudf_fn[3][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};
@@ -98,12 +99,14 @@ bool walls_at_rest(const mpm_specs *s);
 // the initial pass's thermal block; thermal_grid the step's nodal half (after
 // the momentum grid pass); thermal_particles the step's G2P half (before the
 // particle kernel; perm / count: the sorted order, or null); thermal_swap
-// follows the swap of the particle buffers.
+// follows the swap of the particle buffers.  tiled / a non-null perm: the
+// state was cell-sorted in this step or pass, so the deposits go through the
+// tiled deposit (epoch and fallback as for the momentum deposit).
 int thermal_on_upload(mpm_ctx *c);
-int thermal_init(mpm_ctx *c);
-int thermal_grid(mpm_ctx *c, hipStream_t st);
-int thermal_particles(mpm_ctx *c, bool musl, const int *perm, const int *count,
-                      hipStream_t st);
+int thermal_init(mpm_ctx *c, bool tiled);
+int thermal_grid(mpm_ctx *c, bool tiled, int epoch, int fallback, hipStream_t st);
+int thermal_particles(mpm_ctx *c, bool musl, const int *perm, const int *count, int epoch,
+                      int fallback, hipStream_t st);
 void thermal_swap(mpm_ctx *c);
 // The temperature wall functions evaluated at time t into the face tables.
 int thermal_udf_tables(mpm_ctx *c, double t);

@@ -815,7 +815,7 @@ int mpm_init_internal_forces(mpm_ctx *c) {
                     nullptr, c->stream);
     int rc = check_launch();
     if (rc) return rc;
-    return thermal_init(c);
+    return thermal_init(c, c->sorted && c->P.np > 0);
 }
 
 int mpm_step(mpm_ctx *c, int nsteps) {
@@ -865,7 +865,7 @@ int mpm_step(mpm_ctx *c, int nsteps) {
                     step);
         c->all_dirty = false;
         if (c->thermal) {
-            int rc = thermal_grid(c, st);
+            int rc = thermal_grid(c, sorted, step, nr > 1, st);
             if (rc) return rc;
         }
         if (prof) HIPCHK(hipEventRecord(c->ev[4], st));
@@ -884,7 +884,8 @@ int mpm_step(mpm_ctx *c, int nsteps) {
         }
         if (c->thermal) {
             int rc = thermal_particles(c, musl, sorted ? c->S.perm : nullptr,
-                                       sorted ? c->S.cell_start + c->S.nkeys : nullptr, st);
+                                       sorted ? c->S.cell_start + c->S.nkeys : nullptr, step,
+                                       nr > 1, st);
             if (rc) return rc;
             if (sorted) thermal_swap(c);
         }

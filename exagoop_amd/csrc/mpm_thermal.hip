@@ -104,6 +104,53 @@ __global__ __launch_bounds__(kThreads) void k_th_p2g(GridP g, PState P, const do
     }
 }
 
+// The same deposit for the cell-sorted state, through the tiled momentum
+// deposit (p2g_tile): thermal P2G packets at the sorted positions in the
+// layout of the momentum packets, so that the packet's "mass" deposits
+// m c_p N, its momentum components m c_p T N and m k N, and its first force
+// component a N + B grad N with a = V Q and B's first row V q (the sum of
+// the internal and external source).  Zero for a particle with phase != 0.
+template <bool MT, bool SRC>
+__global__ __launch_bounds__(kThreads) void k_th_packets(PState P, SortState S, const double *th,
+                                                         double *pkt) {
+    const int64_t s = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (s >= (int64_t)S.cell_start[S.nkeys]) return;
+    const int64_t j = S.perm[s], cap = P.cap, pc = S.pcap;
+    double v[13];
+    for (int e = 0; e < 13; ++e) v[e] = 0.0;
+    if (P.idata[MPM_I_PHASE * cap + j] == 0) {
+        const double m = P.r[MPM_R_MASS * cap + j], V = P.r[MPM_R_VOLUME * cap + j];
+        if (MT) {
+            const double cp = th[MPM_TH_SPECIFIC_HEAT * cap + j];
+            v[0] = m * cp;
+            v[1] = m * cp * th[MPM_TH_TEMPERATURE * cap + j];
+            v[2] = m * th[MPM_TH_CONDUCTIVITY * cap + j];
+        }
+        if (SRC) {
+            v[4] = V * th[MPM_TH_HEAT_SOURCE * cap + j];
+            for (int d = 0; d < 3; ++d) v[7 + d] = V * th[(MPM_TH_HEAT_FLUX + d) * cap + j];
+        }
+    }
+    for (int d = 0; d < 3; ++d) pkt[d * pc + s] = P.pos[d * cap + j];
+    for (int e = 0; e < 13; ++e) pkt[(3 + e) * pc + s] = v[e];
+}
+
+// The tiled deposit's sums into the thermal components; the accumulators are
+// left zero, as the momentum grid pass leaves them.
+template <bool MT, bool SRC>
+__global__ __launch_bounds__(kThreads) void k_th_take(GridP g, double *acc, double *tn) {
+    const int64_t q = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    const int64_t nn = g.nn;
+    if (q >= nn) return;
+    if (MT) {
+        tn[TN_MSP * nn + q] = acc[q];
+        tn[TN_MSPT * nn + q] = acc[nn + q];
+        tn[TN_MC * nn + q] = acc[2 * nn + q];
+    }
+    if (SRC) tn[TN_SRC * nn + q] = acc[4 * nn + q];
+    for (int c = 0; c < 7; ++c) acc[c * nn + q] = 0.0;
+}
+
 // FabArray::SumBoundary over periodic direction d (AMReX, called at
 // mpm_particle_grid_ops.cpp:848-853): the two face nodes of each periodic
 // pair both take their sum.  One thread per pair; directions in turn.
@@ -497,10 +544,32 @@ __global__ __launch_bounds__(kThreads) void k_th_g2p(GridP g, PState P, const in
     for (int c = 0; c < MPM_NTHERMAL; ++c) thout[c * cap + i] = f[c];
 }
 
-void launch_th_p2g(mpm_ctx *c, bool mass_temp, bool source, hipStream_t st) {
+// The thermal deposit into tn (the components the flags name, on top of what
+// they hold), then SumBoundary.  tiled: the particle state was sorted this
+// step (perm, cell starts and tile list valid), so the deposit goes through
+// the tiled momentum deposit (fallback: particles may lie outside their
+// tile); otherwise one thread per particle with global atomics.
+void launch_th_p2g(mpm_ctx *c, bool mass_temp, bool source, bool tiled, int epoch, int fallback,
+                   hipStream_t st) {
     const GridP &g = c->g;
-    const int64_t np = c->P.np;
-    if (np > 0) {
+    const int64_t np = c->P.np, nn = g.nn;
+    if (np > 0 && tiled) {
+        SortState S = c->S;
+        S.pkt = c->tpkt;
+        if (mass_temp && source)
+            k_th_packets<true, true><<<blocks(np), kThreads, 0, st>>>(c->P, S, c->th, c->tpkt);
+        else if (mass_temp)
+            k_th_packets<true, false><<<blocks(np), kThreads, 0, st>>>(c->P, S, c->th, c->tpkt);
+        else
+            k_th_packets<false, true><<<blocks(np), kThreads, 0, st>>>(c->P, S, c->th, c->tpkt);
+        launch_p2g_tiles(g, c->ph, c->P, S, c->N, epoch, 0, fallback, c->sc, st);
+        if (mass_temp && source)
+            k_th_take<true, true><<<blocks(nn), kThreads, 0, st>>>(g, c->N.acc, c->tn);
+        else if (mass_temp)
+            k_th_take<true, false><<<blocks(nn), kThreads, 0, st>>>(g, c->N.acc, c->tn);
+        else
+            k_th_take<false, true><<<blocks(nn), kThreads, 0, st>>>(g, c->N.acc, c->tn);
+    } else if (np > 0) {
         if (mass_temp && source)
             k_th_p2g<true, true><<<blocks(np), kThreads, 0, st>>>(g, c->P, c->th, c->tn);
         else if (mass_temp)
@@ -576,11 +645,15 @@ int check_specs(const mpm_thermal_specs *t) {
 int alloc_particle_thermal(mpm_ctx *c) {
     (void)hipFree(c->th);
     (void)hipFree(c->th_alt);
-    c->th = c->th_alt = nullptr;
+    (void)hipFree(c->tpkt);
+    c->th = c->th_alt = c->tpkt = nullptr;
     const int64_t cap = c->P.cap > 0 ? c->P.cap : 1;
     const size_t bytes = (size_t)MPM_NTHERMAL * cap * sizeof(double);
     if (hipMalloc(&c->th, bytes) != hipSuccess || hipMalloc(&c->th_alt, bytes) != hipSuccess)
         return set_err(MPM_E_OOM, "thermal particle buffers");
+    if (c->S.pcap > 0 &&
+        hipMalloc(&c->tpkt, (size_t)kPktComps * c->S.pcap * sizeof(double)) != hipSuccess)
+        return set_err(MPM_E_OOM, "thermal packet buffer");
     launch_zero(c->th, (int64_t)MPM_NTHERMAL * cap, c->stream);
     launch_zero(c->th_alt, (int64_t)MPM_NTHERMAL * cap, c->stream);
     c->th_cap = cap;
@@ -611,12 +684,12 @@ int thermal_on_upload(mpm_ctx *c) {
 // The thermal block of the initial pass (mpm_init.cpp:417-441) on freshly
 // zeroed thermal nodal data (mpm_init.cpp:280-281): deposit (mass terms, no
 // source), backup, walls, delta, G2P of temperature and heat flux.
-int thermal_init(mpm_ctx *c) {
+int thermal_init(mpm_ctx *c, bool tiled) {
     if (!c->thermal) return MPM_OK;
     hipStream_t st = c->stream;
     const int64_t nn = c->g.nn;
     launch_zero(c->tn, TN_COMPS * nn, st);
-    launch_th_p2g(c, true, false, st);
+    launch_th_p2g(c, true, false, tiled && c->tpkt, (int)c->steps, 1, st);
     k_th_nodal<false><<<blocks(nn), kThreads, 0, st>>>(c->g, c->s.mass_tolerance, c->tn, nullptr);
     launch_th_walls(c, true, false, LS_NONE, st);
     launch_th_g2p(c, true, true, nullptr, nullptr, c->th, st);
@@ -627,10 +700,10 @@ int thermal_init(mpm_ctx *c) {
 // momentum grid pass has finished the nodal mass: Reset_Nodaldata_to_Zero's
 // share of the thermal components, deposit, normalise + backup + update,
 // walls, delta.
-int thermal_grid(mpm_ctx *c, hipStream_t st) {
+int thermal_grid(mpm_ctx *c, bool tiled, int epoch, int fallback, hipStream_t st) {
     const int64_t nn = c->g.nn;
     launch_zero(c->tn, TN_COMPS * nn, st);
-    launch_th_p2g(c, true, true, st);
+    launch_th_p2g(c, true, true, tiled && c->tpkt, epoch, fallback, st);
     k_th_nodal<true><<<blocks(nn), kThreads, 0, st>>>(c->g, c->s.mass_tolerance, c->tn, c->sc);
     launch_th_walls(c, true, true, c->s.stress_update_scheme == MPM_MUSL ? LS_DIRICHLET : LS_ALL,
                     st);
@@ -643,8 +716,8 @@ int thermal_grid(mpm_ctx *c, hipStream_t st) {
 // heat flux (main.cpp:130-135).  With perm (the sorted step) the last G2P
 // writes the second buffer by sorted position; thermal_swap follows the
 // particle buffers' swap.
-int thermal_particles(mpm_ctx *c, bool musl, const int *perm, const int *count,
-                      hipStream_t st) {
+int thermal_particles(mpm_ctx *c, bool musl, const int *perm, const int *count, int epoch,
+                      int fallback, hipStream_t st) {
     double *out = perm ? c->th_alt : c->th;
     if (!musl) {
         launch_th_g2p(c, true, true, perm, count, out, st);
@@ -656,7 +729,7 @@ int thermal_particles(mpm_ctx *c, bool musl, const int *perm, const int *count,
     // clears the thermal mass, its temperature sum and the source.
     launch_zero(c->tn + TN_MSP * nn, 2 * nn, st);
     launch_zero(c->tn + TN_SRC * nn, nn, st);
-    launch_th_p2g(c, false, true, st);
+    launch_th_p2g(c, false, true, perm != nullptr && c->tpkt, epoch, fallback, st);
     launch_th_walls(c, false, true, LS_ALL, st);
     launch_th_g2p(c, false, true, perm, count, out, st);
     return check_launch();
@@ -692,6 +765,8 @@ int thermal_udf_tables(mpm_ctx *c, double t) {
 }
 
 void thermal_free(mpm_ctx *c) {
+    (void)hipFree(c->tpkt);
+    c->tpkt = nullptr;
     for (int d = 0; d < 3; ++d)
         for (int h = 0; h < 2; ++h) {
             (void)hipFree(c->tudf_dev[d][h]);
