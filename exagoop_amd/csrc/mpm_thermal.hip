@@ -17,6 +17,7 @@
 
 #include "mpm_context.h"
 #include "mpm_device.h"
+#include "mpm_tiles.h"
 
 using namespace xmpm;
 
@@ -549,6 +550,117 @@ __global__ __launch_bounds__(kThreads) void k_th_g2p(GridP g, PState P, const in
 // step (perm, cell starts and tile list valid), so the deposit goes through
 // the tiled momentum deposit (fallback: particles may lie outside their
 // tile); otherwise one thread per particle with global atomics.
+// interpolate_from_grid_temperature over the cell-sorted state, one
+// workgroup per non-empty tile (uniform order): the tile's node block of T
+// and delta T is staged in LDS, each thread takes particles of the tile.  A
+// particle outside its tile (not redistributed this step) reads the nodes
+// from global memory; the arithmetic is that of k_th_g2p.  Sorted position i
+// reads slot perm[i] and writes slot i (sorted) or perm[i] (in place).
+template <int O, bool UT, bool UQ>
+__global__ __launch_bounds__(kBlock) void k_th_g2p_tile(GridP g, PState P, SortState S,
+                                                        const double *thin, double *thout,
+                                                        int sorted_out, const double *tn) {
+    __shared__ double nodes[kTNN * 2];
+    const int tile = active_tile(S, blockIdx.x);
+    if (tile < 0) return;  // uniform over the block
+    const int beg = S.cell_start[tile * kCellsPerTile];
+    const int end = S.cell_start[tile * kCellsPerTile + kCellsPerTile];
+    if (beg == end) return;
+    const int tx = tile % g.ntc[0];
+    const int ty = (tile / g.ntc[0]) % g.ntc[1];
+    const int tz = tile / (g.ntc[0] * g.ntc[1]);
+    const int T0[3] = {tx * kTileCells, ty * kTileCells, tz * kTileCells};
+    const int64_t nn = g.nn, cap = P.cap;
+    const double *Tg = tn + TN_T * nn, *DTg = tn + TN_DT * nn;
+    for (int e = threadIdx.x; e < kTNN; e += kBlock) {
+        const int a[3] = {e % kTN, (e / kTN) % kTN, e / (kTN * kTN)};
+        bool ok = true;
+        int I[3];
+        for (int d = 0; d < 3; ++d) I[d] = wrap_read(T0[d] - 1 + a[d], g.n[d], g.per[d], ok);
+        const int64_t q = nidx(g, I[0], I[1], I[2]);
+        nodes[2 * e] = ok ? Tg[q] : 0.0;
+        nodes[2 * e + 1] = ok ? DTg[q] : 0.0;
+    }
+    __syncthreads();
+    for (int i = beg + (int)threadIdx.x; i < end; i += kBlock) {
+        const int64_t j = S.perm[i];
+        double f[MPM_NTHERMAL];
+        for (int c = 0; c < MPM_NTHERMAL; ++c) f[c] = thin[c * cap + j];
+        if (P.id[j] >= 0 && P.idata[MPM_I_PHASE * cap + j] == 0) {
+            double x[3];
+            int iv[3], mn[3], mx[3];
+            bool inside = true;
+            for (int d = 0; d < 3; ++d) {
+                x[d] = P.pos[d * cap + j];
+                iv[d] = (int)floor((x[d] - g.plo[d]) * g.dxi[d]);
+                bounds(iv[d], 0, g.hi[d], O, g.per[d], mn[d], mx[d]);
+                inside &= iv[d] >= T0[d] && iv[d] < T0[d] + kTileCells;
+            }
+            // node (I, J, K) of the stencil, component 0 (T) or 1 (delta T)
+            auto nv = [&](int comp, int I, int J, int K) {
+                if (inside)
+                    return nodes[2 * (((K - T0[2] + 1) * kTN + (J - T0[1] + 1)) * kTN +
+                                      (I - T0[0] + 1)) + comp];
+                return nodal_read(g, comp ? DTg : Tg, I, J, K);
+            };
+            if (UT) {
+                double value;
+                if (O == 1) {
+                    const int ii = iv[0], jj = iv[1], kk = iv[2];
+                    const double a = (x[0] - (g.plo[0] + ii * g.dx[0])) / g.dx[0];
+                    const double b = (x[1] - (g.plo[1] + jj * g.dx[1])) / g.dx[1];
+                    const double c = (x[2] - (g.plo[2] + kk * g.dx[2])) / g.dx[2];
+                    value = (1.0 - a) * (1.0 - b) * (1.0 - c) * nv(1, ii, jj, kk);
+                    value += a * (1.0 - b) * (1.0 - c) * nv(1, ii + 1, jj, kk);
+                    value += (1.0 - a) * b * (1.0 - c) * nv(1, ii, jj + 1, kk);
+                    value += a * b * (1.0 - c) * nv(1, ii + 1, jj + 1, kk);
+                    value += (1.0 - a) * (1.0 - b) * c * nv(1, ii, jj, kk + 1);
+                    value += a * (1.0 - b) * c * nv(1, ii + 1, jj, kk + 1);
+                    value += (1.0 - a) * b * c * nv(1, ii, jj + 1, kk + 1);
+                    value += a * b * c * nv(1, ii + 1, jj + 1, kk + 1);
+                } else {
+                    double wv[3][4];
+                    for (int d = 0; d < 3; ++d)
+                        for (int l = mn[d]; l < mx[d]; ++l)
+                            wv[d][l - mn[d]] = interp1d(O, g.per[d], iv[d] + l, 0, g.hi[d], x[d],
+                                                        g.plo[d], g.dx[d]);
+                    value = 0.0;
+                    for (int nz = mn[2]; nz < mx[2]; ++nz)
+                        for (int ny = mn[1]; ny < mx[1]; ++ny)
+                            for (int nx = mn[0]; nx < mx[0]; ++nx)
+                                value += wv[0][nx - mn[0]] * wv[1][ny - mn[1]] *
+                                         wv[2][nz - mn[2]] *
+                                         nv(1, iv[0] + nx, iv[1] + ny, iv[2] + nz);
+                }
+                f[MPM_TH_TEMPERATURE] += value;
+            }
+            if (UQ) {
+                double bv[3][4], bd[3][4];
+                for (int d = 0; d < 3; ++d)
+                    for (int l = mn[d]; l < mx[d]; ++l)
+                        basis1d(O, g.per[d], iv[d], l, 0, g.hi[d], x[d], g.plo[d], g.dx[d],
+                                bv[d][l - mn[d]], bd[d][l - mn[d]]);
+                double gT[3] = {0.0, 0.0, 0.0};
+                for (int nz = mn[2]; nz < mx[2]; ++nz)
+                    for (int ny = mn[1]; ny < mx[1]; ++ny)
+                        for (int nx = mn[0]; nx < mx[0]; ++nx) {
+                            const double vx = bv[0][nx - mn[0]], vy = bv[1][ny - mn[1]],
+                                         vz = bv[2][nz - mn[2]];
+                            const double gr[3] = {bd[0][nx - mn[0]] * vy * vz,
+                                                  vx * bd[1][ny - mn[1]] * vz,
+                                                  vx * vy * bd[2][nz - mn[2]]};
+                            const double Tn = nv(0, iv[0] + nx, iv[1] + ny, iv[2] + nz);
+                            for (int d = 0; d < 3; ++d) gT[d] += Tn * gr[d];
+                        }
+                for (int d = 0; d < 3; ++d)
+                    f[MPM_TH_HEAT_FLUX + d] = -f[MPM_TH_CONDUCTIVITY] * gT[d];
+            }
+        }
+        const int64_t o = sorted_out ? (int64_t)i : j;
+        for (int c = 0; c < MPM_NTHERMAL; ++c) thout[c * cap + o] = f[c];
+    }
+}
+
 void launch_th_p2g(mpm_ctx *c, bool mass_temp, bool source, bool tiled, int epoch, int fallback,
                    hipStream_t st) {
     const GridP &g = c->g;
@@ -616,6 +728,31 @@ void launch_th_walls(mpm_ctx *c, bool delta, bool udf, int ls, hipStream_t st) {
         k_th_apply_walls<<<blocks(nn), kThreads, 0, st>>>(c->g, u, 2, c->tn, c->tscr);
     }
     if (delta) k_th_delta<<<blocks(nn), kThreads, 0, st>>>(c->g, c->tn);
+}
+
+// Thermal G2P.  With perm (the state sorted this step) and a uniform order,
+// the tiled kernel; sorted_out: write sorted positions (else in place).
+void launch_th_g2p_tiles(mpm_ctx *c, bool ut, bool uq, bool sorted_out, double *out,
+                         hipStream_t st) {
+    const GridP &g = c->g;
+    const dim3 gr((unsigned)((c->S.ntiles + 7) / 8 * 8)), bl(kBlock);
+    const int so = sorted_out ? 1 : 0;
+#define XMPM_TG(o)                                                                            \
+    do {                                                                                      \
+        if (ut && uq)                                                                         \
+            k_th_g2p_tile<o, true, true><<<gr, bl, 0, st>>>(g, c->P, c->S, c->th, out, so,    \
+                                                            c->tn);                           \
+        else if (ut)                                                                          \
+            k_th_g2p_tile<o, true, false><<<gr, bl, 0, st>>>(g, c->P, c->S, c->th, out, so,   \
+                                                             c->tn);                          \
+        else                                                                                  \
+            k_th_g2p_tile<o, false, true><<<gr, bl, 0, st>>>(g, c->P, c->S, c->th, out, so,   \
+                                                             c->tn);                          \
+    } while (0)
+    if (g.od[0] == 1) XMPM_TG(1);
+    else if (g.od[0] == 2) XMPM_TG(2);
+    else XMPM_TG(3);
+#undef XMPM_TG
 }
 
 void launch_th_g2p(mpm_ctx *c, bool ut, bool uq, const int *perm, const int *count,
@@ -719,19 +856,23 @@ int thermal_grid(mpm_ctx *c, bool tiled, int epoch, int fallback, hipStream_t st
 int thermal_particles(mpm_ctx *c, bool musl, const int *perm, const int *count, int epoch,
                       int fallback, hipStream_t st) {
     double *out = perm ? c->th_alt : c->th;
+    const bool tiles = perm != nullptr && c->g.uniform;
     if (!musl) {
-        launch_th_g2p(c, true, true, perm, count, out, st);
+        if (tiles) launch_th_g2p_tiles(c, true, true, true, out, st);
+        else launch_th_g2p(c, true, true, perm, count, out, st);
         return check_launch();
     }
     const int64_t nn = c->g.nn;
-    launch_th_g2p(c, true, false, nullptr, nullptr, c->th, st);
+    if (tiles) launch_th_g2p_tiles(c, true, false, false, c->th, st);
+    else launch_th_g2p(c, true, false, nullptr, nullptr, c->th, st);
     // P2G_Temperature(reset = 1, mass_temp = 0, source = 1): the reset
     // clears the thermal mass, its temperature sum and the source.
     launch_zero(c->tn + TN_MSP * nn, 2 * nn, st);
     launch_zero(c->tn + TN_SRC * nn, nn, st);
     launch_th_p2g(c, false, true, perm != nullptr && c->tpkt, epoch, fallback, st);
     launch_th_walls(c, false, true, LS_ALL, st);
-    launch_th_g2p(c, false, true, perm, count, out, st);
+    if (tiles) launch_th_g2p_tiles(c, false, true, true, out, st);
+    else launch_th_g2p(c, false, true, perm, count, out, st);
     return check_launch();
 }
 
