@@ -13,10 +13,9 @@
 // 16 lanes per cell: lane (ky, kz) owns the four stencil nodes along x of one
 // (y, z) stencil row and accumulates the cell's particles into registers;
 // the 12 1-D weights of a particle are computed once (one per lane) and
-// shared by lane shuffles.  Per-cell sums go into a private LDS slab of the
-// wavefront with plain read-modify-write (the 64 lanes always address 64
-// distinct nodes), the four slabs are summed at the end and flushed with one
-// global FP64 atomic per non-zero node component.  No LDS atomics.
+// shared by lane shuffles.  Per-row sums go into the tile's LDS node block
+// with LDS FP64 atomic adds (the four wavefronts' z-ranges overlap), and the
+// block is flushed with one global FP64 atomic per non-zero node component.
 // The fused particle body (particle_tile) stages the tile's node block (v,
 // dv) in LDS, runs G2P / F / D / move / volume / stress per particle and
 // writes the particle to the other buffer in sorted order, so particle state
@@ -36,7 +35,6 @@ constexpr int kBlock = 256;
 constexpr int kTN = kTileCells + 3;                    // LDS nodes per tile edge (offsets -1..+2)
 constexpr int kTNN = kTN * kTN * kTN;                  // LDS nodes per tile
 constexpr int kCellsPerTile = kTileCells * kTileCells * kTileCells;
-constexpr int kSlab = kTN * kTN * kTileCells;          // nodes of one z-slice's slab (7 x 7 x 4)
 
 static_assert(kTileCells == 4 && kBlock == 256, "wavefront/cell mapping assumes 4^3 tiles");
 
@@ -314,9 +312,10 @@ struct StageRaw {
 // owning the four stencil nodes along x of one (y, z) stencil row, read the
 // packets and accumulate into registers; offsets compute_bounds excludes
 // have zero factors (they only address nodes outside the domain, which the
-// reference clips).  After a row the sums go into the wavefront's private
-// LDS slab with plain read-modify-write (the 64 lanes address 64 distinct
-// nodes), and at the end the four slabs are summed per node and flushed with
+// reference clips).  After a row the sums go into the tile's LDS node block
+// with LDS FP64 atomic adds: one block instead of a private slab per
+// wavefront keeps the LDS at 42 KB, three workgroups per CU (the register
+// count allows three waves per SIMD).  At the end the block is flushed with
 // one global FP64 atomic per non-zero node component.  FALLBACK: a particle
 // outside its cell (not redistributed this step) deposits through
 // deposit_global; without it such a particle raises the error flag.
@@ -324,7 +323,7 @@ struct StageRaw {
 // LDS of one P2G tile (NC accumulated components).
 template <int NC>
 struct P2gLds {
-    double slab[4][NC * kSlab];  // one private slab per wavefront
+    double slab[NC * kTNN];      // the tile's node block, shared by the wavefronts
     double pk[4][kPkArea];       // staging area per wavefront
     int cs[kCellsPerTile + 1];
     SplineTables tab;
@@ -355,12 +354,11 @@ __device__ __forceinline__ void p2g_tile(const GridP &g, const PhysP &ph, const 
         for (int e = tid; e < (int)(sizeof(SplineTables) / sizeof(int)); e += kBlock)
             dst[e] = src[e];
     }
+    for (int e = tid; e < NC * kTNN; e += kBlock) slab[e] = 0.0;
     __syncthreads();
     if (cs[kCellsPerTile] == cs[0]) return;  // empty tile (uniform over the block)
     const int wave = tid >> 6, lane = tid & 63;
-    double *my = slab[wave];
     double *pkw = pk[wave];
-    for (int e = lane; e < NC * kSlab; e += 64) my[e] = 0.0;
     const int tx = tile % g.ntc[0];
     const int ty = (tile / g.ntc[0]) % g.ntc[1];
     const int tz = tile / (g.ntc[0] * g.ntc[1]);
@@ -538,20 +536,18 @@ __device__ __forceinline__ void p2g_tile(const GridP &g, const PhysP &ph, const 
         }
         asm volatile("" ::: "memory");  // reads of this round before the next writes
         if (nry != ry) {
-            // Row done.  For a given k the 64 lanes address 64 distinct slab
-            // nodes (x = q + k, y = ry + ky, z = kz), so the read-modify-write
-            // needs no atomics; but node q + k of step k is node (q+1) + (k-1)
-            // of the next group's step k-1, so the compiler must keep the
-            // steps apart (LDS operations of a wavefront execute in order).
+            // Row done: lane (q, ky, kz) adds its four nodes (x = q + k,
+            // y = ry + ky, z = cz + kz); the other wavefronts' slices reach the
+            // same z-levels, hence the atomics.
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
-                const int idx = (kz * kTN + (ry + ky)) * kTN + (q + k);
+                const int idx = ((cz + kz) * kTN + (ry + ky)) * kTN + (q + k);
 #pragma unroll
                 for (int c = 0; c < NC; ++c) {
-                    my[c * kSlab + idx] += a4[k][c];
+                    __hip_atomic_fetch_add(&slab[c * kTNN + idx], a4[k][c], __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP);
                     a4[k][c] = 0.0;
                 }
-                asm volatile("" ::: "memory");
             }
         }
         ry = nry;
@@ -559,20 +555,14 @@ __device__ __forceinline__ void p2g_tile(const GridP &g, const PhysP &ph, const 
         cur = nxt;
     }
     __syncthreads();
-    // Sum the four slabs per tile node and flush: one global atomic per
-    // non-zero node component; periodic image targets as in deposit_global;
-    // mark the node tiles that received data.
+    // Flush the node block: one global atomic per non-zero node component;
+    // periodic image targets as in deposit_global; mark the node tiles that
+    // received data.
     const int64_t nn = g.nn;
     for (int e = tid; e < kTNN; e += kBlock) {
         const int ax = e % kTN, ay = (e / kTN) % kTN, az = e / (kTN * kTN);
         double v[NC];
-        for (int c = 0; c < NC; ++c) v[c] = 0.0;
-        for (int w = 0; w < 4; ++w) {
-            const int k = az - w;
-            if (k < 0 || k >= kTileCells) continue;
-            const int idx = (k * kTN + ay) * kTN + ax;
-            for (int c = 0; c < NC; ++c) v[c] += slab[w][c * kSlab + idx];
-        }
+        for (int c = 0; c < NC; ++c) v[c] = slab[c * kTNN + e];
         bool any = false;
         for (int c = 0; c < NC; ++c) any |= (v[c] != 0.0);
         if (!any) continue;
