@@ -767,28 +767,49 @@ __device__ __forceinline__ void particle_tile(const GridP &g, const PhysP &ph, c
         const int64_t i = beg + (int64_t)it * kBlock + tid;
         if (i >= end) continue;
         const int64_t j = S.perm[i];
-        const int phase = Pin.idata[MPM_I_PHASE * ci + j];
         double x[3], v[3], vp[3], D[6], F[9];
         for (int d = 0; d < 3; ++d) x[d] = Pin.pos[d * ci + j];
-        if (phase == 0) {
+        // The gather waits for x only if x is read first: reads complete in
+        // issue order.
+        asm volatile("" ::: "memory");
+        const int phase = Pin.idata[MPM_I_PHASE * ci + j];
+        const bool rigid = phase != 0;
+        // Every remaining read of the particle, in one batch ahead of its
+        // first write and of the G2P gather, which hides the batch's
+        // latency: Pin and Pout never overlap, but the compiler cannot know
+        // that, so a read placed after a write would wait a memory round
+        // trip of its own.
+        for (int d = 0; d < 3; ++d) v[d] = Pin.r[(MPM_R_VEL + d) * ci + j];
+        if (!VEL)
+            for (int d = 0; d < 3; ++d) vp[d] = Pin.r[(MPM_R_VELP + d) * ci + j];
+        for (int q = 0; q < 9; ++q) F[q] = Pin.r[(MPM_R_DEFGRAD + q) * ci + j];
+        int64_t id = Pin.id[j];
+        const int model = Pin.idata[MPM_I_MODEL * ci + j];
+        const double m = Pin.r[MPM_R_MASS * ci + j];
+        const double V0 = Pin.r[MPM_R_VOLINIT * ci + j];
+        const double E = Pin.r[MPM_R_E * ci + j], nu = Pin.r[MPM_R_NU * ci + j];
+        const double K = Pin.r[MPM_R_BULK * ci + j];
+        const double gam = Pin.r[MPM_R_GAMMA * ci + j], mu = Pin.r[MPM_R_VISC * ci + j];
+        double eps[6], sig[6];
+        for (int q = 0; q < 6; ++q) eps[q] = Pin.r[(MPM_R_STRAIN + q) * ci + j];
+        double J = 0.0, V = 0.0, rho = 0.0, p = 0.0;
+        if (DELTA)
+            for (int q = 0; q < 6; ++q) sig[q] = Pin.r[(MPM_R_STRESS + q) * ci + j];
+        if (!rigid) {
             double vals0[3], vals1[3], L[9];
             // Every particle lies in its tile when particles are
             // redistributed each step (the only configuration this kernel
             // is launched for).
             if (!g2p_block<O, VEL>(g, O > 1 ? tab : kTab, nodes, T0, x, vals0, vals1, L))
                 *err = 1;
-            for (int d = 0; d < 3; ++d) v[d] = Pin.r[(MPM_R_VEL + d) * ci + j];
             if (VEL) {
                 const double alpha = ph.alpha;
                 for (int d = 0; d < 3; ++d) {
                     vp[d] = vals0[d];
                     v[d] = alpha * v[d] + alpha * vals1[d] + (1 - alpha) * vals0[d];
                 }
-            } else {
-                for (int d = 0; d < 3; ++d) vp[d] = Pin.r[(MPM_R_VELP + d) * ci + j];
             }
             // get_deformation_gradient_tensor (interpolants.H:1732-1787)
-            for (int q = 0; q < 9; ++q) F[q] = Pin.r[(MPM_R_DEFGRAD + q) * ci + j];
             double nrm = 0.0;
             for (int q = 0; q < 9; q++) nrm += L[q] * L[q];
             if (!(nrm >= 1.0e10)) {
@@ -810,15 +831,23 @@ __device__ __forceinline__ void particle_tile(const GridP &g, const PhysP &ph, c
                     D[ind] = 0.5 * (L[d1 * 3 + d2] + L[d2 * 3 + d1]);
                     ind++;
                 }
-        } else {
-            for (int d = 0; d < 3; ++d) {
-                v[d] = Pin.r[(MPM_R_VEL + d) * ci + j];
-                vp[d] = Pin.r[(MPM_R_VELP + d) * ci + j];
-            }
-            for (int q = 0; q < 6; ++q) D[q] = Pin.r[(MPM_R_STRAINRATE + q) * ci + j];
-            for (int q = 0; q < 9; ++q) F[q] = Pin.r[(MPM_R_DEFGRAD + q) * ci + j];
         }
-        int64_t id = Pin.id[j];
+        // Reads that depend on the phase or the model come after the
+        // gather: in the batch above, the branch on the model would wait for
+        // the model's read, and registers shared between the rigid-only
+        // values and the gather's accumulators would make the gather wait
+        // for most of the batch.
+        if (rigid) {
+            if (VEL)
+                for (int d = 0; d < 3; ++d) vp[d] = Pin.r[(MPM_R_VELP + d) * ci + j];
+            for (int q = 0; q < 6; ++q) D[q] = Pin.r[(MPM_R_STRAINRATE + q) * ci + j];
+            J = Pin.r[MPM_R_JACOBIAN * ci + j];
+            V = Pin.r[MPM_R_VOLUME * ci + j];
+            rho = Pin.r[MPM_R_DENSITY * ci + j];
+            if (!DELTA)
+                for (int q = 0; q < 6; ++q) sig[q] = Pin.r[(MPM_R_STRESS + q) * ci + j];
+        }
+        if (rigid || model != 1) p = Pin.r[MPM_R_PRESSURE * ci + j];
         double *R = Pout.r;
         // moveParticles without level sets or UDF tables
         // (mpm_particle_timestep.cpp:329-576), then Redistribute.  Each
@@ -856,46 +885,32 @@ __device__ __forceinline__ void particle_tile(const GridP &g, const PhysP &ph, c
         const double vmag2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
         for (int q = 0; q < 6; ++q) R[(MPM_R_STRAINRATE + q) * co + i] = D[q];
         // updateVolume and the constitutive update
-        const int model = Pin.idata[MPM_I_MODEL * ci + j];
-        const double m = Pin.r[MPM_R_MASS * ci + j];
-        const double V0 = Pin.r[MPM_R_VOLINIT * ci + j];
         R[MPM_R_MASS * co + i] = m;
         R[MPM_R_VOLINIT * co + i] = V0;
         Pout.idata[MPM_I_MODEL * co + i] = model;
-        double J, V, rho;
-        if (phase == 0) {
+        if (!rigid) {
             J = det3(F);
             V = V0 * J;
             rho = m / V;
-        } else {
-            J = Pin.r[MPM_R_JACOBIAN * ci + j];
-            V = Pin.r[MPM_R_VOLUME * ci + j];
-            rho = Pin.r[MPM_R_DENSITY * ci + j];
         }
         R[MPM_R_VOLUME * co + i] = V;
         R[MPM_R_DENSITY * co + i] = rho;
         R[MPM_R_JACOBIAN * co + i] = J;
         for (int q = 0; q < 9; ++q) R[(MPM_R_DEFGRAD + q) * co + i] = F[q];
-        const double E = Pin.r[MPM_R_E * ci + j], nu = Pin.r[MPM_R_NU * ci + j];
-        const double K = Pin.r[MPM_R_BULK * ci + j];
         R[MPM_R_E * co + i] = E;
         R[MPM_R_NU * co + i] = nu;
         R[MPM_R_BULK * co + i] = K;
         // The dt candidate depends only on state final at this point.
-        if (id >= 0 && phase == 0) cmin = fmin(cmin, dt_candidate(g, model, K, E, nu, rho, vmag2));
-        const double gam = Pin.r[MPM_R_GAMMA * ci + j], mu = Pin.r[MPM_R_VISC * ci + j];
+        if (id >= 0 && !rigid) cmin = fmin(cmin, dt_candidate(g, model, K, E, nu, rho, vmag2));
         R[MPM_R_GAMMA * co + i] = gam;
         R[MPM_R_VISC * co + i] = mu;
-        double eps[6], sig[6], p;
-        for (int q = 0; q < 6; ++q) eps[q] = Pin.r[(MPM_R_STRAIN + q) * ci + j];
-        if (phase == 0) {
+        if (!rigid) {
             const double rate = (time < ph.applied_time) ? ph.applied_rate : 0.0;
             for (int q = 0; q < 6; ++q) eps[q] = eps[q] + dt * D[q];
             eps[XX] += dt * rate;
             eps[YY] += dt * rate;
             eps[ZZ] += dt * rate;
             for (int q = 0; q < 6; ++q) R[(MPM_R_STRAIN + q) * co + i] = eps[q];
-            p = (model == 1) ? 0.0 : Pin.r[MPM_R_PRESSURE * ci + j];
             if (DELTA) {
                 double de[6], ds[6];
                 for (int q = 0; q < 6; ++q) de[q] = dt * D[q];
@@ -903,7 +918,7 @@ __device__ __forceinline__ void particle_tile(const GridP &g, const PhysP &ph, c
                 de[YY] += dt * rate;
                 de[ZZ] += dt * rate;
                 linear_elastic(de, ds, E, nu);
-                for (int q = 0; q < 6; ++q) sig[q] = Pin.r[(MPM_R_STRESS + q) * ci + j] + ds[q];
+                for (int q = 0; q < 6; ++q) sig[q] = sig[q] + ds[q];
             } else if (model == 2) {
                 // (the Tait branch goes last: F, eps, E, nu are dead there)
                 neo_hookean(F, sig, E, nu);
@@ -916,8 +931,6 @@ __device__ __forceinline__ void particle_tile(const GridP &g, const PhysP &ph, c
             }
         } else {
             for (int q = 0; q < 6; ++q) R[(MPM_R_STRAIN + q) * co + i] = eps[q];
-            p = Pin.r[MPM_R_PRESSURE * ci + j];
-            for (int q = 0; q < 6; ++q) sig[q] = Pin.r[(MPM_R_STRESS + q) * ci + j];
         }
         for (int q = 0; q < 6; ++q) R[(MPM_R_STRESS + q) * co + i] = sig[q];
         R[MPM_R_PRESSURE * co + i] = p;
