@@ -763,12 +763,25 @@ __device__ __forceinline__ void particle_tile(const GridP &g, const PhysP &ph, c
     __syncthreads();
     const int64_t ci = Pin.cap, co = Pout.cap;
     const int niter = (end - beg + kBlock - 1) / kBlock;
+    // The permutation entry and the position of a thread's next particle are
+    // read one iteration ahead, before the current particle's writes (reads
+    // and writes complete in issue order, so a read issued after the writes
+    // would wait for them).
+    int64_t jn = 0;
+    double xn[3] = {0.0, 0.0, 0.0};
+    if (beg + tid < end) {
+        jn = S.perm[beg + tid];
+        for (int d = 0; d < 3; ++d) xn[d] = Pin.pos[d * ci + jn];
+    }
     for (int it = 0; it < niter; ++it) {
         const int64_t i = beg + (int64_t)it * kBlock + tid;
         if (i >= end) continue;
-        const int64_t j = S.perm[i];
+        const int64_t j = jn;
         double x[3], v[3], vp[3], D[6], F[9];
-        for (int d = 0; d < 3; ++d) x[d] = Pin.pos[d * ci + j];
+        for (int d = 0; d < 3; ++d) x[d] = xn[d];
+        const bool has_next = i + kBlock < end;
+        // unconditional (a clamped index), so no branch waits for it
+        const int jnext = S.perm[has_next ? i + kBlock : i];
         // The gather waits for x only if x is read first: reads complete in
         // issue order.
         asm volatile("" ::: "memory");
@@ -795,13 +808,15 @@ __device__ __forceinline__ void particle_tile(const GridP &g, const PhysP &ph, c
         double J = 0.0, V = 0.0, rho = 0.0, p = 0.0;
         if (DELTA)
             for (int q = 0; q < 6; ++q) sig[q] = Pin.r[(MPM_R_STRESS + q) * ci + j];
+        // The gather runs for every particle (rigid ones are few and drop
+        // the result), so no branch on the phase waits for the phase's read
+        // ahead of the rest of the batch.
+        double vals0[3], vals1[3], L[9];
+        // Every particle lies in its tile when particles are redistributed
+        // each step (the only configuration this kernel is launched for).
+        const bool in_tile = g2p_block<O, VEL>(g, O > 1 ? tab : kTab, nodes, T0, x, vals0, vals1, L);
         if (!rigid) {
-            double vals0[3], vals1[3], L[9];
-            // Every particle lies in its tile when particles are
-            // redistributed each step (the only configuration this kernel
-            // is launched for).
-            if (!g2p_block<O, VEL>(g, O > 1 ? tab : kTab, nodes, T0, x, vals0, vals1, L))
-                *err = 1;
+            if (!in_tile) *err = 1;
             if (VEL) {
                 const double alpha = ph.alpha;
                 for (int d = 0; d < 3; ++d) {
@@ -848,6 +863,10 @@ __device__ __forceinline__ void particle_tile(const GridP &g, const PhysP &ph, c
                 for (int q = 0; q < 6; ++q) sig[q] = Pin.r[(MPM_R_STRESS + q) * ci + j];
         }
         if (rigid || model != 1) p = Pin.r[MPM_R_PRESSURE * ci + j];
+        if (has_next) {
+            jn = jnext;
+            for (int d = 0; d < 3; ++d) xn[d] = Pin.pos[d * ci + jn];
+        }
         double *R = Pout.r;
         // moveParticles without level sets or UDF tables
         // (mpm_particle_timestep.cpp:329-576), then Redistribute.  Each
