@@ -53,11 +53,13 @@ struct mpm_ctx {
     int keep_forces = 1;
     // Learned-correction work buffers (allocated by its first call, nodal sized).
     int *ml_nodes = nullptr;      // active node list / selected rows
-    int *ml_count = nullptr;      // [0] active count, [1] selected count, [2] appended count
+    int *ml_count = nullptr;      // [0] active count, [1] selected, [2] appended, [3] guard flag,
+                                  // [4] guard workgroups done, [8] alternate active count
     __bf16 *ml_x = nullptr;       // feature rows, padded to 32 rows
     float *ml_y = nullptr;        // network outputs [row][3]
     double *ml_vc = nullptr;      // coarse nodal velocity of selected rows [row][3]
     double *ml_guard = nullptr;   // guard sums
+    int *ml_hflag = nullptr;      // pinned host copy of the guard flag
     int64_t ml_cap = 0;
     // Thermal module (mpm_enable_thermal): particle fields in slot order,
     // MPM_NTHERMAL * th_cap each (th_alt: the sorted step's output), nodal

@@ -330,6 +330,7 @@ void mpm_destroy(mpm_ctx *c) {
     (void)hipFree(c->ml_y);
     (void)hipFree(c->ml_vc);
     (void)hipFree(c->ml_guard);
+    if (c->ml_hflag) (void)hipHostFree(c->ml_hflag);
     for (double *p : c->ls_bufs) (void)hipFree(p);
     for (int d = 0; d < 3; ++d)
         for (int h = 0; h < 2; ++h) (void)hipFree(c->udf_dev[d][h]);

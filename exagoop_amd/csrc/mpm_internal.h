@@ -119,9 +119,11 @@ void launch_particle(const GridP &g, const PhysP &ph, const LsP &ls, const UdfP 
                      const PState &P, const NState &N, int flags, double dt, double time,
                      const Scalars *sc, hipStream_t s);
 // fused: take the minimum the previous step's particle kernel accumulated;
-// advance: first add the previous dt to the time (the previous step's end).
+// advance: first add the previous dt to the time (the previous step's end);
+// scale: multiplies the final dt (a coarse step of that many fine steps).
 void launch_dt(const GridP &g, const PhysP &ph, const PState &P, Scalars *sc, int fixed,
-               double fixed_dt, hipStream_t s, int fused = 0, int advance = 0);
+               double fixed_dt, hipStream_t s, int fused = 0, int advance = 0,
+               double scale = 1.0);
 void launch_advance_time(Scalars *sc, hipStream_t s);
 void launch_energies(const PState &P, double *out6, hipStream_t s);
 void launch_snapshot_vel(const PState &P, hipStream_t s);

@@ -98,7 +98,7 @@ __global__ void k_dt_init(Scalars *sc) {
 }
 
 __global__ void k_dt_final(Scalars *sc, PhysP ph, int fixed, double fixed_dt, int fused,
-                           int advance) {
+                           int advance, double scale) {
     // The step about to run accumulates the following step's minimum, so the
     // running value is reset here.  Read and reset are one atomic exchange: a
     // plain load of a uniform address becomes a scalar-cache load, which the
@@ -115,6 +115,7 @@ __global__ void k_dt_final(Scalars *sc, PhysP ph, int fixed, double fixed_dt, in
         dt = ph.cfl * dt;
         dt = fmax(fmin(dt, ph.dt_max), ph.dt_min);
     }
+    dt = dt * scale;  // a coarse step of several fine steps (1 otherwise)
     // Same exchange for dt: the previous step's dt advances the time
     // (the end of that step, folded into this launch).
     const unsigned long long old = atomicExch(reinterpret_cast<unsigned long long *>(&sc->dt),
@@ -227,14 +228,14 @@ void launch_particle(const GridP &g, const PhysP &ph, const LsP &ls, const UdfP 
 }
 
 void launch_dt(const GridP &g, const PhysP &ph, const PState &P, Scalars *sc, int fixed,
-               double fixed_dt, hipStream_t s, int fused, int advance) {
+               double fixed_dt, hipStream_t s, int fused, int advance, double scale) {
     if (!fixed && !fused) {
         hipLaunchKernelGGL(k_dt_init, dim3(1), dim3(1), 0, s, sc);
         if (P.np > 0)
             hipLaunchKernelGGL(k_dt, dim3(blocks_for(P.np)), dim3(kBlock), 0, s, g, P, sc);
     }
     hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(1), 0, s, sc, ph, fixed, fixed_dt,
-                       (fixed || !fused) ? 0 : 1, advance);
+                       (fixed || !fused) ? 0 : 1, advance, scale);
 }
 
 void launch_advance_time(Scalars *sc, hipStream_t s) {

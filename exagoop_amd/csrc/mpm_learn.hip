@@ -44,6 +44,8 @@ struct mpm_dataset {
 
 namespace {
 
+constexpr int kMlCounts = 16;  // ints of mpm_ctx::ml_count
+
 // float -> bfloat16 bits, round to nearest even (the conversion the device
 // applies to the same values).
 uint16_t bf16_bits(float x) {
@@ -92,11 +94,18 @@ int ctx_ml_buffers(mpm_ctx *c) {
     const int64_t pad = (nn + kMRows - 1) / kMRows * kMRows;
     if (c->ml_cap >= pad) return MPM_OK;
     HIPCHK(hipMalloc(&c->ml_nodes, 2 * pad * sizeof(int)));  // active list, selected rows
-    HIPCHK(hipMalloc(&c->ml_count, 4 * sizeof(int)));
+    // [0] active count, [1] selected, [2] appended, [3] guard flag, [4] guard
+    // workgroups done, [8] the other step parity's active count
+    HIPCHK(hipMalloc(&c->ml_count, kMlCounts * sizeof(int)));
+    HIPCHK(hipMemset(c->ml_count, 0, kMlCounts * sizeof(int)));
     HIPCHK(hipMalloc(&c->ml_x, pad * kMInPad * sizeof(__bf16)));
     HIPCHK(hipMalloc(&c->ml_y, pad * kMOut * sizeof(float)));
     HIPCHK(hipMalloc(&c->ml_vc, pad * 3 * sizeof(double)));
     HIPCHK(hipMalloc(&c->ml_guard, 8 * sizeof(double)));
+    HIPCHK(hipMemset(c->ml_guard, 0, 8 * sizeof(double)));
+    // the per-step decision is read into pinned memory: a copy into pageable
+    // memory goes through a staging buffer
+    if (!c->ml_hflag) HIPCHK(hipHostMalloc(&c->ml_hflag, 4 * sizeof(int), hipHostMallocDefault));
     c->ml_cap = pad;
     return MPM_OK;
 }
@@ -117,17 +126,16 @@ int learned_ok(const mpm_ctx *c) {
     return MPM_OK;
 }
 
-__global__ void k_scale_dt(Scalars *sc, double f) { sc->dt = sc->dt * f; }
-
 // Coarse step up to the nodal update: dt = nfine x the CFL dt, sort, P2G,
-// nodal update + BCs.  Leaves the step counter advanced.
-int coarse_begin(mpm_ctx *c, int nfine) {
+// nodal update + BCs.  Leaves the step counter advanced.  advance: the
+// previous step's time advance is still due (it is folded into the dt
+// launch).
+int coarse_begin(mpm_ctx *c, int nfine, int advance = 0) {
     hipStream_t st = c->stream;
     c->steps++;
     const int step = (int)c->steps;
     launch_dt(c->g, c->ph, c->P, c->sc, c->s.fixed_timestep == 1, c->s.timestep, st,
-              c->dt_next_valid ? 1 : 0, 0);
-    hipLaunchKernelGGL(k_scale_dt, dim3(1), dim3(1), 0, st, c->sc, (double)nfine);
+              c->dt_next_valid ? 1 : 0, advance, (double)nfine);
     launch_sort(c->g, c->ph, c->P, c->S, c->Q.id, 1, st);
     launch_p2g_tiles(c->g, c->ph, c->P, c->S, c->N, step, 0, 0, c->sc, st);
     const int *te = (c->skip_ok && !c->all_dirty) ? c->S.tile_epoch : nullptr;
@@ -216,9 +224,12 @@ __global__ void k_append_rows(const int *sel, int nsel, const __bf16 *X, const d
 
 // Guard sums over the active nodes (row count on the device): [0] sum m,
 // [1..3] sum m dv, [4] sum m |v|^2, [5] sum m |v + dv|^2, [6] non-finite
-// count.
+// count.  The last workgroup to finish takes the decision (see
+// mpm_step_learned: flag = 1 keeps the corrected coarse step) and clears the
+// sums and *done for the next call; both must be zero on the first call.
 __global__ void k_guard_dev(const int *nodes, const int *count, const float *Y, double dv_ref,
-                            NState N, int64_t nn, double *sums) {
+                            NState N, int64_t nn, double *sums, int *done, const Scalars *sc,
+                            double gmag, double gp, double ge, int *flag) {
     const int n = *count;
     double a[7] = {0, 0, 0, 0, 0, 0, 0};
     for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
@@ -239,6 +250,7 @@ __global__ void k_guard_dev(const int *nodes, const int *count, const float *Y, 
     }
     // wavefront sums, then one atomic per workgroup and sum
     __shared__ double red[7][4];
+    __shared__ bool last;
     const int w = threadIdx.x >> 6;
     for (int k = 0; k < 7; ++k) {
         double x = a[k];
@@ -251,14 +263,16 @@ __global__ void k_guard_dev(const int *nodes, const int *count, const float *Y, 
                          red[threadIdx.x][3];
         if (x != 0.0) atomicAdd(&sums[threadIdx.x], x);
     }
-}
-
-// Guard decision (see mpm_step_learned): flag = 1 keeps the corrected coarse
-// step.  The sums are read back with atomics, where they were accumulated.
-__global__ void k_guard_decide(double *sums, const Scalars *sc, double gmag, double gp, double ge,
-                               int *flag) {
+    // This workgroup's sums are visible device-wide before its count.
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) last = atomicAdd(done, 1) == (int)gridDim.x - 1;
+    __syncthreads();
+    if (!last || threadIdx.x != 0) return;
+    __threadfence();
     double gs[7];
-    for (int k = 0; k < 7; ++k) gs[k] = atomicAdd(&sums[k], 0.0);
+    for (int k = 0; k < 7; ++k) gs[k] = atomicExch(&sums[k], 0.0);  // read and clear
+    atomicExch(done, 0);
     const double dt = sc->dt;
     const double M = gs[0];
     const double u = M > 0 ? sqrt(gs[4] / M) : 0.0;
@@ -284,10 +298,6 @@ __global__ void k_apply_dev(const int *nodes, const int *count, const float *Y, 
             N.dvel[d * nn + q] += dv;
         }
     }
-}
-
-__global__ void k_advance_time_if(Scalars *sc, const int *flag) {
-    if (*flag) sc->time = sc->time + sc->dt;
 }
 
 __global__ void k_zero_d(double *p, int n) {
@@ -618,42 +628,49 @@ int mpm_step_learned(mpm_ctx *c, mpm_mlp *m, int ncoarse, int nfine, const doubl
     const double gmag = std::sqrt(c->ph.grav[0] * c->ph.grav[0] + c->ph.grav[1] * c->ph.grav[1] +
                                   c->ph.grav[2] * c->ph.grav[2]);
     int *flag = c->ml_count + 3;
+    int *done = c->ml_count + 4;
+    int *slot[2] = {c->ml_count, c->ml_count + 8};  // active counts by step parity
+    hipLaunchKernelGGL(k_zero_f, dim3(1), dim3(64), 0, st, reinterpret_cast<float *>(slot[0]), 1);
     int fb = 0;
+    bool pending = false;  // an accepted step's time advance is still due
     for (int it = 0; it < ncoarse; ++it) {
-        if ((rc = coarse_begin(c, nfine))) return rc;
+        if ((rc = coarse_begin(c, nfine, pending ? 1 : 0))) return rc;
+        pending = false;
         if (!m) {  // uncorrected coarse step: nothing to decide, no round trip
             if ((rc = coarse_finish(c))) return rc;
             continue;
         }
         // Correction, guard and the particle kernel (into the spare buffer)
         // are enqueued back to back; one round trip reads the decision.  A
-        // rejected step leaves the particles in P untouched.
-        launch_active_nodes(c->g, c->N, c->ph.mass_tol, c->ml_nodes, c->ml_count, nullptr, st);
-        launch_mlp_features_dev(c->g, c->N, c->ml_nodes, c->ml_count, m->sc, c->ml_x, st);
-        launch_mlp_infer_dev(m->d, c->ml_x, c->ml_count, c->ml_y, st);
-        hipLaunchKernelGGL(k_zero_d, dim3(1), dim3(64), 0, st, c->ml_guard, 8);
-        hipLaunchKernelGGL(k_guard_dev, dim3(256), dim3(256), 0, st, c->ml_nodes, c->ml_count,
-                           c->ml_y, m->sc.dv_ref, c->N, c->g.nn, c->ml_guard);
-        hipLaunchKernelGGL(k_guard_decide, dim3(1), dim3(1), 0, st, c->ml_guard, c->sc, gmag,
+        // rejected step leaves the particles in P untouched.  The list
+        // launch clears the other parity's count for the next step, and the
+        // guard's last workgroup clears the guard sums.
+        int *cnt = slot[it & 1];
+        launch_active_nodes_rolling(c->g, c->N, c->ph.mass_tol, c->ml_nodes, cnt,
+                                    slot[(it + 1) & 1], st);
+        launch_mlp_features_dev(c->g, c->N, c->ml_nodes, cnt, m->sc, c->ml_x, st);
+        launch_mlp_infer_dev(m->d, c->ml_x, cnt, c->ml_y, st);
+        hipLaunchKernelGGL(k_guard_dev, dim3(256), dim3(256), 0, st, c->ml_nodes, cnt, c->ml_y,
+                           m->sc.dv_ref, c->N, c->g.nn, c->ml_guard, done, c->sc, gmag,
                            guards[0], guards[1], flag);
-        hipLaunchKernelGGL(k_apply_dev, dim3(512), dim3(256), 0, st, c->ml_nodes, c->ml_count,
-                           c->ml_y, m->sc.dv_ref, c->N, c->g.nn, flag);
+        hipLaunchKernelGGL(k_apply_dev, dim3(512), dim3(256), 0, st, c->ml_nodes, cnt, c->ml_y,
+                           m->sc.dv_ref, c->N, c->g.nn, flag);
         launch_particle_fast(c->g, c->ph, c->P, c->Q, c->S, c->N, 1, 1, c->sc, st);
-        hipLaunchKernelGGL(k_advance_time_if, dim3(1), dim3(1), 0, st, c->sc, flag);
         if ((rc = check_launch())) return rc;
-        int ok = 0;
-        HIPCHK(hipMemcpyAsync(&ok, flag, sizeof(int), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(c->ml_hflag, flag, sizeof(int), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
-        if (ok) {
+        if (c->ml_hflag[0]) {
             std::swap(c->P, c->Q);
             c->dt_next_valid = c->fuse_dt != 0;
             c->dirty = false;
+            pending = true;
         } else {
             coarse_drop(c);  // also drops the dt candidates of the rejected kernel
             ++fb;
             if ((rc = mpm_step(c, nfine))) return rc;
         }
     }
+    if (pending) launch_advance_time(c->sc, st);
     if (fallbacks) *fallbacks = fb;
     return finish_call(c);
 }

@@ -63,5 +63,9 @@ void launch_mlp_pack(const MlpDev &w, hipStream_t s);
 // Active node list: nodes with mass >= tol, in index order, count in *count.
 void launch_active_nodes(const GridP &g, const NState &N, double tol, int *list, int *count,
                          int *scratch, hipStream_t s);
+// The same without clearing *count first (it must be zero on entry); the
+// launch zeroes *next_count for the following step's list instead.
+void launch_active_nodes_rolling(const GridP &g, const NState &N, double tol, int *list,
+                                 int *count, int *next_count, hipStream_t s);
 
 }  // namespace xmpm

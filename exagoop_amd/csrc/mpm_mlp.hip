@@ -392,12 +392,14 @@ __global__ void k_mlp_features(GridP g, NState N, const int *nodes, int nrows,
 }
 
 // Active nodes (mass >= tol): one atomic per workgroup reserves its run.
+// next_count (if given) is zeroed for the following step's list.
 __global__ __launch_bounds__(256) void k_active_nodes(GridP g, NState N, double tol, int *list,
-                                                      int *count) {
+                                                      int *count, int *next_count) {
     __shared__ int wsum[4];
     __shared__ int base;
     const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (next_count && blockIdx.x == 0 && threadIdx.x == 0) *next_count = 0;
     const bool f = q < g.nn && N.mass[q] >= tol;
     const unsigned long long b = __ballot(f);
     const int inwave = __popcll(b & ((1ull << lane) - 1ull));
@@ -473,7 +475,13 @@ void launch_active_nodes(const GridP &g, const NState &N, double tol, int *list,
     (void)scratch;
     hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, s, count, 1);
     hipLaunchKernelGGL(k_active_nodes, dim3((unsigned)((g.nn + 255) / 256)), dim3(256), 0, s, g,
-                       N, tol, list, count);
+                       N, tol, list, count, (int *)nullptr);
+}
+
+void launch_active_nodes_rolling(const GridP &g, const NState &N, double tol, int *list,
+                                 int *count, int *next_count, hipStream_t s) {
+    hipLaunchKernelGGL(k_active_nodes, dim3((unsigned)((g.nn + 255) / 256)), dim3(256), 0, s, g,
+                       N, tol, list, count, next_count);
 }
 
 }  // namespace xmpm

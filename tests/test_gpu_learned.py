@@ -121,6 +121,29 @@ def test_uncorrected_single_fine_coarse_step_is_the_plain_step(gpu):
     assert a.time()[:2] == b.time()[:2] or abs(a.time()[0] - b.time()[0]) <= 1e-15
 
 
+def test_accepted_zero_correction_of_one_fine_dt_is_the_plain_step(gpu):
+    say("[learned] zero network, coarse step of one fine dt, accepted == plain step")
+    s, pos, r, idata = small_fluid()
+    a = capi.Context(s)
+    b = capi.Context(s)
+    for c in (a, b):
+        c.upload(pos, r, idata)
+        c.init_internal_forces()
+        c.step(2)
+    net = capi.MLP(np.zeros(NP, np.float32), scales=(1.0, 1.0, 1.0))  # correction 0
+    fb = capi.step_learned(a, net, 5, 1)
+    b.step(5)
+    assert fb == 0
+    pa, ra, _, ia = a.download()
+    pb, rb, _, ib = b.download()
+    assert np.array_equal(ia, ib)
+    assert rel(pa, pb) <= 1e-13
+    assert rel(ra, rb) <= 1e-12
+    # the accepted steps' time advances (deferred into the next dt launch)
+    assert a.time()[1] == b.time()[1]
+    assert abs(a.time()[0] - b.time()[0]) <= 1e-15
+
+
 def test_rejected_correction_falls_back_to_plain_steps(gpu):
     say("[learned] guard rejects a wild network -> plain steps")
     s, pos, r, idata = small_fluid()
