@@ -54,7 +54,7 @@ struct mpm_ctx {
     // Learned-correction work buffers (allocated by its first call, nodal sized).
     int *ml_nodes = nullptr;      // active node list / selected rows
     int *ml_count = nullptr;      // [0] active count, [1] selected, [2] appended, [3] guard flag,
-                                  // [4] guard workgroups done, [8] alternate active count
+                                  // [8] alternate active count
     __bf16 *ml_x = nullptr;       // feature rows, padded to 32 rows
     float *ml_y = nullptr;        // network outputs [row][3]
     double *ml_vc = nullptr;      // coarse nodal velocity of selected rows [row][3]

@@ -94,8 +94,8 @@ int ctx_ml_buffers(mpm_ctx *c) {
     const int64_t pad = (nn + kMRows - 1) / kMRows * kMRows;
     if (c->ml_cap >= pad) return MPM_OK;
     HIPCHK(hipMalloc(&c->ml_nodes, 2 * pad * sizeof(int)));  // active list, selected rows
-    // [0] active count, [1] selected, [2] appended, [3] guard flag, [4] guard
-    // workgroups done, [8] the other step parity's active count
+    // [0] active count, [1] selected, [2] appended, [3] guard flag, [8] the
+    // other step parity's active count
     HIPCHK(hipMalloc(&c->ml_count, kMlCounts * sizeof(int)));
     HIPCHK(hipMemset(c->ml_count, 0, kMlCounts * sizeof(int)));
     HIPCHK(hipMalloc(&c->ml_x, pad * kMInPad * sizeof(__bf16)));
@@ -224,12 +224,9 @@ __global__ void k_append_rows(const int *sel, int nsel, const __bf16 *X, const d
 
 // Guard sums over the active nodes (row count on the device): [0] sum m,
 // [1..3] sum m dv, [4] sum m |v|^2, [5] sum m |v + dv|^2, [6] non-finite
-// count.  The last workgroup to finish takes the decision (see
-// mpm_step_learned: flag = 1 keeps the corrected coarse step) and clears the
-// sums and *done for the next call; both must be zero on the first call.
+// count.  The sums must be zero on entry (k_guard_decide clears them).
 __global__ void k_guard_dev(const int *nodes, const int *count, const float *Y, double dv_ref,
-                            NState N, int64_t nn, double *sums, int *done, const Scalars *sc,
-                            double gmag, double gp, double ge, int *flag) {
+                            NState N, int64_t nn, double *sums) {
     const int n = *count;
     double a[7] = {0, 0, 0, 0, 0, 0, 0};
     for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
@@ -250,7 +247,6 @@ __global__ void k_guard_dev(const int *nodes, const int *count, const float *Y, 
     }
     // wavefront sums, then one atomic per workgroup and sum
     __shared__ double red[7][4];
-    __shared__ bool last;
     const int w = threadIdx.x >> 6;
     for (int k = 0; k < 7; ++k) {
         double x = a[k];
@@ -263,16 +259,15 @@ __global__ void k_guard_dev(const int *nodes, const int *count, const float *Y, 
                          red[threadIdx.x][3];
         if (x != 0.0) atomicAdd(&sums[threadIdx.x], x);
     }
-    // This workgroup's sums are visible device-wide before its count.
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) last = atomicAdd(done, 1) == (int)gridDim.x - 1;
-    __syncthreads();
-    if (!last || threadIdx.x != 0) return;
-    __threadfence();
+}
+
+// Guard decision (see mpm_step_learned): flag = 1 keeps the corrected coarse
+// step.  The sums are read back with atomics, where they were accumulated,
+// and cleared for the next step in the same exchange.
+__global__ void k_guard_decide(double *sums, const Scalars *sc, double gmag, double gp, double ge,
+                               int *flag) {
     double gs[7];
-    for (int k = 0; k < 7; ++k) gs[k] = atomicExch(&sums[k], 0.0);  // read and clear
-    atomicExch(done, 0);
+    for (int k = 0; k < 7; ++k) gs[k] = atomicExch(&sums[k], 0.0);
     const double dt = sc->dt;
     const double M = gs[0];
     const double u = M > 0 ? sqrt(gs[4] / M) : 0.0;
@@ -628,9 +623,9 @@ int mpm_step_learned(mpm_ctx *c, mpm_mlp *m, int ncoarse, int nfine, const doubl
     const double gmag = std::sqrt(c->ph.grav[0] * c->ph.grav[0] + c->ph.grav[1] * c->ph.grav[1] +
                                   c->ph.grav[2] * c->ph.grav[2]);
     int *flag = c->ml_count + 3;
-    int *done = c->ml_count + 4;
     int *slot[2] = {c->ml_count, c->ml_count + 8};  // active counts by step parity
     hipLaunchKernelGGL(k_zero_f, dim3(1), dim3(64), 0, st, reinterpret_cast<float *>(slot[0]), 1);
+    hipLaunchKernelGGL(k_zero_d, dim3(1), dim3(64), 0, st, c->ml_guard, 8);  // once per call
     int fb = 0;
     bool pending = false;  // an accepted step's time advance is still due
     for (int it = 0; it < ncoarse; ++it) {
@@ -644,14 +639,15 @@ int mpm_step_learned(mpm_ctx *c, mpm_mlp *m, int ncoarse, int nfine, const doubl
         // are enqueued back to back; one round trip reads the decision.  A
         // rejected step leaves the particles in P untouched.  The list
         // launch clears the other parity's count for the next step, and the
-        // guard's last workgroup clears the guard sums.
+        // decision clears the guard sums.
         int *cnt = slot[it & 1];
         launch_active_nodes_rolling(c->g, c->N, c->ph.mass_tol, c->ml_nodes, cnt,
                                     slot[(it + 1) & 1], st);
         launch_mlp_features_dev(c->g, c->N, c->ml_nodes, cnt, m->sc, c->ml_x, st);
         launch_mlp_infer_dev(m->d, c->ml_x, cnt, c->ml_y, st);
         hipLaunchKernelGGL(k_guard_dev, dim3(256), dim3(256), 0, st, c->ml_nodes, cnt, c->ml_y,
-                           m->sc.dv_ref, c->N, c->g.nn, c->ml_guard, done, c->sc, gmag,
+                           m->sc.dv_ref, c->N, c->g.nn, c->ml_guard);
+        hipLaunchKernelGGL(k_guard_decide, dim3(1), dim3(1), 0, st, c->ml_guard, c->sc, gmag,
                            guards[0], guards[1], flag);
         hipLaunchKernelGGL(k_apply_dev, dim3(512), dim3(256), 0, st, c->ml_nodes, cnt, c->ml_y,
                            m->sc.dv_ref, c->N, c->g.nn, flag);
