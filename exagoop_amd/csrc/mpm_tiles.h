@@ -61,6 +61,19 @@ __device__ __forceinline__ int active_tile(const SortState &S, int b) {
     return p < na ? S.tiles[1 + p] : -1;
 }
 
+// Store of a particle field the step will not read back before the next
+// step: non-temporal, so the streaming writes do not evict the tile's staged
+// node data and tables from the caches.
+template <typename T>
+__device__ __forceinline__ void stream_store(T *p, T v) {
+    __builtin_nontemporal_store(v, p);
+}
+
+template <typename T>
+__device__ __forceinline__ T stream_load(const T *p) {
+    return __builtin_nontemporal_load(p);
+}
+
 // a[d] for a per-lane d in 0..2 (d = 3 gives a[2]).
 template <typename T>
 __device__ __forceinline__ T pick3(const T (&a)[3], int d) {
@@ -792,22 +805,22 @@ __device__ __forceinline__ void particle_tile(const GridP &g, const PhysP &ph, c
         // latency: Pin and Pout never overlap, but the compiler cannot know
         // that, so a read placed after a write would wait a memory round
         // trip of its own.
-        for (int d = 0; d < 3; ++d) v[d] = Pin.r[(MPM_R_VEL + d) * ci + j];
+        for (int d = 0; d < 3; ++d) v[d] = stream_load(&Pin.r[(MPM_R_VEL + d) * ci + j]);
         if (!VEL)
-            for (int d = 0; d < 3; ++d) vp[d] = Pin.r[(MPM_R_VELP + d) * ci + j];
-        for (int q = 0; q < 9; ++q) F[q] = Pin.r[(MPM_R_DEFGRAD + q) * ci + j];
-        int64_t id = Pin.id[j];
-        const int model = Pin.idata[MPM_I_MODEL * ci + j];
-        const double m = Pin.r[MPM_R_MASS * ci + j];
-        const double V0 = Pin.r[MPM_R_VOLINIT * ci + j];
-        const double E = Pin.r[MPM_R_E * ci + j], nu = Pin.r[MPM_R_NU * ci + j];
-        const double K = Pin.r[MPM_R_BULK * ci + j];
-        const double gam = Pin.r[MPM_R_GAMMA * ci + j], mu = Pin.r[MPM_R_VISC * ci + j];
+            for (int d = 0; d < 3; ++d) vp[d] = stream_load(&Pin.r[(MPM_R_VELP + d) * ci + j]);
+        for (int q = 0; q < 9; ++q) F[q] = stream_load(&Pin.r[(MPM_R_DEFGRAD + q) * ci + j]);
+        int64_t id = stream_load(&Pin.id[j]);
+        const int model = stream_load(&Pin.idata[MPM_I_MODEL * ci + j]);
+        const double m = stream_load(&Pin.r[MPM_R_MASS * ci + j]);
+        const double V0 = stream_load(&Pin.r[MPM_R_VOLINIT * ci + j]);
+        const double E = Pin.r[MPM_R_E * ci + j], nu = stream_load(&Pin.r[MPM_R_NU * ci + j]);
+        const double K = stream_load(&Pin.r[MPM_R_BULK * ci + j]);
+        const double gam = Pin.r[MPM_R_GAMMA * ci + j], mu = stream_load(&Pin.r[MPM_R_VISC * ci + j]);
         double eps[6], sig[6];
-        for (int q = 0; q < 6; ++q) eps[q] = Pin.r[(MPM_R_STRAIN + q) * ci + j];
+        for (int q = 0; q < 6; ++q) eps[q] = stream_load(&Pin.r[(MPM_R_STRAIN + q) * ci + j]);
         double J = 0.0, V = 0.0, rho = 0.0, p = 0.0;
         if (DELTA)
-            for (int q = 0; q < 6; ++q) sig[q] = Pin.r[(MPM_R_STRESS + q) * ci + j];
+            for (int q = 0; q < 6; ++q) sig[q] = stream_load(&Pin.r[(MPM_R_STRESS + q) * ci + j]);
         // The gather runs for every particle (rigid ones are few and drop
         // the result), so no branch on the phase waits for the phase's read
         // ahead of the rest of the batch.
@@ -895,41 +908,41 @@ __device__ __forceinline__ void particle_tile(const GridP &g, const PhysP &ph, c
             if (redist) stage_redistribute(g, x, id);
         }
         for (int d = 0; d < 3; ++d) {
-            Pout.pos[d * co + i] = x[d];
-            R[(MPM_R_VEL + d) * co + i] = v[d];
-            R[(MPM_R_VELP + d) * co + i] = vp[d];
+            stream_store(&Pout.pos[d * co + i], x[d]);
+            stream_store(&R[(MPM_R_VEL + d) * co + i], v[d]);
+            stream_store(&R[(MPM_R_VELP + d) * co + i], vp[d]);
         }
-        Pout.id[i] = id;
-        Pout.idata[MPM_I_PHASE * co + i] = phase;
+        stream_store(&Pout.id[i], id);
+        stream_store(&Pout.idata[MPM_I_PHASE * co + i], phase);
         const double vmag2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
-        for (int q = 0; q < 6; ++q) R[(MPM_R_STRAINRATE + q) * co + i] = D[q];
+        for (int q = 0; q < 6; ++q) stream_store(&R[(MPM_R_STRAINRATE + q) * co + i], D[q]);
         // updateVolume and the constitutive update
-        R[MPM_R_MASS * co + i] = m;
-        R[MPM_R_VOLINIT * co + i] = V0;
-        Pout.idata[MPM_I_MODEL * co + i] = model;
+        stream_store(&R[MPM_R_MASS * co + i], m);
+        stream_store(&R[MPM_R_VOLINIT * co + i], V0);
+        stream_store(&Pout.idata[MPM_I_MODEL * co + i], model);
         if (!rigid) {
             J = det3(F);
             V = V0 * J;
             rho = m / V;
         }
-        R[MPM_R_VOLUME * co + i] = V;
-        R[MPM_R_DENSITY * co + i] = rho;
-        R[MPM_R_JACOBIAN * co + i] = J;
-        for (int q = 0; q < 9; ++q) R[(MPM_R_DEFGRAD + q) * co + i] = F[q];
-        R[MPM_R_E * co + i] = E;
-        R[MPM_R_NU * co + i] = nu;
-        R[MPM_R_BULK * co + i] = K;
+        stream_store(&R[MPM_R_VOLUME * co + i], V);
+        stream_store(&R[MPM_R_DENSITY * co + i], rho);
+        stream_store(&R[MPM_R_JACOBIAN * co + i], J);
+        for (int q = 0; q < 9; ++q) stream_store(&R[(MPM_R_DEFGRAD + q) * co + i], F[q]);
+        stream_store(&R[MPM_R_E * co + i], E);
+        stream_store(&R[MPM_R_NU * co + i], nu);
+        stream_store(&R[MPM_R_BULK * co + i], K);
         // The dt candidate depends only on state final at this point.
         if (id >= 0 && !rigid) cmin = fmin(cmin, dt_candidate(g, model, K, E, nu, rho, vmag2));
-        R[MPM_R_GAMMA * co + i] = gam;
-        R[MPM_R_VISC * co + i] = mu;
+        stream_store(&R[MPM_R_GAMMA * co + i], gam);
+        stream_store(&R[MPM_R_VISC * co + i], mu);
         if (!rigid) {
             const double rate = (time < ph.applied_time) ? ph.applied_rate : 0.0;
             for (int q = 0; q < 6; ++q) eps[q] = eps[q] + dt * D[q];
             eps[XX] += dt * rate;
             eps[YY] += dt * rate;
             eps[ZZ] += dt * rate;
-            for (int q = 0; q < 6; ++q) R[(MPM_R_STRAIN + q) * co + i] = eps[q];
+            for (int q = 0; q < 6; ++q) stream_store(&R[(MPM_R_STRAIN + q) * co + i], eps[q]);
             if (DELTA) {
                 double de[6], ds[6];
                 for (int q = 0; q < 6; ++q) de[q] = dt * D[q];
@@ -949,10 +962,10 @@ __device__ __forceinline__ void particle_tile(const GridP &g, const PhysP &ph, c
                 newtonian(D, sig, mu, p);
             }
         } else {
-            for (int q = 0; q < 6; ++q) R[(MPM_R_STRAIN + q) * co + i] = eps[q];
+            for (int q = 0; q < 6; ++q) stream_store(&R[(MPM_R_STRAIN + q) * co + i], eps[q]);
         }
-        for (int q = 0; q < 6; ++q) R[(MPM_R_STRESS + q) * co + i] = sig[q];
-        R[MPM_R_PRESSURE * co + i] = p;
+        for (int q = 0; q < 6; ++q) stream_store(&R[(MPM_R_STRESS + q) * co + i], sig[q]);
+        stream_store(&R[MPM_R_PRESSURE * co + i], p);
     }
 }
 
