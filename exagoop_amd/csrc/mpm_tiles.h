@@ -61,9 +61,10 @@ __device__ __forceinline__ int active_tile(const SortState &S, int b) {
     return p < na ? S.tiles[1 + p] : -1;
 }
 
-// Store of a particle field the step will not read back before the next
-// step: non-temporal, so the streaming writes do not evict the tile's staged
-// node data and tables from the caches.
+// Accesses to the particle streams of the fused particle pass, which touches
+// each field once: non-temporal, so the streams do not evict the nodal data
+// the following grid update and sort reuse from the caches.  (The sort's
+// packet stream is read back by P2G within the step and stays cached.)
 template <typename T>
 __device__ __forceinline__ void stream_store(T *p, T v) {
     __builtin_nontemporal_store(v, p);
