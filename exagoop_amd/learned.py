@@ -177,7 +177,11 @@ def train(ds, epochs=24, batch=4096, lr=2e-3, seed=0, log=print, prob=None):
         o += n + fan_out
     net = capi.MLP(p, scales=(prob.m_ref, prob.v_ref, rms))
     losses = []
-    for phase_lr, phase_ep in ((lr, epochs * 2 // 3), (lr * 0.2, epochs - epochs * 2 // 3)):
+    # step schedule: half the epochs at lr, 3/10 at lr / 5, the rest at lr / 25
+    e1, e2 = epochs // 2, (epochs * 3) // 10
+    for phase_lr, phase_ep in ((lr, e1), (lr * 0.2, e2), (lr * 0.04, epochs - e1 - e2)):
+        if phase_ep <= 0:
+            continue
         l = net.train(ds, phase_ep, batch, phase_lr, seed=seed + len(losses))
         losses += [float(x) for x in l]
         if log:
@@ -286,11 +290,13 @@ def main(argv=None):
     ap.add_argument("--res", type=float, default=1.0, help="column resolution (1 = full size)")
     ap.add_argument("--train-seeds", type=int, default=None)
     ap.add_argument("--eval-seeds", type=int, default=None)
-    ap.add_argument("--epochs", type=int, default=24)
+    ap.add_argument("--epochs", type=int, default=400)
     ap.add_argument("--eval-steps", type=int, default=400)
     ap.add_argument("--samples", type=int, default=10)
     ap.add_argument("--spacing", type=int, default=40)
-    ap.add_argument("--rows", type=int, default=8000)
+    ap.add_argument("--rows", type=int, default=16000)
+    ap.add_argument("--capacity", type=int, default=12_000_000,
+                    help="training rows held on the device (rows beyond it are dropped)")
     a = ap.parse_args(argv)
     if a.demo:
         demo()
@@ -302,7 +308,8 @@ def main(argv=None):
     train_seeds = prob.train_seeds[:a.train_seeds] if a.train_seeds else prob.train_seeds
     eval_seeds = prob.eval_seeds[:a.eval_seeds] if a.eval_seeds else prob.eval_seeds
     t0 = time.perf_counter()
-    ds = collect(train_seeds, samples=a.samples, spacing=a.spacing, rows=a.rows, prob=prob)
+    ds = collect(train_seeds, samples=a.samples, spacing=a.spacing, rows=a.rows, prob=prob,
+                 capacity=a.capacity)
     t1 = time.perf_counter()
     net, losses = train(ds, epochs=a.epochs, prob=prob)
     t2 = time.perf_counter()
