@@ -182,10 +182,19 @@ def train(ds, epochs=24, batch=4096, lr=2e-3, seed=0, log=print, prob=None):
     for phase_lr, phase_ep in ((lr, e1), (lr * 0.2, e2), (lr * 0.04, epochs - e1 - e2)):
         if phase_ep <= 0:
             continue
-        l = net.train(ds, phase_ep, batch, phase_lr, seed=seed + len(losses))
-        losses += [float(x) for x in l]
+        # in chunks of at most 20 epochs, each reported (the Adam step count
+        # carries over between calls; each chunk draws its own shuffles)
+        first = len(losses)
+        done = 0
+        while done < phase_ep:
+            n = min(20, phase_ep - done)
+            l = net.train(ds, n, batch, phase_lr, seed=seed + len(losses))
+            losses += [float(x) for x in l]
+            done += n
+            if log:
+                log(f"[learned] lr {phase_lr:g}: {done}/{phase_ep} epochs, loss {l[-1]:.4f}")
         if log:
-            log(f"[learned] lr {phase_lr:g}: loss {l[0]:.4f} -> {l[-1]:.4f}")
+            log(f"[learned] lr {phase_lr:g}: loss {losses[first]:.4f} -> {losses[-1]:.4f}")
     return net, losses
 
 
