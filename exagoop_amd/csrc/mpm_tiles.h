@@ -785,7 +785,7 @@ __device__ __forceinline__ void particle_tile(const GridP &g, const PhysP &ph, c
     double xn[3] = {0.0, 0.0, 0.0};
     if (beg + tid < end) {
         jn = S.perm[beg + tid];
-        for (int d = 0; d < 3; ++d) xn[d] = Pin.pos[d * ci + jn];
+        for (int d = 0; d < 3; ++d) xn[d] = stream_load(&Pin.pos[d * ci + jn]);
     }
     for (int it = 0; it < niter; ++it) {
         const int64_t i = beg + (int64_t)it * kBlock + tid;
@@ -799,7 +799,7 @@ __device__ __forceinline__ void particle_tile(const GridP &g, const PhysP &ph, c
         // The gather waits for x only if x is read first: reads complete in
         // issue order.
         asm volatile("" ::: "memory");
-        const int phase = Pin.idata[MPM_I_PHASE * ci + j];
+        const int phase = stream_load(&Pin.idata[MPM_I_PHASE * ci + j]);
         const bool rigid = phase != 0;
         // Every remaining read of the particle, in one batch ahead of its
         // first write and of the G2P gather, which hides the batch's
@@ -814,9 +814,11 @@ __device__ __forceinline__ void particle_tile(const GridP &g, const PhysP &ph, c
         const int model = stream_load(&Pin.idata[MPM_I_MODEL * ci + j]);
         const double m = stream_load(&Pin.r[MPM_R_MASS * ci + j]);
         const double V0 = stream_load(&Pin.r[MPM_R_VOLINIT * ci + j]);
-        const double E = Pin.r[MPM_R_E * ci + j], nu = stream_load(&Pin.r[MPM_R_NU * ci + j]);
+        const double E = stream_load(&Pin.r[MPM_R_E * ci + j]);
+        const double nu = stream_load(&Pin.r[MPM_R_NU * ci + j]);
         const double K = stream_load(&Pin.r[MPM_R_BULK * ci + j]);
-        const double gam = Pin.r[MPM_R_GAMMA * ci + j], mu = stream_load(&Pin.r[MPM_R_VISC * ci + j]);
+        const double gam = stream_load(&Pin.r[MPM_R_GAMMA * ci + j]);
+        const double mu = stream_load(&Pin.r[MPM_R_VISC * ci + j]);
         double eps[6], sig[6];
         for (int q = 0; q < 6; ++q) eps[q] = stream_load(&Pin.r[(MPM_R_STRAIN + q) * ci + j]);
         double J = 0.0, V = 0.0, rho = 0.0, p = 0.0;
@@ -868,18 +870,18 @@ __device__ __forceinline__ void particle_tile(const GridP &g, const PhysP &ph, c
         // for most of the batch.
         if (rigid) {
             if (VEL)
-                for (int d = 0; d < 3; ++d) vp[d] = Pin.r[(MPM_R_VELP + d) * ci + j];
-            for (int q = 0; q < 6; ++q) D[q] = Pin.r[(MPM_R_STRAINRATE + q) * ci + j];
-            J = Pin.r[MPM_R_JACOBIAN * ci + j];
-            V = Pin.r[MPM_R_VOLUME * ci + j];
-            rho = Pin.r[MPM_R_DENSITY * ci + j];
+                for (int d = 0; d < 3; ++d) vp[d] = stream_load(&Pin.r[(MPM_R_VELP + d) * ci + j]);
+            for (int q = 0; q < 6; ++q) D[q] = stream_load(&Pin.r[(MPM_R_STRAINRATE + q) * ci + j]);
+            J = stream_load(&Pin.r[MPM_R_JACOBIAN * ci + j]);
+            V = stream_load(&Pin.r[MPM_R_VOLUME * ci + j]);
+            rho = stream_load(&Pin.r[MPM_R_DENSITY * ci + j]);
             if (!DELTA)
-                for (int q = 0; q < 6; ++q) sig[q] = Pin.r[(MPM_R_STRESS + q) * ci + j];
+                for (int q = 0; q < 6; ++q) sig[q] = stream_load(&Pin.r[(MPM_R_STRESS + q) * ci + j]);
         }
-        if (rigid || model != 1) p = Pin.r[MPM_R_PRESSURE * ci + j];
+        if (rigid || model != 1) p = stream_load(&Pin.r[MPM_R_PRESSURE * ci + j]);
         if (has_next) {
             jn = jnext;
-            for (int d = 0; d < 3; ++d) xn[d] = Pin.pos[d * ci + jn];
+            for (int d = 0; d < 3; ++d) xn[d] = stream_load(&Pin.pos[d * ci + jn]);
         }
         double *R = Pout.r;
         // moveParticles without level sets or UDF tables
