@@ -12,25 +12,18 @@ namespace {
 
 __global__ __launch_bounds__(kBlock) void k_keys(GridP g, PState P, SortState S) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) S.tiles[0] = 0;  // non-empty tile count, filled by k_tile_list
+    if (i == 0) S.tiles[0] = 0;  // non-empty tile count, filled by the scatter launch
     keys_wave(g, P, S, i, P.np);
 }
 
-// perm[start[key] + rank] = i; re-zero the counts; mark output slots past the
-// live count as free (removed particles are dropped by the reorder).  With
-// packets, also write particle i's P2G packet at its sorted position.
-__global__ __launch_bounds__(kBlock) void k_scatter(PState P, SortState S, int64_t *out_id,
-                                                    PhysP ph, int packets) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= P.np) return;
-    scatter_one(P, S, out_id, ph, packets, i);
-}
-
 // Non-empty tiles into S.tiles[1..], their number into S.tiles[0] (zeroed
-// by k_keys): each workgroup compacts 256 consecutive tiles in order and
-// reserves its range with one atomic, so runs of neighbouring tiles stay
-// together while the order of the runs depends on scheduling.
-__global__ __launch_bounds__(kBlock) void k_tile_list(SortState S) {
+// by k_keys), run by the scatter launch's first workgroups (the scan is
+// complete there): each workgroup compacts 256 consecutive tiles in order
+// and reserves its range with one atomic, so runs of neighbouring tiles stay
+// together while the order of the runs depends on scheduling.  Every thread
+// of the workgroup must call it (barriers).
+__device__ __forceinline__ void tile_list_block(const SortState &S) {
+    if ((int64_t)blockIdx.x * kBlock >= S.ntiles) return;  // uniform over the block
     __shared__ int wsum[kBlock / 64];
     __shared__ int base;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -49,6 +42,17 @@ __global__ __launch_bounds__(kBlock) void k_tile_list(SortState S) {
     if (tid == 0) base = tot ? atomicAdd(&S.tiles[0], tot) : 0;
     __syncthreads();
     if (f) S.tiles[1 + base + off + inwave] = t;
+}
+
+// perm[start[key] + rank] = i; re-zero the counts; mark output slots past the
+// live count as free (removed particles are dropped by the reorder).  With
+// packets, also write particle i's P2G packet at its sorted position.
+__global__ __launch_bounds__(kBlock) void k_scatter(PState P, SortState S, int64_t *out_id,
+                                                    PhysP ph, int packets) {
+    tile_list_block(S);  // the first ceil(ntiles / 256) workgroups; every thread
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.np) return;
+    scatter_one(P, S, out_id, ph, packets, i);
 }
 
 // Fused per-particle pipeline of the sorted step (generic: level sets,
@@ -122,9 +126,9 @@ void launch_sort(const GridP &g, const PhysP &ph, const PState &P, SortState &S,
     size_t bytes = S.scan_bytes;
     (void)hipcub::DeviceScan::ExclusiveSum(S.scan_tmp, bytes, S.count, S.cell_start, S.nkeys + 1,
                                            s);
-    hipLaunchKernelGGL(k_scatter, dim3(blocks_for(P.np)), dim3(kBlock), 0, s, P, S, out_id, ph,
-                       packets);
-    hipLaunchKernelGGL(k_tile_list, dim3((S.ntiles + kBlock - 1) / kBlock), dim3(kBlock), 0, s, S);
+    // the scatter launch also builds the non-empty tile list
+    const unsigned nb = std::max(blocks_for(P.np), (unsigned)((S.ntiles + kBlock - 1) / kBlock));
+    hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(kBlock), 0, s, P, S, out_id, ph, packets);
 }
 
 void launch_p2g_tiles(const GridP &g, const PhysP &ph, const PState &P, const SortState &S,
