@@ -840,9 +840,21 @@ int mpm_step(mpm_ctx *c, int nsteps) {
         if (musl && c->P.vsnap) launch_snapshot_vel(c->P, st);
         // The previous step of this call ends here: its dt advances the time
         // inside the dt launch; the last step's advance follows the loop.
-        launch_dt(c->g, c->ph, c->P, c->sc, c->s.fixed_timestep == 1, c->s.timestep, st,
-                  fused_dt ? 1 : 0, it > 0 ? 1 : 0);
+        // Without a reduction over the particles, and with no host read of
+        // the time before the sort, the sort's first launch computes the dt.
         const bool temp_udf = c->thermal && c->any_tudf;
+        const bool fixed_dt = c->s.fixed_timestep == 1;
+        DtFold fold;
+        if (c->sorted && c->P.np > 0 && !(c->any_udf || temp_udf) && (fixed_dt || fused_dt)) {
+            fold.on = 1;
+            fold.fixed = fixed_dt ? 1 : 0;
+            fold.fixed_dt = c->s.timestep;
+            fold.advance = it > 0 ? 1 : 0;
+            fold.sc = c->sc;
+        } else {
+            launch_dt(c->g, c->ph, c->P, c->sc, fixed_dt, c->s.timestep, st, fused_dt ? 1 : 0,
+                      it > 0 ? 1 : 0);
+        }
         if (c->any_udf || temp_udf) {
             Scalars h;
             HIPCHK(hipMemcpyAsync(&h, c->sc, sizeof(Scalars), hipMemcpyDeviceToHost, st));
@@ -853,7 +865,7 @@ int mpm_step(mpm_ctx *c, int nsteps) {
         }
         const bool sorted = c->sorted != 0;
         if (prof) HIPCHK(hipEventRecord(c->ev[1], st));
-        if (sorted) launch_sort(c->g, c->ph, c->P, c->S, c->Q.id, 1, st);
+        if (sorted) launch_sort(c->g, c->ph, c->P, c->S, c->Q.id, 1, st, fold);
         if (prof) HIPCHK(hipEventRecord(c->ev[2], st));
         if (sorted) launch_p2g_tiles(c->g, c->ph, c->P, c->S, c->N, step, 0, nr > 1, c->sc, st);
         else launch_p2g(c->g, c->ph, c->P, c->N, 1, 1, 1, nullptr, st);

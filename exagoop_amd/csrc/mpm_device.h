@@ -10,6 +10,38 @@
 
 namespace xmpm {
 
+// The step's dt (Calculate_time_step, mpm_particle_timestep.cpp:38-111) from
+// the minimum candidate: fused takes the minimum the previous step's particle
+// kernel accumulated, else the one k_dt reduced; fixed takes fixed_dt; the
+// result times scale (a coarse step of several fine steps, 1 otherwise).
+// advance: the previous dt first advances the time (that step's end).  One
+// thread runs it.
+__device__ __forceinline__ void dt_final_one(Scalars *sc, const PhysP &ph, int fixed,
+                                             double fixed_dt, int fused, int advance,
+                                             double scale) {
+    // The step about to run accumulates the following step's minimum, so the
+    // running value is reset here.  Read and reset are one atomic exchange: a
+    // plain load of a uniform address becomes a scalar-cache load, which the
+    // hardware does not order against the vector store of the reset.
+    const unsigned long long nxt = atomicExch(
+        &sc->dtmin_next_bits, (unsigned long long)__double_as_longlong(__builtin_huge_val()));
+    const unsigned long long bits = fused ? nxt : sc->dtmin_bits;
+    double dt;
+    if (fixed) {
+        dt = fixed_dt;
+    } else {
+        dt = __longlong_as_double((long long)bits);
+        if (dt == __builtin_huge_val()) dt = 1.79769313486231570815e+308;  // numeric_limits max
+        dt = ph.cfl * dt;
+        dt = fmax(fmin(dt, ph.dt_max), ph.dt_min);
+    }
+    dt = dt * scale;
+    // Same exchange for dt: the previous step's dt advances the time.
+    const unsigned long long old = atomicExch(reinterpret_cast<unsigned long long *>(&sc->dt),
+                                              (unsigned long long)__double_as_longlong(dt));
+    if (advance) sc->time = sc->time + __longlong_as_double((long long)old);
+}
+
 __device__ __forceinline__ int64_t nidx(const GridP &g, int I, int J, int K) {
     return ((int64_t)K * (g.n[1] + 1) + J) * (g.n[0] + 1) + I;
 }

@@ -158,8 +158,19 @@ size_t sort_scan_bytes(int nkeys);
 // out_id: id array of the buffer the step writes into (slots past the live
 // count are marked free), or null.
 // packets: also build the P2G packets of the sorted particles.
+// The step's dt, computed by the sort's first launch when it needs no
+// reduction over the particles (fixed dt, or the minimum the previous
+// particle kernel accumulated): the arguments of launch_dt.
+struct DtFold {
+    int on = 0;
+    int fixed = 0;
+    double fixed_dt = 0.0;
+    int advance = 0;
+    double scale = 1.0;
+    Scalars *sc = nullptr;
+};
 void launch_sort(const GridP &g, const PhysP &ph, const PState &P, SortState &S,
-                 int64_t *out_id, int packets, hipStream_t s);
+                 int64_t *out_id, int packets, hipStream_t s, const DtFold &dt = DtFold());
 // fallback: particles may lie outside their tile (not redistributed every
 // step); they deposit straight to global memory.  Without it such a particle
 // sets Scalars::err.

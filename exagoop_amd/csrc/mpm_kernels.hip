@@ -99,28 +99,7 @@ __global__ void k_dt_init(Scalars *sc) {
 
 __global__ void k_dt_final(Scalars *sc, PhysP ph, int fixed, double fixed_dt, int fused,
                            int advance, double scale) {
-    // The step about to run accumulates the following step's minimum, so the
-    // running value is reset here.  Read and reset are one atomic exchange: a
-    // plain load of a uniform address becomes a scalar-cache load, which the
-    // hardware does not order against the vector store of the reset.
-    const unsigned long long nxt = atomicExch(
-        &sc->dtmin_next_bits, (unsigned long long)__double_as_longlong(__builtin_huge_val()));
-    const unsigned long long bits = fused ? nxt : sc->dtmin_bits;
-    double dt;
-    if (fixed) {
-        dt = fixed_dt;
-    } else {
-        dt = __longlong_as_double((long long)bits);
-        if (dt == __builtin_huge_val()) dt = 1.79769313486231570815e+308;  // numeric_limits max
-        dt = ph.cfl * dt;
-        dt = fmax(fmin(dt, ph.dt_max), ph.dt_min);
-    }
-    dt = dt * scale;  // a coarse step of several fine steps (1 otherwise)
-    // Same exchange for dt: the previous step's dt advances the time
-    // (the end of that step, folded into this launch).
-    const unsigned long long old = atomicExch(reinterpret_cast<unsigned long long *>(&sc->dt),
-                                              (unsigned long long)__double_as_longlong(dt));
-    if (advance) sc->time = sc->time + __longlong_as_double((long long)old);
+    dt_final_one(sc, ph, fixed, fixed_dt, fused, advance, scale);
 }
 
 __global__ void k_advance_time(Scalars *sc) { sc->time = sc->time + sc->dt; }

@@ -134,9 +134,20 @@ int coarse_begin(mpm_ctx *c, int nfine, int advance = 0) {
     hipStream_t st = c->stream;
     c->steps++;
     const int step = (int)c->steps;
-    launch_dt(c->g, c->ph, c->P, c->sc, c->s.fixed_timestep == 1, c->s.timestep, st,
-              c->dt_next_valid ? 1 : 0, advance, (double)nfine);
-    launch_sort(c->g, c->ph, c->P, c->S, c->Q.id, 1, st);
+    const bool fixed_dt = c->s.fixed_timestep == 1;
+    DtFold fold;  // the dt in the sort's first launch when it needs no reduction
+    if (c->P.np > 0 && (fixed_dt || c->dt_next_valid)) {
+        fold.on = 1;
+        fold.fixed = fixed_dt ? 1 : 0;
+        fold.fixed_dt = c->s.timestep;
+        fold.advance = advance;
+        fold.scale = (double)nfine;
+        fold.sc = c->sc;
+    } else {
+        launch_dt(c->g, c->ph, c->P, c->sc, fixed_dt, c->s.timestep, st,
+                  c->dt_next_valid ? 1 : 0, advance, (double)nfine);
+    }
+    launch_sort(c->g, c->ph, c->P, c->S, c->Q.id, 1, st, fold);
     launch_p2g_tiles(c->g, c->ph, c->P, c->S, c->N, step, 0, 0, c->sc, st);
     const int *te = (c->skip_ok && !c->all_dirty) ? c->S.tile_epoch : nullptr;
     launch_grid(c->g, c->ph, c->ls, c->udf, c->N, grid_flags_full(c), 0.0, c->sc, st, te, step);

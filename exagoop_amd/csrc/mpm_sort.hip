@@ -10,9 +10,13 @@ namespace xmpm {
 
 namespace {
 
-__global__ __launch_bounds__(kBlock) void k_keys(GridP g, PState P, SortState S) {
+__global__ __launch_bounds__(kBlock) void k_keys(GridP g, PState P, SortState S, PhysP ph,
+                                                 DtFold dt) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) S.tiles[0] = 0;  // non-empty tile count, filled by the scatter launch
+    if (i == 0) {
+        S.tiles[0] = 0;  // non-empty tile count, filled by the scatter launch
+        if (dt.on) dt_final_one(dt.sc, ph, dt.fixed, dt.fixed_dt, 1, dt.advance, dt.scale);
+    }
     keys_wave(g, P, S, i, P.np);
 }
 
@@ -120,9 +124,9 @@ size_t sort_scan_bytes(int nkeys) {
 }
 
 void launch_sort(const GridP &g, const PhysP &ph, const PState &P, SortState &S,
-                 int64_t *out_id, int packets, hipStream_t s) {
-    if (P.np <= 0) return;
-    hipLaunchKernelGGL(k_keys, dim3(blocks_for(P.np)), dim3(kBlock), 0, s, g, P, S);
+                 int64_t *out_id, int packets, hipStream_t s, const DtFold &dt) {
+    if (P.np <= 0) return;  // (callers fold the dt only with particles)
+    hipLaunchKernelGGL(k_keys, dim3(blocks_for(P.np)), dim3(kBlock), 0, s, g, P, S, ph, dt);
     size_t bytes = S.scan_bytes;
     (void)hipcub::DeviceScan::ExclusiveSum(S.scan_tmp, bytes, S.count, S.cell_start, S.nkeys + 1,
                                            s);
