@@ -159,14 +159,16 @@ __device__ __forceinline__ void grid_at(const GridP &g, const PhysP &ph, const L
         }
     }
     double m = N.mass[node];
-    double v[3], dv[3];
+    double v[3] = {0.0, 0.0, 0.0}, dv[3] = {0.0, 0.0, 0.0};
+    const bool full = (flags & G_FINISH_MASS) && (flags & G_BACKUP);
+    // v is rebuilt from the momentum sums under G_FINISH_VEL and dv is reset
+    // in a full pass: only the other cases read them.
     for (int d = 0; d < 3; ++d) {
-        v[d] = N.vel[d * nn + node];
-        dv[d] = N.dvel[d * nn + node];
+        if (!(flags & G_FINISH_VEL)) v[d] = N.vel[d * nn + node];
+        if (!full) dv[d] = N.dvel[d * nn + node];
     }
     double f[3] = {0, 0, 0};
     double mold = 0.0;
-    const bool full = (flags & G_FINISH_MASS) && (flags & G_BACKUP);
     if (flags & G_FINISH_MASS) {
         m = N.acc[node];
         N.acc[node] = 0.0;
