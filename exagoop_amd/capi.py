@@ -165,6 +165,8 @@ SIGNATURES = {
     "mpm_add_levelset_body": (ctypes.c_int, [_P, _DP, ctypes.c_int, ctypes.c_double, _DP]),
     "mpm_set_wall_udf": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, UDF_FN]),
     "mpm_energies": (ctypes.c_int, [_P, _DP]),
+    "mpm_energies_noalloc": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p,
+                                            ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "mpm_set_option": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int]),
     "mpm_last_step_timing": (ctypes.c_int, [_P, _DP, _DP]),
     "mpm_last_error": (ctypes.c_char_p, []),

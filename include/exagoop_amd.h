@@ -330,6 +330,23 @@ int mpm_set_wall_udf(mpm_ctx *ctx, int dir, int hi, mpm_wall_udf_fn fn);
 /* out[0]=TKE, out[1]=TSE, out[2..4]=sum m*v, out[5]=sum m (all particles). */
 int mpm_energies(mpm_ctx *ctx, double out[6]);
 
+/* The sums of mpm_energies over particles the caller holds in device memory,
+ * allocating nothing.  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc).  rdata: device array of the
+ * MPM_NREAL real fields, component-major with stride `stride` >= np (field f
+ * of particle i at rdata[f * stride + i]); id: device array of np ids, a
+ * particle with id < 0 is skipped, or NULL (all counted); out: 6 doubles of
+ * device memory in the order of mpm_energies; stream: the hipStream_t to
+ * enqueue on (NULL = the null stream); the call does not synchronise.
+ * Partial sums go to one library-owned device array per GPU, sized for
+ * np <= MPM_ENERGIES_NOALLOC_MAX_PARTICLES (larger np: MPM_E_ARG); calls that
+ * overlap on one GPU race on it.  The sums run in a fixed order: equal inputs
+ * give equal bits. */
+#define MPM_ENERGIES_NOALLOC_MAX_PARTICLES (1 << 24)
+int mpm_energies_noalloc(int64_t np, int64_t stride, const double *rdata, const int64_t *id,
+                         double *out, void *stream);
+
 /* Context options without a reference counterpart (tooling for callers that
  * measure or inspect the step): MPM_OPT_PROFILE records HIP events around
  * each kernel of mpm_step (read back with mpm_last_step_timing);
