@@ -33,6 +33,23 @@ def hipcc():
     return "hipcc"
 
 
+RUN = os.path.join(HERE, "mpm_run")
+
+
+def build_driver():
+    """The command-line time-loop driver (csrc/mpm_run.cpp) over the C++
+    facade, linked against the in-tree library it sits beside."""
+    tmp = RUN + ".tmp"
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-I" + os.path.join(HERE, "..", "include"),
+           "-I" + CSRC, os.path.join(CSRC, "mpm_run.cpp"), "-o", tmp, "-L" + HERE,
+           "-lexagoop_amd", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("g++ failed on mpm_run.cpp:\n" + r.stdout + r.stderr)
+    os.replace(tmp, RUN)
+    return RUN
+
+
 def build(verbose=False, jobs=None):
     """Compile libexagoop_amd.so in place; raises on failure.  Each source is
     compiled to an object in parallel (the kernels of one source call no
@@ -68,6 +85,7 @@ def build(verbose=False, jobs=None):
         os.replace(tmp, LIB)
     finally:
         shutil.rmtree(tmpdir, ignore_errors=True)
+    build_driver()
     return LIB
 
 

@@ -156,6 +156,7 @@ SIGNATURES = {
     "mpm_apply_nodal_bcs": (ctypes.c_int, [_P, ctypes.c_double, ctypes.c_double]),
     "mpm_g2p": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_double]),
     "mpm_move": (ctypes.c_int, [_P, ctypes.c_double, ctypes.c_int]),
+    "mpm_redistribute": (ctypes.c_int, [_P, ctypes.c_int]),
     "mpm_update_volume": (ctypes.c_int, [_P]),
     "mpm_stress": (ctypes.c_int, [_P, ctypes.c_double, ctypes.c_double]),
     "mpm_calculate_dt": (ctypes.c_int, [_P, _DP]),
@@ -337,6 +338,9 @@ class _Base:
 
     def move(self, dt, step):
         self._call("move", self._ctx, dt, step)
+
+    def redistribute(self, step):
+        self._call("redistribute", self._ctx, step)
 
     def update_volume(self):
         self._call("update_volume", self._ctx)

@@ -61,6 +61,10 @@ class MPMParticleContainer {
     void updateVolume() { check(mpm_update_volume(ctx_), "mpm_update_volume"); }
     // moveParticles + the following Redistribute_Fill_Update.
     void moveParticles(double dt, int step) { check(mpm_move(ctx_, dt, step), "mpm_move"); }
+    // Redistribute_Fill_Update at the top of a step (utilities.cpp:420-434).
+    void Redistribute_Fill_Update(int step) {
+        check(mpm_redistribute(ctx_, step), "mpm_redistribute");
+    }
     // Calculate_time_step (mpm_particle_timestep.cpp:38-111).
     double Calculate_time_step() {
         double dt = 0.0;

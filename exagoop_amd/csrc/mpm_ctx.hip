@@ -710,8 +710,13 @@ int mpm_reset_nodal(mpm_ctx *c) {
 
 int mpm_p2g(mpm_ctx *c, int um, int uv, int uf) {
     if (!c) return set_err(MPM_E_ARG, "null context");
-    if (uf < 0 || uf > 1) return set_err(MPM_E_ARG, "update_forces must be 0 or 1");
+    if (uf < 0 || uf > 2) return set_err(MPM_E_ARG, "update_forces must be 0, 1 or 2");
     c->all_dirty = true;
+    // update_forces == 2 also zeroes nodal component 15 and divides it by the
+    // mass (:398-400, 617-625), but no deposit writes that component, so it
+    // stays zero: the step is that of update_forces == 1 (component 15 is not
+    // stored and reads as zero).
+    if (uf == 2) uf = 1;
     // The reference zeroes the selected components first (:368-403); the
     // accumulators are already zero.
     launch_p2g(c->g, c->ph, c->P, c->N, um, uv, uf, nullptr, c->stream);
@@ -764,6 +769,16 @@ int mpm_move(mpm_ctx *c, double dt, int step) {
     launch_particle(c->g, c->ph, c->ls, c->udf, c->P, c->N, P_MOVE | (redist ? P_REDIST : 0), dt,
                     0.0, nullptr, c->stream);
     c->dirty = !redist;
+    return check_launch();
+}
+
+int mpm_redistribute(mpm_ctx *c, int step) {
+    if (!c) return set_err(MPM_E_ARG, "null context");
+    const int nr = c->s.num_redist > 0 ? c->s.num_redist : 1;
+    if (step % nr != 0) return MPM_OK;
+    launch_redistribute(c->g, c->P, c->stream);
+    c->dirty = false;
+    c->dt_next_valid = false;  // removals change the dt reduction's set
     return check_launch();
 }
 

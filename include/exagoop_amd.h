@@ -240,7 +240,8 @@ int mpm_get_nodal(mpm_ctx *ctx, int comp, double *out);
 int mpm_reset_nodal(mpm_ctx *ctx);
 
 /* deposit_onto_grid_momentum (mpm_particle_grid_ops.cpp:333-628), including
- * the v = p/m normalisation.  Flags as in the reference. */
+ * the v = p/m normalisation.  Flags as in the reference; update_forces 2
+ * deposits as 1 and leaves nodal component 15 zero (:398-400, 617-625). */
 int mpm_p2g(mpm_ctx *ctx, int update_mass, int update_vel, int update_forces);
 
 /* backup_current_velocity (nodal_data_ops.cpp:57-81). */
@@ -262,6 +263,13 @@ int mpm_g2p(mpm_ctx *ctx, int update_vel, int update_strainrate, double dt);
  * (main.cpp:142, utilities.cpp:420-434): periodic wrap and removal of
  * particles outside a non-periodic domain when step % num_redist == 0. */
 int mpm_move(mpm_ctx *ctx, double dt, int step);
+
+/* Redistribute_Fill_Update (utilities.cpp:420-434) as the time loop calls it
+ * at the top of a step (main.cpp:88): when step % num_redist == 0, periodic
+ * wrap and removal of particles outside a non-periodic domain (the
+ * neighbour images it refreshes are index arithmetic here).  mpm_move runs
+ * the same after the move (main.cpp:142). */
+int mpm_redistribute(mpm_ctx *ctx, int step);
 
 /* updateVolume (mpm_particle_timestep.cpp:131-182). */
 int mpm_update_volume(mpm_ctx *ctx);

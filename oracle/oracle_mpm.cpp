@@ -1094,6 +1094,15 @@ int orc_move(orc_ctx *c, double dt, int step) {
     return MPM_OK;
 }
 
+// Redistribute_Fill_Update (utilities.cpp:420-434) on its own: the
+// redistribution every num_redist steps, the images refreshed every call.
+int orc_redistribute(orc_ctx *c, int step) {
+    const int nr = c->s.num_redist > 0 ? c->s.num_redist : 1;
+    if (step % nr == 0) redistribute(c);
+    fill_neighbors(c);
+    return MPM_OK;
+}
+
 // updateVolume (mpm_particle_timestep.cpp:131-182).
 int orc_update_volume(orc_ctx *c) {
     const int64_t np = (int64_t)c->parts.size();

@@ -44,6 +44,7 @@ int orc_nodal_update(orc_ctx *ctx, double dt);
 int orc_apply_nodal_bcs(orc_ctx *ctx, double dt, double t);
 int orc_g2p(orc_ctx *ctx, int update_vel, int update_strainrate, double dt);
 int orc_move(orc_ctx *ctx, double dt, int step);
+int orc_redistribute(orc_ctx *ctx, int step);
 int orc_update_volume(orc_ctx *ctx);
 int orc_stress(orc_ctx *ctx, double dt, double time);
 int orc_calculate_dt(orc_ctx *ctx, double *dt);
