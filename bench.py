@@ -8,8 +8,10 @@ redistribute, volume, stress).  value = particles x steps / seconds of the
 timed region, inputs resident in HBM.
 
 The cpu_baseline leg times the CPU restatement of the reference step
-(oracle/, OpenMP, all allowed host cores) on a bounded sample of the same
-workload.
+(oracle/, OpenMP with atomic deposits) on the same workload: 50 steps after 5
+warm-up steps on the job's host cores, and a 1-thread figure on 4 steps after
+one warm-up step; the
+host's core count, CPU model and sockets go into the record.
 
 The batch object runs the batched ensemble (exagoop_amd/ensemble.py): 256
 independent scenes (seeds 0..255: elastic disks or dam break, 2k-64k
@@ -27,11 +29,14 @@ The thermal object runs the same dam break with the thermal module on
 (uniform temperature, adiabatic walls): milliseconds per step and the
 thermal stages' share of them.
 
-The learned object compares, on the same dam break, the plain run at a fixed
-fine dt = 2.5e-5 with the learned coarse step (4 fine dt per step, the
-committed network exagoop_amd/data/learned_dam_break.npz) and the uncorrected
-coarse step over 200 fine steps: fine-step-equivalent particle-updates/s of
-each form and the relative L2 particle-velocity error against the plain run.
+The learned object compares, on the same dam break over 200 fine steps of
+dt = 2.5e-5 (a quarter of the CFL-0.3 step; the plain run at that dt is the
+reference), the learned coarse step (4 fine dt per step, the committed
+network exagoop_amd/data/learned_dam_break.npz), the uncorrected coarse step
+and plain runs at larger fixed dt: the relative L2 particle-velocity error of
+each against the reference, and the speed-up of the learned form over the
+plain run at the largest dt whose error does not exceed the learned form's
+(learned vs plain at equal error).
 """
 import argparse
 import json
@@ -56,7 +61,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--cpu-steps", type=int, default=50)
+    ap.add_argument("--cpu-warmup", type=int, default=5)
+    ap.add_argument("--cpu-1t-steps", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--scheme", default="USL", choices=["USL", "MUSL"])
     ap.add_argument("--no-learned", action="store_true")
@@ -128,21 +135,7 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        import oracle
-        threads = min(16, os.cpu_count() or 1)
-        o = oracle.Oracle(s, threads=threads)
-        o.upload(pos, r, idata)
-        o.init_internal_forces()
-        o.step(1)
-        t0 = time.perf_counter()
-        o.step(args.cpu_steps)
-        ct = time.perf_counter() - t0
-        cpu = {"value": round(npart * args.cpu_steps / ct, 1), "unit": "particle-updates/s",
-               "cores": threads, "kind": "port",
-               "sample": f"full dam break ({npart} particles), {args.cpu_steps} steps after "
-                         f"1 warm-up step, oracle OpenMP {threads} threads, {ct:.1f} s"}
-        print(f"[bench] cpu baseline {cpu['value']:.0f} p-upd/s on {threads} threads",
-              file=sys.stderr, flush=True)
+        cpu = cpu_leg(s, pos, r, idata, args.cpu_warmup, args.cpu_steps, args.cpu_1t_steps)
 
     if args.calib_copy_mib:
         calib_copy(args.calib_copy_mib)
@@ -181,6 +174,60 @@ def main():
         "thermal": thermal,
     }
     print(json.dumps(out), flush=True)
+
+
+def host_cpu():
+    """nproc, the CPU model and the socket count of this host (lscpu)."""
+    import subprocess
+    info = {"nproc": os.cpu_count(), "model": None, "sockets": None,
+            "threads_allowed": len(os.sched_getaffinity(0))}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() == "Model name":
+                info["model"] = v.strip()
+            elif k.strip() == "Socket(s)":
+                info["sockets"] = int(v.strip())
+    except (OSError, ValueError, subprocess.SubprocessError):
+        pass
+    return info
+
+
+def cpu_leg(s, pos, r, idata, warmup, nsteps, nsteps_1t):
+    """The oracle (the CPU restatement of the reference step, OpenMP with
+    atomic deposits as the reference's CPU build) on the benchmark dam break:
+    warm-up steps, then nsteps timed on the cores this job may use
+    (OMP_NUM_THREADS, else the affinity mask), and a 1-thread figure over
+    nsteps_1t steps."""
+    import oracle
+    host = host_cpu()
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    threads = int(env) if env.isdigit() and int(env) > 0 else host["threads_allowed"]
+    npart = pos.shape[1]
+
+    def run(nthreads, n, w):
+        o = oracle.Oracle(s, threads=nthreads)
+        o.upload(pos, r, idata)
+        o.init_internal_forces()
+        o.step(w)
+        t0 = time.perf_counter()
+        o.step(n)
+        return time.perf_counter() - t0
+
+    ct = run(threads, nsteps, warmup)
+    c1 = run(1, nsteps_1t, 1)
+    out = {"value": round(npart * nsteps / ct, 1), "unit": "particle-updates/s",
+           "cores": threads, "kind": "port",
+           "sample": f"full dam break ({npart} particles): {nsteps} steps timed after "
+                     f"{warmup} warm-up steps, oracle OpenMP on {threads} threads, {ct:.1f} s",
+           "value_1_thread": round(npart * nsteps_1t / c1, 1),
+           "sample_1_thread": f"{nsteps_1t} steps after 1 warm-up step, {c1:.1f} s",
+           "host": host,
+           "threads_note": "the job's core share (OMP_NUM_THREADS) of the GPU host"}
+    print(f"[bench] cpu baseline {out['value']:.0f} p-upd/s on {threads} threads, "
+          f"{out['value_1_thread']:.0f} on 1 thread ({host})", file=sys.stderr, flush=True)
+    return out
 
 
 def thermal_leg(s, pos, r, idata, nsteps, plain_ms):
@@ -274,55 +321,94 @@ def batch_leg(nscenes, nsteps):
 
 
 def learned_leg(nsteps):
-    """Plain fine run vs learned and uncorrected coarse runs (same scene)."""
+    """Learned vs plain at equal error, on the benchmark dam break.
+
+    The reference is the plain run at the fine fixed dt = 2.5e-5 (CFL about
+    0.075 at rest, a quarter of the CFL-0.3 step) over nsteps fine steps.
+    The learned form takes coarse steps of 4 fine dt (1e-4, CFL about 0.3)
+    with the committed network; the uncorrected form the same coarse steps
+    without it.  Plain runs at fixed dt = T / n over the same horizon T give
+    the plain error at each dt; the equal-error plain run is the one with the
+    largest dt whose error does not exceed the learned form's, and the
+    equal-error speed-up is its time over the learned form's time."""
     import numpy as np
     from exagoop_amd import capi, learned, scenes
 
-    def ctx():
+    def ctx(dt):
         s, pos, r, idata = scenes.dam_break()
         s.fixed_timestep = 1
-        s.timestep = learned.DT
+        s.timestep = dt
         g = capi.Context(s)
         g.upload(pos, r, idata)
         g.init_internal_forces()
         g.time()
         return g, pos.shape[1]
 
-    net = learned.load_net()
-    nf = learned.NFINE
-    nsteps -= nsteps % nf
-    g, npart = ctx()
-    t0 = time.perf_counter()
-    g.step(nsteps)
-    g.time()
-    t_fine = time.perf_counter() - t0
-    vf, idf = learned.velocities(g)
-    g.close()
-    res = {}
-    for name, model in (("learned", net), ("coarse", None)):
-        g, _ = ctx()
+    def timed(dt, n, fn):
+        g, npart = ctx(dt)
         t0 = time.perf_counter()
-        fb = capi.step_learned(g, model, nsteps // nf, nf)
-        g.time()
+        extra = fn(g, n)
+        g.time()  # synchronises
         t = time.perf_counter() - t0
         v, ids = learned.velocities(g)
         g.close()
-        err = learned.rel_l2(v, vf) if np.array_equal(ids, idf) else float("inf")
-        res[name] = (t, err, fb)
+        return t, v, ids, npart, extra
+
+    net = learned.load_net()
+    nf = learned.NFINE
+    nsteps -= nsteps % nf
+    horizon = nsteps * learned.DT
+    c_rest = (2e6 / 1000.0) ** 0.5 / (1.0 / 64.0)  # sound speed / dx
+    t_fine, vf, idf, npart, _ = timed(learned.DT, nsteps, lambda g, n: g.step(n))
+
+    def err(v, ids):
+        return learned.rel_l2(v, vf) if np.array_equal(ids, idf) else float("inf")
+
+    res = {}
+    for name, model in (("learned", net), ("coarse", None)):
+        t, v, ids, _, fb = timed(learned.DT, nsteps // nf,
+                                 lambda g, n, m=model: capi.step_learned(g, m, n, nf))
+        res[name] = (t, err(v, ids), fb)
     t_l, err_l, fb_l = res["learned"]
+    sweep = []
+    # dt = horizon / n, from just above the fine dt to the CFL-0.3 step
+    for n in (190, 180, 170, 160, 128, 100, 80, 64, 50):
+        dt = horizon / n
+        t, v, ids, _, _ = timed(dt, n, lambda g, k: g.step(k))
+        sweep.append({"dt": dt, "cfl_at_rest": round(dt * c_rest, 4), "steps": n,
+                      "rel_l2_velocity_err": float(f"{err(v, ids):.4g}"),
+                      "seconds": round(t, 5)})
+    ok = [x for x in sweep if x["rel_l2_velocity_err"] <= err_l]
+    if ok:
+        eq = max(ok, key=lambda x: x["dt"])
+        t_eq = eq["seconds"]
+    else:  # no coarser plain run is as accurate: the fine run itself
+        eq = {"dt": learned.DT, "cfl_at_rest": round(learned.DT * c_rest, 4), "steps": nsteps,
+              "rel_l2_velocity_err": 0.0, "seconds": round(t_fine, 5)}
+        t_eq = t_fine
     out = {"value": round(npart * nsteps / t_l, 1), "unit": "particle-updates/s",
+           "speedup_equal_error": round(t_eq / t_l, 3),
+           "equal_error_plain": eq,
+           "plain_dt_sweep": sweep,
            "plain_value": round(npart * nsteps / t_fine, 1),
-           "speedup": round(t_fine / t_l, 3),
+           "speedup_vs_fine": round(t_fine / t_l, 3),
            "rel_l2_velocity_err": float(f"{err_l:.4g}"),
            "uncorrected_rel_l2_velocity_err": float(f"{res['coarse'][1]:.4g}"),
            "uncorrected_value": round(npart * nsteps / res["coarse"][0], 1),
-           "fallbacks": fb_l, "fine_dt": learned.DT, "fine_steps_per_learned_step": nf,
-           "horizon_fine_steps": nsteps,
+           "fallbacks": fb_l, "fine_dt": learned.DT,
+           "fine_dt_note": "reference run at a quarter of the CFL-0.3 step (CFL about 0.075)",
+           "fine_steps_per_learned_step": nf, "horizon_fine_steps": nsteps,
+           "horizon_s": horizon,
            "network": "27x4 -> 64 -> 64 -> 3 GELU, BF16 MFMA / FP32 accumulate",
            "weights": "exagoop_amd/data/learned_dam_break.npz"}
-    print(f"[bench] learned: {out['speedup']}x the plain fine run, rel L2 velocity error "
-          f"{out['rel_l2_velocity_err']} (uncorrected {out['uncorrected_rel_l2_velocity_err']})",
-          file=sys.stderr, flush=True)
+    print(f"[bench] learned: rel L2 velocity error {out['rel_l2_velocity_err']} "
+          f"(uncorrected {out['uncorrected_rel_l2_velocity_err']}); equal-error plain dt "
+          f"{eq['dt']:.4g} (error {eq['rel_l2_velocity_err']}): {out['speedup_equal_error']}x; "
+          f"{out['speedup_vs_fine']}x the fine run", file=sys.stderr, flush=True)
+    for x in sweep:
+        print(f"[bench]   plain dt {x['dt']:.4g} (CFL {x['cfl_at_rest']}): error "
+              f"{x['rel_l2_velocity_err']}, {x['seconds'] * 1e3:.2f} ms", file=sys.stderr,
+              flush=True)
     return out
 
 
