@@ -83,6 +83,9 @@ class Column:
         self.v_ref = 1.0
         self.train_seeds = list(range(32))
         self.eval_seeds = list(range(1000, 1004))
+        # where a column network would be read from; none ships, because the
+        # trained one's held-out error was worse than the uncorrected coarse
+        # step's (--out DIR writes DIR/learned_column.npz)
         self.data_file = os.path.join(os.path.dirname(DATA_FILE), "learned_column.npz")
 
     @staticmethod
