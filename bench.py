@@ -25,6 +25,10 @@ cells, 100 663 296 particles, cubic, exagoop_amd/column.py): 20 timed steps
 after 3 warm-up steps, particle-updates/s, per-kernel times, and the
 size-independent checks of its final state.
 
+The disks object runs the elastic-disk collision (two linear-elastic disks,
+64x64x4 cells, 33 008 particles, quadratic B-splines, dt = 1e-4): 200 timed
+steps on the GPU and the oracle over the same steps.
+
 The thermal object runs the same dam break with the thermal module on
 (uniform temperature, adiabatic walls): milliseconds per step and the
 thermal stages' share of them.
@@ -72,6 +76,7 @@ def main():
     ap.add_argument("--no-column", action="store_true")
     ap.add_argument("--column-steps", type=int, default=20)
     ap.add_argument("--no-thermal", action="store_true")
+    ap.add_argument("--no-disks", action="store_true")
     ap.add_argument("--thermal-steps", type=int, default=50)
     ap.add_argument("--batch-scenes", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=500)
@@ -152,6 +157,10 @@ def main():
     if not args.no_thermal:
         thermal = thermal_leg(s, pos, r, idata, args.thermal_steps, ev_ms / args.steps)
 
+    disks = None
+    if not args.no_disks:
+        disks = disks_leg(not args.no_cpu)
+
     column = None
     if not args.no_column:
         from exagoop_amd import column as column_mod
@@ -172,6 +181,7 @@ def main():
         "batch": batch,
         "column": column,
         "thermal": thermal,
+        "disks": disks,
     }
     print(json.dumps(out), flush=True)
 
@@ -227,6 +237,43 @@ def cpu_leg(s, pos, r, idata, warmup, nsteps, nsteps_1t):
            "threads_note": "the job's core share (OMP_NUM_THREADS) of the GPU host"}
     print(f"[bench] cpu baseline {out['value']:.0f} p-upd/s on {threads} threads, "
           f"{out['value_1_thread']:.0f} on 1 thread ({host})", file=sys.stderr, flush=True)
+    return out
+
+
+def disks_leg(with_cpu):
+    """The elastic-disk collision (two linear-elastic disks, 64x64x4 cells,
+    z periodic, 8 particles per cell, quadratic B-splines, fixed dt = 1e-4,
+    USL, FP64): 200 timed steps after 5 warm-up steps on the GPU, and the
+    oracle over the same 200 steps on the job's cores."""
+    from exagoop_amd import capi, scenes
+
+    s, pos, r, idata = scenes.elastic_disks()
+    n = pos.shape[1]
+    g = capi.Context(s)
+    g.upload(pos, r, idata)
+    g.init_internal_forces()
+    g.step(5)
+    g.step(200)
+    ms, _ = g.last_step_timing()
+    out = {"workload": "two elastic disks colliding, 64x64x4 cells, quadratic, dt 1e-4",
+           "particles": n, "steps": 200, "ms_per_step": round(ms / 200, 5),
+           "value": round(n * 200 / (ms * 1e-3), 1), "unit": "particle-updates/s"}
+    if with_cpu:
+        import oracle
+        env = os.environ.get("OMP_NUM_THREADS", "")
+        threads = int(env) if env.isdigit() and int(env) > 0 else len(os.sched_getaffinity(0))
+        o = oracle.Oracle(s, threads=threads)
+        o.upload(pos, r, idata)
+        o.init_internal_forces()
+        o.step(5)
+        t0 = time.perf_counter()
+        o.step(200)
+        ct = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(n * 200 / ct, 1), "unit": "particle-updates/s",
+                               "cores": threads, "kind": "port",
+                               "sample": f"200 steps after 5 warm-up steps, {ct:.1f} s"}
+    print(f"[bench] elastic disks: {n} particles, {out['ms_per_step'] * 1e3:.1f} us/step, "
+          f"{out['value'] / 1e9:.3f} G p-upd/s", file=sys.stderr, flush=True)
     return out
 
 

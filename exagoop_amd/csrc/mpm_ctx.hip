@@ -529,8 +529,10 @@ int mpm_write_ascii_particles(mpm_ctx *c, const char *prefix, double time, char 
 }
 
 // readCheckpointFile (mpm_particle_outputs.cpp:537-589): step, output index
-// and time from dir/Header, the particles from dir/particles.bin.  As in the
-// reference, the caller runs Initialise_Internal_Forces next.
+// and time from dir/Header (either Header variant of WriteHeader: with or
+// without the EB line), the particles from dir/particles.bin.  As in the
+// reference, the caller runs Initialise_Internal_Forces next.  A context with
+// the thermal module on needs dir/thermal.bin.
 int mpm_read_checkpoint(mpm_ctx *c, const char *dir, int *output_it) {
     if (!c || !dir) return set_err(MPM_E_ARG, "null argument");
     std::ifstream h(std::string(dir) + "/Header");
@@ -541,7 +543,24 @@ int mpm_read_checkpoint(mpm_ctx *c, const char *dir, int *output_it) {
     int64_t steps = 0;
     int oit = 0;
     double time = 0.0;
-    if (!(h >> steps >> oit >> time)) return set_err(MPM_E_ARG, "bad checkpoint Header");
+    if (!(h >> steps >> oit)) return set_err(MPM_E_ARG, "bad checkpoint Header");
+    // The time is the last line: the reference's default EB build writes
+    // EB_generate_max_level (an integer) before it (:381-385), a build
+    // without EB (and mpm_write_checkpoint) does not.
+    std::vector<std::string> rest;
+    while (std::getline(h, line))
+        if (line.find_first_not_of(" \t\r") != std::string::npos) rest.push_back(line);
+    if (rest.empty() || rest.size() > 2) return set_err(MPM_E_ARG, "bad checkpoint Header");
+    {
+        std::istringstream ts(rest.back());
+        if (!(ts >> time)) return set_err(MPM_E_ARG, "bad checkpoint Header time");
+    }
+    if (c->thermal) {
+        std::ifstream probe(std::string(dir) + "/thermal.bin", std::ios::binary);
+        if (!probe.good())
+            return set_err(MPM_E_ARG, "the thermal module is on but the checkpoint holds no "
+                                      "thermal fields (thermal.bin)");
+    }
     std::ifstream f(std::string(dir) + "/particles.bin", std::ios::binary);
     char magic[8];
     int64_t hdr[4];

@@ -196,8 +196,9 @@ int mpm_write_particle_file_thermal(const char *path, int64_t np, const double *
 
 /* Checkpoint / restart.  mpm_write_checkpoint replaces writeCheckpointFile
  * (mpm_particle_outputs.cpp:355-499): dir/Header in the reference's layout
- * ("Checkpoint version: 1", step, output index, time to 17 digits; WriteHeader
- * :311-353) and dir/particles.bin holding every particle field, the ids and
+ * without EB ("Checkpoint version: 1", step, output index, time to 17 digits;
+ * WriteHeader :311-353; the reader also takes the EB build's variant with
+ * EB_generate_max_level before the time) and dir/particles.bin holding every particle field, the ids and
  * the cold fields (the reference stores the particles in AMReX's checkpoint
  * format instead).  mpm_read_checkpoint replaces readCheckpointFile
  * (:537-589): the context takes the particles (ids kept), step count and
@@ -205,7 +206,8 @@ int mpm_write_particle_file_thermal(const char *path, int64_t np, const double *
  * mpm_init_internal_forces.  No nodal data is stored (the reference stores
  * none).  With the thermal module enabled, dir/thermal.bin holds the thermal
  * fields by upload index (the USE_TEMP build's record carries them), and a
- * context with the module enabled reads them back. */
+ * context with the module enabled reads them back (MPM_E_ARG when the
+ * checkpoint has none). */
 int mpm_write_checkpoint(mpm_ctx *ctx, const char *dir, int output_it);
 /* writeAsciiFiles (mpm_particle_outputs.cpp:188-196): the live particles as
  * text in "<prefix>_t<time with 6 decimals>" (the reference's name rule; the

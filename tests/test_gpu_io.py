@@ -229,6 +229,36 @@ def test_checkpoint_carries_the_thermal_fields(gpu, tmp_path):
     assert np.array_equal(pc, pa)
 
 
+def test_checkpoint_header_of_the_eb_build(gpu, tmp_path):
+    """The reference's default build (EB on) writes EB_generate_max_level
+    before the time (mpm_particle_outputs.cpp:381-385); the reader takes the
+    time from the last line in either layout."""
+    say("[io] checkpoint Header with the EB line")
+    s, pos, r, idata = elastic_scene()
+    ck = str(tmp_path / "chk_eb")
+    a = run(s, pos, r, idata, 7, ckpt=ck)
+    ta, sa, _ = a.time()
+    hp = os.path.join(ck, "Header")
+    lines = open(hp).read().split("\n")
+    assert float(lines[3]) == ta and ta > 0.0
+    open(hp, "w").write("\n".join(lines[:3] + ["0"] + lines[3:]))
+    b = capi.Context(s)
+    assert b.read_checkpoint(ck) == 7
+    tb, sb, _ = b.time()
+    assert sb == sa == 7 and tb == ta
+
+
+def test_thermal_context_needs_thermal_fields_in_the_checkpoint(gpu, tmp_path):
+    say("[io] thermal context, checkpoint without thermal.bin")
+    s, pos, r, idata = elastic_scene()
+    ck = str(tmp_path / "chk_mech")
+    run(s, pos, r, idata, 3, ckpt=ck)
+    b = capi.Context(s)
+    b.enable_thermal()
+    with pytest.raises(capi.MPMError, match="thermal"):
+        b.read_checkpoint(ck)
+
+
 def test_thermal_particle_file_loads_both_records(gpu, tmp_path):
     say("[io] thermal particle file -> context")
     s, pos, r, idata = scenes.random_block(43, order=2, model=1)
