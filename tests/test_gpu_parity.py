@@ -328,6 +328,28 @@ def test_twisted_column_small_parity(gpu):
     assert rotation_asymmetry(pos, idg, rg) <= 1e-12
 
 
+def test_twisted_column_full_size_checks(gpu):
+    """Twisted column at full size (256 x 256 x 896 cells, 100 663 296
+    particles), 6 steps: every particle kept, the total mass unchanged, zero
+    net horizontal momentum, finite energies, and the initial kinetic energy
+    equal to its closed form 1/2 rho int omega(z)^2 r^2 dV = 1/2 * 1000 *
+    (1/6) * 300 = 25 000 J for omega(z) = 10 sin(pi z / 12) over the
+    1 x 1 x 6 column (the particle lattice is a midpoint rule of it)."""
+    from exagoop_amd import column
+    say("[properties] twisted column full size, 1 + 5 steps")
+    out = column.run(1.0, steps=5, warmup=1, log=say)
+    c = out["checks"]
+    assert out["particles"] == 100663296
+    assert c["particles_kept"] and c["live_particles"] == out["particles"]
+    assert c["mass_rel_change"] <= 1e-14
+    assert max(c["net_momentum_xy_rel"]) <= 1e-10
+    assert c["energies_finite"]
+    ke0, ke1 = c["kinetic_energy"]
+    assert abs(ke0 - 25000.0) <= 1e-4 * 25000.0
+    # the twist turns kinetic into strain energy; the total does not grow
+    assert ke1 < ke0 and ke1 + c["strain_energy"] <= ke0 * (1 + 1e-6)
+
+
 def test_twisted_column_quarter_size_properties(gpu):
     """Twisted column at 1/4 resolution (64 x 64 x 224 cells, 1 572 864
     particles): size-independent properties after 40 steps -- every particle
