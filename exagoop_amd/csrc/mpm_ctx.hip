@@ -838,7 +838,7 @@ int mpm_init_internal_forces(mpm_ctx *c) {
     // The sorted deposit (particles outside their cell go through the global
     // path) replaces the atomic one-thread-per-particle deposit when enabled.
     if (c->sorted && c->P.np > 0) {
-        launch_sort(c->g, c->ph, c->P, c->S, nullptr, 1, c->stream);
+        launch_sort(c->g, c->ph, c->P, c->S, nullptr, c->ph.extloads ? 1 : 0, c->stream);
         launch_p2g_tiles(c->g, c->ph, c->P, c->S, c->N, (int)c->steps, 0, 1, c->sc, c->stream);
     } else {
         launch_p2g(c->g, c->ph, c->P, c->N, 1, 1, 1, nullptr, c->stream);
@@ -899,7 +899,8 @@ int mpm_step(mpm_ctx *c, int nsteps) {
         }
         const bool sorted = c->sorted != 0;
         if (prof) HIPCHK(hipEventRecord(c->ev[1], st));
-        if (sorted) launch_sort(c->g, c->ph, c->P, c->S, c->Q.id, 1, st, fold);
+        if (sorted)
+            launch_sort(c->g, c->ph, c->P, c->S, c->Q.id, c->ph.extloads ? 1 : 0, st, fold);
         if (prof) HIPCHK(hipEventRecord(c->ev[2], st));
         if (sorted) launch_p2g_tiles(c->g, c->ph, c->P, c->S, c->N, step, 0, nr > 1, c->sc, st);
         else launch_p2g(c->g, c->ph, c->P, c->N, 1, 1, 1, nullptr, st);

@@ -148,6 +148,9 @@ struct SortState {
     // xz, yy, yz, zz); all but x zero for a particle with phase != 0.
     double *pkt;
     int64_t pcap;
+    // the last sort wrote the packets (otherwise P2G gathers the particle
+    // fields through perm)
+    int pkt_valid;
     // Non-empty cell tiles after the sort: tiles[0] = count, then the tile
     // indices in increasing order (ntiles + 1 entries).
     int *tiles;
@@ -157,7 +160,8 @@ constexpr int kPktComps = 16;
 size_t sort_scan_bytes(int nkeys);
 // out_id: id array of the buffer the step writes into (slots past the live
 // count are marked free), or null.
-// packets: also build the P2G packets of the sorted particles.
+// packets: also build the P2G packets of the sorted particles (needed with
+// external slab loads; without them P2G can gather the fields through perm).
 // The step's dt, computed by the sort's first launch when it needs no
 // reduction over the particles (fixed dt, or the minimum the previous
 // particle kernel accumulated): the arguments of launch_dt.

@@ -87,13 +87,13 @@ __global__ __launch_bounds__(kBlock) void k_particle_sorted(GridP g, PhysP ph, L
 
 // deposit_onto_grid_momentum over the non-empty tiles, one workgroup each
 // (see p2g_tile).
-template <int O, bool FORCES, bool FALLBACK>
+template <int O, bool FORCES, bool FALLBACK, bool GATHER>
 __global__ __launch_bounds__(kBlock) void k_p2g_cells(GridP g, PhysP ph, PState P, SortState S,
                                                       double *acc, int epoch, Scalars *sc) {
     __shared__ P2gLds<FORCES ? 7 : 3> lds;
     const int tile = active_tile(S, blockIdx.x);
     if (tile < 0) return;  // uniform over the block
-    p2g_tile<O, FORCES, FALLBACK>(g, ph, P, S, acc, epoch, &sc->err, tile, lds);
+    p2g_tile<O, FORCES, FALLBACK, GATHER>(g, ph, P, S, acc, epoch, &sc->err, tile, lds);
 }
 
 // Fused particle pipeline over the non-empty tiles, one workgroup each (see
@@ -126,6 +126,7 @@ size_t sort_scan_bytes(int nkeys) {
 void launch_sort(const GridP &g, const PhysP &ph, const PState &P, SortState &S,
                  int64_t *out_id, int packets, hipStream_t s, const DtFold &dt) {
     if (P.np <= 0) return;  // (callers fold the dt only with particles)
+    S.pkt_valid = packets;
     hipLaunchKernelGGL(k_keys, dim3(blocks_for(P.np)), dim3(kBlock), 0, s, g, P, S, ph, dt);
     size_t bytes = S.scan_bytes;
     (void)hipcub::DeviceScan::ExclusiveSum(S.scan_tmp, bytes, S.count, S.cell_start, S.nkeys + 1,
@@ -141,7 +142,14 @@ void launch_p2g_tiles(const GridP &g, const PhysP &ph, const PState &P, const So
     if (P.np <= 0) return;
     const dim3 gr((unsigned)((S.ntiles + 7) / 8 * 8)), bl(kBlock);
 #define XMPM_PG(o, f, fb)                                                                     \
-    hipLaunchKernelGGL((k_p2g_cells<o, f, fb>), gr, bl, 0, s, g, ph, P, S, N.acc, epoch, sc)
+    do {                                                                                      \
+        if (f && !S.pkt_valid)                                                                \
+            hipLaunchKernelGGL((k_p2g_cells<o, f, fb, true>), gr, bl, 0, s, g, ph, P, S, N.acc, \
+                               epoch, sc);                                                    \
+        else                                                                                  \
+            hipLaunchKernelGGL((k_p2g_cells<o, f, fb, false>), gr, bl, 0, s, g, ph, P, S,      \
+                               N.acc, epoch, sc);                                             \
+    } while (0)
 #define XMPM_PGO(f, fb)                                                                       \
     do {                                                                                      \
         if (!g.uniform) XMPM_PG(0, f, fb);                                                    \

@@ -668,6 +668,7 @@ void launch_th_p2g(mpm_ctx *c, bool mass_temp, bool source, bool tiled, int epoc
     if (np > 0 && tiled) {
         SortState S = c->S;
         S.pkt = c->tpkt;
+        S.pkt_valid = 1;
         if (mass_temp && source)
             k_th_packets<true, true><<<blocks(np), kThreads, 0, st>>>(c->P, S, c->th, c->tpkt);
         else if (mass_temp)

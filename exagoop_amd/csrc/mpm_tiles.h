@@ -308,7 +308,8 @@ __device__ __forceinline__ int sym_idx(int a, int b) {
 struct StageRaw {
     double xd;
     double val[4];
-    int64_t slot;  // sorted position (FORCES) or particle index (momentum only)
+    int64_t slot;  // sorted position (packets), else particle index
+    int phase;     // gathered forces deposit: the particle's phase (part-0 lane)
     bool act;
 };
 
@@ -350,7 +351,7 @@ struct PartLds {
     SplineTables tab;
 };
 
-template <int O, bool FORCES, bool FALLBACK>
+template <int O, bool FORCES, bool FALLBACK, bool GATHER = false>
 __device__ __forceinline__ void p2g_tile(const GridP &g, const PhysP &ph, const PState &P,
                                          const SortState &S, double *acc, int epoch, int *err,
                                          int tile, P2gLds<FORCES ? 7 : 3> &L) {
@@ -410,6 +411,7 @@ __device__ __forceinline__ void p2g_tile(const GridP &g, const PhysP &ph, const 
         R.xd = 0.0;
         for (int e = 0; e < 4; ++e) R.val[e] = 0.0;
         R.slot = 0;
+        R.phase = 0;
         if (R.act) {
             if (FORCES) {
                 R.slot = sbeg + t;
@@ -433,6 +435,41 @@ __device__ __forceinline__ void p2g_tile(const GridP &g, const PhysP &ph, const 
         }
         return R;
     };
+    // GATHER: the permutation entry of this lane's slot in round (ry, base),
+    // -1 for an empty slot or past the last round; the packet's inputs of
+    // particle j: part 0 m and v, part 1 m, parts 2 and 3 V and three stress
+    // components (the products are formed at staging).
+    auto fetch_j = [&](int ry, int base) -> int {
+        if (ry >= kTileCells) return -1;
+        const int c0 = kTileCells * (ry + kTileCells * cz) + s_cell;
+        const int sbeg = cs[c0], scnt = cs[c0 + 1] - sbeg;
+        const int t = base + s_t;
+        return t < scnt ? S.perm[sbeg + t] : -1;
+    };
+    // The phase is read by the slot's part-0 lane only (R.phase) and shared
+    // at staging; no field read waits for it.
+    auto load_g = [&](int j) {
+        StageRaw R;
+        R.act = j >= 0;
+        R.xd = 0.0;
+        for (int e = 0; e < 4; ++e) R.val[e] = 0.0;
+        R.slot = j;
+        R.phase = 0;
+        if (R.act) {
+            if (s_part == 0) R.phase = P.idata[MPM_I_PHASE * cap + j];
+            if (s_part < 3) R.xd = P.pos[s_part * cap + j];
+            if (s_part <= 1) {
+                R.val[0] = P.r[MPM_R_MASS * cap + j];
+                if (s_part == 0)
+                    for (int d = 0; d < 3; ++d) R.val[1 + d] = P.r[(MPM_R_VEL + d) * cap + j];
+            } else {
+                R.val[0] = P.r[MPM_R_VOLUME * cap + j];
+                const int e0 = (s_part == 2) ? 0 : 3;
+                for (int e = 0; e < 3; ++e) R.val[1 + e] = P.r[(MPM_R_STRESS + e0 + e) * cap + j];
+            }
+        }
+        return R;
+    };
     auto mc_of = [&](int r) {
         return r == 0 ? mc[0] : (r == 1 ? mc[1] : (r == 2 ? mc[2] : mc[3]));
     };
@@ -451,13 +488,20 @@ __device__ __forceinline__ void p2g_tile(const GridP &g, const PhysP &ph, const 
 #pragma unroll
         for (int c = 0; c < NC; ++c) a4[k][c] = 0.0;
     StageRaw cur;
-    if (ry < kTileCells) cur = load(ry, base);
+    if (ry < kTileCells) cur = (FORCES && GATHER) ? load_g(fetch_j(ry, base)) : load(ry, base);
     while (ry < kTileCells) {
         int nry = ry, nbase = base;
         advance(nry, nbase);
         StageRaw nxt;
-        if (nry < kTileCells) nxt = load(nry, nbase);
+        if (nry < kTileCells)
+            nxt = (FORCES && GATHER) ? load_g(fetch_j(nry, nbase)) : load(nry, nbase);
         // ---- staging: this round's 16 packets
+        if (FORCES && GATHER) {
+            // a particle with phase != 0 deposits nothing (the packet path's
+            // zero values); the phase comes from the slot's part-0 lane
+            const int ph0 = __shfl(cur.phase, s_slot, 64);
+            if (ph0 != 0) cur.act = false;
+        }
         {
             const int ic = (s_part == 0) ? T0[0] + s_cell
                                          : ((s_part == 1) ? T0[1] + ry : T0[2] + cz);
@@ -472,7 +516,20 @@ __device__ __forceinline__ void p2g_tile(const GridP &g, const PhysP &ph, const 
                 if (FALLBACK) {
                     if (s_part == 3 && cur.act) {
                         double x[3], vals[13];
-                        if (FORCES) {
+                        if (FORCES && GATHER) {
+                            // the packet of particle j as scatter_one builds it
+                            // (no slab loads on this path)
+                            const int64_t j = cur.slot;
+                            for (int d = 0; d < 3; ++d) x[d] = P.pos[d * cap + j];
+                            const double m = P.r[MPM_R_MASS * cap + j];
+                            vals[0] = m;
+                            for (int d = 0; d < 3; ++d)
+                                vals[1 + d] = m * P.r[(MPM_R_VEL + d) * cap + j];
+                            for (int d = 0; d < 3; ++d) vals[4 + d] = m * ph.grav[d] + 0.0;
+                            const double V = P.r[MPM_R_VOLUME * cap + j];
+                            for (int e = 0; e < 6; ++e)
+                                vals[7 + e] = -V * P.r[(MPM_R_STRESS + e) * cap + j];
+                        } else if (FORCES) {
                             for (int d = 0; d < 3; ++d) x[d] = S.pkt[d * pc + cur.slot];
                             for (int e = 0; e < 13; ++e) vals[e] = S.pkt[(3 + e) * pc + cur.slot];
                         } else {
@@ -502,9 +559,29 @@ __device__ __forceinline__ void p2g_tile(const GridP &g, const PhysP &ph, const 
 #pragma unroll
                 for (int e = 0; e < 8; ++e) pw[kPkFac + s_part * 8 + e] = f[e];
             }
+            if (FORCES && GATHER) {
+                // packet values from the gathered fields, with the arithmetic
+                // of scatter_one: m, m v (part 0); m g (part 1); -V sigma
+                // (parts 2, 3)
+                double pv[4];
+                if (s_part == 0) {
+                    pv[0] = cur.val[0];
+                    for (int d = 0; d < 3; ++d) pv[1 + d] = cur.val[0] * cur.val[1 + d];
+                } else if (s_part == 1) {
+                    for (int d = 0; d < 3; ++d) pv[d] = cur.val[0] * ph.grav[d] + 0.0;
+                    pv[3] = 0.0;
+                } else {
+                    for (int e = 0; e < 3; ++e) pv[e] = -cur.val[0] * cur.val[1 + e];
+                    pv[3] = 0.0;
+                }
 #pragma unroll
-            for (int e = 0; e < 4; ++e)
-                if (e < v_n) pw[kPkM - 3 + v_first + e] = use ? cur.val[e] : 0.0;
+                for (int e = 0; e < 4; ++e)
+                    if (e < v_n) pw[kPkM - 3 + v_first + e] = use ? pv[e] : 0.0;
+            } else {
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    if (e < v_n) pw[kPkM - 3 + v_first + e] = use ? cur.val[e] : 0.0;
+            }
         }
         // The staging area is private to the wavefront and LDS operations of
         // one wavefront execute in issue order: the compiler only has to keep
