@@ -377,7 +377,9 @@ def learned_leg(nsteps):
     without it.  Plain runs at fixed dt = T / n over the same horizon T give
     the plain error at each dt; the equal-error plain run is the one with the
     largest dt whose error does not exceed the learned form's, and the
-    equal-error speed-up is its time over the learned form's time."""
+    equal-error speed-up is its time over the learned form's time.  Each time
+    is the median of three runs from fresh contexts, after an untimed warm-up
+    of both paths."""
     import numpy as np
     from exagoop_amd import capi, learned, scenes
 
@@ -391,19 +393,29 @@ def learned_leg(nsteps):
         g.time()
         return g, pos.shape[1]
 
-    def timed(dt, n, fn):
-        g, npart = ctx(dt)
-        t0 = time.perf_counter()
-        extra = fn(g, n)
-        g.time()  # synchronises
-        t = time.perf_counter() - t0
-        v, ids = learned.velocities(g)
-        g.close()
-        return t, v, ids, npart, extra
+    def timed(dt, n, fn, reps=3):
+        # median wall time of reps runs from fresh contexts (each run takes a
+        # host round trip per learned step, so single runs jitter)
+        ts = []
+        for _ in range(reps):
+            g, npart = ctx(dt)
+            t0 = time.perf_counter()
+            extra = fn(g, n)
+            g.time()  # synchronises
+            ts.append(time.perf_counter() - t0)
+            v, ids = learned.velocities(g)
+            g.close()
+        return float(np.median(ts)), v, ids, npart, extra
 
     net = learned.load_net()
     nf = learned.NFINE
     nsteps -= nsteps % nf
+    # untimed warm-up of the plain and learned paths (first launches load code)
+    g, _ = ctx(learned.DT)
+    g.step(nf)
+    capi.step_learned(g, net, 2, nf)
+    g.time()
+    g.close()
     horizon = nsteps * learned.DT
     c_rest = (2e6 / 1000.0) ** 0.5 / (1.0 / 64.0)  # sound speed / dx
     t_fine, vf, idf, npart, _ = timed(learned.DT, nsteps, lambda g, n: g.step(n))

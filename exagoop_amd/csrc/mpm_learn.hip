@@ -233,45 +233,6 @@ __global__ void k_append_rows(const int *sel, int nsel, const __bf16 *X, const d
     for (int c = 0; c < 3; ++c) d.t[3 * k + c] = (float)(N.vel[c * nn + q] - vc[3 * r + c]);
 }
 
-// Guard sums over the active nodes (row count on the device): [0] sum m,
-// [1..3] sum m dv, [4] sum m |v|^2, [5] sum m |v + dv|^2, [6] non-finite
-// count.  The sums must be zero on entry (k_guard_decide clears them).
-__global__ void k_guard_dev(const int *nodes, const int *count, const float *Y, double dv_ref,
-                            NState N, int64_t nn, double *sums) {
-    const int n = *count;
-    double a[7] = {0, 0, 0, 0, 0, 0, 0};
-    for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
-        const int q = nodes[r];
-        const double m = N.mass[q];
-        double v2 = 0.0, w2 = 0.0;
-        for (int d = 0; d < 3; ++d) {
-            const double v = N.vel[d * nn + q];
-            const double dv = (double)Y[3 * r + d] * dv_ref;
-            if (!isfinite(dv)) a[6] += 1.0;
-            a[1 + d] += m * dv;
-            v2 += v * v;
-            w2 += (v + dv) * (v + dv);
-        }
-        a[0] += m;
-        a[4] += m * v2;
-        a[5] += m * w2;
-    }
-    // wavefront sums, then one atomic per workgroup and sum
-    __shared__ double red[7][4];
-    const int w = threadIdx.x >> 6;
-    for (int k = 0; k < 7; ++k) {
-        double x = a[k];
-        for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
-        if ((threadIdx.x & 63) == 0) red[k][w] = x;
-    }
-    __syncthreads();
-    if (threadIdx.x < 7) {
-        const double x = red[threadIdx.x][0] + red[threadIdx.x][1] + red[threadIdx.x][2] +
-                         red[threadIdx.x][3];
-        if (x != 0.0) atomicAdd(&sums[threadIdx.x], x);
-    }
-}
-
 // Guard decision (see mpm_step_learned): flag = 1 keeps the corrected coarse
 // step.  The sums are read back with atomics, where they were accumulated,
 // and cleared for the next step in the same exchange.
@@ -338,10 +299,11 @@ int mpm_mlp_create(mpm_mlp **out) {
         hipMalloc(&d.m, kMParams * sizeof(float)) != hipSuccess ||
         hipMalloc(&d.v, kMParams * sizeof(float)) != hipSuccess ||
         hipMalloc(&d.grad, kMParams * sizeof(float)) != hipSuccess ||
+        hipMalloc(&d.part, (size_t)kMTrainParts * kMPartStride * sizeof(float)) != hipSuccess ||
         hipMalloc(&d.w1t, kMH * kMInPad * sizeof(__bf16)) != hipSuccess ||
-        hipMalloc(&d.w2t, kMH * kMH * sizeof(__bf16)) != hipSuccess ||
-        hipMalloc(&d.w2, kMH * kMH * sizeof(__bf16)) != hipSuccess ||
-        hipMalloc(&d.w3t, 32 * kMH * sizeof(__bf16)) != hipSuccess ||
+        hipMalloc(&d.w2tp, kMH * kMH * sizeof(__bf16)) != hipSuccess ||
+        hipMalloc(&d.w2p, kMH * kMH * sizeof(__bf16)) != hipSuccess ||
+        hipMalloc(&d.w3tp, 32 * kMH * sizeof(__bf16)) != hipSuccess ||
         hipMalloc(&d.w3, kMH * 16 * sizeof(__bf16)) != hipSuccess ||
         hipMalloc(&d.loss, sizeof(double)) != hipSuccess)
         return fail(set_err(MPM_E_OOM, "network allocation failed"));
@@ -355,11 +317,11 @@ int mpm_mlp_create(mpm_mlp **out) {
     hipLaunchKernelGGL(k_zero_f, dim3(blocks256(kMH * kMInPad / 2)), dim3(256), 0, m->stream,
                        reinterpret_cast<float *>(d.w1t), kMH * kMInPad / 2);
     hipLaunchKernelGGL(k_zero_f, dim3(blocks256(kMH * kMH / 2)), dim3(256), 0, m->stream,
-                       reinterpret_cast<float *>(d.w2t), kMH * kMH / 2);
+                       reinterpret_cast<float *>(d.w2tp), kMH * kMH / 2);
     hipLaunchKernelGGL(k_zero_f, dim3(blocks256(kMH * kMH / 2)), dim3(256), 0, m->stream,
-                       reinterpret_cast<float *>(d.w2), kMH * kMH / 2);
+                       reinterpret_cast<float *>(d.w2p), kMH * kMH / 2);
     hipLaunchKernelGGL(k_zero_f, dim3(blocks256(32 * kMH / 2)), dim3(256), 0, m->stream,
-                       reinterpret_cast<float *>(d.w3t), 32 * kMH / 2);
+                       reinterpret_cast<float *>(d.w3tp), 32 * kMH / 2);
     hipLaunchKernelGGL(k_zero_f, dim3(blocks256(kMH * 16 / 2)), dim3(256), 0, m->stream,
                        reinterpret_cast<float *>(d.w3), kMH * 16 / 2);
     hipLaunchKernelGGL(k_zero_d, dim3(1), dim3(64), 0, m->stream, d.loss, 1);
@@ -373,8 +335,9 @@ void mpm_mlp_destroy(mpm_mlp *m) {
     if (!m) return;
     if (m->stream) (void)hipStreamSynchronize(m->stream);
     MlpDev &d = m->d;
-    for (void *p : {(void *)d.params, (void *)d.m, (void *)d.v, (void *)d.grad, (void *)d.w1t,
-                    (void *)d.w2t, (void *)d.w2, (void *)d.w3t, (void *)d.w3, (void *)d.loss,
+    for (void *p : {(void *)d.params, (void *)d.m, (void *)d.v, (void *)d.grad, (void *)d.part,
+                    (void *)d.w1t, (void *)d.w2tp, (void *)d.w2p, (void *)d.w3tp, (void *)d.w3,
+                    (void *)d.loss,
                     (void *)m->x, (void *)m->t, (void *)m->idx, (void *)m->y})
         (void)hipFree(p);
     if (m->stream) (void)hipStreamDestroy(m->stream);
@@ -436,10 +399,9 @@ int mpm_mlp_gradient(mpm_mlp *m, int64_t nrows, const float *X, const float *T, 
                           m->stream));
     HIPCHK(hipMemcpyAsync(m->t, T, nrows * kMOut * sizeof(float), hipMemcpyHostToDevice,
                           m->stream));
-    hipLaunchKernelGGL(k_zero_f, dim3(blocks256(kMParams)), dim3(256), 0, m->stream, m->d.grad,
-                       kMParams);
     hipLaunchKernelGGL(k_zero_d, dim3(1), dim3(64), 0, m->stream, m->d.loss, 1);
-    launch_mlp_train(m->d, m->x, m->t, m->idx, (int)nrows, 1.0f, m->stream);
+    const int nparts = launch_mlp_train(m->d, m->x, m->t, m->idx, (int)nrows, 1.0f, m->stream);
+    launch_mlp_reduce(m->d, nparts, m->stream);
     if ((rc = check_launch())) return rc;
     double l = 0.0;
     HIPCHK(hipMemcpyAsync(grad, m->d.grad, kMParams * sizeof(float), hipMemcpyDeviceToHost,
@@ -458,7 +420,7 @@ int mpm_mlp_adam(mpm_mlp *m, const float *grad, float lr, int t) {
                           m->stream));
     const float b1 = 0.9f, b2 = 0.999f;
     const float c1 = 1.0f - std::pow(b1, (float)t), c2 = 1.0f - std::pow(b2, (float)t);
-    launch_mlp_adam(m->d, lr, b1, b2, 1e-8f, c1, c2, m->stream);
+    launch_mlp_adam(m->d, 0, lr, b1, b2, 1e-8f, c1, c2, m->stream);
     HIPCHK(hipStreamSynchronize(m->stream));
     return check_launch();
 }
@@ -608,10 +570,11 @@ int mpm_mlp_train(mpm_mlp *m, mpm_dataset *d, int epochs, int batch, float lr, u
         hipLaunchKernelGGL(k_zero_d, dim3(1), dim3(64), 0, m->stream, m->d.loss, 1);
         for (int64_t off = 0; off < n; off += batch) {
             const int nb = (int)std::min<int64_t>(batch, n - off);
-            launch_mlp_train(m->d, d->x, d->t, m->idx + off, nb, inv_t, m->stream);
+            const int nparts = launch_mlp_train(m->d, d->x, d->t, m->idx + off, nb, inv_t,
+                                                m->stream);
             ++t;
             const float c1 = 1.0f - std::pow(b1, (float)t), c2 = 1.0f - std::pow(b2, (float)t);
-            launch_mlp_adam(m->d, lr, b1, b2, 1e-8f, c1, c2, m->stream);
+            launch_mlp_adam(m->d, nparts, lr, b1, b2, 1e-8f, c1, c2, m->stream);
         }
         if ((rc = check_launch())) return rc;
         double l = 0.0;
@@ -654,10 +617,8 @@ int mpm_step_learned(mpm_ctx *c, mpm_mlp *m, int ncoarse, int nfine, const doubl
         int *cnt = slot[it & 1];
         launch_active_nodes_rolling(c->g, c->N, c->ph.mass_tol, c->ml_nodes, cnt,
                                     slot[(it + 1) & 1], st);
-        launch_mlp_features_dev(c->g, c->N, c->ml_nodes, cnt, m->sc, c->ml_x, st);
-        launch_mlp_infer_dev(m->d, c->ml_x, cnt, c->ml_y, st);
-        hipLaunchKernelGGL(k_guard_dev, dim3(256), dim3(256), 0, st, c->ml_nodes, cnt, c->ml_y,
-                           m->sc.dv_ref, c->N, c->g.nn, c->ml_guard);
+        launch_mlp_infer_nodes(m->d, c->g, c->N, c->ml_nodes, cnt, m->sc, c->ml_y, c->ml_guard,
+                               st);
         hipLaunchKernelGGL(k_guard_decide, dim3(1), dim3(1), 0, st, c->ml_guard, c->sc, gmag,
                            guards[0], guards[1], flag);
         hipLaunchKernelGGL(k_apply_dev, dim3(512), dim3(256), 0, st, c->ml_nodes, cnt, c->ml_y,

@@ -21,17 +21,24 @@ constexpr int kOffW1 = 0, kOffB1 = kOffW1 + kMIn * kMH, kOffW2 = kOffB1 + kMH,
               kOffB3 = kOffW3 + kMH * kMOut, kMParams = kOffB3 + kMOut;
 static_assert(kMParams == 11331, "parameter count");
 constexpr int kMRows = 32;  // rows (nodes) per wavefront tile
+// Training partial rows: kMParams gradient entries, the loss sum, padding
+// to a multiple of 64 floats; at most kMTrainParts rows (one per wavefront).
+constexpr int kMPartStride = 11392, kMTrainParts = 512;
+static_assert(kMPartStride >= kMParams + 1 && kMPartStride % 64 == 0, "partial row");
 
 // Device copies of one network.  params/m/v/grad: FP32 [kMParams] in the
-// layout of include/exagoop_amd.h (W[in][out]).  The BF16 operand copies:
-//   w1t [64][128]  W1 transposed, inputs padded to 128 with zeros
-//   w2t [64][64]   W2 transposed (forward B operand)
-//   w2  [64][64]   W2 as stored (backward B operand)
-//   w3t [32][64]   W3 transposed, outputs padded to 32
-//   w3  [64][16]   W3 as stored, outputs padded to 16
+// layout of include/exagoop_amd.h (W[in][out]).  The BF16 operand copies,
+// with swap23(i) = i with bits 2 and 3 exchanged (the slot order in which a
+// transposed 32x32 MFMA result holds its columns):
+//   w1t  [64][128]  w1t[n][k]  = W1[k][n], inputs padded to 128 with zeros
+//   w2tp [64][64]   w2tp[n][s] = W2[swap23(s)][n]   (forward B operand)
+//   w2p  [64][64]   w2p[k][s]  = W2[k][swap23(s)]   (backward B operand)
+//   w3tp [32][64]   w3tp[n][s] = W3[swap23(s)][n], outputs padded to 32
+//   w3   [64][16]   w3[k][n]   = W3[k][n], outputs padded to 16
+// part: [kMTrainParts][kMPartStride] FP32 partial gradient rows.
 struct MlpDev {
-    float *params, *m, *v, *grad;
-    __bf16 *w1t, *w2t, *w2, *w3t, *w3;
+    float *params, *m, *v, *grad, *part;
+    __bf16 *w1t, *w2tp, *w2p, *w3tp, *w3;
     double *loss;  // running loss sum of a training pass
 };
 
@@ -45,22 +52,31 @@ void launch_mlp_features(const GridP &g, const NState &N, const int *nodes, int 
                          MlpScales sc, __bf16 *X, hipStream_t s);
 // Y[r][3] = network(X[r]) for r < nrows (X padded to a multiple of 32 rows).
 void launch_mlp_infer(const MlpDev &w, const __bf16 *X, int nrows, float *Y, hipStream_t s);
-// The same with the row count read on the device (*count), no host round trip.
-void launch_mlp_features_dev(const GridP &g, const NState &N, const int *nodes, const int *count,
-                             MlpScales sc, __bf16 *X, hipStream_t s);
-void launch_mlp_infer_dev(const MlpDev &w, const __bf16 *X, const int *count, float *Y,
-                          hipStream_t s);
 // Gradient of the mean squared error over the rows idx[0..nrows) of (X, T)
-// (T scaled by inv_t) added into w.grad; the loss sum into w.loss.
-void launch_mlp_train(const MlpDev &w, const __bf16 *X, const float *T, const int *idx,
-                      int nrows, float inv_t, hipStream_t s);
-// One Adam step from w.grad (then zeroed); refreshes the BF16 copies.
-void launch_mlp_adam(const MlpDev &w, float lr, float beta1, float beta2, float eps, float c1,
-                     float c2, hipStream_t s);
+// (T scaled by inv_t) as partial rows in w.part, in a fixed order (no
+// atomics); returns the number of rows written.
+int launch_mlp_train(const MlpDev &w, const __bf16 *X, const float *T, const int *idx, int nrows,
+                     float inv_t, hipStream_t s);
+// w.grad = the sum of the first nparts partial rows, in row order; their loss
+// sum added to *w.loss.
+void launch_mlp_reduce(const MlpDev &w, int nparts, hipStream_t s);
+// One Adam step; refreshes the BF16 copies.  The gradient is the sum of the
+// first nparts partial rows (their loss sum added to *w.loss) or, with
+// nparts == 0, w.grad (then zeroed).
+void launch_mlp_adam(const MlpDev &w, int nparts, float lr, float beta1, float beta2, float eps,
+                     float c1, float c2, hipStream_t s);
 // BF16 copies from params (after an upload).
 void launch_mlp_pack(const MlpDev &w, hipStream_t s);
 
-// Active node list: nodes with mass >= tol, in index order, count in *count.
+// Y[r][3] = network(features of nodes[r]) for r < *count, the features
+// gathered from the nodal state in the kernel (no row buffer), and the
+// guard sums of dv = Y dv_ref added to sums[0..7): m, m dv (3), m |v|^2,
+// m |v + dv|^2, non-finite count.
+void launch_mlp_infer_nodes(const MlpDev &w, const GridP &g, const NState &N, const int *nodes,
+                            const int *count, MlpScales sc, float *Y, double *sums, hipStream_t s);
+
+// Active node list: nodes with mass >= tol, in index order within each
+// workgroup's span of nodes (spans in arbitrary order), count in *count.
 void launch_active_nodes(const GridP &g, const NState &N, double tol, int *list, int *count,
                          int *scratch, hipStream_t s);
 // The same without clearing *count first (it must be zero on entry); the

@@ -63,6 +63,34 @@ def test_gradient_matches_oracle(gpu):
         assert rel(g[lo:hi], gref[lo:hi]) <= 2e-2, (lo, hi, rel(g[lo:hi], gref[lo:hi]))
 
 
+def test_gradient_is_reproducible(gpu):
+    """The gradient is summed without atomics in a fixed order (per-wavefront
+    partial rows, then rows in order): the same bits on every call."""
+    say("[learned] gradient twice, bitwise")
+    p = rand_params(11)
+    X, T = rand_rows(12, 3000)
+    net = capi.MLP(p)
+    l1, g1 = net.gradient(X, T)
+    l2, g2 = net.gradient(X, T)
+    assert l1 == l2
+    assert np.array_equal(g1, g2)
+
+
+def test_gradient_of_a_batch_above_one_launch(gpu):
+    """20 000 rows = 625 tiles: the first 512 tiles store their partial rows,
+    a second launch adds the other 113 into rows 0..112."""
+    say("[learned] gradient, 20000 rows (two launches)")
+    p = rand_params(13)
+    X, T = rand_rows(14, 20000)
+    net = capi.MLP(p)
+    loss, g = net.gradient(X, T)
+    lref, gref = oracle.mlp_grad(p, X, T, exact=False)
+    assert abs(loss - lref) <= 2e-3 * abs(lref), (loss, lref)
+    assert rel(g, gref) <= 1e-2, rel(g, gref)
+    _, g2 = net.gradient(X, T)
+    assert np.array_equal(g, g2)
+
+
 def test_adam_matches_oracle(gpu):
     say("[learned] adam vs oracle")
     p = rand_params(5)
@@ -202,6 +230,44 @@ def test_collect_rows_from_the_solver(gpu):
     n = ds.size()
     assert 1000 <= n <= 3000, n
     assert np.isfinite(ds.target_rms()) and ds.target_rms() > 0
+
+
+def test_learned_step_applies_features_then_forward(gpu):
+    """The learned step gathers each active node's features inside its
+    inference kernel.  After one accepted coarse step the nodal velocity
+    minus that of the same coarse step without a network must be
+    dv_ref * forward(features) at the active nodes (features from the
+    separate feature kernel, forward from the separate inference kernel) and
+    zero elsewhere.  The two runs deposit with FP64 atomics, so their nodal
+    states agree to rounding, not bitwise."""
+    from exagoop_amd import learned
+    say("[learned] fused features + inference + correction vs the separate kernels")
+    net = learned.load_net()
+    _, sc = net.get_params()
+    s, _, _, _ = learned.scene(1005)
+    runs = {}
+    for name, model in (("learned", net), ("coarse", None)):
+        g = learned.context(1005)
+        g.step(40)
+        fb = capi.step_learned(g, model, 1, learned.NFINE)
+        assert fb == 0
+        runs[name] = g
+    a, b = runs["learned"], runs["coarse"]
+    mass = b.nodal(capi.N_MASS)
+    nodes = np.flatnonzero(mass >= s.mass_tolerance)
+    assert nodes.size > 1000
+    Y = net.forward(capi.learn_features(b, net, nodes))
+    dv = Y.astype(np.float64) * sc[2]
+    va = np.stack([a.nodal(capi.N_VEL + d) for d in range(3)])
+    vb = np.stack([b.nodal(capi.N_VEL + d) for d in range(3)])
+    diff = (va - vb)[:, nodes].T
+    assert np.abs(dv).max() > 0
+    assert rel(diff, dv) <= 1e-3, rel(diff, dv)
+    rest = np.ones(mass.size, bool)
+    rest[nodes] = False
+    assert np.abs((va - vb)[:, rest]).max() <= 1e-9 * max(np.abs(vb).max(), 1.0)
+    for g in (a, b):
+        g.close()
 
 
 def test_trained_network_gains_over_the_uncorrected_coarse_step(gpu):
