@@ -7,6 +7,10 @@ One step = the full time-loop body (dt, P2G, grid update + BCs, G2P, move,
 redistribute, volume, stress).  value = particles x steps / seconds of the
 timed region, inputs resident in HBM.
 
+The deterministic object runs the same dam break with the bitwise-
+reproducible P2G (MPM_OPT_DETERMINISTIC) over the same steps: milliseconds
+per step, the ratio to the default step, and its per-kernel times.
+
 The cpu_baseline leg times the CPU restatement of the reference step
 (oracle/, OpenMP with atomic deposits) on the same workload: 50 steps after 5
 warm-up steps on the job's host cores, and a 1-thread figure on 4 steps after
@@ -77,6 +81,7 @@ def main():
     ap.add_argument("--column-steps", type=int, default=20)
     ap.add_argument("--no-thermal", action="store_true")
     ap.add_argument("--no-disks", action="store_true")
+    ap.add_argument("--no-det", action="store_true")
     ap.add_argument("--thermal-steps", type=int, default=50)
     ap.add_argument("--batch-scenes", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=500)
@@ -138,6 +143,10 @@ def main():
                 "kernels_gbs": {k: round(v[1] / (v[0] * 1e-3) / 1e9, 2) for k, v in kern.items()},
                 "sort_ms": round(kms[3], 5)}
 
+    det = None
+    if not args.no_det:
+        det = deterministic_leg(s, pos, r, idata, args.warmup, args.steps, ev_ms / args.steps)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_leg(s, pos, r, idata, args.cpu_warmup, args.cpu_steps, args.cpu_1t_steps)
@@ -176,6 +185,7 @@ def main():
                    "particles": npart, "nodes": nn, "order": 3, "scheme": args.scheme,
                    "constitutive": "Tait EOS K=2e6 gamma=7 mu=1e-3", "cfl": 0.3},
         "roofline": roofline,
+        "deterministic": det,
         "cpu_baseline": cpu,
         "learned": learned,
         "batch": batch,
@@ -184,6 +194,34 @@ def main():
         "disks": disks,
     }
     print(json.dumps(out), flush=True)
+
+
+def deterministic_leg(s, pos, r, idata, warmup, nsteps, plain_ms):
+    """The same dam break with MPM_OPT_DETERMINISTIC (bitwise-reproducible
+    P2G: per-cell particle order, private wavefront slabs, per-node gather of
+    the tile blocks): milliseconds per step over the same steps, and its
+    per-kernel times (p2g includes the gather pass)."""
+    from exagoop_amd import capi
+    g = capi.Context(s)
+    g.set_option(capi.OPT_DETERMINISTIC, 1)
+    g.upload(pos, r, idata)
+    g.init_internal_forces()
+    g.step(warmup)
+    g.step(nsteps)
+    ev_ms, _ = g.last_step_timing()
+    g.set_option(capi.OPT_PROFILE, 1)
+    g.step(min(nsteps, 20))
+    _, kms = g.last_step_timing()
+    g.close()
+    ms = ev_ms / nsteps
+    out = {"ms_per_step": round(ms, 5), "value": round(pos.shape[1] * 1e3 / ms, 1),
+           "unit": "particle-updates/s", "vs_default_time": round(ms / plain_ms, 3),
+           "kernels_ms": {"p2g_and_gather": round(kms[0], 5), "grid": round(kms[1], 5),
+                          "g2p_move_stress": round(kms[2], 5)},
+           "sort_ms": round(kms[3], 5)}
+    print(f"[bench] deterministic P2G: {ms:.4f} ms/step ({out['vs_default_time']}x the default "
+          f"step), p2g+gather {kms[0]:.4f} ms, sort {kms[3]:.4f} ms", file=sys.stderr, flush=True)
+    return out
 
 
 def host_cpu():

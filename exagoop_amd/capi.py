@@ -39,6 +39,7 @@ LS_TEMP_ADIABATIC, LS_TEMP_ISOTHERMAL, LS_TEMP_HEATFLUX, LS_TEMP_CONVECTIVE = 0,
 USL, MUSL = 0, 1
 
 OPT_PROFILE, OPT_KEEP_FORCES, OPT_SORTED, OPT_FAST, OPT_FUSED_DT = 1, 2, 3, 4, 5
+OPT_DETERMINISTIC = 6
 LS_SPHERE, LS_PLANE, LS_CYLINDER, LS_BOX = 0, 1, 2, 3
 MLP_NPARAMS = 11331
 MLP_IN = 128

@@ -51,6 +51,12 @@ struct mpm_ctx {
     double kernel_ms[4] = {0, 0, 0, 0};
     int profile = 0;
     int keep_forces = 1;
+    // Deterministic P2G (MPM_OPT_DETERMINISTIC): requested, and its buffers
+    // (allocated by the option, kept until destroy); det_sync hands them to
+    // S where the mode applies.
+    int det = 0;
+    double *det_blk = nullptr;
+    int64_t *det_slot = nullptr;
     // Learned-correction work buffers (allocated by its first call, nodal sized).
     int *ml_nodes = nullptr;      // active node list / selected rows
     int *ml_count = nullptr;      // [0] active count, [1] selected, [2] appended, [3] guard flag,
@@ -84,6 +90,15 @@ struct mpm_ctx {
 };
 
 namespace xmpm {
+
+// The deterministic P2G applies to the sorted mechanical step without
+// packets: not with the thermal module or external slab loads, whose
+// packets are written at the scatter's (atomic) positions.
+inline void det_sync(mpm_ctx *c) {
+    const bool on = c->det && !c->thermal && !c->ph.extloads;
+    c->S.blk = on ? c->det_blk : nullptr;
+    c->S.tile_slot = on ? c->det_slot : nullptr;
+}
 
 // Record the message of a failed call and return its status.
 int set_err(int code, const char *what);

@@ -324,6 +324,8 @@ void mpm_destroy(mpm_ctx *c) {
     (void)hipFree(c->S.scan_tmp);
     (void)hipFree(c->S.tile_epoch);
     (void)hipFree(c->S.tiles);
+    (void)hipFree(c->det_blk);
+    (void)hipFree(c->det_slot);
     (void)hipFree(c->ml_nodes);
     (void)hipFree(c->ml_count);
     (void)hipFree(c->ml_x);
@@ -838,6 +840,7 @@ int mpm_init_internal_forces(mpm_ctx *c) {
     // The sorted deposit (particles outside their cell go through the global
     // path) replaces the atomic one-thread-per-particle deposit when enabled.
     if (c->sorted && c->P.np > 0) {
+        det_sync(c);
         launch_sort(c->g, c->ph, c->P, c->S, nullptr, c->ph.extloads ? 1 : 0, c->stream);
         launch_p2g_tiles(c->g, c->ph, c->P, c->S, c->N, (int)c->steps, 0, 1, c->sc, c->stream);
     } else {
@@ -899,8 +902,10 @@ int mpm_step(mpm_ctx *c, int nsteps) {
         }
         const bool sorted = c->sorted != 0;
         if (prof) HIPCHK(hipEventRecord(c->ev[1], st));
-        if (sorted)
+        if (sorted) {
+            det_sync(c);
             launch_sort(c->g, c->ph, c->P, c->S, c->Q.id, c->ph.extloads ? 1 : 0, st, fold);
+        }
         if (prof) HIPCHK(hipEventRecord(c->ev[2], st));
         if (sorted) launch_p2g_tiles(c->g, c->ph, c->P, c->S, c->N, step, 0, nr > 1, c->sc, st);
         else launch_p2g(c->g, c->ph, c->P, c->N, 1, 1, 1, nullptr, st);
@@ -1110,6 +1115,23 @@ int mpm_set_option(mpm_ctx *c, int option, int value) {
     } else if (option == MPM_OPT_FUSED_DT) {
         c->fuse_dt = value;
         c->dt_next_valid = false;
+    } else if (option == MPM_OPT_DETERMINISTIC) {
+        if (value && !c->det_blk) {
+            // one node block per tile (7 components), and the tile stamps
+            // (-1: no pass)
+            const size_t nb = (size_t)c->S.ntiles * 7 * xmpm::kTNNodes;
+            if (hipMalloc(&c->det_blk, nb * sizeof(double)) != hipSuccess ||
+                hipMalloc(&c->det_slot, (size_t)c->S.ntiles * sizeof(int64_t)) != hipSuccess) {
+                (void)hipFree(c->det_blk);
+                c->det_blk = nullptr;
+                c->det_slot = nullptr;
+                return set_err(MPM_E_OOM, "deterministic P2G buffers");
+            }
+            HIPCHK(hipMemsetAsync(c->det_slot, 0xff, (size_t)c->S.ntiles * sizeof(int64_t),
+                                  c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+        }
+        c->det = value ? 1 : 0;
     }
     else return set_err(MPM_E_ARG, "unknown option");
     return MPM_OK;

@@ -93,6 +93,60 @@ __device__ __forceinline__ int targets2(int I, int n, int per, int &t0, int &t1)
     return (in ? 1 : 0) + (im >= 0 ? 1 : 0);
 }
 
+__device__ __forceinline__ int floor4(int x) { return x >= 0 ? x / 4 : -((3 - x) / 4); }
+
+// Block sources of node index I along one direction for the deterministic
+// P2G: the pairs (cell tile t, block offset a) whose block node, unwrapped
+// index G = kTileCells t - 1 + a, the tile flush sends to I (targets2): G = I
+// and, in a periodic direction, the images G = I - n and G = I + n; in that
+// order, then by increasing t.
+__device__ __forceinline__ int block_sources(int I, int n, int per, int ntc, int t[6], int a[6]) {
+    static_assert(kTileCells == 4, "floor4");
+    int c = 0;
+    for (int s = 0; s < 3; ++s) {
+        if (s != 1 && !per) continue;
+        const int G = I + (s - 1) * n;
+        int lo = floor4(G - 2), hi = floor4(G + 1);  // 4 t - 1 <= G <= 4 t + 5
+        lo = lo < 0 ? 0 : lo;
+        hi = hi > ntc - 1 ? ntc - 1 : hi;
+        for (int tt = lo; tt <= hi && c < 6; ++tt) {
+            t[c] = tt;
+            a[c] = G - 4 * tt + 1;
+            ++c;
+        }
+    }
+    return c;
+}
+
+// Sum of node (I, J, K)'s entries in the blocks of the tiles stamped with
+// bg.pass, in a fixed order (z, then y, then x sources): out[0..NC), NC =
+// bg.nc.  Returns whether any block contributed.
+template <int NC>
+__device__ __forceinline__ bool blk_gather(const GridP &g, const BlkGather &bg, int I, int J,
+                                           int K, double out[NC]) {
+    constexpr int kBN = kTileCells + 3;
+    int tx[6], ax[6], ty[6], ay[6], tz[6], az[6];
+    const int cx = block_sources(I, g.n[0], g.per[0], g.ntc[0], tx, ax);
+    const int cy = block_sources(J, g.n[1], g.per[1], g.ntc[1], ty, ay);
+    const int cz = block_sources(K, g.n[2], g.per[2], g.ntc[2], tz, az);
+#pragma unroll
+    for (int c = 0; c < NC; ++c) out[c] = 0.0;
+    bool any = false;
+    for (int iz = 0; iz < cz; ++iz)
+        for (int iy = 0; iy < cy; ++iy)
+            for (int ix = 0; ix < cx; ++ix) {
+                const int tile = (tz[iz] * g.ntc[1] + ty[iy]) * g.ntc[0] + tx[ix];
+                const int64_t e = bg.slot[tile];
+                if ((int)(e >> 32) != bg.pass) continue;
+                const double *b = bg.blk + (int64_t)(uint32_t)e * NC * kTNNodes +
+                                  (az[iz] * kBN + ay[iy]) * kBN + ax[ix];
+#pragma unroll
+                for (int c = 0; c < NC; ++c) out[c] += b[c * kTNNodes];
+                any = true;
+            }
+    return any;
+}
+
 // Node read by G2P for an unwrapped index: periodic ghost nodes hold the
 // value one period away (FillBoundary); non-periodic ghost nodes hold zero.
 __device__ __forceinline__ int wrap_read(int I, int n, int per, bool &ok) {

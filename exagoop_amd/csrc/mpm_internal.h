@@ -27,6 +27,7 @@ struct GridP {
 // kTileCells^3-cell tiles; a P2G workgroup owns one tile and accumulates
 // into an LDS block of (kTileCells + 3)^3 nodes (stencil offsets -1..+2).
 constexpr int kTileCells = 4;
+constexpr int kTNNodes = (kTileCells + 3) * (kTileCells + 3) * (kTileCells + 3);  // P2G node block
 
 // Step physics and wall data (MPMspecs subset).
 struct PhysP {
@@ -108,6 +109,16 @@ enum {
     P_STRESS_INIT = 64  // apply_constitutive_model(dt=0, applied rate) of the init pass
 };
 
+// The node blocks of a deterministic P2G pass (SortState::blk): nc
+// components per block (7 with forces, 3 momentum only), tiles stamped with
+// pass in slot.
+struct BlkGather {
+    const double *blk = nullptr;
+    const int64_t *slot = nullptr;
+    int pass = 0;
+    int nc = 0;
+};
+
 // Launchers (mpm_kernels.hip).
 void launch_zero(double *p, int64_t n, hipStream_t s);
 void launch_p2g(const GridP &g, const PhysP &ph, const PState &P, const NState &N, int um,
@@ -152,9 +163,19 @@ struct SortState {
     // fields through perm)
     int pkt_valid;
     // Non-empty cell tiles after the sort: tiles[0] = count, then the tile
-    // indices in increasing order (ntiles + 1 entries).
+    // indices (ntiles + 1 entries), in runs of increasing index whose order
+    // depends on scheduling.
     int *tiles;
     int ntiles;
+    // Deterministic P2G (MPM_OPT_DETERMINISTIC; blk null when off).  The
+    // sort orders each cell's particles by their current index, the tile
+    // kernel writes the node block of the tile at list position p whole to
+    // blk[(p * NC + c) * kTNN + e] (NC = 7 with forces, 3 without) and
+    // tile_slot[t] = (pass << 32) | p; a gather pass then sums every node's
+    // blocks in a fixed order.  pass counts the deterministic P2G launches.
+    double *blk = nullptr;
+    int64_t *tile_slot = nullptr;
+    int pass = 0;
 };
 constexpr int kPktComps = 16;
 size_t sort_scan_bytes(int nkeys);
@@ -177,8 +198,10 @@ void launch_sort(const GridP &g, const PhysP &ph, const PState &P, SortState &S,
                  int64_t *out_id, int packets, hipStream_t s, const DtFold &dt = DtFold());
 // fallback: particles may lie outside their tile (not redistributed every
 // step); they deposit straight to global memory.  Without it such a particle
-// sets Scalars::err.
-void launch_p2g_tiles(const GridP &g, const PhysP &ph, const PState &P, const SortState &S,
+// sets Scalars::err.  With S.blk set (deterministic P2G) the tiles write
+// their node blocks instead of adding them to N.acc, and a gather pass adds
+// each node's blocks to N.acc in a fixed order.
+void launch_p2g_tiles(const GridP &g, const PhysP &ph, const PState &P, SortState &S,
                       const NState &N, int epoch, int momentum_only, int fallback, Scalars *sc,
                       hipStream_t s);
 void launch_particle_sorted(const GridP &g, const PhysP &ph, const LsP &ls, const UdfP &udf,

@@ -147,6 +147,7 @@ int coarse_begin(mpm_ctx *c, int nfine, int advance = 0) {
         launch_dt(c->g, c->ph, c->P, c->sc, fixed_dt, c->s.timestep, st,
                   c->dt_next_valid ? 1 : 0, advance, (double)nfine);
     }
+    det_sync(c);
     launch_sort(c->g, c->ph, c->P, c->S, c->Q.id, c->ph.extloads ? 1 : 0, st, fold);
     launch_p2g_tiles(c->g, c->ph, c->P, c->S, c->N, step, 0, 0, c->sc, st);
     const int *te = (c->skip_ok && !c->all_dirty) ? c->S.tile_epoch : nullptr;

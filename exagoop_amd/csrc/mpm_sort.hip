@@ -85,15 +85,80 @@ __global__ __launch_bounds__(kBlock) void k_particle_sorted(GridP g, PhysP ph, L
     store_all(Pout, i, R);
 }
 
+// Deterministic sort: each cell's particles in increasing current index
+// (the scatter's ranks come from atomics).  A workgroup takes 256
+// consecutive cells, whose particles are one contiguous range of perm: the
+// range is staged in LDS with coalesced loads, each thread sorts its cell
+// there (insertion sort; cells hold a few particles) and the range is
+// written back coalesced.  A range longer than kOrderLds entries is sorted
+// in global memory, one thread per cell.
+constexpr int kOrderLds = 8192;
+__global__ __launch_bounds__(kBlock) void k_cell_order(SortState S) {
+    __shared__ int seg[kOrderLds];
+    const int c0 = blockIdx.x * kBlock;
+    const int c = c0 + threadIdx.x;
+    const int cend = min(c0 + kBlock, S.nkeys);
+    const int r0 = S.cell_start[c0], r1 = S.cell_start[cend];
+    const int b = c < S.nkeys ? S.cell_start[c] : r1;
+    const int e = c < S.nkeys ? S.cell_start[c + 1] : r1;
+    if (r1 - r0 > kOrderLds) {  // uniform over the block
+        for (int i = b + 1; i < e; ++i) {
+            const int x = S.perm[i];
+            int j = i - 1;
+            while (j >= b && S.perm[j] > x) {
+                S.perm[j + 1] = S.perm[j];
+                --j;
+            }
+            S.perm[j + 1] = x;
+        }
+        return;
+    }
+    for (int i = threadIdx.x; i < r1 - r0; i += kBlock) seg[i] = S.perm[r0 + i];
+    __syncthreads();
+    for (int i = b + 1 - r0; i < e - r0; ++i) {
+        const int x = seg[i];
+        int j = i - 1;
+        while (j >= b - r0 && seg[j] > x) {
+            seg[j + 1] = seg[j];
+            --j;
+        }
+        seg[j + 1] = x;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < r1 - r0; i += kBlock) S.perm[r0 + i] = seg[i];
+}
+
 // deposit_onto_grid_momentum over the non-empty tiles, one workgroup each
 // (see p2g_tile).
-template <int O, bool FORCES, bool FALLBACK, bool GATHER>
+template <int O, bool FORCES, bool FALLBACK, bool GATHER, bool DET>
 __global__ __launch_bounds__(kBlock) void k_p2g_cells(GridP g, PhysP ph, PState P, SortState S,
                                                       double *acc, int epoch, Scalars *sc) {
-    __shared__ P2gLds<FORCES ? 7 : 3> lds;
-    const int tile = active_tile(S, blockIdx.x);
-    if (tile < 0) return;  // uniform over the block
-    p2g_tile<O, FORCES, FALLBACK, GATHER>(g, ph, P, S, acc, epoch, &sc->err, tile, lds);
+    __shared__ P2gLds<FORCES ? 7 : 3, DET> lds;
+    const int pos = active_pos(S, blockIdx.x);
+    if (pos < 0) return;  // uniform over the block
+    p2g_tile<O, FORCES, FALLBACK, GATHER, DET>(g, ph, P, S, acc, epoch, &sc->err,
+                                               S.tiles[1 + pos], lds, pos, S.pass);
+}
+
+// The deterministic P2G's gather pass: node (I, J, K) adds its blocks
+// (blk_gather, a fixed order) to the accumulators, which hold only the
+// FALLBACK path's global deposits.  Node tiles the tile flush did not mark
+// in this epoch received only zeros.
+template <int NC>
+__global__ __launch_bounds__(kBlock) void k_p2g_gather(GridP g, BlkGather bg,
+                                                       const int *tile_epoch, int epoch,
+                                                       double *acc) {
+    const int64_t node = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (node >= g.nn) return;
+    const int I = (int)(node % (g.n[0] + 1));
+    const int J = (int)((node / (g.n[0] + 1)) % (g.n[1] + 1));
+    const int K = (int)(node / ((int64_t)(g.n[0] + 1) * (g.n[1] + 1)));
+    if (tile_epoch[node_tile(g, I, J, K)] != epoch) return;
+    double s[NC];
+    if (!blk_gather<NC>(g, bg, I, J, K, s)) return;
+    const int64_t nn = g.nn;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) acc[(NC == 7 ? c : c + 1) * nn + node] += s[c];
 }
 
 // Fused particle pipeline over the non-empty tiles, one workgroup each (see
@@ -134,21 +199,30 @@ void launch_sort(const GridP &g, const PhysP &ph, const PState &P, SortState &S,
     // the scatter launch also builds the non-empty tile list
     const unsigned nb = std::max(blocks_for(P.np), (unsigned)((S.ntiles + kBlock - 1) / kBlock));
     hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(kBlock), 0, s, P, S, out_id, ph, packets);
+    // deterministic P2G: a fixed particle order in each cell (the packets,
+    // written at the scatter's positions, are not reordered: no packets then)
+    if (S.blk && !packets)
+        hipLaunchKernelGGL(k_cell_order, dim3(blocks_for(S.nkeys)), dim3(kBlock), 0, s, S);
 }
 
-void launch_p2g_tiles(const GridP &g, const PhysP &ph, const PState &P, const SortState &S,
+void launch_p2g_tiles(const GridP &g, const PhysP &ph, const PState &P, SortState &S,
                       const NState &N, int epoch, int momentum_only, int fallback, Scalars *sc,
                       hipStream_t s) {
     if (P.np <= 0) return;
     const dim3 gr((unsigned)((S.ntiles + 7) / 8 * 8)), bl(kBlock);
+    const bool det = S.blk && !S.pkt_valid;
+    if (det) ++S.pass;
 #define XMPM_PG(o, f, fb)                                                                     \
     do {                                                                                      \
-        if (f && !S.pkt_valid)                                                                \
-            hipLaunchKernelGGL((k_p2g_cells<o, f, fb, true>), gr, bl, 0, s, g, ph, P, S, N.acc, \
-                               epoch, sc);                                                    \
-        else                                                                                  \
-            hipLaunchKernelGGL((k_p2g_cells<o, f, fb, false>), gr, bl, 0, s, g, ph, P, S,      \
+        if (det)                                                                              \
+            hipLaunchKernelGGL((k_p2g_cells<o, f, fb, f, true>), gr, bl, 0, s, g, ph, P, S,    \
                                N.acc, epoch, sc);                                             \
+        else if (f && !S.pkt_valid)                                                           \
+            hipLaunchKernelGGL((k_p2g_cells<o, f, fb, true, false>), gr, bl, 0, s, g, ph, P, S, \
+                               N.acc, epoch, sc);                                             \
+        else                                                                                  \
+            hipLaunchKernelGGL((k_p2g_cells<o, f, fb, false, false>), gr, bl, 0, s, g, ph, P,  \
+                               S, N.acc, epoch, sc);                                          \
     } while (0)
 #define XMPM_PGO(f, fb)                                                                       \
     do {                                                                                      \
@@ -164,6 +238,19 @@ void launch_p2g_tiles(const GridP &g, const PhysP &ph, const PState &P, const So
     }
 #undef XMPM_PGO
 #undef XMPM_PG
+    if (det) {
+        BlkGather bg;
+        bg.blk = S.blk;
+        bg.slot = S.tile_slot;
+        bg.pass = S.pass;
+        bg.nc = momentum_only ? 3 : 7;
+        if (momentum_only)
+            hipLaunchKernelGGL(k_p2g_gather<3>, dim3(blocks_for(g.nn)), dim3(kBlock), 0, s, g, bg,
+                               S.tile_epoch, epoch, N.acc);
+        else
+            hipLaunchKernelGGL(k_p2g_gather<7>, dim3(blocks_for(g.nn)), dim3(kBlock), 0, s, g, bg,
+                               S.tile_epoch, epoch, N.acc);
+    }
 }
 
 void launch_particle_sorted(const GridP &g, const PhysP &ph, const LsP &ls, const UdfP &udf,

@@ -37,6 +37,7 @@ constexpr int kTNN = kTN * kTN * kTN;                  // LDS nodes per tile
 constexpr int kCellsPerTile = kTileCells * kTileCells * kTileCells;
 
 static_assert(kTileCells == 4 && kBlock == 256, "wavefront/cell mapping assumes 4^3 tiles");
+static_assert(kTNN == kTNNodes, "node block size");
 
 // Blocks are dispatched round-robin over the 8 XCDs; give each XCD a
 // contiguous range of tiles so neighbouring tiles (which share halo nodes
@@ -59,6 +60,16 @@ __device__ __forceinline__ int active_tile(const SortState &S, int b) {
     if (k >= per) return -1;
     const int p = x * per + k;
     return p < na ? S.tiles[1 + p] : -1;
+}
+
+// The list position active_tile maps workgroup b to (-1: none).
+__device__ __forceinline__ int active_pos(const SortState &S, int b) {
+    const int na = S.tiles[0];
+    const int per = (na + 7) >> 3;
+    const int x = b & 7, k = b >> 3;
+    if (k >= per) return -1;
+    const int p = x * per + k;
+    return p < na ? p : -1;
 }
 
 // Accesses to the particle streams of the fused particle pass, which touches
@@ -335,10 +346,14 @@ struct StageRaw {
 // outside its cell (not redistributed this step) deposits through
 // deposit_global; without it such a particle raises the error flag.
 
-// LDS of one P2G tile (NC accumulated components).
-template <int NC>
+// LDS of one P2G tile (NC accumulated components).  DET false: the tile's
+// node block, shared by the wavefronts (LDS atomic adds).  DET true: one
+// block of kZN nodes per wavefront (its cells' four z-levels), plain adds,
+// summed in wavefront order at the flush.
+constexpr int kZN = kTN * kTN * 4;
+template <int NC, bool DET = false>
 struct P2gLds {
-    double slab[NC * kTNN];      // the tile's node block, shared by the wavefronts
+    double slab[DET ? 4 * NC * kZN : NC * kTNN];
     double pk[4][kPkArea];       // staging area per wavefront
     int cs[kCellsPerTile + 1];
     SplineTables tab;
@@ -351,11 +366,16 @@ struct PartLds {
     SplineTables tab;
 };
 
-template <int O, bool FORCES, bool FALLBACK, bool GATHER = false>
+// DET: the deterministic form (SortState::blk): private wavefront slabs and
+// the whole node block written to list position pos of S.blk, stamped with
+// pass in S.tile_slot; no atomics (but those of the FALLBACK global path).
+template <int O, bool FORCES, bool FALLBACK, bool GATHER = false, bool DET = false>
 __device__ __forceinline__ void p2g_tile(const GridP &g, const PhysP &ph, const PState &P,
                                          const SortState &S, double *acc, int epoch, int *err,
-                                         int tile, P2gLds<FORCES ? 7 : 3> &L) {
+                                         int tile, P2gLds<FORCES ? 7 : 3, DET> &L, int pos = 0,
+                                         int pass = 0) {
     constexpr int NC = FORCES ? 7 : 3;
+    constexpr int kSlabN = DET ? 4 * NC * kZN : NC * kTNN;
     auto &slab = L.slab;
     auto &pk = L.pk;
     auto &cs = L.cs;
@@ -369,7 +389,7 @@ __device__ __forceinline__ void p2g_tile(const GridP &g, const PhysP &ph, const 
         for (int e = tid; e < (int)(sizeof(SplineTables) / sizeof(int)); e += kBlock)
             dst[e] = src[e];
     }
-    for (int e = tid; e < NC * kTNN; e += kBlock) slab[e] = 0.0;
+    for (int e = tid; e < kSlabN; e += kBlock) slab[e] = 0.0;
     __syncthreads();
     if (cs[kCellsPerTile] == cs[0]) return;  // empty tile (uniform over the block)
     const int wave = tid >> 6, lane = tid & 63;
@@ -628,16 +648,33 @@ __device__ __forceinline__ void p2g_tile(const GridP &g, const PhysP &ph, const 
         asm volatile("" ::: "memory");  // reads of this round before the next writes
         if (nry != ry) {
             // Row done: lane (q, ky, kz) adds its four nodes (x = q + k,
-            // y = ry + ky, z = cz + kz); the other wavefronts' slices reach the
-            // same z-levels, hence the atomics.
+            // y = ry + ky, z = cz + kz).  The other wavefronts' slices reach
+            // the same z-levels: atomics into the shared block, or (DET) into
+            // this wavefront's own slab (z-level kz).  Lanes q and q + 1 share
+            // three nodes, so even the private slab takes atomic adds (plain
+            // read-add-writes of one lane may be batched past another lane's);
+            // one wavefront's LDS instructions run in issue order and, for one
+            // k, its lanes' nodes are distinct, so the order of the adds to a
+            // node is fixed by the code.
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
-                const int idx = ((cz + kz) * kTN + (ry + ky)) * kTN + (q + k);
+                if (DET) {
+                    double *sw = slab + wave * NC * kZN;
+                    const int idx = (kz * kTN + (ry + ky)) * kTN + (q + k);
 #pragma unroll
-                for (int c = 0; c < NC; ++c) {
-                    __hip_atomic_fetch_add(&slab[c * kTNN + idx], a4[k][c], __ATOMIC_RELAXED,
-                                           __HIP_MEMORY_SCOPE_WORKGROUP);
-                    a4[k][c] = 0.0;
+                    for (int c = 0; c < NC; ++c) {
+                        __hip_atomic_fetch_add(&sw[c * kZN + idx], a4[k][c], __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_WAVEFRONT);
+                        a4[k][c] = 0.0;
+                    }
+                } else {
+                    const int idx = ((cz + kz) * kTN + (ry + ky)) * kTN + (q + k);
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) {
+                        __hip_atomic_fetch_add(&slab[c * kTNN + idx], a4[k][c], __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_WORKGROUP);
+                        a4[k][c] = 0.0;
+                    }
                 }
             }
         }
@@ -648,12 +685,25 @@ __device__ __forceinline__ void p2g_tile(const GridP &g, const PhysP &ph, const 
     __syncthreads();
     // Flush the node block: one global atomic per non-zero node component;
     // periodic image targets as in deposit_global; mark the node tiles that
-    // received data.
+    // received data.  DET: the wavefront slabs summed in wavefront order,
+    // the whole block stored at list position pos (the gather pass adds it
+    // to the nodes), the node tiles marked alike.
     const int64_t nn = g.nn;
+    double *blk = DET ? S.blk + (int64_t)pos * NC * kTNN : nullptr;
+    if (DET && tid == 0) S.tile_slot[tile] = ((int64_t)pass << 32) | (uint32_t)pos;
     for (int e = tid; e < kTNN; e += kBlock) {
         const int ax = e % kTN, ay = (e / kTN) % kTN, az = e / (kTN * kTN);
         double v[NC];
-        for (int c = 0; c < NC; ++c) v[c] = slab[c * kTNN + e];
+        if (DET) {
+            for (int c = 0; c < NC; ++c) v[c] = 0.0;
+            const int w0 = az > 3 ? az - 3 : 0, w1 = az < 3 ? az : 3;
+            for (int w = w0; w <= w1; ++w)
+                for (int c = 0; c < NC; ++c)
+                    v[c] += slab[w * NC * kZN + c * kZN + ((az - w) * kTN + ay) * kTN + ax];
+            for (int c = 0; c < NC; ++c) blk[c * kTNN + e] = v[c];
+        } else {
+            for (int c = 0; c < NC; ++c) v[c] = slab[c * kTNN + e];
+        }
         bool any = false;
         for (int c = 0; c < NC; ++c) any |= (v[c] != 0.0);
         if (!any) continue;
@@ -669,10 +719,12 @@ __device__ __forceinline__ void p2g_tile(const GridP &g, const PhysP &ph, const 
                 for (int qx = 0; qx < 2; ++qx) {
                     if (qz >= cz2 || qy >= cy || qx >= cx) continue;
                     const int I = qx ? tx1 : tx0, J = qy ? ty1 : ty0, K = qz ? tz1 : tz0;
-                    const int64_t node = nidx(g, I, J, K);
-                    for (int c = 0; c < NC; ++c)
-                        if (v[c] != 0.0)
-                            unsafeAtomicAdd(&acc[(FORCES ? c : c + 1) * nn + node], v[c]);
+                    if (!DET) {
+                        const int64_t node = nidx(g, I, J, K);
+                        for (int c = 0; c < NC; ++c)
+                            if (v[c] != 0.0)
+                                unsafeAtomicAdd(&acc[(FORCES ? c : c + 1) * nn + node], v[c]);
+                    }
                     S.tile_epoch[node_tile(g, I, J, K)] = epoch;
                 }
     }

@@ -366,12 +366,24 @@ int mpm_energies_noalloc(int64_t np, int64_t stride, const double *rdata, const 
  * specialised particle kernel where it applies (default 1) or the generic
  * one (0); MPM_OPT_FUSED_DT lets the specialised particle kernel reduce the
  * next step's dt candidates on its way out (default 1) instead of a separate
- * pass over the particles at the top of the next step (0). */
+ * pass over the particles at the top of the next step (0);
+ * MPM_OPT_DETERMINISTIC (default 0) makes the sorted step's P2G free of
+ * floating-point atomics, so that equal inputs give equal bits run to run:
+ * each cell's particles are taken in a fixed order, the four wavefronts of a
+ * tile sum private slabs in a fixed order, and each node adds the node blocks
+ * of its tiles in a fixed order (an extra pass; the reference deposits with
+ * order-dependent atomics, mpm_particle_grid_ops.cpp:484-578).  It applies
+ * without the thermal module and external slab loads, to particles inside
+ * their sorted tile (redistributed every step, or not yet outside the
+ * domain), and not to MUSL's re-projection deposit with periodic faces (the
+ * images' start-of-step velocities go through the atomic per-particle
+ * deposit); it allocates one 7 x 343-double node block per cell tile. */
 #define MPM_OPT_PROFILE 1
 #define MPM_OPT_KEEP_FORCES 2
 #define MPM_OPT_SORTED 3
 #define MPM_OPT_FAST 4
 #define MPM_OPT_FUSED_DT 5
+#define MPM_OPT_DETERMINISTIC 6
 int mpm_set_option(mpm_ctx *ctx, int option, int value);
 
 /* Last HIP error string (static storage). */

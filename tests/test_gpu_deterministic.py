@@ -1,0 +1,121 @@
+"""The deterministic sorted step (MPM_OPT_DETERMINISTIC) on the GPU.
+
+The reference deposits with floating-point atomics (mpm_particle_grid_ops.cpp:
+484-578), so its GPU results depend on scheduling.  With the option on, the
+P2G takes each cell's particles in a fixed order, sums the wavefronts' slabs
+in a fixed order and adds each node's tile blocks in a fixed order: two runs
+on the same input give the same bits.  The option changes only the order of
+FP64 sums, so it meets the same parity bars against the oracle as the default
+step: nodal state and particles <= 1e-12 after one step, particles <= 1e-10
+after 40 steps (USL and MUSL, orders 2 and 3, walls and periodic faces, whose
+image deposits go through the gather's image sources).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from exagoop_amd import capi, scenes
+from exagoop_amd.capi import (BC_NOSLIP, BC_PARTIALSLIP, BC_SLIP, MUSL, N_DVEL, N_FRC, N_MASS,
+                              N_MASS_OLD, N_VEL, OPT_DETERMINISTIC, USL)
+
+pytestmark = pytest.mark.gpu
+
+NODAL = (N_MASS, N_VEL, N_VEL + 1, N_VEL + 2, N_FRC, N_FRC + 1, N_FRC + 2, N_DVEL, N_DVEL + 1,
+         N_DVEL + 2, N_MASS_OLD)
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def rel(a, b):
+    a = np.asarray(a, dtype=float).ravel()
+    b = np.asarray(b, dtype=float).ravel()
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a) if nb == 0.0 else np.linalg.norm(a - b) / nb
+
+
+CASES = [
+    ("walls_usl_o3", dict(order=3, frac=0.9, speed=1.5,
+                          bcs=[(BC_NOSLIP, BC_SLIP), (BC_PARTIALSLIP, BC_SLIP),
+                               (BC_SLIP, BC_NOSLIP)])),
+    ("walls_musl_o2", dict(order=2, frac=0.9, speed=1.5, scheme=MUSL,
+                           bcs=[(BC_SLIP, BC_SLIP), (BC_NOSLIP, BC_SLIP), (BC_SLIP, BC_NOSLIP)])),
+    ("periodic_x_z_usl_o3", dict(order=3, periodic=(1, 0, 1), speed=0.5)),
+    ("periodic_all_musl_o3", dict(order=3, periodic=(1, 1, 1), speed=0.5, scheme=MUSL)),
+    ("periodic_y_usl_o2", dict(order=2, periodic=(0, 1, 0), speed=0.5)),
+    ("fluid_usl_o3", dict(order=3, model=1, frac=0.8, speed=0.5,
+                          bcs=[(BC_NOSLIP, BC_SLIP), (BC_SLIP, BC_NOSLIP),
+                               (BC_NOSLIP, BC_NOSLIP)])),
+]
+
+
+def det_context(s, pos, r, idata):
+    g = capi.Context(s)
+    g.set_option(OPT_DETERMINISTIC, 1)
+    g.upload(pos, r, idata)
+    return g
+
+
+@pytest.mark.parametrize("name,kw", CASES, ids=[c[0] for c in CASES])
+def test_deterministic_step_matches_oracle(gpu, name, kw):
+    say(f"[deterministic] {name}: 1 and 40 steps vs oracle")
+    s, pos, r, idata = scenes.random_block(29, **kw)
+    g, o = det_context(s, pos, r, idata), oracle.Oracle(s)
+    o.upload(pos, r, idata)
+    for c in (g, o):
+        c.init_internal_forces()
+        c.step(1)
+    bad = {c: rel(g.nodal(c), o.nodal(c)) for c in NODAL if not rel(g.nodal(c), o.nodal(c)) <= 1e-12}
+    assert not bad, bad
+    for c in (g, o):
+        c.step(39)
+    pg, rg, ig, idg = g.download()
+    po, ro, io, ido = o.download()
+    assert np.array_equal(idg, ido) and np.array_equal(ig, io)
+    assert rel(pg, po) <= 1e-10, rel(pg, po)
+    bad = {f: rel(rg[f], ro[f]) for f in range(rg.shape[0]) if not rel(rg[f], ro[f]) <= 1e-10}
+    assert not bad, bad
+
+
+def run_bits(s, pos, r, idata, nsteps):
+    g = det_context(s, pos, r, idata)
+    g.init_internal_forces()
+    g.step(nsteps)
+    p, rr, ii, ids = g.download()
+    nodal = np.stack([g.nodal(c) for c in NODAL])
+    t = g.time()
+    g.close()
+    return p, rr, ii, ids, nodal, t
+
+
+BITWISE = [CASES[0], CASES[1], CASES[2], CASES[4]]
+
+
+@pytest.mark.parametrize("name,kw", BITWISE, ids=[c[0] for c in BITWISE])
+def test_deterministic_step_is_bitwise_reproducible(gpu, name, kw):
+    """Two runs of 60 steps on the same input: byte-identical particles and
+    nodal state.  (MUSL with periodic faces is not among them: its
+    re-projection deposit gives the periodic images their start-of-step
+    velocities through the atomic one-thread-per-particle path.)"""
+    say(f"[deterministic] {name}: two runs, bitwise")
+    s, pos, r, idata = scenes.random_block(31, **kw)
+    a = run_bits(s, pos, r, idata, 60)
+    b = run_bits(s, pos, r, idata, 60)
+    for x, y in zip(a[:5], b[:5]):
+        assert x.tobytes() == y.tobytes()
+    assert a[5][:2] == b[5][:2]
+
+
+def test_deterministic_dam_break_is_bitwise_reproducible(gpu):
+    """The benchmark's dam break at half resolution (65 536 particles), two
+    runs of 100 steps: byte-identical particles and nodal state."""
+    say("[deterministic] dam break 64x32x32: two runs of 100 steps, bitwise")
+    s, pos, r, idata = scenes.dam_break(nx=64, ny=32, nz=32)
+    a = run_bits(s, pos, r, idata, 100)
+    b = run_bits(s, pos, r, idata, 100)
+    for x, y in zip(a[:5], b[:5]):
+        assert x.tobytes() == y.tobytes()
