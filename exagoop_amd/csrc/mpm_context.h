@@ -23,6 +23,7 @@ struct mpm_ctx {
     UdfP udf;
     PState P;      // current particle buffers
     PState Q;      // second buffer the sorted step writes into
+    PState R;      // third buffer: the learned step's speculative next step writes into it
     NState N;
     SortState S;
     // Cold particle fields (never read by the step), by upload index.
@@ -65,7 +66,9 @@ struct mpm_ctx {
     float *ml_y = nullptr;        // network outputs [row][3]
     double *ml_vc = nullptr;      // coarse nodal velocity of selected rows [row][3]
     double *ml_guard = nullptr;   // guard sums
-    int *ml_hflag = nullptr;      // pinned host copy of the guard flag
+    int *ml_hflag = nullptr;      // pinned host copies of the guard flag, by step parity
+    Scalars *ml_scbak = nullptr;  // step scalars before a speculative learned step
+    hipEvent_t ml_ev[2] = {nullptr, nullptr};  // a step's flag copy done, by step parity
     int64_t ml_cap = 0;
     // Thermal module (mpm_enable_thermal): particle fields in slot order,
     // MPM_NTHERMAL * th_cap each (th_alt: the sorted step's output), nodal

@@ -143,6 +143,7 @@ void free_particles(mpm_ctx *c) {
     c->P.vsnap = nullptr;
     free_pstate(c->P);
     free_pstate(c->Q);
+    free_pstate(c->R);
     (void)hipFree(c->S.key);
     (void)hipFree(c->S.rank);
     (void)hipFree(c->S.perm);
@@ -333,6 +334,9 @@ void mpm_destroy(mpm_ctx *c) {
     (void)hipFree(c->ml_vc);
     (void)hipFree(c->ml_guard);
     if (c->ml_hflag) (void)hipHostFree(c->ml_hflag);
+    (void)hipFree(c->ml_scbak);
+    for (int q = 0; q < 2; ++q)
+        if (c->ml_ev[q]) (void)hipEventDestroy(c->ml_ev[q]);
     for (double *p : c->ls_bufs) (void)hipFree(p);
     for (int d = 0; d < 3; ++d)
         for (int h = 0; h < 2; ++h) (void)hipFree(c->udf_dev[d][h]);

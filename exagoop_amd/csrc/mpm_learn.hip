@@ -106,7 +106,30 @@ int ctx_ml_buffers(mpm_ctx *c) {
     // the per-step decision is read into pinned memory: a copy into pageable
     // memory goes through a staging buffer
     if (!c->ml_hflag) HIPCHK(hipHostMalloc(&c->ml_hflag, 4 * sizeof(int), hipHostMallocDefault));
+    if (!c->ml_scbak) HIPCHK(hipMalloc(&c->ml_scbak, sizeof(Scalars)));
+    for (int q = 0; q < 2; ++q)
+        if (!c->ml_ev[q]) HIPCHK(hipEventCreateWithFlags(&c->ml_ev[q], hipEventDisableTiming));
     c->ml_cap = pad;
+    return MPM_OK;
+}
+
+// The third particle buffer of the speculative learned step (the size of P;
+// freed and re-made with the particles).
+int ctx_spec_buffer(mpm_ctx *c) {
+    if (c->R.pos && c->R.cap == c->P.cap) return MPM_OK;
+    (void)hipFree(c->R.pos);
+    (void)hipFree(c->R.r);
+    (void)hipFree(c->R.idata);
+    (void)hipFree(c->R.id);
+    c->R = PState();
+    const int64_t cap = c->P.cap;
+    HIPCHK(hipMalloc(&c->R.pos, 3 * cap * sizeof(double)));
+    HIPCHK(hipMalloc(&c->R.r, (int64_t)MPM_NREAL * cap * sizeof(double)));
+    HIPCHK(hipMalloc(&c->R.idata, (int64_t)MPM_NINT * cap * sizeof(int)));
+    HIPCHK(hipMalloc(&c->R.id, cap * sizeof(int64_t)));
+    c->R.cap = cap;
+    c->R.np = c->P.np;
+    c->R.vsnap = c->P.vsnap;
     return MPM_OK;
 }
 
@@ -130,13 +153,16 @@ int learned_ok(const mpm_ctx *c) {
 // nodal update + BCs.  Leaves the step counter advanced.  advance: the
 // previous step's time advance is still due (it is folded into the dt
 // launch).
-int coarse_begin(mpm_ctx *c, int nfine, int advance = 0) {
+int coarse_begin(mpm_ctx *c, int nfine, int advance = 0, const PState *in = nullptr,
+                 const PState *out = nullptr) {
     hipStream_t st = c->stream;
+    const PState &Pin = in ? *in : c->P;
+    const PState &Pout = out ? *out : c->Q;
     c->steps++;
     const int step = (int)c->steps;
     const bool fixed_dt = c->s.fixed_timestep == 1;
     DtFold fold;  // the dt in the sort's first launch when it needs no reduction
-    if (c->P.np > 0 && (fixed_dt || c->dt_next_valid)) {
+    if (Pin.np > 0 && (fixed_dt || c->dt_next_valid)) {
         fold.on = 1;
         fold.fixed = fixed_dt ? 1 : 0;
         fold.fixed_dt = c->s.timestep;
@@ -144,12 +170,12 @@ int coarse_begin(mpm_ctx *c, int nfine, int advance = 0) {
         fold.scale = (double)nfine;
         fold.sc = c->sc;
     } else {
-        launch_dt(c->g, c->ph, c->P, c->sc, fixed_dt, c->s.timestep, st,
+        launch_dt(c->g, c->ph, Pin, c->sc, fixed_dt, c->s.timestep, st,
                   c->dt_next_valid ? 1 : 0, advance, (double)nfine);
     }
     det_sync(c);
-    launch_sort(c->g, c->ph, c->P, c->S, c->Q.id, c->ph.extloads ? 1 : 0, st, fold);
-    launch_p2g_tiles(c->g, c->ph, c->P, c->S, c->N, step, 0, 0, c->sc, st);
+    launch_sort(c->g, c->ph, Pin, c->S, Pout.id, c->ph.extloads ? 1 : 0, st, fold);
+    launch_p2g_tiles(c->g, c->ph, Pin, c->S, c->N, step, 0, 0, c->sc, st);
     const int *te = (c->skip_ok && !c->all_dirty) ? c->S.tile_epoch : nullptr;
     launch_grid(c->g, c->ph, c->ls, c->udf, c->N, grid_flags_full(c), 0.0, c->sc, st, te, step);
     c->all_dirty = false;
@@ -603,18 +629,24 @@ int mpm_step_learned(mpm_ctx *c, mpm_mlp *m, int ncoarse, int nfine, const doubl
     hipLaunchKernelGGL(k_zero_d, dim3(1), dim3(64), 0, st, c->ml_guard, 8);  // once per call
     int fb = 0;
     bool pending = false;  // an accepted step's time advance is still due
-    for (int it = 0; it < ncoarse; ++it) {
-        if ((rc = coarse_begin(c, nfine, pending ? 1 : 0))) return rc;
-        pending = false;
-        if (!m) {  // uncorrected coarse step: nothing to decide, no round trip
+    if (!m) {  // uncorrected coarse steps: nothing to decide, no round trip
+        for (int it = 0; it < ncoarse; ++it) {
+            if ((rc = coarse_begin(c, nfine, pending ? 1 : 0))) return rc;
+            pending = false;
             if ((rc = coarse_finish(c))) return rc;
-            continue;
         }
-        // Correction, guard and the particle kernel (into the spare buffer)
-        // are enqueued back to back; one round trip reads the decision.  A
-        // rejected step leaves the particles in P untouched.  The list
-        // launch clears the other parity's count for the next step, and the
-        // decision clears the guard sums.
+        if (fallbacks) *fallbacks = 0;
+        return finish_call(c);
+    }
+    if ((rc = ctx_spec_buffer(c))) return rc;
+    // One coarse step from particles `in` into `out`: correction, guard and
+    // the particle kernel enqueued back to back, then the decision copied to
+    // the host slot of the step's parity with an event behind it.  A rejected
+    // step leaves `in` untouched.  The list launch clears the other parity's
+    // count for the next step, and the decision clears the guard sums.
+    auto enqueue = [&](int it, int advance, const PState &in, const PState &out) -> int {
+        int r = coarse_begin(c, nfine, advance, &in, &out);
+        if (r) return r;
         int *cnt = slot[it & 1];
         launch_active_nodes_rolling(c->g, c->N, c->ph.mass_tol, c->ml_nodes, cnt,
                                     slot[(it + 1) & 1], st);
@@ -624,19 +656,64 @@ int mpm_step_learned(mpm_ctx *c, mpm_mlp *m, int ncoarse, int nfine, const doubl
                            guards[0], guards[1], flag);
         hipLaunchKernelGGL(k_apply_dev, dim3(512), dim3(256), 0, st, c->ml_nodes, cnt, c->ml_y,
                            m->sc.dv_ref, c->N, c->g.nn, flag);
-        launch_particle_fast(c->g, c->ph, c->P, c->Q, c->S, c->N, 1, 1, c->sc, st);
-        if ((rc = check_launch())) return rc;
-        HIPCHK(hipMemcpyAsync(c->ml_hflag, flag, sizeof(int), hipMemcpyDeviceToHost, st));
-        HIPCHK(hipStreamSynchronize(st));
-        if (c->ml_hflag[0]) {
-            std::swap(c->P, c->Q);
+        launch_particle_fast(c->g, c->ph, in, out, c->S, c->N, 1, 1, c->sc, st);
+        if ((r = check_launch())) return r;
+        HIPCHK(hipMemcpyAsync(c->ml_hflag + (it & 1), flag, sizeof(int), hipMemcpyDeviceToHost,
+                              st));
+        HIPCHK(hipEventRecord(c->ml_ev[it & 1], st));
+        return MPM_OK;
+    };
+    // Speculation: step it + 1 (from Q into R) is enqueued as if step it
+    // (from P into Q) were accepted, before its decision is read, so the GPU
+    // does not idle across the round trip.  Accepted: the buffers rotate and
+    // step it + 1 is in flight.  Rejected: the speculative step is drained
+    // and undone (the step scalars from the copy taken before its dt launch,
+    // the step counter, the active counts; P, its input of record, was not
+    // written), and the plain fine steps run from P.
+    bool inflight = false;  // step it is enqueued (from P into Q)
+    for (int it = 0; it < ncoarse; ++it) {
+        if (!inflight) {
+            if ((rc = enqueue(it, pending ? 1 : 0, c->P, c->Q))) return rc;
+            pending = false;
+        }
+        const bool spec = it + 1 < ncoarse;
+        if (spec) {
+            HIPCHK(hipMemcpyAsync(c->ml_scbak, c->sc, sizeof(Scalars), hipMemcpyDeviceToDevice,
+                                  st));
+            c->dt_next_valid = c->fuse_dt != 0;  // as after an accepted step
+            if ((rc = enqueue(it + 1, 1, c->Q, c->R))) return rc;
+        }
+        HIPCHK(hipEventSynchronize(c->ml_ev[it & 1]));
+        if (c->ml_hflag[it & 1]) {
             c->dt_next_valid = c->fuse_dt != 0;
             c->dirty = false;
-            pending = true;
+            if (spec) {
+                // step it + 1 reads the new P and writes the new Q
+                const PState p0 = c->P;
+                c->P = c->Q;
+                c->Q = c->R;
+                c->R = p0;
+                inflight = true;
+            } else {
+                std::swap(c->P, c->Q);
+                pending = true;
+                inflight = false;
+            }
         } else {
+            if (spec) {
+                HIPCHK(hipStreamSynchronize(st));
+                HIPCHK(hipMemcpyAsync(c->sc, c->ml_scbak, sizeof(Scalars),
+                                      hipMemcpyDeviceToDevice, st));
+                hipLaunchKernelGGL(k_zero_f, dim3(1), dim3(64), 0, st,
+                                   reinterpret_cast<float *>(slot[0]), 1);
+                hipLaunchKernelGGL(k_zero_f, dim3(1), dim3(64), 0, st,
+                                   reinterpret_cast<float *>(slot[1]), 1);
+                c->steps--;  // the speculative step's coarse_begin
+            }
             coarse_drop(c);  // also drops the dt candidates of the rejected kernel
             ++fb;
             if ((rc = mpm_step(c, nfine))) return rc;
+            inflight = false;
         }
     }
     if (pending) launch_advance_time(c->sc, st);

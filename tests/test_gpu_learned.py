@@ -270,6 +270,46 @@ def test_learned_step_applies_features_then_forward(gpu):
         g.close()
 
 
+def wild_net():
+    p = np.zeros(NP, np.float32)
+    p[-3:] = [50.0, -40.0, 30.0]  # b3: a large constant correction, always rejected
+    return capi.MLP(p, scales=(1.0, 1.0, 1.0))
+
+
+@pytest.mark.parametrize("case", ["trained_accepts", "wild_rejects", "tight_guard"])
+def test_speculative_steps_equal_one_step_calls(gpu, case):
+    """Within one call the learned step enqueues step k + 1 before reading
+    step k's decision and undoes it when step k is rejected.  One call of 8
+    steps must give the bits of 8 calls of one step (no speculation across
+    calls), with the same fallbacks; the deterministic P2G makes the runs
+    bitwise comparable."""
+    from exagoop_amd import learned
+    say(f"[learned] speculative steps vs one-step calls: {case}")
+    net = {"trained_accepts": learned.load_net, "wild_rejects": wild_net,
+           "tight_guard": learned.load_net}[case]()
+    guards = (1e-3, 0.02) if case == "tight_guard" else (0.05, 0.5)
+    runs = []
+    for calls in ((8,), (1,) * 8):
+        g = learned.context(1007)
+        g.set_option(capi.OPT_DETERMINISTIC, 1)
+        g.step(20)
+        fb = sum(capi.step_learned(g, net, n, learned.NFINE, guards) for n in calls)
+        p, r, i, ids = g.download()
+        runs.append((fb, p, r, ids, g.time()))
+        g.close()
+    (fa, pa, ra, ida, ta), (fb, pb, rb, idb, tb) = runs
+    say(f"    fallbacks {fa} (one call) {fb} (one-step calls)")
+    if case == "trained_accepts":
+        assert fa == 0
+    if case == "wild_rejects":
+        assert fa == 8
+    assert fa == fb
+    assert np.array_equal(ida, idb)
+    assert pa.tobytes() == pb.tobytes()
+    assert ra.tobytes() == rb.tobytes()
+    assert ta[:2] == tb[:2]
+
+
 def test_trained_network_gains_over_the_uncorrected_coarse_step(gpu):
     """The committed network (exagoop_amd/data/learned_dam_break.npz, trained
     by exagoop_amd.learned on seeds 0..63) on a held-out dam-break geometry:
