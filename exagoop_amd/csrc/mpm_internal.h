@@ -193,6 +193,7 @@ struct DtFold {
     int advance = 0;
     double scale = 1.0;
     Scalars *sc = nullptr;
+    Scalars *bak = nullptr;  // *sc is copied here first (the learned step's undo), when set
 };
 void launch_sort(const GridP &g, const PhysP &ph, const PState &P, SortState &S,
                  int64_t *out_id, int packets, hipStream_t s, const DtFold &dt = DtFold());

@@ -153,8 +153,9 @@ int learned_ok(const mpm_ctx *c) {
 // nodal update + BCs.  Leaves the step counter advanced.  advance: the
 // previous step's time advance is still due (it is folded into the dt
 // launch).
+// bak: copy the step scalars there before the dt launch changes them.
 int coarse_begin(mpm_ctx *c, int nfine, int advance = 0, const PState *in = nullptr,
-                 const PState *out = nullptr) {
+                 const PState *out = nullptr, Scalars *bak = nullptr) {
     hipStream_t st = c->stream;
     const PState &Pin = in ? *in : c->P;
     const PState &Pout = out ? *out : c->Q;
@@ -169,7 +170,10 @@ int coarse_begin(mpm_ctx *c, int nfine, int advance = 0, const PState *in = null
         fold.advance = advance;
         fold.scale = (double)nfine;
         fold.sc = c->sc;
+        fold.bak = bak;
     } else {
+        if (bak)
+            HIPCHK(hipMemcpyAsync(bak, c->sc, sizeof(Scalars), hipMemcpyDeviceToDevice, st));
         launch_dt(c->g, c->ph, Pin, c->sc, fixed_dt, c->s.timestep, st,
                   c->dt_next_valid ? 1 : 0, advance, (double)nfine);
     }
@@ -263,8 +267,9 @@ __global__ void k_append_rows(const int *sel, int nsel, const __bf16 *X, const d
 // Guard decision (see mpm_step_learned): flag = 1 keeps the corrected coarse
 // step.  The sums are read back with atomics, where they were accumulated,
 // and cleared for the next step in the same exchange.
+// hflag (pinned host memory, or null): the decision is also stored there.
 __global__ void k_guard_decide(double *sums, const Scalars *sc, double gmag, double gp, double ge,
-                               int *flag) {
+                               int *flag, int *hflag) {
     double gs[7];
     for (int k = 0; k < 7; ++k) gs[k] = atomicExch(&sums[k], 0.0);
     const double dt = sc->dt;
@@ -276,6 +281,10 @@ __global__ void k_guard_decide(double *sums, const Scalars *sc, double gmag, dou
     const bool ok = gs[6] == 0.0 && isfinite(unew) && dp <= gp * M * vs + 1e-300 &&
                     unew <= (1.0 + ge) * vs + 1e-300;
     flag[0] = ok ? 1 : 0;
+    if (hflag) {
+        *hflag = ok ? 1 : 0;
+        __threadfence_system();
+    }
 }
 
 // Add the correction to the nodal velocity and to the increment used by
@@ -640,12 +649,13 @@ int mpm_step_learned(mpm_ctx *c, mpm_mlp *m, int ncoarse, int nfine, const doubl
     }
     if ((rc = ctx_spec_buffer(c))) return rc;
     // One coarse step from particles `in` into `out`: correction, guard and
-    // the particle kernel enqueued back to back, then the decision copied to
-    // the host slot of the step's parity with an event behind it.  A rejected
-    // step leaves `in` untouched.  The list launch clears the other parity's
+    // the particle kernel enqueued back to back; the decision kernel also
+    // stores the decision in the host slot of the step's parity, with an
+    // event behind it.  A rejected step leaves `in` untouched.  The list launch clears the other parity's
     // count for the next step, and the decision clears the guard sums.
-    auto enqueue = [&](int it, int advance, const PState &in, const PState &out) -> int {
-        int r = coarse_begin(c, nfine, advance, &in, &out);
+    auto enqueue = [&](int it, int advance, const PState &in, const PState &out,
+                       Scalars *bak) -> int {
+        int r = coarse_begin(c, nfine, advance, &in, &out, bak);
         if (r) return r;
         int *cnt = slot[it & 1];
         launch_active_nodes_rolling(c->g, c->N, c->ph.mass_tol, c->ml_nodes, cnt,
@@ -653,35 +663,30 @@ int mpm_step_learned(mpm_ctx *c, mpm_mlp *m, int ncoarse, int nfine, const doubl
         launch_mlp_infer_nodes(m->d, c->g, c->N, c->ml_nodes, cnt, m->sc, c->ml_y, c->ml_guard,
                                st);
         hipLaunchKernelGGL(k_guard_decide, dim3(1), dim3(1), 0, st, c->ml_guard, c->sc, gmag,
-                           guards[0], guards[1], flag);
+                           guards[0], guards[1], flag, c->ml_hflag + (it & 1));
+        HIPCHK(hipEventRecord(c->ml_ev[it & 1], st));
         hipLaunchKernelGGL(k_apply_dev, dim3(512), dim3(256), 0, st, c->ml_nodes, cnt, c->ml_y,
                            m->sc.dv_ref, c->N, c->g.nn, flag);
         launch_particle_fast(c->g, c->ph, in, out, c->S, c->N, 1, 1, c->sc, st);
-        if ((r = check_launch())) return r;
-        HIPCHK(hipMemcpyAsync(c->ml_hflag + (it & 1), flag, sizeof(int), hipMemcpyDeviceToHost,
-                              st));
-        HIPCHK(hipEventRecord(c->ml_ev[it & 1], st));
-        return MPM_OK;
+        return check_launch();
     };
     // Speculation: step it + 1 (from Q into R) is enqueued as if step it
     // (from P into Q) were accepted, before its decision is read, so the GPU
     // does not idle across the round trip.  Accepted: the buffers rotate and
     // step it + 1 is in flight.  Rejected: the speculative step is drained
-    // and undone (the step scalars from the copy taken before its dt launch,
+    // and undone (the step scalars from the copy its dt launch took first,
     // the step counter, the active counts; P, its input of record, was not
     // written), and the plain fine steps run from P.
     bool inflight = false;  // step it is enqueued (from P into Q)
     for (int it = 0; it < ncoarse; ++it) {
         if (!inflight) {
-            if ((rc = enqueue(it, pending ? 1 : 0, c->P, c->Q))) return rc;
+            if ((rc = enqueue(it, pending ? 1 : 0, c->P, c->Q, nullptr))) return rc;
             pending = false;
         }
         const bool spec = it + 1 < ncoarse;
         if (spec) {
-            HIPCHK(hipMemcpyAsync(c->ml_scbak, c->sc, sizeof(Scalars), hipMemcpyDeviceToDevice,
-                                  st));
             c->dt_next_valid = c->fuse_dt != 0;  // as after an accepted step
-            if ((rc = enqueue(it + 1, 1, c->Q, c->R))) return rc;
+            if ((rc = enqueue(it + 1, 1, c->Q, c->R, c->ml_scbak))) return rc;
         }
         HIPCHK(hipEventSynchronize(c->ml_ev[it & 1]));
         if (c->ml_hflag[it & 1]) {

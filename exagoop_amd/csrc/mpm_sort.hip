@@ -15,7 +15,10 @@ __global__ __launch_bounds__(kBlock) void k_keys(GridP g, PState P, SortState S,
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) {
         S.tiles[0] = 0;  // non-empty tile count, filled by the scatter launch
-        if (dt.on) dt_final_one(dt.sc, ph, dt.fixed, dt.fixed_dt, 1, dt.advance, dt.scale);
+        if (dt.on) {
+            if (dt.bak) *dt.bak = *dt.sc;
+            dt_final_one(dt.sc, ph, dt.fixed, dt.fixed_dt, 1, dt.advance, dt.scale);
+        }
     }
     keys_wave(g, P, S, i, P.np);
 }
