@@ -63,6 +63,7 @@ struct mpm_ctx {
     int *ml_count = nullptr;      // [0] active count, [1] selected, [2] appended, [3] guard flag,
                                   // [8] alternate active count
     __bf16 *ml_x = nullptr;       // feature rows, padded to 32 rows
+    void *ml_feat = nullptr;      // every node's four BF16 features (nn x 8 bytes)
     float *ml_y = nullptr;        // network outputs [row][3]
     double *ml_vc = nullptr;      // coarse nodal velocity of selected rows [row][3]
     double *ml_guard = nullptr;   // guard sums

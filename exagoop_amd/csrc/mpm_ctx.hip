@@ -330,6 +330,7 @@ void mpm_destroy(mpm_ctx *c) {
     (void)hipFree(c->ml_nodes);
     (void)hipFree(c->ml_count);
     (void)hipFree(c->ml_x);
+    (void)hipFree(c->ml_feat);
     (void)hipFree(c->ml_y);
     (void)hipFree(c->ml_vc);
     (void)hipFree(c->ml_guard);

@@ -99,10 +99,10 @@ int ctx_ml_buffers(mpm_ctx *c) {
     HIPCHK(hipMalloc(&c->ml_count, kMlCounts * sizeof(int)));
     HIPCHK(hipMemset(c->ml_count, 0, kMlCounts * sizeof(int)));
     HIPCHK(hipMalloc(&c->ml_x, pad * kMInPad * sizeof(__bf16)));
+    HIPCHK(hipMalloc(&c->ml_feat, nn * 8));
     HIPCHK(hipMalloc(&c->ml_y, pad * kMOut * sizeof(float)));
     HIPCHK(hipMalloc(&c->ml_vc, pad * 3 * sizeof(double)));
-    HIPCHK(hipMalloc(&c->ml_guard, 8 * sizeof(double)));
-    HIPCHK(hipMemset(c->ml_guard, 0, 8 * sizeof(double)));
+    HIPCHK(hipMalloc(&c->ml_guard, kInferBlocks * 8 * sizeof(double)));
     // the per-step decision is read into pinned memory: a copy into pageable
     // memory goes through a staging buffer
     if (!c->ml_hflag) HIPCHK(hipHostMalloc(&c->ml_hflag, 4 * sizeof(int), hipHostMallocDefault));
@@ -265,13 +265,29 @@ __global__ void k_append_rows(const int *sel, int nsel, const __bf16 *X, const d
 }
 
 // Guard decision (see mpm_step_learned): flag = 1 keeps the corrected coarse
-// step.  The sums are read back with atomics, where they were accumulated,
-// and cleared for the next step in the same exchange.
-// hflag (pinned host memory, or null): the decision is also stored there.
-__global__ void k_guard_decide(double *sums, const Scalars *sc, double gmag, double gp, double ge,
-                               int *flag, int *hflag) {
+// step.  One workgroup sums the inference's kInferBlocks partial rows in a
+// fixed order (thread t: rows t, t + 256, ...; then a fixed tree), so equal
+// states give equal decisions.  hflag (pinned host memory, or null): the
+// decision is also stored there.
+__global__ __launch_bounds__(256) void k_guard_decide(const double *part, const Scalars *sc,
+                                                      double gmag, double gp, double ge, int *flag,
+                                                      int *hflag) {
+    __shared__ double red[7][256];
+    const int t = threadIdx.x;
+    for (int k = 0; k < 7; ++k) {
+        double x = 0.0;
+        for (int b = t; b < kInferBlocks; b += 256) x += part[b * 8 + k];
+        red[k][t] = x;
+    }
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (t < h)
+            for (int k = 0; k < 7; ++k) red[k][t] += red[k][t + h];
+        __syncthreads();
+    }
+    if (t != 0) return;
     double gs[7];
-    for (int k = 0; k < 7; ++k) gs[k] = atomicExch(&sums[k], 0.0);
+    for (int k = 0; k < 7; ++k) gs[k] = red[k][0];
     const double dt = sc->dt;
     const double M = gs[0];
     const double u = M > 0 ? sqrt(gs[4] / M) : 0.0;
@@ -635,7 +651,6 @@ int mpm_step_learned(mpm_ctx *c, mpm_mlp *m, int ncoarse, int nfine, const doubl
     int *flag = c->ml_count + 3;
     int *slot[2] = {c->ml_count, c->ml_count + 8};  // active counts by step parity
     hipLaunchKernelGGL(k_zero_f, dim3(1), dim3(64), 0, st, reinterpret_cast<float *>(slot[0]), 1);
-    hipLaunchKernelGGL(k_zero_d, dim3(1), dim3(64), 0, st, c->ml_guard, 8);  // once per call
     int fb = 0;
     bool pending = false;  // an accepted step's time advance is still due
     if (!m) {  // uncorrected coarse steps: nothing to decide, no round trip
@@ -651,18 +666,18 @@ int mpm_step_learned(mpm_ctx *c, mpm_mlp *m, int ncoarse, int nfine, const doubl
     // One coarse step from particles `in` into `out`: correction, guard and
     // the particle kernel enqueued back to back; the decision kernel also
     // stores the decision in the host slot of the step's parity, with an
-    // event behind it.  A rejected step leaves `in` untouched.  The list launch clears the other parity's
-    // count for the next step, and the decision clears the guard sums.
+    // event behind it.  A rejected step leaves `in` untouched.  The list
+    // launch clears the other parity's count for the next step.
     auto enqueue = [&](int it, int advance, const PState &in, const PState &out,
                        Scalars *bak) -> int {
         int r = coarse_begin(c, nfine, advance, &in, &out, bak);
         if (r) return r;
         int *cnt = slot[it & 1];
         launch_active_nodes_rolling(c->g, c->N, c->ph.mass_tol, c->ml_nodes, cnt,
-                                    slot[(it + 1) & 1], st);
-        launch_mlp_infer_nodes(m->d, c->g, c->N, c->ml_nodes, cnt, m->sc, c->ml_y, c->ml_guard,
-                               st);
-        hipLaunchKernelGGL(k_guard_decide, dim3(1), dim3(1), 0, st, c->ml_guard, c->sc, gmag,
+                                    slot[(it + 1) & 1], c->ml_feat, m->sc, st);
+        launch_mlp_infer_nodes(m->d, c->g, c->N, c->ml_feat, c->ml_nodes, cnt, m->sc, c->ml_y,
+                               c->ml_guard, st);
+        hipLaunchKernelGGL(k_guard_decide, dim3(1), dim3(256), 0, st, c->ml_guard, c->sc, gmag,
                            guards[0], guards[1], flag, c->ml_hflag + (it & 1));
         HIPCHK(hipEventRecord(c->ml_ev[it & 1], st));
         hipLaunchKernelGGL(k_apply_dev, dim3(512), dim3(256), 0, st, c->ml_nodes, cnt, c->ml_y,

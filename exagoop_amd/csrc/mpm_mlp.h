@@ -21,6 +21,7 @@ constexpr int kOffW1 = 0, kOffB1 = kOffW1 + kMIn * kMH, kOffW2 = kOffB1 + kMH,
               kOffB3 = kOffW3 + kMH * kMOut, kMParams = kOffB3 + kMOut;
 static_assert(kMParams == 11331, "parameter count");
 constexpr int kMRows = 32;  // rows (nodes) per wavefront tile
+constexpr int kInferBlocks = 1024;  // workgroups of the learned step's inference
 // Training partial rows: kMParams gradient entries, the loss sum, padding
 // to a multiple of 64 floats; at most kMTrainParts rows (one per wavefront).
 constexpr int kMPartStride = 11392, kMTrainParts = 512;
@@ -68,20 +69,25 @@ void launch_mlp_adam(const MlpDev &w, int nparts, float lr, float beta1, float b
 // BF16 copies from params (after an upload).
 void launch_mlp_pack(const MlpDev &w, hipStream_t s);
 
-// Y[r][3] = network(features of nodes[r]) for r < *count, the features
-// gathered from the nodal state in the kernel (no row buffer), and the
-// guard sums of dv = Y dv_ref added to sums[0..7): m, m dv (3), m |v|^2,
-// m |v + dv|^2, non-finite count.
-void launch_mlp_infer_nodes(const MlpDev &w, const GridP &g, const NState &N, const int *nodes,
-                            const int *count, MlpScales sc, float *Y, double *sums, hipStream_t s);
+// Y[r][3] = network(features of nodes[r]) for r < *count, the rows
+// gathered in the kernel from feat (every node's four BF16 features, 8 bytes,
+// written by launch_active_nodes_rolling; no row buffer), and the guard sums
+// of dv = Y dv_ref as kInferBlocks partial rows sums[b * 8 + k], k < 7: m,
+// m dv (3), m |v|^2, m |v + dv|^2, non-finite count (every row written).
+void launch_mlp_infer_nodes(const MlpDev &w, const GridP &g, const NState &N, const void *feat,
+                            const int *nodes, const int *count, MlpScales sc, float *Y,
+                            double *sums, hipStream_t s);
 
 // Active node list: nodes with mass >= tol, in index order within each
 // workgroup's span of nodes (spans in arbitrary order), count in *count.
 void launch_active_nodes(const GridP &g, const NState &N, double tol, int *list, int *count,
                          int *scratch, hipStream_t s);
 // The same without clearing *count first (it must be zero on entry); the
-// launch zeroes *next_count for the following step's list instead.
+// launch zeroes *next_count for the following step's list instead, and
+// writes every node's four BF16 features (m / m_ref, m v / (m_ref v_ref), the
+// conversions of launch_mlp_features) to feat (nn x 8 bytes).
 void launch_active_nodes_rolling(const GridP &g, const NState &N, double tol, int *list,
-                                 int *count, int *next_count, hipStream_t s);
+                                 int *count, int *next_count, void *feat, MlpScales sc,
+                                 hipStream_t s);
 
 }  // namespace xmpm

@@ -489,22 +489,39 @@ __global__ void k_mlp_features(GridP g, NState N, const int *nodes, int nrows,
     }
 }
 
+// The four BF16 features of one node (m / m_ref, m v / (m_ref v_ref)), in
+// the conversions of k_mlp_features, as one 8-byte word.
+__device__ __forceinline__ uint2 node_features(const NState &N, int64_t nn, int64_t p, double im,
+                                               double imv) {
+    const double m = N.mass[p];
+    __bf16 f[4];
+    f[0] = (__bf16)(float)(m * im);
+    for (int d = 0; d < 3; ++d) f[1 + d] = (__bf16)(float)(m * N.vel[d * nn + p] * imv);
+    uint2 w;
+    __builtin_memcpy(&w, f, sizeof(w));
+    return w;
+}
+
 // Active nodes (mass >= tol).  A workgroup takes kActiveSpan consecutive
 // nodes in 16 coalesced rounds, keeps its flags in a bit mask, reserves its
 // run with one atomic and writes the run in node order.  next_count (if
-// given) is zeroed for the following step's list.
+// given) is zeroed for the following step's list.  feat (if given): every
+// node's features (node_features), for the inference's gather.
 constexpr int kActiveRounds = 16, kActiveSpan = 256 * kActiveRounds;
 __global__ __launch_bounds__(256) void k_active_nodes(GridP g, NState N, double tol, int *list,
-                                                      int *count, int *next_count) {
+                                                      int *count, int *next_count, uint2 *feat,
+                                                      MlpScales sc) {
     __shared__ int off[kActiveRounds * 4];  // counts, then offsets, of (round, wavefront)
     __shared__ int base;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int64_t q0 = (int64_t)blockIdx.x * kActiveSpan + threadIdx.x;
     if (next_count && blockIdx.x == 0 && threadIdx.x == 0) *next_count = 0;
     unsigned mask = 0;
+    const double im = 1.0 / sc.m_ref, imv = 1.0 / (sc.m_ref * sc.v_ref);
 #pragma unroll
     for (int k = 0; k < kActiveRounds; ++k) {
         const int64_t q = q0 + 256 * k;
+        if (feat && q < g.nn) feat[q] = node_features(N, g.nn, q, im, imv);
         const bool f = q < g.nn && N.mass[q] >= tol;
         mask |= (unsigned)f << k;
         const unsigned long long b = __ballot(f);
@@ -531,8 +548,9 @@ __global__ __launch_bounds__(256) void k_active_nodes(GridP g, NState N, double 
     }
 }
 
-// Inference on the listed nodes with the feature rows gathered from the
-// nodal state in the kernel (the rows of k_mlp_features, never stored) and
+// Inference on the listed nodes with the feature rows gathered in the kernel
+// from the per-node features k_active_nodes wrote (the rows of
+// k_mlp_features, never stored) and
 // the guard sums of the corrections added on the way: sums[0] += m,
 // sums[1+d] += m dv_d, sums[4] += m |v|^2, sums[5] += m |v + dv|^2,
 // sums[6] += count of non-finite dv (dv = Y dv_ref, the FP64 nodal mass m
@@ -540,8 +558,8 @@ __global__ __launch_bounds__(256) void k_active_nodes(GridP g, NState N, double 
 // Two wavefronts per SIMD: the register budget then holds one tile's 64
 // FP64 gathers in flight with the network, without spills.
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_mlp_infer_nodes(
-    MlpDev w, GridP g, NState N, const int *nodes, const int *count, MlpScales sc, float *Y,
-    double *sums) {
+    MlpDev w, GridP g, NState N, const uint2 *feat, const int *nodes, const int *count,
+    MlpScales sc, float *Y, double *sums) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, col = lane & 31;
     const int h = lane >> 5;
     const int nrows = *count;
@@ -549,7 +567,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     const bf16x8 p0 = ident(0, lane, true), p1 = ident(1, lane, true);
     float b1[2], b2[2], b3;
     load_biases(w, col, b1, b2, b3);
-    const double im = 1.0 / sc.m_ref, imv = 1.0 / (sc.m_ref * sc.v_ref);
     const int nx = g.n[0] + 1, ny = g.n[1] + 1, nz = g.n[2] + 1;
     double gm = 0.0, gmdv = 0.0, gv2 = 0.0, gw2 = 0.0, gbad = 0.0;
     for (int t = blockIdx.x * 4 + wave; t < ntiles; t += gridDim.x * 4) {
@@ -573,13 +590,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
                 const bool in = q >= 0 && nb < 27 && a >= 0 && a < nx && b >= 0 && b < ny &&
                                 cz >= 0 && cz < nz;
                 const int64_t p = in ? a + (int64_t)nx * (b + (int64_t)ny * cz) : 0;
-                const double m = in ? N.mass[p] : 0.0;
-                xa[ks][4 * u] = (__bf16)(float)(m * im);
+                uint2 wv = in ? feat[p] : make_uint2(0u, 0u);
+                __bf16 f[4];
+                __builtin_memcpy(f, &wv, sizeof(wv));
 #pragma unroll
-                for (int d = 0; d < 3; ++d) {
-                    const double v = in ? N.vel[d * g.nn + p] : 0.0;
-                    xa[ks][4 * u + 1 + d] = (__bf16)(float)(m * v * imv);
-                }
+                for (int e = 0; e < 4; ++e) xa[ks][4 * u + e] = f[e];
             }
         }
         bf16x8 h1[2][2], h2[2][2];
@@ -618,11 +633,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         red[6][wave] = gbad;
     }
     __syncthreads();
-    if (threadIdx.x < 7) {
-        const double x = red[threadIdx.x][0] + red[threadIdx.x][1] + red[threadIdx.x][2] +
-                         red[threadIdx.x][3];
-        if (x != 0.0) atomicAdd(&sums[threadIdx.x], x);
-    }
+    if (threadIdx.x < 7)
+        sums[blockIdx.x * 8 + threadIdx.x] = red[threadIdx.x][0] + red[threadIdx.x][1] +
+                                             red[threadIdx.x][2] + red[threadIdx.x][3];
 }
 
 __global__ void k_zero_int(int *p, int n) {
@@ -691,20 +704,23 @@ void launch_active_nodes(const GridP &g, const NState &N, double tol, int *list,
     (void)scratch;
     hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, s, count, 1);
     hipLaunchKernelGGL(k_active_nodes, dim3((unsigned)((g.nn + kActiveSpan - 1) / kActiveSpan)),
-                       dim3(256), 0, s, g, N, tol, list, count, (int *)nullptr);
+                       dim3(256), 0, s, g, N, tol, list, count, (int *)nullptr, (uint2 *)nullptr,
+                       MlpScales{1.0, 1.0, 1.0});
 }
 
 void launch_active_nodes_rolling(const GridP &g, const NState &N, double tol, int *list,
-                                 int *count, int *next_count, hipStream_t s) {
+                                 int *count, int *next_count, void *feat, MlpScales sc,
+                                 hipStream_t s) {
     hipLaunchKernelGGL(k_active_nodes, dim3((unsigned)((g.nn + kActiveSpan - 1) / kActiveSpan)),
-                       dim3(256), 0, s, g, N, tol, list, count, next_count);
+                       dim3(256), 0, s, g, N, tol, list, count, next_count,
+                       reinterpret_cast<uint2 *>(feat), sc);
 }
 
-void launch_mlp_infer_nodes(const MlpDev &w, const GridP &g, const NState &N, const int *nodes,
-                            const int *count, MlpScales sc, float *Y, double *sums,
-                            hipStream_t s) {
-    hipLaunchKernelGGL(k_mlp_infer_nodes, dim3(1024), dim3(256), 0, s, w, g, N, nodes, count, sc,
-                       Y, sums);
+void launch_mlp_infer_nodes(const MlpDev &w, const GridP &g, const NState &N, const void *feat,
+                            const int *nodes, const int *count, MlpScales sc, float *Y,
+                            double *sums, hipStream_t s) {
+    hipLaunchKernelGGL(k_mlp_infer_nodes, dim3(kInferBlocks), dim3(256), 0, s, w, g, N,
+                       reinterpret_cast<const uint2 *>(feat), nodes, count, sc, Y, sums);
 }
 
 }  // namespace xmpm
