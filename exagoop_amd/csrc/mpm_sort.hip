@@ -136,7 +136,7 @@ __global__ __launch_bounds__(kBlock) void k_cell_order(SortState S) {
 template <int O, bool FORCES, bool FALLBACK, bool GATHER, bool DET>
 __global__ __launch_bounds__(kBlock) void k_p2g_cells(GridP g, PhysP ph, PState P, SortState S,
                                                       double *acc, int epoch, Scalars *sc) {
-    __shared__ P2gLds<FORCES ? 7 : 3, DET> lds;
+    __shared__ P2gLdsOf<FORCES, GATHER, DET> lds;
     const int pos = active_pos(S, blockIdx.x);
     if (pos < 0) return;  // uniform over the block
     p2g_tile<O, FORCES, FALLBACK, GATHER, DET>(g, ph, P, S, acc, epoch, &sc->err,

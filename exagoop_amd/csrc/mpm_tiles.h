@@ -351,13 +351,21 @@ struct StageRaw {
 // block of kZN nodes per wavefront (its cells' four z-levels), plain adds,
 // summed in wavefront order at the flush.
 constexpr int kZN = kTN * kTN * 4;
-template <int NC, bool DET = false>
+// Permutation entries of a tile staged in LDS by the P2G forms that read the
+// particle state through the permutation (a tile with more particles reads
+// the rest from global memory).
+constexpr int kPermLds = 1024;
+template <int NC, bool DET = false, int NPERM = 1>
 struct P2gLds {
     double slab[DET ? 4 * NC * kZN : NC * kTNN];
     double pk[4][kPkArea];       // staging area per wavefront
     int cs[kCellsPerTile + 1];
+    int perm[NPERM];
     SplineTables tab;
 };
+// The LDS of p2g_tile<O, FORCES, FALLBACK, GATHER, DET>.
+template <bool FORCES, bool GATHER, bool DET>
+using P2gLdsOf = P2gLds<FORCES ? 7 : 3, DET, (GATHER || !FORCES) ? kPermLds : 1>;
 
 // LDS of one particle tile (NCV node components).
 template <int NCV>
@@ -372,8 +380,8 @@ struct PartLds {
 template <int O, bool FORCES, bool FALLBACK, bool GATHER = false, bool DET = false>
 __device__ __forceinline__ void p2g_tile(const GridP &g, const PhysP &ph, const PState &P,
                                          const SortState &S, double *acc, int epoch, int *err,
-                                         int tile, P2gLds<FORCES ? 7 : 3, DET> &L, int pos = 0,
-                                         int pass = 0) {
+                                         int tile, P2gLdsOf<FORCES, GATHER, DET> &L,
+                                         int pos = 0, int pass = 0) {
     constexpr int NC = FORCES ? 7 : 3;
     constexpr int kSlabN = DET ? 4 * NC * kZN : NC * kTNN;
     auto &slab = L.slab;
@@ -381,8 +389,17 @@ __device__ __forceinline__ void p2g_tile(const GridP &g, const PhysP &ph, const 
     auto &cs = L.cs;
     auto &tab = L.tab;
     const int tid = threadIdx.x;
+    // Forms that read through the permutation stage the tile's entries in
+    // LDS, so a round's field reads wait on no global read of their own.
+    constexpr bool kPermL = GATHER || !FORCES;
+    const int pbeg = S.cell_start[tile * kCellsPerTile];
+    const int pnum = kPermL ? min(S.cell_start[tile * kCellsPerTile + kCellsPerTile] - pbeg,
+                                  kPermLds)
+                            : 0;
     __syncthreads();  // the workgroup's previous tile is done with the LDS
     if (tid <= kCellsPerTile) cs[tid] = S.cell_start[tile * kCellsPerTile + tid];
+    if (kPermL)
+        for (int e = tid; e < pnum; e += kBlock) L.perm[e] = S.perm[pbeg + e];
     if (O != 1) {
         const int *src = reinterpret_cast<const int *>(&kTab);
         int *dst = reinterpret_cast<int *>(&tab);
@@ -439,7 +456,8 @@ __device__ __forceinline__ void p2g_tile(const GridP &g, const PhysP &ph, const 
                 for (int e = 0; e < 4; ++e)
                     if (e < v_n) R.val[e] = S.pkt[(v_first + e) * pc + R.slot];
             } else {
-                const int64_t j = S.perm[sbeg + t];
+                const int o = sbeg + t - pbeg;
+                const int64_t j = o < pnum ? L.perm[o] : S.perm[sbeg + t];
                 R.slot = j;
                 if (P.idata[MPM_I_PHASE * cap + j] != 0) {
                     R.act = false;
@@ -464,7 +482,9 @@ __device__ __forceinline__ void p2g_tile(const GridP &g, const PhysP &ph, const 
         const int c0 = kTileCells * (ry + kTileCells * cz) + s_cell;
         const int sbeg = cs[c0], scnt = cs[c0 + 1] - sbeg;
         const int t = base + s_t;
-        return t < scnt ? S.perm[sbeg + t] : -1;
+        if (t >= scnt) return -1;
+        const int o = sbeg + t - pbeg;
+        return o < pnum ? L.perm[o] : S.perm[sbeg + t];
     };
     // The phase is read by the slot's part-0 lane only (R.phase) and shared
     // at staging; no field read waits for it.
@@ -635,7 +655,11 @@ __device__ __forceinline__ void p2g_tile(const GridP &g, const PhysP &ph, const 
                 for (int k = 0; k < 4; ++k) {
                     a4[k][0] += wx[k] * Mr;
                     for (int d = 0; d < 3; ++d) a4[k][1 + d] += wx[k] * Pr[d];
-                    for (int d = 0; d < 3; ++d) a4[k][4 + d] += wx[k] * C[d] + dwx[k] * E[d];
+                    // two fused multiply-adds into the accumulator
+                    for (int d = 0; d < 3; ++d) {
+                        a4[k][4 + d] += wx[k] * C[d];
+                        a4[k][4 + d] += dwx[k] * E[d];
+                    }
                 }
             } else {
                 double Pr[3];
