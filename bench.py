@@ -44,7 +44,10 @@ network exagoop_amd/data/learned_dam_break.npz), the uncorrected coarse step
 and plain runs at larger fixed dt: the relative L2 particle-velocity error of
 each against the reference, and the speed-up of the learned form over the
 plain run at the largest dt whose error does not exceed the learned form's
-(learned vs plain at equal error).
+(learned vs plain at equal error).  It also reports the coarse step without
+the network with the PIC/FLIP blend alpha^4 (the blend of the four fine steps
+a coarse step stands for, MPM_OPT_COARSE_BLEND), and the learned and that
+uncorrected form's errors over four times the horizon.
 """
 import argparse
 import json
@@ -405,6 +408,32 @@ def batch_leg(nscenes, nsteps):
     return out
 
 
+def long_horizon(ctx, net, nsteps):
+    """Errors against the fine run over a longer horizon (no timing): the
+    learned form, and the coarse step without the network with the PIC/FLIP
+    blend alpha^4."""
+    import numpy as np
+    from exagoop_amd import capi, learned
+
+    g, _ = ctx(learned.DT)
+    g.step(nsteps)
+    vf, idf = learned.velocities(g)
+    g.close()
+    out = {"horizon_fine_steps": nsteps}
+    for name, model, blend in (("learned", net, 0), ("uncorrected_blend", None, 1)):
+        g, _ = ctx(learned.DT, blend)
+        fb = capi.step_learned(g, model, nsteps // learned.NFINE, learned.NFINE)
+        v, ids = learned.velocities(g)
+        g.close()
+        e = learned.rel_l2(v, vf) if np.array_equal(ids, idf) else float("inf")
+        out[name + "_rel_l2_velocity_err"] = float(f"{e:.4g}")
+        out[name + "_fallbacks"] = fb
+    print(f"[bench] learned over {nsteps} fine steps: error {out['learned_rel_l2_velocity_err']} "
+          f"({out['learned_fallbacks']} fallbacks), uncorrected with alpha^4 blend "
+          f"{out['uncorrected_blend_rel_l2_velocity_err']}", file=sys.stderr, flush=True)
+    return out
+
+
 def learned_leg(nsteps):
     """Learned vs plain at equal error, on the benchmark dam break.
 
@@ -421,22 +450,23 @@ def learned_leg(nsteps):
     import numpy as np
     from exagoop_amd import capi, learned, scenes
 
-    def ctx(dt):
+    def ctx(dt, blend=0):
         s, pos, r, idata = scenes.dam_break()
         s.fixed_timestep = 1
         s.timestep = dt
         g = capi.Context(s)
         g.upload(pos, r, idata)
         g.init_internal_forces()
+        g.set_option(capi.OPT_COARSE_BLEND, blend)
         g.time()
         return g, pos.shape[1]
 
-    def timed(dt, n, fn, reps=3):
+    def timed(dt, n, fn, reps=3, blend=0):
         # median wall time of reps runs from fresh contexts (each run takes a
         # host round trip per learned step, so single runs jitter)
         ts = []
         for _ in range(reps):
-            g, npart = ctx(dt)
+            g, npart = ctx(dt, blend)
             t0 = time.perf_counter()
             extra = fn(g, n)
             g.time()  # synchronises
@@ -462,9 +492,11 @@ def learned_leg(nsteps):
         return learned.rel_l2(v, vf) if np.array_equal(ids, idf) else float("inf")
 
     res = {}
-    for name, model in (("learned", net), ("coarse", None)):
+    for name, model, blend in (("learned", net, 0), ("coarse", None, 0),
+                               ("coarse_blend", None, 1)):
         t, v, ids, _, fb = timed(learned.DT, nsteps // nf,
-                                 lambda g, n, m=model: capi.step_learned(g, m, n, nf))
+                                 lambda g, n, m=model: capi.step_learned(g, m, n, nf),
+                                 blend=blend)
         res[name] = (t, err(v, ids), fb)
     t_l, err_l, fb_l = res["learned"]
     sweep = []
@@ -492,6 +524,10 @@ def learned_leg(nsteps):
            "rel_l2_velocity_err": float(f"{err_l:.4g}"),
            "uncorrected_rel_l2_velocity_err": float(f"{res['coarse'][1]:.4g}"),
            "uncorrected_value": round(npart * nsteps / res["coarse"][0], 1),
+           "uncorrected_blend_rel_l2_velocity_err": float(f"{res['coarse_blend'][1]:.4g}"),
+           "uncorrected_blend_note": "the coarse step without the network, PIC/FLIP blend "
+                                     "alpha^4 (MPM_OPT_COARSE_BLEND)",
+           "long_horizon": long_horizon(ctx, net, 4 * nsteps),
            "fallbacks": fb_l, "fine_dt": learned.DT,
            "fine_dt_note": "reference run at a quarter of the CFL-0.3 step (CFL about 0.075)",
            "fine_steps_per_learned_step": nf, "horizon_fine_steps": nsteps,
@@ -499,7 +535,8 @@ def learned_leg(nsteps):
            "network": "27x4 -> 64 -> 64 -> 3 GELU, BF16 MFMA / FP32 accumulate",
            "weights": "exagoop_amd/data/learned_dam_break.npz"}
     print(f"[bench] learned: rel L2 velocity error {out['rel_l2_velocity_err']} "
-          f"(uncorrected {out['uncorrected_rel_l2_velocity_err']}); equal-error plain dt "
+          f"(uncorrected {out['uncorrected_rel_l2_velocity_err']}, with alpha^4 blend "
+          f"{out['uncorrected_blend_rel_l2_velocity_err']}); equal-error plain dt "
           f"{eq['dt']:.4g} (error {eq['rel_l2_velocity_err']}): {out['speedup_equal_error']}x; "
           f"{out['speedup_vs_fine']}x the fine run", file=sys.stderr, flush=True)
     for x in sweep:

@@ -40,6 +40,8 @@ USL, MUSL = 0, 1
 
 OPT_PROFILE, OPT_KEEP_FORCES, OPT_SORTED, OPT_FAST, OPT_FUSED_DT = 1, 2, 3, 4, 5
 OPT_DETERMINISTIC = 6
+OPT_COARSE_BLEND = 7
+OPT_TILE_WAVES = 8
 LS_SPHERE, LS_PLANE, LS_CYLINDER, LS_BOX = 0, 1, 2, 3
 MLP_NPARAMS = 11331
 MLP_IN = 128

@@ -34,6 +34,7 @@ struct mpm_ctx {
     int sorted = 1;         // use the cell-sorted step
     int fast = 1;           // use the specialised particle kernel where it applies
     int fuse_dt = 1;        // the specialised kernel reduces the next step's dt
+    int coarse_blend = 0;   // learned coarse step blends PIC/FLIP with alpha^nfine
     bool dt_next_valid = false;  // the last step reduced the next step's dt candidates
     Scalars *sc = nullptr;
     double *d_energy = nullptr;

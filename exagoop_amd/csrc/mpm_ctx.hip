@@ -1120,6 +1120,12 @@ int mpm_set_option(mpm_ctx *c, int option, int value) {
     } else if (option == MPM_OPT_FUSED_DT) {
         c->fuse_dt = value;
         c->dt_next_valid = false;
+    } else if (option == MPM_OPT_TILE_WAVES) {
+        if (value != 0 && value != 4 && value != 8)
+            return set_err(MPM_E_ARG, "tile wavefronts must be 0, 4 or 8");
+        c->S.waves = value;
+    } else if (option == MPM_OPT_COARSE_BLEND) {
+        c->coarse_blend = value ? 1 : 0;
     } else if (option == MPM_OPT_DETERMINISTIC) {
         if (value && !c->det_blk) {
             // one node block per tile (7 components), and the tile stamps

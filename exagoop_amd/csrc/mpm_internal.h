@@ -176,6 +176,9 @@ struct SortState {
     double *blk = nullptr;
     int64_t *tile_slot = nullptr;
     int pass = 0;
+    // Wavefronts per tile of the tile kernels (MPM_OPT_TILE_WAVES): 4 or 8,
+    // 0 to choose by the particle count.
+    int waves = 0;
 };
 constexpr int kPktComps = 16;
 size_t sort_scan_bytes(int nkeys);

@@ -154,6 +154,21 @@ int learned_ok(const mpm_ctx *c) {
 // previous step's time advance is still due (it is folded into the dt
 // launch).
 // bak: copy the step scalars there before the dt launch changes them.
+// Step parameters of a coarse step's particle update: with
+// MPM_OPT_COARSE_BLEND the PIC/FLIP blend is alpha^nfine, the blend of the
+// nfine fine steps the coarse step stands for (each fine step keeps the
+// fraction alpha of the particle's own velocity deviation from the grid
+// field; the blend is mpm_particle_grid_ops.cpp:1030-1034).
+PhysP coarse_ph(const mpm_ctx *c, int nfine) {
+    PhysP p = c->ph;
+    if (c->coarse_blend) {
+        double a = 1.0;
+        for (int k = 0; k < nfine; ++k) a *= c->ph.alpha;
+        p.alpha = a;
+    }
+    return p;
+}
+
 int coarse_begin(mpm_ctx *c, int nfine, int advance = 0, const PState *in = nullptr,
                  const PState *out = nullptr, Scalars *bak = nullptr) {
     hipStream_t st = c->stream;
@@ -187,9 +202,9 @@ int coarse_begin(mpm_ctx *c, int nfine, int advance = 0, const PState *in = null
 }
 
 // The particle kernel of the coarse step and the time advance.
-int coarse_finish(mpm_ctx *c) {
+int coarse_finish(mpm_ctx *c, int nfine) {
     hipStream_t st = c->stream;
-    launch_particle_fast(c->g, c->ph, c->P, c->Q, c->S, c->N, 1, 1, c->sc, st);
+    launch_particle_fast(c->g, coarse_ph(c, nfine), c->P, c->Q, c->S, c->N, 1, 1, c->sc, st);
     std::swap(c->P, c->Q);
     c->dt_next_valid = c->fuse_dt != 0;
     c->dirty = false;
@@ -657,7 +672,7 @@ int mpm_step_learned(mpm_ctx *c, mpm_mlp *m, int ncoarse, int nfine, const doubl
         for (int it = 0; it < ncoarse; ++it) {
             if ((rc = coarse_begin(c, nfine, pending ? 1 : 0))) return rc;
             pending = false;
-            if ((rc = coarse_finish(c))) return rc;
+            if ((rc = coarse_finish(c, nfine))) return rc;
         }
         if (fallbacks) *fallbacks = 0;
         return finish_call(c);
@@ -682,7 +697,7 @@ int mpm_step_learned(mpm_ctx *c, mpm_mlp *m, int ncoarse, int nfine, const doubl
         HIPCHK(hipEventRecord(c->ml_ev[it & 1], st));
         hipLaunchKernelGGL(k_apply_dev, dim3(512), dim3(256), 0, st, c->ml_nodes, cnt, c->ml_y,
                            m->sc.dv_ref, c->N, c->g.nn, flag);
-        launch_particle_fast(c->g, c->ph, in, out, c->S, c->N, 1, 1, c->sc, st);
+        launch_particle_fast(c->g, coarse_ph(c, nfine), in, out, c->S, c->N, 1, 1, c->sc, st);
         return check_launch();
     };
     // Speculation: step it + 1 (from Q into R) is enqueued as if step it

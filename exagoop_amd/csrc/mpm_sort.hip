@@ -133,14 +133,14 @@ __global__ __launch_bounds__(kBlock) void k_cell_order(SortState S) {
 
 // deposit_onto_grid_momentum over the non-empty tiles, one workgroup each
 // (see p2g_tile).
-template <int O, bool FORCES, bool FALLBACK, bool GATHER, bool DET>
-__global__ __launch_bounds__(kBlock) void k_p2g_cells(GridP g, PhysP ph, PState P, SortState S,
-                                                      double *acc, int epoch, Scalars *sc) {
-    __shared__ P2gLdsOf<FORCES, GATHER, DET> lds;
+template <int O, bool FORCES, bool FALLBACK, bool GATHER, bool DET, int NW>
+__global__ __launch_bounds__(NW * 64) void k_p2g_cells(GridP g, PhysP ph, PState P, SortState S,
+                                                       double *acc, int epoch, Scalars *sc) {
+    __shared__ P2gLdsOf<FORCES, GATHER, DET, NW> lds;
     const int pos = active_pos(S, blockIdx.x);
     if (pos < 0) return;  // uniform over the block
-    p2g_tile<O, FORCES, FALLBACK, GATHER, DET>(g, ph, P, S, acc, epoch, &sc->err,
-                                               S.tiles[1 + pos], lds, pos, S.pass);
+    p2g_tile<O, FORCES, FALLBACK, GATHER, DET, NW>(g, ph, P, S, acc, epoch, &sc->err,
+                                                   S.tiles[1 + pos], lds, pos, S.pass);
 }
 
 // The deterministic P2G's gather pass: node (I, J, K) adds its blocks
@@ -166,21 +166,42 @@ __global__ __launch_bounds__(kBlock) void k_p2g_gather(GridP g, BlkGather bg,
 
 // Fused particle pipeline over the non-empty tiles, one workgroup each (see
 // particle_tile); the next step's dt candidates go into the running minimum.
-template <int O, bool VEL, bool DELTA>
-__global__ __launch_bounds__(kBlock) void k_particle_tile(GridP g, PhysP ph, PState Pin,
-                                                          PState Pout, SortState S, NState N,
-                                                          int redist, Scalars *sc) {
+template <int O, bool VEL, bool DELTA, int NT>
+__global__ __launch_bounds__(NT) void k_particle_tile(GridP g, PhysP ph, PState Pin,
+                                                      PState Pout, SortState S, NState N,
+                                                      int redist, Scalars *sc) {
     __shared__ PartLds<VEL ? 6 : 3> lds;
     const int tile = active_tile(S, blockIdx.x);
     if (tile < 0) return;  // uniform over the block
     double cmin = __builtin_huge_val();
-    particle_tile<O, VEL, DELTA>(g, ph, Pin, Pout, S, N, redist, sc->dt, sc->time, &sc->err, tile,
-                                 lds, cmin);
+    particle_tile<O, VEL, DELTA, NT>(g, ph, Pin, Pout, S, N, redist, sc->dt, sc->time, &sc->err,
+                                     tile, lds, cmin);
     dt_wave_min(cmin, sc);  // every lane reaches this point
 }
 
 
 inline unsigned blocks_for(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+// Few tiles: with fewer particles than one full tile (4^3 cells x 8) per
+// compute unit, each tile's serial chain sets the kernel time and the tile
+// kernels run 8 wavefronts per tile (P2G: a z-slice's row pair per
+// wavefront; particle kernel: 512 threads, one pass over a full tile).  With
+// more, 4 wavefronts per tile keep more tiles resident per CU (elastic disks,
+// 33 k particles: 88.5 -> 70.3 us per step; dam break, 524 k: 240 us with 4
+// wavefronts against 282 us with 8).
+bool few_tiles(int64_t np) {
+    static int cus[64] = {0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
+    if (cus[dev] <= 0) {
+        int n = 0;
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            n <= 0)
+            n = 256;
+        cus[dev] = n;
+    }
+    return np < (int64_t)cus[dev] * 512;
+}
 
 }  // namespace
 
@@ -212,20 +233,23 @@ void launch_p2g_tiles(const GridP &g, const PhysP &ph, const PState &P, SortStat
                       const NState &N, int epoch, int momentum_only, int fallback, Scalars *sc,
                       hipStream_t s) {
     if (P.np <= 0) return;
-    const dim3 gr((unsigned)((S.ntiles + 7) / 8 * 8)), bl(kBlock);
+    const dim3 gr((unsigned)((S.ntiles + 7) / 8 * 8)), bl(kBlock), bl8(8 * 64);
     const bool det = S.blk && !S.pkt_valid;
+    const bool few = S.waves ? S.waves == 8 : few_tiles(P.np);
     if (det) ++S.pass;
+#define XMPM_PGW(o, f, fb, ga, w)                                                             \
+    hipLaunchKernelGGL((k_p2g_cells<o, f, fb, ga, false, w>), gr, (w) == 8 ? bl8 : bl, 0, s, \
+                       g, ph, P, S, N.acc, epoch, sc)
 #define XMPM_PG(o, f, fb)                                                                     \
     do {                                                                                      \
         if (det)                                                                              \
-            hipLaunchKernelGGL((k_p2g_cells<o, f, fb, f, true>), gr, bl, 0, s, g, ph, P, S,    \
+            hipLaunchKernelGGL((k_p2g_cells<o, f, fb, f, true, 4>), gr, bl, 0, s, g, ph, P, S, \
                                N.acc, epoch, sc);                                             \
-        else if (f && !S.pkt_valid)                                                           \
-            hipLaunchKernelGGL((k_p2g_cells<o, f, fb, true, false>), gr, bl, 0, s, g, ph, P, S, \
-                               N.acc, epoch, sc);                                             \
-        else                                                                                  \
-            hipLaunchKernelGGL((k_p2g_cells<o, f, fb, false, false>), gr, bl, 0, s, g, ph, P,  \
-                               S, N.acc, epoch, sc);                                          \
+        else if (f && !S.pkt_valid) {                                                         \
+            if (few) XMPM_PGW(o, f, fb, true, 8); else XMPM_PGW(o, f, fb, true, 4);           \
+        } else {                                                                              \
+            if (few) XMPM_PGW(o, f, fb, false, 8); else XMPM_PGW(o, f, fb, false, 4);         \
+        }                                                                                     \
     } while (0)
 #define XMPM_PGO(f, fb)                                                                       \
     do {                                                                                      \
@@ -241,6 +265,7 @@ void launch_p2g_tiles(const GridP &g, const PhysP &ph, const PState &P, SortStat
     }
 #undef XMPM_PGO
 #undef XMPM_PG
+#undef XMPM_PGW
     if (det) {
         BlkGather bg;
         bg.blk = S.blk;
@@ -275,11 +300,18 @@ void launch_particle_fast(const GridP &g, const PhysP &ph, const PState &Pin, co
                           const SortState &S, const NState &N, int vel, int redist, Scalars *sc,
                           hipStream_t s) {
     if (Pin.np <= 0) return;
-    const dim3 gr((unsigned)((S.ntiles + 7) / 8 * 8)), bl(kBlock);
+    const dim3 gr((unsigned)((S.ntiles + 7) / 8 * 8)), bl(kBlock), bl2(2 * kBlock);
     const int O = g.od[0];
+    const bool few = S.waves ? S.waves == 8 : few_tiles(Pin.np);
 #define XMPM_PT(o, v, dl)                                                                     \
-    hipLaunchKernelGGL((k_particle_tile<o, v, dl>), gr, bl, 0, s, g, ph, Pin, Pout, S, N,   \
-                       redist, sc)
+    do {                                                                                      \
+        if (few)                                                                              \
+            hipLaunchKernelGGL((k_particle_tile<o, v, dl, 2 * kBlock>), gr, bl2, 0, s, g, ph, \
+                               Pin, Pout, S, N, redist, sc);                                  \
+        else                                                                                  \
+            hipLaunchKernelGGL((k_particle_tile<o, v, dl, kBlock>), gr, bl, 0, s, g, ph, Pin,  \
+                               Pout, S, N, redist, sc);                                       \
+    } while (0)
     if (ph.delta) {
         if (O == 1) { if (vel) XMPM_PT(1, true, true); else XMPM_PT(1, false, true); }
         else if (O == 2) { if (vel) XMPM_PT(2, true, true); else XMPM_PT(2, false, true); }

@@ -355,17 +355,17 @@ constexpr int kZN = kTN * kTN * 4;
 // particle state through the permutation (a tile with more particles reads
 // the rest from global memory).
 constexpr int kPermLds = 1024;
-template <int NC, bool DET = false, int NPERM = 1>
+template <int NC, bool DET = false, int NPERM = 1, int NW = 4>
 struct P2gLds {
     double slab[DET ? 4 * NC * kZN : NC * kTNN];
-    double pk[4][kPkArea];       // staging area per wavefront
+    double pk[NW][kPkArea];      // staging area per wavefront
     int cs[kCellsPerTile + 1];
     int perm[NPERM];
     SplineTables tab;
 };
 // The LDS of p2g_tile<O, FORCES, FALLBACK, GATHER, DET>.
-template <bool FORCES, bool GATHER, bool DET>
-using P2gLdsOf = P2gLds<FORCES ? 7 : 3, DET, (GATHER || !FORCES) ? kPermLds : 1>;
+template <bool FORCES, bool GATHER, bool DET, int NW = 4>
+using P2gLdsOf = P2gLds<FORCES ? 7 : 3, DET, (GATHER || !FORCES) ? kPermLds : 1, NW>;
 
 // LDS of one particle tile (NCV node components).
 template <int NCV>
@@ -377,11 +377,15 @@ struct PartLds {
 // DET: the deterministic form (SortState::blk): private wavefront slabs and
 // the whole node block written to list position pos of S.blk, stamped with
 // pass in S.tile_slot; no atomics (but those of the FALLBACK global path).
-template <int O, bool FORCES, bool FALLBACK, bool GATHER = false, bool DET = false>
+// NW wavefronts per tile: 4 (one z-slice each) or 8 (a z-slice's first or
+// second pair of cell rows each, half the chain of rounds per wavefront).
+template <int O, bool FORCES, bool FALLBACK, bool GATHER = false, bool DET = false, int NW = 4>
 __device__ __forceinline__ void p2g_tile(const GridP &g, const PhysP &ph, const PState &P,
                                          const SortState &S, double *acc, int epoch, int *err,
-                                         int tile, P2gLdsOf<FORCES, GATHER, DET> &L,
+                                         int tile, P2gLdsOf<FORCES, GATHER, DET, NW> &L,
                                          int pos = 0, int pass = 0) {
+    static_assert(NW == 4 || (NW == 8 && !DET), "8 wavefronts per tile: atomic form only");
+    constexpr int NT = NW * 64;
     constexpr int NC = FORCES ? 7 : 3;
     constexpr int kSlabN = DET ? 4 * NC * kZN : NC * kTNN;
     auto &slab = L.slab;
@@ -399,14 +403,14 @@ __device__ __forceinline__ void p2g_tile(const GridP &g, const PhysP &ph, const 
     __syncthreads();  // the workgroup's previous tile is done with the LDS
     if (tid <= kCellsPerTile) cs[tid] = S.cell_start[tile * kCellsPerTile + tid];
     if (kPermL)
-        for (int e = tid; e < pnum; e += kBlock) L.perm[e] = S.perm[pbeg + e];
+        for (int e = tid; e < pnum; e += NT) L.perm[e] = S.perm[pbeg + e];
     if (O != 1) {
         const int *src = reinterpret_cast<const int *>(&kTab);
         int *dst = reinterpret_cast<int *>(&tab);
-        for (int e = tid; e < (int)(sizeof(SplineTables) / sizeof(int)); e += kBlock)
+        for (int e = tid; e < (int)(sizeof(SplineTables) / sizeof(int)); e += NT)
             dst[e] = src[e];
     }
-    for (int e = tid; e < kSlabN; e += kBlock) slab[e] = 0.0;
+    for (int e = tid; e < kSlabN; e += NT) slab[e] = 0.0;
     __syncthreads();
     if (cs[kCellsPerTile] == cs[0]) return;  // empty tile (uniform over the block)
     const int wave = tid >> 6, lane = tid & 63;
@@ -417,7 +421,10 @@ __device__ __forceinline__ void p2g_tile(const GridP &g, const PhysP &ph, const 
     const int T0[3] = {tx * kTileCells, ty * kTileCells, tz * kTileCells};
     const int64_t cap = P.cap, pc = S.pcap;
     const SplineTables &TT = (O != 1) ? tab : kTab;
-    const int cz = wave;
+    const int cz = wave & 3;
+    // the wavefront's cell rows [r0, r1) of its z-slice
+    const int r0 = (NW == 4) ? 0 : 2 * (wave >> 2);
+    const int r1 = (NW == 4) ? kTileCells : r0 + 2;
     // accumulation role: group q (cell x inside the tile), lane (ky, kz)
     const int q = lane >> 4;
     const int gl = lane & 15;
@@ -478,7 +485,7 @@ __device__ __forceinline__ void p2g_tile(const GridP &g, const PhysP &ph, const 
     // particle j: part 0 m and v, part 1 m, parts 2 and 3 V and three stress
     // components (the products are formed at staging).
     auto fetch_j = [&](int ry, int base) -> int {
-        if (ry >= kTileCells) return -1;
+        if (ry >= r1) return -1;
         const int c0 = kTileCells * (ry + kTileCells * cz) + s_cell;
         const int sbeg = cs[c0], scnt = cs[c0 + 1] - sbeg;
         const int t = base + s_t;
@@ -515,12 +522,12 @@ __device__ __forceinline__ void p2g_tile(const GridP &g, const PhysP &ph, const 
     };
     auto advance = [&](int &ry, int &base) {
         base += 4;
-        while (ry < kTileCells && base >= mc_of(ry)) {
+        while (ry < r1 && base >= mc_of(ry)) {
             ++ry;
             base = 0;
         }
     };
-    int ry = 0, base = -4;
+    int ry = r0, base = -4;
     advance(ry, base);
     double a4[4][NC];
 #pragma unroll
@@ -528,12 +535,12 @@ __device__ __forceinline__ void p2g_tile(const GridP &g, const PhysP &ph, const 
 #pragma unroll
         for (int c = 0; c < NC; ++c) a4[k][c] = 0.0;
     StageRaw cur;
-    if (ry < kTileCells) cur = (FORCES && GATHER) ? load_g(fetch_j(ry, base)) : load(ry, base);
-    while (ry < kTileCells) {
+    if (ry < r1) cur = (FORCES && GATHER) ? load_g(fetch_j(ry, base)) : load(ry, base);
+    while (ry < r1) {
         int nry = ry, nbase = base;
         advance(nry, nbase);
         StageRaw nxt;
-        if (nry < kTileCells)
+        if (nry < r1)
             nxt = (FORCES && GATHER) ? load_g(fetch_j(nry, nbase)) : load(nry, nbase);
         // ---- staging: this round's 16 packets
         if (FORCES && GATHER) {
@@ -715,7 +722,7 @@ __device__ __forceinline__ void p2g_tile(const GridP &g, const PhysP &ph, const 
     const int64_t nn = g.nn;
     double *blk = DET ? S.blk + (int64_t)pos * NC * kTNN : nullptr;
     if (DET && tid == 0) S.tile_slot[tile] = ((int64_t)pass << 32) | (uint32_t)pos;
-    for (int e = tid; e < kTNN; e += kBlock) {
+    for (int e = tid; e < kTNN; e += NT) {
         const int ax = e % kTN, ay = (e / kTN) % kTN, az = e / (kTN * kTN);
         double v[NC];
         if (DET) {
@@ -890,7 +897,7 @@ __device__ __forceinline__ bool g2p_block(const GridP &g, const SplineTables &T,
 // in LDS, then each thread takes one particle of the tile: G2P (g2p_block),
 // F, D, move + wall BCs, redistribution, volume, stress, written in sorted
 // order.  The next step's dt candidates are min-reduced into cmin.
-template <int O, bool VEL, bool DELTA>
+template <int O, bool VEL, bool DELTA, int NT = kBlock>
 __device__ __forceinline__ void particle_tile(const GridP &g, const PhysP &ph, const PState &Pin,
                                               const PState &Pout, const SortState &S,
                                               const NState &N, int redist, double dt,
@@ -909,7 +916,7 @@ __device__ __forceinline__ void particle_tile(const GridP &g, const PhysP &ph, c
     const int tz = tile / (g.ntc[0] * g.ntc[1]);
     const int T0[3] = {tx * kTileCells, ty * kTileCells, tz * kTileCells};
     const int64_t nn = g.nn;
-    for (int e = tid; e < kTNN; e += kBlock) {
+    for (int e = tid; e < kTNN; e += NT) {
         const int a[3] = {e % kTN, (e / kTN) % kTN, e / (kTN * kTN)};
         bool ok = true;
         int I[3];
@@ -924,12 +931,12 @@ __device__ __forceinline__ void particle_tile(const GridP &g, const PhysP &ph, c
         static_assert(sizeof(SplineTables) % sizeof(int) == 0, "table copy by words");
         const int *src = reinterpret_cast<const int *>(&kTab);
         int *dst = reinterpret_cast<int *>(&tab);
-        for (int e = tid; e < (int)(sizeof(SplineTables) / sizeof(int)); e += kBlock)
+        for (int e = tid; e < (int)(sizeof(SplineTables) / sizeof(int)); e += NT)
             dst[e] = src[e];
     }
     __syncthreads();
     const int64_t ci = Pin.cap, co = Pout.cap;
-    const int niter = (end - beg + kBlock - 1) / kBlock;
+    const int niter = (end - beg + NT - 1) / NT;
     // The permutation entry and the position of a thread's next particle are
     // read one iteration ahead, before the current particle's writes (reads
     // and writes complete in issue order, so a read issued after the writes
@@ -941,14 +948,14 @@ __device__ __forceinline__ void particle_tile(const GridP &g, const PhysP &ph, c
         for (int d = 0; d < 3; ++d) xn[d] = stream_load(&Pin.pos[d * ci + jn]);
     }
     for (int it = 0; it < niter; ++it) {
-        const int64_t i = beg + (int64_t)it * kBlock + tid;
+        const int64_t i = beg + (int64_t)it * NT + tid;
         if (i >= end) continue;
         const int64_t j = jn;
         double x[3], v[3], vp[3], D[6], F[9];
         for (int d = 0; d < 3; ++d) x[d] = xn[d];
-        const bool has_next = i + kBlock < end;
+        const bool has_next = i + NT < end;
         // unconditional (a clamped index), so no branch waits for it
-        const int jnext = S.perm[has_next ? i + kBlock : i];
+        const int jnext = S.perm[has_next ? i + NT : i];
         // The gather waits for x only if x is read first: reads complete in
         // issue order.
         asm volatile("" ::: "memory");

@@ -16,6 +16,11 @@ dv_ref = rms of the training targets.
 
     python -m exagoop_amd.learned --out DIR [--train-seeds 64] [--eval-seeds 16]
 
+Training states are taken every 40 fine steps over the first 800 (samples =
+20), so the network sees the flow as far as the long evaluation horizon; a
+network trained on the first 400 steps only had held-out errors of 10-70 %
+with dozens of guard fallbacks at 800 steps.
+
 writes DIR/learned_dam_break.npz (parameters + scales) and
 DIR/learned_dam_break.json (training losses; per held-out seed: relative L2
 velocity error of the learned and of the uncorrected coarse run against the
@@ -147,7 +152,7 @@ def context(seed, **kw):
     return g
 
 
-def collect(seeds, samples=10, spacing=40, rows=8000, capacity=6_000_000, log=print,
+def collect(seeds, samples=20, spacing=40, rows=8000, capacity=6_000_000, log=print,
             prob=None, **kw):
     """Training rows from the plain trajectories of the given seeds: at
     `samples` points `spacing` fine steps apart, one collection of up to
@@ -212,8 +217,10 @@ def rel_l2(a, b):
 
 
 def evaluate(net, seeds, nsteps=400, guards=(0.05, 0.5), log=print, prob=None, **kw):
-    """Fine plain run of nsteps vs nsteps/NFINE learned (and uncorrected)
-    coarse steps from the same initial state."""
+    """Fine plain run of nsteps vs nsteps/NFINE learned and uncorrected
+    coarse steps from the same initial state; the uncorrected steps both with
+    the plain PIC/FLIP blend alpha ("coarse") and with alpha^NFINE
+    ("coarse_blend", MPM_OPT_COARSE_BLEND)."""
     prob = prob or DAM_BREAK
     out = []
     for seed in seeds:
@@ -231,12 +238,14 @@ def evaluate(net, seeds, nsteps=400, guards=(0.05, 0.5), log=print, prob=None, *
         else:
             vf = velocities_by_id(g, n)
         g.close()
-        res = {"seed": seed, "fine_s": t_fine, **prob.describe(seed)}
-        for name, model in (("learned", net), ("coarse", None)):
+        res = {"seed": seed, "steps": nsteps, "fine_s": t_fine, **prob.describe(seed)}
+        for name, model, blend in (("learned", net, 0), ("coarse", None, 0),
+                                   ("coarse_blend", None, 1)):
             if n is None:
                 g = context(seed, **kw)
             else:
                 g, _ = prob_context(prob, seed)
+            g.set_option(capi.OPT_COARSE_BLEND, blend)
             capi_sync(g)
             t0 = time.perf_counter()
             fb = capi.step_learned(g, model, nsteps // NFINE, NFINE, guards)
@@ -258,8 +267,9 @@ def evaluate(net, seeds, nsteps=400, guards=(0.05, 0.5), log=print, prob=None, *
         res["speedup"] = res["fine_s"] / res["learned_s"]
         out.append(res)
         if log:
-            log(f"[learned] seed {seed}: err learned {res['learned_err']:.3e} coarse "
-                f"{res['coarse_err']:.3e} fallbacks {res['learned_fallbacks']} speedup "
+            log(f"[learned] seed {seed}, {nsteps} fine steps: err learned "
+                f"{res['learned_err']:.3e} coarse {res['coarse_err']:.3e} (alpha^{NFINE} blend "
+                f"{res['coarse_blend_err']:.3e}) fallbacks {res['learned_fallbacks']} speedup "
                 f"{res['speedup']:.2f}")
     return out
 
@@ -293,6 +303,22 @@ def demo():
     print(f"[learned demo] training losses {losses}", flush=True)
 
 
+def eval_summary(net, seeds, steps, steps_long=0, prob=None, log=print):
+    """Held-out evaluation at one or two horizons, with per-horizon means."""
+    out = {}
+    for key, n in (("eval", steps), ("eval_long", steps_long)):
+        if not n:
+            continue
+        ev = evaluate(net, seeds, nsteps=n, prob=prob, log=log)
+        sfx = "" if key == "eval" else "_long"
+        out[key] = ev
+        out["steps" + sfx] = n
+        for f in ("learned_err", "coarse_err", "coarse_blend_err", "speedup"):
+            out["mean_" + f + sfx] = float(np.mean([e[f] for e in ev]))
+        out["fallbacks" + sfx] = int(sum(e["learned_fallbacks"] for e in ev))
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--demo", action="store_true",
@@ -304,11 +330,15 @@ def main(argv=None):
     ap.add_argument("--eval-seeds", type=int, default=None)
     ap.add_argument("--epochs", type=int, default=400)
     ap.add_argument("--eval-steps", type=int, default=400)
-    ap.add_argument("--samples", type=int, default=10)
+    ap.add_argument("--eval-steps-long", type=int, default=800,
+                    help="second evaluation horizon (0: none)")
+    ap.add_argument("--samples", type=int, default=20)
     ap.add_argument("--spacing", type=int, default=40)
     ap.add_argument("--rows", type=int, default=16000)
-    ap.add_argument("--capacity", type=int, default=12_000_000,
+    ap.add_argument("--capacity", type=int, default=24_000_000,
                     help="training rows held on the device (rows beyond it are dropped)")
+    ap.add_argument("--eval-only", metavar="NPZ",
+                    help="evaluate this network (no collection or training)")
     a = ap.parse_args(argv)
     if a.demo:
         demo()
@@ -319,6 +349,15 @@ def main(argv=None):
     prob = problem(a.problem, a.res)
     train_seeds = prob.train_seeds[:a.train_seeds] if a.train_seeds else prob.train_seeds
     eval_seeds = prob.eval_seeds[:a.eval_seeds] if a.eval_seeds else prob.eval_seeds
+    if a.eval_only:
+        summary = {"problem": prob.name, "network": os.path.basename(a.eval_only)}
+        summary.update(eval_summary(load_net(a.eval_only), eval_seeds, a.eval_steps,
+                                    a.eval_steps_long, prob))
+        with open(os.path.join(a.out, "learned_%s_eval.json" % prob.name), "w") as f:
+            json.dump(summary, f, indent=1)
+        print(json.dumps({k: v for k, v in summary.items() if k not in ("eval", "eval_long")}),
+              flush=True)
+        return
     t0 = time.perf_counter()
     ds = collect(train_seeds, samples=a.samples, spacing=a.spacing, rows=a.rows, prob=prob,
                  capacity=a.capacity)
@@ -326,21 +365,17 @@ def main(argv=None):
     net, losses = train(ds, epochs=a.epochs, prob=prob)
     t2 = time.perf_counter()
     save(net, os.path.join(a.out, "learned_%s.npz" % prob.name))
-    ev = evaluate(net, eval_seeds, nsteps=a.eval_steps, prob=prob)
     summary = {
         "problem": prob.name, "train_seeds": len(train_seeds), "fine_dt": prob.dt,
+        "samples": a.samples, "spacing": a.spacing,
         "rows": ds.size(), "dv_ref": float(net.get_params()[1][2]),
         "collect_s": t1 - t0, "train_s": t2 - t1, "losses": losses,
-        "eval": ev,
-        "mean_learned_err": float(np.mean([e["learned_err"] for e in ev])),
-        "mean_coarse_err": float(np.mean([e["coarse_err"] for e in ev])),
-        "mean_speedup": float(np.mean([e["speedup"] for e in ev])),
-        "fallbacks": int(sum(e["learned_fallbacks"] for e in ev)),
     }
+    summary.update(eval_summary(net, eval_seeds, a.eval_steps, a.eval_steps_long, prob))
     with open(os.path.join(a.out, "learned_%s.json" % prob.name), "w") as f:
         json.dump(summary, f, indent=1)
-    print(json.dumps({k: v for k, v in summary.items() if k not in ("eval", "losses")}),
-          flush=True)
+    print(json.dumps({k: v for k, v in summary.items()
+                      if k not in ("eval", "eval_long", "losses")}), flush=True)
 
 
 if __name__ == "__main__":

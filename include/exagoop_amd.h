@@ -377,13 +377,22 @@ int mpm_energies_noalloc(int64_t np, int64_t stride, const double *rdata, const 
  * their sorted tile (redistributed every step, or not yet outside the
  * domain), and not to MUSL's re-projection deposit with periodic faces (the
  * images' start-of-step velocities go through the atomic per-particle
- * deposit); it allocates one 7 x 343-double node block per cell tile. */
+ * deposit); it allocates one 7 x 343-double node block per cell tile.
+ * MPM_OPT_TILE_WAVES: wavefronts per cell tile of the sorted step's P2G and
+ * fused particle kernels, 4 or 8 (8: shorter per-tile chains for scenes with
+ * fewer than one full tile per compute unit), 0 (default) to choose by the
+ * particle count.  The deterministic P2G always runs 4.
+ * MPM_OPT_COARSE_BLEND (default 0): the coarse steps of mpm_step_learned
+ * blend PIC and FLIP with alpha^nfine instead of alpha in their particle
+ * update, the blend of the nfine fine steps a coarse step stands for. */
 #define MPM_OPT_PROFILE 1
 #define MPM_OPT_KEEP_FORCES 2
 #define MPM_OPT_SORTED 3
 #define MPM_OPT_FAST 4
 #define MPM_OPT_FUSED_DT 5
 #define MPM_OPT_DETERMINISTIC 6
+#define MPM_OPT_COARSE_BLEND 7
+#define MPM_OPT_TILE_WAVES 8
 int mpm_set_option(mpm_ctx *ctx, int option, int value);
 
 /* Last HIP error string (static storage). */

@@ -29,6 +29,14 @@ def test_library_exports_every_declared_symbol():
     assert set(names) == set(capi.SIGNATURES), set(names) ^ set(capi.SIGNATURES)
 
 
+def test_option_numbers_match_header():
+    txt = open(HEADER).read()
+    defs = dict((k, int(v)) for k, v in re.findall(r"#define MPM_OPT_([A-Z_]+) (\d+)", txt))
+    assert len(defs) >= 7
+    for name, value in defs.items():
+        assert getattr(capi, "OPT_" + name) == value, name
+
+
 def test_specs_layout_matches_c():
     fields = [f[0] for f in capi.Specs._fields_]
     prog = ["#include <stdio.h>", "#include <stddef.h>", f'#include "{HEADER}"', "int main(){",

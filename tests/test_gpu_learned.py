@@ -129,8 +129,10 @@ def test_features_match_oracle(gpu):
     assert np.array_equal(X, oracle.bf16_round(ref))
 
 
-def test_uncorrected_single_fine_coarse_step_is_the_plain_step(gpu):
-    say("[learned] coarse step of one fine dt, no network == plain step")
+@pytest.mark.parametrize("blend", [0, 1])
+def test_uncorrected_single_fine_coarse_step_is_the_plain_step(gpu, blend):
+    # with nfine = 1 the coarse blend alpha^nfine is alpha itself
+    say(f"[learned] coarse step of one fine dt, no network == plain step (blend option {blend})")
     s, pos, r, idata = small_fluid()
     a = capi.Context(s)
     b = capi.Context(s)
@@ -138,6 +140,7 @@ def test_uncorrected_single_fine_coarse_step_is_the_plain_step(gpu):
         c.upload(pos, r, idata)
         c.init_internal_forces()
         c.step(2)
+    a.set_option(capi.OPT_COARSE_BLEND, blend)
     fb = capi.step_learned(a, None, 5, 1)
     b.step(5)
     assert fb == 0
@@ -310,22 +313,28 @@ def test_speculative_steps_equal_one_step_calls(gpu, case):
     assert ta[:2] == tb[:2]
 
 
-def test_trained_network_gains_over_the_uncorrected_coarse_step(gpu):
+@pytest.mark.parametrize("nsteps", [100, 800])
+def test_trained_network_gains_over_the_uncorrected_coarse_step(gpu, nsteps):
     """The committed network (exagoop_amd/data/learned_dam_break.npz, trained
-    by exagoop_amd.learned on seeds 0..63) on a held-out dam-break geometry:
-    100 fine steps of dt = 2.5e-5 against 25 coarse steps of 1e-4 with and
-    without the correction."""
+    by exagoop_amd.learned on seeds 0..63 over their first 800 fine steps) on
+    a held-out dam-break geometry: nsteps fine steps of dt = 2.5e-5 against
+    nsteps / 4 coarse steps of 1e-4 with the correction, and without it both
+    with the plain PIC/FLIP blend and with the blend alpha^4 of the four fine
+    steps (MPM_OPT_COARSE_BLEND), the better of the two uncorrected forms."""
     from exagoop_amd import learned
-    say("[learned] trained network vs uncorrected coarse step, held-out seed 1003")
+    say(f"[learned] trained network vs uncorrected coarse steps, held-out seed 1003, "
+        f"{nsteps} fine steps")
     net = learned.load_net()
-    seed, nsteps = 1003, 100
+    seed = 1003
     g = learned.context(seed)
     g.step(nsteps)
     vf, idf = learned.velocities(g)
     g.close()
     errs = {}
-    for name, model in (("learned", net), ("coarse", None)):
+    for name, model, blend in (("learned", net, 0), ("coarse", None, 0),
+                               ("coarse_blend", None, 1)):
         g = learned.context(seed)
+        g.set_option(capi.OPT_COARSE_BLEND, blend)
         fb = capi.step_learned(g, model, nsteps // learned.NFINE, learned.NFINE)
         v, ids = learned.velocities(g)
         g.close()
@@ -333,4 +342,5 @@ def test_trained_network_gains_over_the_uncorrected_coarse_step(gpu):
         assert fb == 0
         errs[name] = learned.rel_l2(v, vf)
     say(f"    errors {errs}")
-    assert errs["learned"] <= 0.5 * errs["coarse"], errs
+    assert errs["coarse_blend"] < errs["coarse"], errs
+    assert errs["learned"] <= 0.5 * errs["coarse_blend"], errs

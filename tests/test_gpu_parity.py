@@ -91,12 +91,19 @@ def apply_tweaks(s, tw):
             setattr(s, k, v)
 
 
+# The sorted step's tile kernels run 4 or 8 wavefronts per tile
+# (MPM_OPT_TILE_WAVES; by default chosen by the particle count): both forms.
+WAVES = [4, 8]
+
+
+@pytest.mark.parametrize("waves", WAVES)
 @pytest.mark.parametrize("name,kw,tw", CASES, ids=[c[0] for c in CASES])
-def test_one_step_parity(gpu, name, kw, tw):
-    say(f"[parity one-step] {name}")
+def test_one_step_parity(gpu, name, kw, tw, waves):
+    say(f"[parity one-step] {name}, {waves} wavefronts per tile")
     s, pos, r, idata = scenes.random_block(17, **kw)
     apply_tweaks(s, tw)
     g, o = pair(s, pos, r, idata)
+    g.set_option(capi.OPT_TILE_WAVES, waves)
     g.init_internal_forces()
     o.init_internal_forces()
     compare_particles(g, o, 1e-12)
@@ -109,12 +116,14 @@ def test_one_step_parity(gpu, name, kw, tw):
     assert sg == so and dtg == dto and tg == to
 
 
+@pytest.mark.parametrize("waves", WAVES)
 @pytest.mark.parametrize("name,kw,tw", CASES[:6] + CASES[9:11], ids=[c[0] for c in CASES[:6] + CASES[9:11]])
-def test_multi_step_parity(gpu, name, kw, tw):
-    say(f"[parity 40 steps] {name}")
+def test_multi_step_parity(gpu, name, kw, tw, waves):
+    say(f"[parity 40 steps] {name}, {waves} wavefronts per tile")
     s, pos, r, idata = scenes.random_block(23, **kw)
     apply_tweaks(s, tw)
     g, o = pair(s, pos, r, idata)
+    g.set_option(capi.OPT_TILE_WAVES, waves)
     g.init_internal_forces()
     o.init_internal_forces()
     g.step(40)
