@@ -136,6 +136,7 @@ SIGNATURES = {
     "mpm_num_particles": (ctypes.c_int, [_P, _LP]),
     "mpm_download_particles": (ctypes.c_int, [_P, _DP, _DP, _IP, _LP]),
     "mpm_get_nodal": (ctypes.c_int, [_P, ctypes.c_int, _DP]),
+    "mpm_set_nodal": (ctypes.c_int, [_P, ctypes.c_int, _DP]),
     "mpm_read_particle_file": (ctypes.c_int, [ctypes.c_char_p, _LP, _DP, _DP, _IP, _DP]),
     "mpm_load_particle_file": (ctypes.c_int, [_P, ctypes.c_char_p, _DP]),
     "mpm_autogen_particles": (ctypes.c_int, [ctypes.POINTER(Specs), _DP, _DP, _DP, _IP,
@@ -438,6 +439,11 @@ class Context(_Base):
 
     def _errstr(self):
         return self._lib.mpm_last_error().decode()
+
+    def set_nodal(self, comp, values):
+        a = np.ascontiguousarray(values, dtype=np.float64)
+        assert a.shape == (self.nn,)
+        self._call("set_nodal", self._ctx, comp, dptr(a))
 
     def set_option(self, opt, value):
         self._call("set_option", self._ctx, opt, value)

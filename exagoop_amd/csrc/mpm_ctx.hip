@@ -722,6 +722,25 @@ int mpm_get_nodal(mpm_ctx *c, int comp, double *out) {
     return MPM_OK;
 }
 
+int mpm_set_nodal(mpm_ctx *c, int comp, const double *in) {
+    const int ncomp = c && c->thermal ? (int)MPM_N_NCOMP_TEMP : (int)MPM_N_NCOMP;
+    if (!c || !in || comp < 0 || comp >= ncomp) return set_err(MPM_E_ARG, "bad comp");
+    const int64_t nn = c->g.nn;
+    double *dst = nullptr;
+    if (comp >= MPM_N_MASS_SPHEAT) dst = c->tn + (comp - MPM_N_MASS_SPHEAT) * nn;
+    else if (comp == MPM_N_MASS) dst = c->N.mass;
+    else if (comp >= MPM_N_VEL && comp < MPM_N_VEL + 3) dst = c->N.vel + (comp - MPM_N_VEL) * nn;
+    else if (comp >= MPM_N_FRC && comp < MPM_N_FRC + 3) dst = c->N.frc + (comp - MPM_N_FRC) * nn;
+    else if (comp >= MPM_N_DVEL && comp < MPM_N_DVEL + 3)
+        dst = c->N.dvel + (comp - MPM_N_DVEL) * nn;
+    else if (comp == MPM_N_MASS_OLD) dst = c->N.mold;
+    if (!dst) return set_err(MPM_E_ARG, "nodal component is not stored");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(dst, in, nn * sizeof(double), hipMemcpyHostToDevice));
+    c->all_dirty = true;  // the next grid pass visits every node
+    return MPM_OK;
+}
+
 int mpm_reset_nodal(mpm_ctx *c) {
     if (!c) return set_err(MPM_E_ARG, "null context");
     c->all_dirty = true;

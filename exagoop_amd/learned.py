@@ -274,6 +274,78 @@ def evaluate(net, seeds, nsteps=400, guards=(0.05, 0.5), log=print, prob=None, *
     return out
 
 
+def nodal_floor(nsteps=200, modes=("none", "end", "two"), log=print):
+    """Error floor of any nodal velocity correction, on the benchmark dam
+    break: coarse steps of NFINE fine dt through the per-stage entry points,
+    each given the exact correction of its own state, taken from NFINE fine
+    steps run from that state in a second context.  Modes: "none" (no
+    correction), "end" (nodal velocity and FLIP increment corrected to the
+    fine nodal velocity after NFINE steps, the network's form) and "two"
+    (nodal velocity set to the mean of the NFINE fine nodal velocities, which
+    moves the particles and updates F as the fine steps do, and the FLIP
+    increment set so that the PIC/FLIP blend still ends at the last fine
+    nodal velocity).  Returns {mode: relative L2 particle-velocity error
+    against the fine run}."""
+    from .capi import N_DVEL, N_VEL
+
+    def make(dt):
+        s, pos, r, idata = scenes.dam_break()
+        s.fixed_timestep = 1
+        s.timestep = dt
+        g = capi.Context(s)
+        g.upload(pos, r, idata)
+        g.init_internal_forces()
+        return g, s
+
+    F, sf = make(DT)
+    F.step(nsteps)
+    vf = velocities(F)
+    F.close()
+    alpha = sf.alpha_pic_flip
+    out = {}
+    for mode in modes:
+        A, _ = make(NFINE * DT)
+        t, dtc = 0.0, NFINE * DT
+        for k in range(nsteps // NFINE):
+            A.reset_nodal()
+            A.p2g(1, 1, 1)
+            A.backup_velocity()
+            A.nodal_update(dtc)
+            A.apply_nodal_bcs(dtc, t)
+            if mode != "none":
+                vc = np.stack([A.nodal(N_VEL + d) for d in range(3)])
+                dvc = np.stack([A.nodal(N_DVEL + d) for d in range(3)])
+                pos, r, idata, _ = A.download()
+                B = capi.Context(sf)
+                B.upload(pos, r, idata)
+                vs = []
+                for _ in range(NFINE):
+                    B.step(1)
+                    vs.append(np.stack([B.nodal(N_VEL + d) for d in range(3)]))
+                B.close()
+                vend, vmean = vs[-1], sum(vs) / NFINE
+                if mode == "end":
+                    vn, dvn = vend, dvc + (vend - vc)
+                else:
+                    vn = vmean
+                    dvn = dvc + (vend - vc) + ((1 - alpha) / alpha) * (vend - vmean)
+                for d in range(3):
+                    A.set_nodal(N_VEL + d, vn[d])
+                    A.set_nodal(N_DVEL + d, dvn[d])
+            A.g2p(1, 1, dtc)
+            A.move(dtc, k + 1)
+            A.update_volume()
+            A.stress(dtc, t)
+            t += dtc
+        v = velocities(A)
+        A.close()
+        out[mode] = rel_l2(v[0], vf[0]) if np.array_equal(v[1], vf[1]) else float("inf")
+        if log:
+            log(f"[learned] nodal correction floor, {nsteps} fine steps, {mode}: "
+                f"{out[mode]:.3e}")
+    return out
+
+
 def capi_sync(g):
     g.time()  # synchronises the context's stream
 
@@ -337,11 +409,17 @@ def main(argv=None):
     ap.add_argument("--rows", type=int, default=16000)
     ap.add_argument("--capacity", type=int, default=24_000_000,
                     help="training rows held on the device (rows beyond it are dropped)")
+    ap.add_argument("--floor", type=int, metavar="NSTEPS",
+                    help="error floor of exact nodal corrections over NSTEPS fine steps "
+                         "(benchmark dam break), no outputs")
     ap.add_argument("--eval-only", metavar="NPZ",
                     help="evaluate this network (no collection or training)")
     a = ap.parse_args(argv)
     if a.demo:
         demo()
+        return
+    if a.floor:
+        print(json.dumps(nodal_floor(a.floor)), flush=True)
         return
     if not a.out:
         ap.error("--out is required")

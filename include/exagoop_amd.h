@@ -233,6 +233,12 @@ int mpm_get_particle_field(mpm_ctx *ctx, int field, int64_t n_upload, double *ou
  * the thermal module enabled) of the valid nodal box, (n0+1)(n1+1)(n2+1)
  * values, i fastest. */
 int mpm_get_nodal(mpm_ctx *ctx, int comp, double *out);
+/* Overwrites nodal component comp with in[], laid out as mpm_get_nodal
+ * returns it.  In the reference the caller owns the nodal MultiFab
+ * (main.cpp:83-187 hands nodaldata to each step function) and may write it
+ * between calls; components the build does not store (the rigid-body and
+ * normal components) are an error. */
+int mpm_set_nodal(mpm_ctx *ctx, int comp, const double *in);
 
 /* ------------------------------------------------------------------------ */
 /* MPMParticleContainer methods (mpm_particle_container.H:150-577)          */

@@ -17,7 +17,7 @@ import pytest
 import oracle
 from exagoop_amd import capi, scenes
 from exagoop_amd.capi import (BC_NOSLIP, BC_PARTIALSLIP, BC_PERIODIC, BC_SLIP, MUSL, N_MASS,
-                              N_VEL, R_MASS, R_STRESS, R_VEL, USL)
+                              N_VEL, R_MASS, R_STRESS, R_VEL, R_VELP, USL)
 
 pytestmark = pytest.mark.gpu
 
@@ -244,3 +244,42 @@ def test_strain_energy_against_oracle(gpu):
         assert abs(eg[q] - eo[q]) <= 1e-10 * abs(eo[q]), (q, eg, eo)
     scale = np.sqrt(2.0 * eo[0] * eo[5])
     assert np.all(np.abs(eg[2:5] - eo[2:5]) <= 1e-10 * scale)
+
+
+# ---------------------------------------------------------- caller-written nodes
+def test_set_nodal_round_trip_and_uniform_field_gather(gpu):
+    """mpm_set_nodal writes what mpm_get_nodal reads back, and G2P of a
+    caller-written uniform nodal velocity U with zero increment gives every
+    particle v' = S U with one weight sum S per particle for all three
+    directions, S = 1 to the precision of the spline tables (partition of
+    unity, every direction periodic), and v = alpha v + (1 - alpha) v' (the
+    PIC/FLIP blend, mpm_particle_grid_ops.cpp:1030-1034)."""
+    say("[nodal] set_nodal round trip, uniform field through G2P")
+    s, pos, r, idata = scenes.random_block(31, order=3, model=0, periodic=(1, 1, 1), n=(8, 8, 8))
+    g = capi.Context(s)
+    g.upload(pos, r, idata)
+    rng = np.random.default_rng(31)
+    vals = rng.standard_normal(g.nn)
+    g.set_nodal(capi.N_MASS_OLD, vals)
+    assert np.array_equal(g.nodal(capi.N_MASS_OLD), vals)
+    U = (0.3, -1.25, 2.0)
+    for d in range(3):
+        g.set_nodal(capi.N_VEL + d, np.full(g.nn, U[d]))
+        g.set_nodal(capi.N_DVEL + d, np.zeros(g.nn))
+    _, r0, _, id0 = g.download()
+    g.g2p(1, 0, 1e-4)
+    _, r1, _, id1 = g.download()
+    assert np.array_equal(id0, id1)
+    # v' = S U with S = sum of the particle's 64 weights: the cubic partition
+    # of unity holds to ~7e-11 per direction (truncated constants.H literals
+    # in the interpolants.H:32-170 tables), so S - 1 is ~2e-10, the same for
+    # every direction.
+    a = s.alpha_pic_flip
+    S = r1[R_VELP] / U[0]
+    assert np.max(np.abs(S - 1.0)) <= 1e-9
+    for d in range(3):
+        assert np.max(np.abs(r1[R_VELP + d] - S * U[d])) <= 8e-15 * abs(U[d])
+        want = a * r0[R_VEL + d] + (1 - a) * r1[R_VELP + d]
+        assert np.max(np.abs(r1[R_VEL + d] - want)) <= 1e-13 * (1 + np.max(np.abs(want)))
+    with pytest.raises(capi.MPMError):
+        g.set_nodal(99, vals)
