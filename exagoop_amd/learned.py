@@ -274,19 +274,27 @@ def evaluate(net, seeds, nsteps=400, guards=(0.05, 0.5), log=print, prob=None, *
     return out
 
 
-def nodal_floor(nsteps=200, modes=("none", "end", "two"), log=print):
-    """Error floor of any nodal velocity correction, on the benchmark dam
+def correction_floor(nsteps=200, modes=("none", "end", "two", "pvel", "pstate", "pnopos"),
+                     log=print):
+    """Error floor of exact corrections of a coarse step, on the benchmark dam
     break: coarse steps of NFINE fine dt through the per-stage entry points,
-    each given the exact correction of its own state, taken from NFINE fine
-    steps run from that state in a second context.  Modes: "none" (no
+    each given an exact correction of its own state, taken from NFINE fine
+    steps run from that state in a second context.  Nodal modes: "none" (no
     correction), "end" (nodal velocity and FLIP increment corrected to the
     fine nodal velocity after NFINE steps, the network's form) and "two"
     (nodal velocity set to the mean of the NFINE fine nodal velocities, which
     moves the particles and updates F as the fine steps do, and the FLIP
     increment set so that the PIC/FLIP blend still ends at the last fine
-    nodal velocity).  Returns {mode: relative L2 particle-velocity error
-    against the fine run}."""
-    from .capi import N_DVEL, N_VEL
+    nodal velocity).  Particle modes, on top of "end": after the coarse step
+    the particles take the fine run's velocity ("pvel"), its stress,
+    deformation gradient, J, volume, density and pressure ("pstate"), or every
+    field but the position ("pnopos").  Returns {mode: relative L2
+    particle-velocity error against the fine run}."""
+    from .capi import N_DVEL, N_VEL, R_VEL
+
+    stress_like = (list(range(capi.R_STRESS, capi.R_STRESS + 6)) +
+                   list(range(capi.R_DEFGRAD, capi.R_DEFGRAD + 9)) +
+                   [capi.R_JACOBIAN, capi.R_VOLUME, capi.R_DENSITY, capi.R_PRESSURE])
 
     def make(dt):
         s, pos, r, idata = scenes.dam_break()
@@ -296,6 +304,11 @@ def nodal_floor(nsteps=200, modes=("none", "end", "two"), log=print):
         g.upload(pos, r, idata)
         g.init_internal_forces()
         return g, s
+
+    def by_id(state):
+        pos, r, idata, ids = state
+        o = np.argsort(ids)
+        return pos[:, o].copy(), r[:, o].copy(), idata[:, o].copy()
 
     F, sf = make(DT)
     F.step(nsteps)
@@ -312,23 +325,26 @@ def nodal_floor(nsteps=200, modes=("none", "end", "two"), log=print):
             A.backup_velocity()
             A.nodal_update(dtc)
             A.apply_nodal_bcs(dtc, t)
+            fine = None
             if mode != "none":
                 vc = np.stack([A.nodal(N_VEL + d) for d in range(3)])
                 dvc = np.stack([A.nodal(N_DVEL + d) for d in range(3)])
-                pos, r, idata, _ = A.download()
+                # the second context takes the particles in id order, so its
+                # ids are the coarse run's
                 B = capi.Context(sf)
-                B.upload(pos, r, idata)
+                B.upload(*by_id(A.download()))
                 vs = []
                 for _ in range(NFINE):
                     B.step(1)
                     vs.append(np.stack([B.nodal(N_VEL + d) for d in range(3)]))
+                fine = by_id(B.download())
                 B.close()
                 vend, vmean = vs[-1], sum(vs) / NFINE
-                if mode == "end":
-                    vn, dvn = vend, dvc + (vend - vc)
-                else:
+                if mode == "two":
                     vn = vmean
                     dvn = dvc + (vend - vc) + ((1 - alpha) / alpha) * (vend - vmean)
+                else:
+                    vn, dvn = vend, dvc + (vend - vc)
                 for d in range(3):
                     A.set_nodal(N_VEL + d, vn[d])
                     A.set_nodal(N_DVEL + d, dvn[d])
@@ -336,12 +352,21 @@ def nodal_floor(nsteps=200, modes=("none", "end", "two"), log=print):
             A.move(dtc, k + 1)
             A.update_volume()
             A.stress(dtc, t)
+            if mode in ("pvel", "pstate", "pnopos"):
+                pa, ra, ia = by_id(A.download())
+                rb = fine[1]
+                if mode == "pnopos":
+                    ra = rb
+                else:
+                    for f in (range(R_VEL, R_VEL + 3) if mode == "pvel" else stress_like):
+                        ra[f] = rb[f]
+                A.upload(pa, ra, ia)
             t += dtc
         v = velocities(A)
         A.close()
         out[mode] = rel_l2(v[0], vf[0]) if np.array_equal(v[1], vf[1]) else float("inf")
         if log:
-            log(f"[learned] nodal correction floor, {nsteps} fine steps, {mode}: "
+            log(f"[learned] exact-correction floor, {nsteps} fine steps, {mode}: "
                 f"{out[mode]:.3e}")
     return out
 
@@ -410,8 +435,8 @@ def main(argv=None):
     ap.add_argument("--capacity", type=int, default=24_000_000,
                     help="training rows held on the device (rows beyond it are dropped)")
     ap.add_argument("--floor", type=int, metavar="NSTEPS",
-                    help="error floor of exact nodal corrections over NSTEPS fine steps "
-                         "(benchmark dam break), no outputs")
+                    help="error floor of exact nodal and particle corrections over NSTEPS "
+                         "fine steps (benchmark dam break), no outputs")
     ap.add_argument("--eval-only", metavar="NPZ",
                     help="evaluate this network (no collection or training)")
     a = ap.parse_args(argv)
@@ -419,7 +444,7 @@ def main(argv=None):
         demo()
         return
     if a.floor:
-        print(json.dumps(nodal_floor(a.floor)), flush=True)
+        print(json.dumps(correction_floor(a.floor)), flush=True)
         return
     if not a.out:
         ap.error("--out is required")
