@@ -274,8 +274,8 @@ def evaluate(net, seeds, nsteps=400, guards=(0.05, 0.5), log=print, prob=None, *
     return out
 
 
-def correction_floor(nsteps=200, modes=("none", "end", "two", "pvel", "pstate", "pnopos"),
-                     log=print):
+def correction_floor(nsteps=200, modes=("none", "end", "two", "pvel", "pstate", "pnopos", "pvj",
+                                        "gvj"), log=print):
     """Error floor of exact corrections of a coarse step, on the benchmark dam
     break: coarse steps of NFINE fine dt through the per-stage entry points,
     each given an exact correction of its own state, taken from NFINE fine
@@ -288,8 +288,13 @@ def correction_floor(nsteps=200, modes=("none", "end", "two", "pvel", "pstate", 
     nodal velocity).  Particle modes, on top of "end": after the coarse step
     the particles take the fine run's velocity ("pvel"), its stress,
     deformation gradient, J, volume, density and pressure ("pstate"), or every
-    field but the position ("pnopos").  Returns {mode: relative L2
-    particle-velocity error against the fine run}."""
+    field but the position ("pnopos").  Velocity and volume modes, on top of
+    "end" and applied after the move, before the volume and stress update:
+    each particle's fine-minus-coarse velocity and ln J, exact ("pvj") or as
+    grid fields ("gvj": deposited with the particles' mass weights, P2G in a
+    scratch context, and interpolated back, G2P); F is scaled by
+    exp(d ln J / 3).  Returns {mode: relative L2 particle-velocity error
+    against the fine run}."""
     from .capi import N_DVEL, N_VEL, R_VEL
 
     stress_like = (list(range(capi.R_STRESS, capi.R_STRESS + 6)) +
@@ -310,6 +315,33 @@ def correction_floor(nsteps=200, modes=("none", "end", "two", "pvel", "pstate", 
         o = np.argsort(ids)
         return pos[:, o].copy(), r[:, o].copy(), idata[:, o].copy()
 
+    def det_f(r):
+        F = r[capi.R_DEFGRAD:capi.R_DEFGRAD + 9].reshape(3, 3, -1)
+        return (F[0, 0] * (F[1, 1] * F[2, 2] - F[1, 2] * F[2, 1]) -
+                F[0, 1] * (F[1, 0] * F[2, 2] - F[1, 2] * F[2, 0]) +
+                F[0, 2] * (F[1, 0] * F[2, 1] - F[1, 1] * F[2, 0]))
+
+    def through_grid(spec, pos, r, idata, fields, dt):
+        # fields (k, n) -> the particles' mass-weighted nodal projection,
+        # interpolated back: P2G of the fields as velocities, then G2P of
+        # those nodal velocities with zero increment (v' = sum N v_node)
+        k = fields.shape[0]
+        rc = r.copy()
+        rc[R_VEL:R_VEL + 3] = 0.0
+        rc[R_VEL:R_VEL + k] = fields
+        C = capi.Context(spec)
+        C.upload(pos, rc, idata)
+        C.reset_nodal()
+        C.p2g(1, 1, 0)
+        nv = [C.nodal(N_VEL + d) for d in range(3)]
+        for d in range(3):
+            C.set_nodal(N_VEL + d, nv[d])
+            C.set_nodal(N_DVEL + d, np.zeros(C.nn))
+        C.g2p(1, 0, dt)
+        back = by_id(C.download())[1][capi.R_VELP:capi.R_VELP + k]
+        C.close()
+        return back
+
     F, sf = make(DT)
     F.step(nsteps)
     vf = velocities(F)
@@ -317,7 +349,7 @@ def correction_floor(nsteps=200, modes=("none", "end", "two", "pvel", "pstate", 
     alpha = sf.alpha_pic_flip
     out = {}
     for mode in modes:
-        A, _ = make(NFINE * DT)
+        A, spec = make(NFINE * DT)
         t, dtc = 0.0, NFINE * DT
         for k in range(nsteps // NFINE):
             A.reset_nodal()
@@ -350,6 +382,17 @@ def correction_floor(nsteps=200, modes=("none", "end", "two", "pvel", "pstate", 
                     A.set_nodal(N_DVEL + d, dvn[d])
             A.g2p(1, 1, dtc)
             A.move(dtc, k + 1)
+            if mode in ("pvj", "gvj"):
+                pa, ra, ia = by_id(A.download())
+                rb = fine[1]
+                dv = rb[R_VEL:R_VEL + 3] - ra[R_VEL:R_VEL + 3]
+                dl = (np.log(rb[capi.R_JACOBIAN]) - np.log(det_f(ra)))[None, :]
+                if mode == "gvj":
+                    dv = through_grid(spec, pa, ra, ia, dv, dtc)
+                    dl = through_grid(spec, pa, ra, ia, dl, dtc)
+                ra[R_VEL:R_VEL + 3] += dv
+                ra[capi.R_DEFGRAD:capi.R_DEFGRAD + 9] *= np.exp(dl[0] / 3.0)
+                A.upload(pa, ra, ia)
             A.update_volume()
             A.stress(dtc, t)
             if mode in ("pvel", "pstate", "pnopos"):
