@@ -22,7 +22,8 @@ independent scenes (seeds 0..255: elastic disks or dam break, 2k-64k
 particles, 16^3-48^3 cells, quadratic B-splines, FP64), one persistent kernel
 launch per step replayed from a HIP graph, decomposition chosen by the
 library's autotuner, 500 timed steps: particle-updates/s over all scenes and
-the step's algorithmic HBM bytes per second.
+the step's algorithmic HBM bytes per second; then the same ensemble with the
+bitwise-reproducible step (MPM_OPT_DETERMINISTIC).
 
 The column object runs the twisted column at full size (256 x 256 x 896
 cells, 100 663 296 particles, cubic, exagoop_amd/column.py): 20 timed steps
@@ -405,6 +406,19 @@ def batch_leg(nscenes, nsteps):
     print(f"[bench] batch: {nscenes} scenes, {npart} particles, {st['step_ms']:.3f} ms/step, "
           f"{out['value'] / 1e9:.3f} G p-upd/s, tuning {best}", file=sys.stderr, flush=True)
     b.close()
+    # the same ensemble with the bitwise-reproducible step (MPM_OPT_DETERMINISTIC)
+    bd, _ = ensemble.make_batch(range(nscenes), deterministic=True)
+    best_d, _ = bd.autotune(trial_steps=3)
+    bd.step(10)
+    bd.step(nsteps)
+    std = bd.stats()
+    out["deterministic"] = {"ms_per_step": round(std["step_ms"], 5),
+                            "value": round(npart * 1e3 / std["step_ms"], 1),
+                            "ratio": round(std["step_ms"] / st["step_ms"], 3),
+                            "tuning": best_d}
+    print(f"[bench] batch, deterministic step: {std['step_ms']:.3f} ms/step "
+          f"({out['deterministic']['ratio']}x)", file=sys.stderr, flush=True)
+    bd.close()
     return out
 
 

@@ -208,6 +208,7 @@ SIGNATURES = {
     "mpm_batch_autotune": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_void_p, _DP]),
     "mpm_batch_stats": (ctypes.c_int, [_P, _DP, _LP, _IP]),
     "mpm_batch_profile": (ctypes.c_int, [_P, ctypes.c_int, _DP, _DP, _LP, _DP, _DP]),
+    "mpm_batch_set_option": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int]),
     "mpm_thermal_specs_default": (None, [ctypes.POINTER(Specs), ctypes.POINTER(ThermalSpecs)]),
     "mpm_enable_thermal": (ctypes.c_int, [_P, ctypes.POINTER(ThermalSpecs)]),
     "mpm_upload_thermal": (ctypes.c_int, [_P, ctypes.c_int64, _DP]),
@@ -730,12 +731,15 @@ class Batch:
         self._call("autotune", trial_steps, ctypes.byref(t), ctypes.byref(ms))
         return t.as_dict(), ms.value
 
-    STAGES = ("keys", "scan", "scatter", "p2g", "grid", "particle")
+    STAGES = ("keys", "scan", "scatter", "order", "p2g", "grid", "particle")
+
+    def set_option(self, opt, value):
+        self._call("set_option", opt, value)
 
     def profile(self, nsteps=5):
         """Per-stage workgroup-ms per step (busy, wait), tasks, other, step ms."""
-        busy, wait = np.zeros(6), np.zeros(6)
-        tasks = np.zeros(6, dtype=np.int64)
+        busy, wait = np.zeros(7), np.zeros(7)
+        tasks = np.zeros(7, dtype=np.int64)
         other, ms = ctypes.c_double(0), ctypes.c_double(0)
         self._call("profile", nsteps, dptr(busy), dptr(wait), lptr(tasks), ctypes.byref(other),
                    ctypes.byref(ms))

@@ -4,13 +4,19 @@
 // each scene runs the sorted USL step of mpm_step (main.cpp:83-187) built
 // from the same bodies (mpm_tiles.h, mpm_stages.h).
 //
-// Work of one step, per scene, in six stages:
+// Work of one step, per scene, in six stages (seven when deterministic):
 //   KEYS     cell key + rank of each particle          (particle chunks)
 //   SCAN     exclusive scan of the cell counts, list of non-empty tiles
 //   SCATTER  sorted permutation + P2G packets          (particle chunks)
+//   ORDER    (deterministic) each cell's particles by index (tile shares)
 //   P2G      tile deposits                              (tile shares)
 //   GRID     nodal pipeline over the active node tiles   (node-tile chunks)
 //   PART     fused particle pipeline, sorted write, dt candidates (tile shares)
+// Deterministic batches (MPM_OPT_DETERMINISTIC) run the single-scene
+// step's atomic-free P2G: the scatter writes no packets and P2G gathers the
+// fields through the permutation, ORDER fixes the order of each cell's
+// particles, the tile deposits write whole node blocks stamped with the step
+// number, and GRID first adds each node's blocks in a fixed order.
 // The tasks sit in eight queues, stage-major: every task of stage k of a
 // scene comes before every task of stage k + 1 of that scene, and a scene's
 // tasks are all in one queue.  A workgroup takes tasks from its own queue
@@ -40,7 +46,7 @@ namespace xmpm {
 
 namespace {
 
-enum { B_KEYS = 0, B_SCAN, B_SCATTER, B_P2G, B_GRID, B_PART, B_NSTAGE };
+enum { B_KEYS = 0, B_SCAN, B_SCATTER, B_ORDER, B_P2G, B_GRID, B_PART, B_NSTAGE };
 constexpr int kQueues = 8;
 constexpr int kScanChunk = 16 * kBlock;  // cell counts per scan round
 constexpr int kGridFlags = G_FINISH_MASS | G_FINISH_VEL | G_FINISH_FRC | G_BACKUP | G_UPDATE |
@@ -97,11 +103,63 @@ struct ScanLds {
     int tsum[kBlock / 64];
 };
 
-union BLds {
+// Permutation entries of a tile the ORDER stage sorts in LDS (a longer range
+// is sorted in global memory).
+constexpr int kOrderTileLds = 4096;
+
+template <bool DET>
+union BLds;
+template <>
+union BLds<false> {
     P2gLds<7> p2g;
     PartLds<6> part;
     ScanLds scan;
 };
+template <>
+union BLds<true> {
+    P2gLdsOf<true, true, true> p2g;
+    PartLds<6> part;
+    ScanLds scan;
+    int order[kOrderTileLds];
+};
+
+// Each cell of a tile gets its particles in increasing current index (the
+// order k_cell_order gives the single-scene deterministic step): the tile's
+// contiguous range of the permutation is staged in LDS and one thread per
+// cell insertion-sorts its run.
+__device__ void order_tile(const SortState &S, int tile, int *seg) {
+    const int tid = threadIdx.x;
+    const int c0 = tile * kCellsPerTile;
+    const int r0 = S.cell_start[c0], r1 = S.cell_start[c0 + kCellsPerTile];
+    const int b = tid < kCellsPerTile ? S.cell_start[c0 + tid] : r1;
+    const int e = tid < kCellsPerTile ? S.cell_start[c0 + tid + 1] : r1;
+    __syncthreads();  // the workgroup's previous tile is done with the LDS
+    if (r1 - r0 > kOrderTileLds) {  // uniform over the block
+        for (int i = b + 1; i < e; ++i) {
+            const int x = S.perm[i];
+            int j = i - 1;
+            while (j >= b && S.perm[j] > x) {
+                S.perm[j + 1] = S.perm[j];
+                --j;
+            }
+            S.perm[j + 1] = x;
+        }
+        return;
+    }
+    for (int i = tid; i < r1 - r0; i += kBlock) seg[i] = S.perm[r0 + i];
+    __syncthreads();
+    for (int i = b + 1 - r0; i < e - r0; ++i) {
+        const int x = seg[i];
+        int j = i - 1;
+        while (j >= b - r0 && seg[j] > x) {
+            seg[j + 1] = seg[j];
+            --j;
+        }
+        seg[j + 1] = x;
+    }
+    __syncthreads();
+    for (int i = tid; i < r1 - r0; i += kBlock) S.perm[r0 + i] = seg[i];
+}
 
 // Wait until *ctr >= need, then acquire.  Bounded: past `limit` polls the
 // error flag is set, and once it is set every wait of the launch gives up at
@@ -229,11 +287,11 @@ __device__ __forceinline__ int64_t share(int64_t n, int part, int nparts) {
     return n * part / nparts;
 }
 
-template <int O, bool DELTA>
+template <int O, bool DELTA, bool DET>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) void k_batch_step(const BScene *__restrict__ scenes,
                                                        BSync *sync, BQueues Q, BCtl *ctl,
                                                        unsigned spin_limit, int prof) {
-    __shared__ BLds lds;
+    __shared__ BLds<DET> lds;
     __shared__ int4 task_s;
     const int tid = threadIdx.x;
     const int home = blockIdx.x & (kQueues - 1);
@@ -266,7 +324,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
         unsigned long long t_start = 0, t_work = 0;
         if (prof && tid == 0) t_start = wall_clock64();
         if (stage > 0) {
-            if (tid == 0) wait_count(&sync[s].done[stage - 1], sd.ntask[stage - 1], ctl, spin_limit);
+            // the previous stage (ORDER runs only in deterministic batches)
+            const int prev = (!DET && stage == B_P2G) ? B_SCATTER : stage - 1;
+            if (tid == 0) wait_count(&sync[s].done[prev], sd.ntask[prev], ctl, spin_limit);
             __syncthreads();
         }
         if (prof && tid == 0) t_work = wall_clock64();
@@ -283,13 +343,20 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
             scan_scene(S, lds.scan);
         } else if (stage == B_SCATTER) {
             const int64_t a = share(sd.np, part, nparts), b = share(sd.np, part + 1, nparts);
-            for (int64_t i = a + tid; i < b; i += kBlock) scatter_one(Pin, S, Pout.id, sd.ph, 1, i);
+            for (int64_t i = a + tid; i < b; i += kBlock)
+                scatter_one(Pin, S, Pout.id, sd.ph, DET ? 0 : 1, i);
+        } else if (DET && stage == B_ORDER) {
+            const int nt = S.tiles[0];
+            const int a = (int)share(nt, part, nparts), b = (int)share(nt, part + 1, nparts);
+            if constexpr (DET)
+                for (int k = a; k < b; ++k) order_tile(S, S.tiles[1 + k], lds.order);
         } else if (stage == B_P2G) {
             const int nt = S.tiles[0];
             const int a = (int)share(nt, part, nparts), b = (int)share(nt, part + 1, nparts);
+            // deterministic: block of list position k stamped with the step
             for (int k = a; k < b; ++k)
-                p2g_tile<O, true, false>(sd.g, sd.ph, Pin, S, sd.N.acc, epoch, &sd.sc->err,
-                                         S.tiles[1 + k], lds.p2g);
+                p2g_tile<O, true, false, DET, DET>(sd.g, sd.ph, Pin, S, sd.N.acc, epoch,
+                                                   &sd.sc->err, S.tiles[1 + k], lds.p2g, k, epoch);
         } else if (stage == B_GRID) {
             LsP ls;
             ls.nb = 0;
@@ -316,8 +383,21 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
                 const int J = ((t / ntx) % nty) * kTileCells + ((lane >> 2) & 3);
                 const int K = (t / (ntx * nty)) * kTileCells + (lane >> 4);
                 if (I > g.n[0] || J > g.n[1] || K > g.n[2]) continue;
-                grid_at(g, sd.ph, ls, udf, sd.N, kGridFlags, dt, nidx(g, I, J, K), I, J, K,
-                        S.tile_epoch, epoch);
+                const int64_t node = nidx(g, I, J, K);
+                if (DET && e == epoch) {
+                    // the node's blocks, in a fixed order, into the
+                    // accumulators (which hold no other deposit)
+                    BlkGather bg;
+                    bg.blk = S.blk;
+                    bg.slot = S.tile_slot;
+                    bg.pass = epoch;
+                    bg.nc = 7;
+                    double sm[7];
+                    if (blk_gather<7, true>(g, bg, I, J, K, sm))
+                        for (int c = 0; c < 7; ++c) sd.N.acc[c * g.nn + node] += sm[c];
+                }
+                grid_at(g, sd.ph, ls, udf, sd.N, kGridFlags, dt, node, I, J, K, S.tile_epoch,
+                        epoch);
             }
         } else {
             const int nt = S.tiles[0];
@@ -364,15 +444,20 @@ __global__ void k_fill_int(int *p, int64_t n, int v) {
 
 typedef void (*BatchKernel)(const BScene *, BSync *, BQueues, BCtl *, unsigned, int);
 
-BatchKernel batch_kernel(int O, int delta) {
+template <bool DET>
+BatchKernel batch_kernel_of(int O, int delta) {
     if (delta) {
-        if (O == 1) return k_batch_step<1, true>;
-        if (O == 2) return k_batch_step<2, true>;
-        return k_batch_step<3, true>;
+        if (O == 1) return k_batch_step<1, true, DET>;
+        if (O == 2) return k_batch_step<2, true, DET>;
+        return k_batch_step<3, true, DET>;
     }
-    if (O == 1) return k_batch_step<1, false>;
-    if (O == 2) return k_batch_step<2, false>;
-    return k_batch_step<3, false>;
+    if (O == 1) return k_batch_step<1, false, DET>;
+    if (O == 2) return k_batch_step<2, false, DET>;
+    return k_batch_step<3, false, DET>;
+}
+
+BatchKernel batch_kernel(int O, int delta, int det) {
+    return det ? batch_kernel_of<true>(O, delta) : batch_kernel_of<false>(O, delta);
 }
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -416,6 +501,11 @@ struct mpm_batch {
     bool inited = false;
     int64_t steps = 0;
     double step_ms = 0.0;
+    // Deterministic step (MPM_OPT_DETERMINISTIC): requested, and the node
+    // blocks and tile stamps of every scene (allocated by the option).
+    int det = 0;
+    char *det_arena = nullptr;
+    std::vector<size_t> det_slot_off;  // per scene, bytes into det_arena
 };
 
 namespace {
@@ -452,6 +542,7 @@ int build_queues(mpm_batch *b) {
         sd.ntask[B_KEYS] = ks;
         sd.ntask[B_SCAN] = 1;
         sd.ntask[B_SCATTER] = ks;
+        sd.ntask[B_ORDER] = b->det ? kt : 0;
         sd.ntask[B_P2G] = kt;
         sd.ntask[B_GRID] = kg;
         sd.ntask[B_PART] = kt;
@@ -484,7 +575,7 @@ int build_queues(mpm_batch *b) {
 }
 
 int launch_steps(mpm_batch *b, int n, int prof = 0) {
-    BatchKernel k = batch_kernel(b->O, b->delta);
+    BatchKernel k = batch_kernel(b->O, b->delta, b->det);
     for (int i = 0; i < n; ++i)
         hipLaunchKernelGGL(k, dim3(b->tune.workgroups), dim3(kBlock), 0, b->st, b->d_scn, b->sync,
                            b->q, b->ctl, b->spin_limit, prof);
@@ -510,6 +601,30 @@ int check_flags(mpm_batch *b) {
         HIPCHK(hipMemcpy(b->sc, h.data(), b->ns * sizeof(Scalars), hipMemcpyHostToDevice));
         return set_err(MPM_E_UNSUPPORTED, "batched step: a particle lay outside its sorted tile");
     }
+    return MPM_OK;
+}
+
+// Resident workgroups of the step kernel over the device (a multiple of the
+// queue count): the persistent launch's default size.
+int batch_occupancy(mpm_batch *b) {
+    int per_cu = 0;
+    hipDeviceProp_t prop;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, reinterpret_cast<const void *>(batch_kernel(b->O, b->delta, b->det)),
+            kBlock, 0) != hipSuccess ||
+        hipGetDeviceProperties(&prop, 0) != hipSuccess)
+        return set_err(MPM_E_HIP, "occupancy query failed");
+    b->max_wgs = std::max(kQueues, per_cu * prop.multiProcessorCount / kQueues * kQueues);
+    return MPM_OK;
+}
+
+// Marks every deterministic block stamp empty (-1): at set-up and after an
+// arena restore, which rewinds the step numbers the stamps compare with.
+int clear_det_stamps(mpm_batch *b) {
+    if (!b->det_arena) return MPM_OK;
+    for (int s = 0; s < b->ns; ++s)
+        HIPCHK(hipMemsetAsync(b->det_arena + b->det_slot_off[s], 0xff,
+                              (size_t)b->scn[s].S.ntiles * sizeof(int64_t), b->st));
     return MPM_OK;
 }
 
@@ -664,15 +779,8 @@ int mpm_batch_create(int ns, const mpm_specs *specs, const int64_t *np, mpm_batc
     }
     int rc = check_launch();
     if (rc) return fail(rc);
-    int per_cu = 0, cus = 0;
-    hipDeviceProp_t prop;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per_cu, reinterpret_cast<const void *>(batch_kernel(b->O, b->delta)), kBlock, 0) !=
-            hipSuccess ||
-        hipGetDeviceProperties(&prop, 0) != hipSuccess)
-        return fail(set_err(MPM_E_HIP, "occupancy query failed"));
-    cus = prop.multiProcessorCount;
-    b->max_wgs = std::max(kQueues, per_cu * cus / kQueues * kQueues);
+    rc = batch_occupancy(b);
+    if (rc) return fail(rc);
     b->tune.workgroups = b->max_wgs;
     b->tune.sort_chunk = 4096;
     b->tune.tile_chunk = 2048;
@@ -689,6 +797,7 @@ void mpm_batch_destroy(mpm_batch *b) {
     if (b->st) (void)hipStreamSynchronize(b->st);
     drop_graph(b);
     (void)hipFree(b->arena);
+    (void)hipFree(b->det_arena);
     (void)hipFree(b->scan_tmp);
     (void)hipFree(b->d_scn);
     (void)hipFree(b->d_queue);
@@ -754,8 +863,11 @@ int mpm_batch_init(mpm_batch *b) {
         BScene &sd = b->scn[s];
         SortState S = sd.S;
         // Initialise_Internal_Forces (mpm_init.cpp:368-401) as mpm_ctx runs it,
-        // then Calculate_time_step for the first step.
-        launch_sort(sd.g, sd.ph, sd.P[0], S, nullptr, 1, b->st);
+        // then Calculate_time_step for the first step.  Deterministic: the
+        // single-scene deterministic sort and P2G, their block stamps below
+        // every step number of the persistent kernel.
+        S.pass = -(1 << 30);
+        launch_sort(sd.g, sd.ph, sd.P[0], S, nullptr, b->det ? 0 : 1, b->st);
         launch_p2g_tiles(sd.g, sd.ph, sd.P[0], S, sd.N, 0, 0, 1, sd.sc, b->st);
         launch_grid(sd.g, sd.ph, ls, udf, sd.N,
                     G_FINISH_MASS | G_FINISH_VEL | G_FINISH_FRC | G_KEEP_FRC | G_BACKUP, 0.0,
@@ -930,6 +1042,8 @@ int mpm_batch_autotune(mpm_batch *b, int trial_steps, mpm_batch_tuning *best_out
         int rc = build_queues(b);
         if (rc) return rc;
         HIPCHK(hipMemcpyAsync(b->arena, snap, b->arena_bytes, hipMemcpyDeviceToDevice, b->st));
+        rc = clear_det_stamps(b);
+        if (rc) return rc;
         rc = launch_steps(b, 1);  // warm-up
         if (rc) return rc;
         HIPCHK(hipEventRecord(b->ev[0], b->st));
@@ -976,13 +1090,15 @@ int mpm_batch_autotune(mpm_batch *b, int trial_steps, mpm_batch_tuning *best_out
     if (hipMemcpyAsync(b->arena, snap, b->arena_bytes, hipMemcpyDeviceToDevice, b->st) !=
         hipSuccess)
         return done(set_err(MPM_E_HIP, "autotune restore failed"));
+    rc = clear_det_stamps(b);
+    if (rc) return done(rc);
     if (best_out) *best_out = best;
     if (best_ms) *best_ms = bms;
     return done(MPM_OK);
 }
 
-int mpm_batch_profile(mpm_batch *b, int nsteps, double busy_ms[6], double wait_ms[6],
-                      int64_t tasks[6], double *other_ms, double *step_ms) {
+int mpm_batch_profile(mpm_batch *b, int nsteps, double busy_ms[7], double wait_ms[7],
+                      int64_t tasks[7], double *other_ms, double *step_ms) {
     if (!b || nsteps < 1) return set_err(MPM_E_ARG, "bad argument");
     if (!b->inited) return set_err(MPM_E_ARG, "batch: mpm_batch_init first");
     int khz = 0;
@@ -1009,6 +1125,45 @@ int mpm_batch_profile(mpm_batch *b, int nsteps, double busy_ms[6], double wait_m
     if (other_ms) *other_ms = h.other * tick_ms / nsteps;
     if (step_ms) *step_ms = ms / nsteps;
     return check_flags(b);
+}
+
+int mpm_batch_set_option(mpm_batch *b, int option, int value) {
+    if (!b) return set_err(MPM_E_ARG, "null batch");
+    if (option != MPM_OPT_DETERMINISTIC)
+        return set_err(MPM_E_ARG, "batch: the only option is MPM_OPT_DETERMINISTIC");
+    if (b->steps != 0 || b->inited)
+        return set_err(MPM_E_ARG, "batch: set options before mpm_batch_init");
+    HIPCHK(hipStreamSynchronize(b->st));
+    if (value && !b->det_arena) {
+        // per scene: one 7 x 343-double node block per cell tile (the list
+        // position of a non-empty tile selects it) and the tile stamps
+        size_t at = 0;
+        std::vector<size_t> o_blk(b->ns);
+        b->det_slot_off.assign(b->ns, 0);
+        for (int s = 0; s < b->ns; ++s) {
+            const size_t nt = (size_t)b->scn[s].S.ntiles;
+            o_blk[s] = at;
+            at = align256(at + nt * 7 * kTNNodes * sizeof(double));
+            b->det_slot_off[s] = at;
+            at = align256(at + nt * sizeof(int64_t));
+        }
+        if (hipMalloc(&b->det_arena, at) != hipSuccess) {
+            b->det_arena = nullptr;
+            return set_err(MPM_E_OOM, "batch: deterministic node blocks");
+        }
+        for (int s = 0; s < b->ns; ++s) {
+            SortState &S = b->scn[s].S;
+            S.blk = reinterpret_cast<double *>(b->det_arena + o_blk[s]);
+            S.tile_slot = reinterpret_cast<int64_t *>(b->det_arena + b->det_slot_off[s]);
+        }
+        int rc = clear_det_stamps(b);
+        if (rc) return rc;
+    }
+    b->det = value ? 1 : 0;
+    int rc = batch_occupancy(b);
+    if (rc) return rc;
+    if (b->tune.workgroups > b->max_wgs) b->tune.workgroups = b->max_wgs;
+    return build_queues(b);
 }
 
 int mpm_batch_stats(mpm_batch *b, double *step_ms, int64_t *particles, int *tasks) {

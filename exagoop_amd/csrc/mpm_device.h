@@ -120,8 +120,10 @@ __device__ __forceinline__ int block_sources(int I, int n, int per, int ntc, int
 
 // Sum of node (I, J, K)'s entries in the blocks of the tiles stamped with
 // bg.pass, in a fixed order (z, then y, then x sources): out[0..NC), NC =
-// bg.nc.  Returns whether any block contributed.
-template <int NC>
+// bg.nc.  Returns whether any block contributed.  SAME_LAUNCH: the stamps
+// and blocks were written earlier in the same launch by other workgroups
+// (read the stamps with agent-scope atomic loads).
+template <int NC, bool SAME_LAUNCH = false>
 __device__ __forceinline__ bool blk_gather(const GridP &g, const BlkGather &bg, int I, int J,
                                            int K, double out[NC]) {
     constexpr int kBN = kTileCells + 3;
@@ -136,7 +138,11 @@ __device__ __forceinline__ bool blk_gather(const GridP &g, const BlkGather &bg, 
         for (int iy = 0; iy < cy; ++iy)
             for (int ix = 0; ix < cx; ++ix) {
                 const int tile = (tz[iz] * g.ntc[1] + ty[iy]) * g.ntc[0] + tx[ix];
-                const int64_t e = bg.slot[tile];
+                const int64_t e =
+                    SAME_LAUNCH ? (int64_t)__hip_atomic_load(
+                                      reinterpret_cast<const unsigned long long *>(bg.slot + tile),
+                                      __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                : bg.slot[tile];
                 if ((int)(e >> 32) != bg.pass) continue;
                 const double *b = bg.blk + (int64_t)(uint32_t)e * NC * kTNNodes +
                                   (az[iz] * kBN + ay[iy]) * kBN + ax[ix];

@@ -129,12 +129,15 @@ def scene(seed):
     return s, pos, rdata, idata, prm
 
 
-def make_batch(seeds, tuning=None):
-    """A capi.Batch of the given scene seeds, uploaded and initialised."""
+def make_batch(seeds, tuning=None, deterministic=False):
+    """A capi.Batch of the given scene seeds, uploaded and initialised
+    (deterministic: MPM_OPT_DETERMINISTIC, the bitwise-reproducible step)."""
     sc = [scene(int(k)) for k in seeds]
     b = capi.Batch([x[0] for x in sc], [x[1].shape[1] for x in sc])
     for k, (s, pos, rdata, idata, _) in enumerate(sc):
         b.upload(k, pos, rdata, idata)
+    if deterministic:
+        b.set_option(capi.OPT_DETERMINISTIC, 1)
     if tuning:
         b.set_tuning(**tuning)
     b.init()

@@ -533,13 +533,21 @@ int mpm_batch_set_tuning(mpm_batch *b, const mpm_batch_tuning *t);
  * simulation state is unchanged.  best / best_ms may be NULL. */
 int mpm_batch_autotune(mpm_batch *b, int trial_steps, mpm_batch_tuning *best, double *best_ms);
 /* nsteps steps with an instrumented launch (no graph; the scenes advance):
- * per stage (keys, scan, scatter, p2g, grid, particle), workgroup-milliseconds
+ * per stage (keys, scan, scatter, order, p2g, grid, particle; order runs in
+ * deterministic batches only), workgroup-milliseconds
  * per step spent in its tasks (busy_ms) and waiting for the previous stage
  * (wait_ms), and its tasks per step; workgroup-milliseconds per step outside
  * tasks (taking tasks, leaving); milliseconds per step.  Any output may be
  * NULL. */
-int mpm_batch_profile(mpm_batch *b, int nsteps, double busy_ms[6], double wait_ms[6],
-                      int64_t tasks[6], double *other_ms, double *step_ms);
+int mpm_batch_profile(mpm_batch *b, int nsteps, double busy_ms[7], double wait_ms[7],
+                      int64_t tasks[7], double *other_ms, double *step_ms);
+/* MPM_OPT_DETERMINISTIC (the only batch option; before mpm_batch_init): every
+ * scene takes the single-scene deterministic step (each cell's particles in
+ * index order, atomic-free tile deposits into whole node blocks, each node
+ * adding its blocks in a fixed order), so that equal inputs give equal bits
+ * run to run and each scene the bits of a deterministic context of its own.
+ * Allocates one 7 x 343-double node block per cell tile of every scene. */
+int mpm_batch_set_option(mpm_batch *b, int option, int value);
 /* Milliseconds per step of the last mpm_batch_step call (HIP events), the
  * particle slots of all scenes and the tasks of one step. */
 int mpm_batch_stats(mpm_batch *b, double *step_ms, int64_t *particles, int *tasks);

@@ -1,15 +1,15 @@
 """GPU checks of the batched ensemble (mpm_batch_*) through the C ABI.
 
 Each scene of a batch must give what a context of its own gives (the
-single-scene sorted step, pinned to the CPU oracle in test_gpu_parity.py):
-nodal mass / velocity / delta-velocity after one step within 1e-12 relative
-L2, particle state after 25 steps within 1e-10 (the only difference is the
-order of the FP64 atomic sums; a field that is zero up to rounding is held
-to 10 x the run-to-run spread of the single-scene step instead).  Two small scenes are also checked straight
-against the oracle.  Any work decomposition (tuning), the autotuner's trials
-and graph replays leave the results unchanged; the full 256-scene ensemble
-keeps every particle, its masses and a dt equal to the CFL bound recomputed
-on the host from the state the step left.
+single-scene sorted step, pinned to the CPU oracle in test_gpu_parity.py).
+With MPM_OPT_DETERMINISTIC on both sides every sum runs in a fixed order,
+so a scene of the batch gives the bits of its context after 25 steps, two
+runs give the same bits, and any work decomposition (tuning), the
+autotuner's trials and graph replays leave the bits unchanged.  The default
+(atomic) step matches the contexts' nodal fields after one step within 1e-12
+relative L2.  Two small scenes are checked straight against the oracle; the
+full 256-scene ensemble keeps every particle, its masses and a dt equal to the
+CFL bound recomputed on the host from the state the step left.
 """
 import sys
 
@@ -19,7 +19,7 @@ import pytest
 import oracle
 from exagoop_amd import capi, ensemble
 from exagoop_amd.capi import (N_DVEL, N_MASS, N_MASS_OLD, N_VEL, R_BULK, R_DENSITY, R_E, R_MASS,
-                              R_NU, R_VEL, MUSL)
+                              R_NU, R_STRAIN, R_STRAINRATE, R_STRESS, R_VEL, MUSL)
 
 pytestmark = pytest.mark.gpu
 
@@ -58,31 +58,45 @@ def context_of(sc):
     return g
 
 
-def compare_particles(a, b, tol, floor=None):
-    """Particle state a against b: ids and integer fields equal, positions and
-    every real field within tol relative L2, or, where `floor` (per-field
-    relative differences between two runs of the same single-scene step) is
-    given, within 10 x that run-to-run spread if larger.  A field that is
-    zero up to rounding (the strain rate of a rigid motion) carries only
-    the rounding noise of the FP64 atomic sums, so its relative difference is
-    set by that noise, not by tol."""
+def same_bits(a, b):
+    """Two downloads hold the same bits (positions, fields, ints, ids)."""
+    return all(x.tobytes() == y.tobytes() for x, y in zip(a, b))
+
+
+def compare_to_oracle(a, b, spec, t, tol):
+    """Particle state a (GPU) against b (the CPU oracle): ids and integer
+    fields equal, positions and every real field within tol relative L2 of
+    the field, where a field's norm is at least its natural scale.  The two
+    sides sum in different orders, and a field that is zero in exact
+    arithmetic (the strain rate, strain and deviatoric stress of a rigid
+    translation) carries only rounding noise of the velocity gradient: its
+    scale is |v|max / dx for the strain rate, that times the elapsed time
+    for the strain, and the largest modulus E (or K) times the strain scale
+    for the stress."""
     pa, ra, ia, ida = a
     pb, rb, ib, idb = b
     assert np.array_equal(ida, idb)
     assert np.array_equal(ia, ib)
     assert rel(pa, pb) <= tol, rel(pa, pb)
+    n = rb.shape[1]
+    dx = min((spec.phi[d] - spec.plo[d]) / spec.ncells[d] for d in range(3))
+    vmax = np.sqrt((rb[R_VEL:R_VEL + 3] ** 2).sum(axis=0)).max()
+    rate = vmax / dx
+    modulus = max(rb[R_E].max(), rb[R_BULK].max())
+    natural = {}
+    for f in range(R_STRAINRATE, R_STRAINRATE + 6):
+        natural[f] = rate
+    for f in range(R_STRAIN, R_STRAIN + 6):
+        natural[f] = rate * t
+    for f in range(R_STRESS, R_STRESS + 6):
+        natural[f] = modulus * rate * t
     bad = {}
     for f in range(ra.shape[0]):
-        t = tol if floor is None else max(tol, 10.0 * floor[f])
-        e = rel(ra[f], rb[f])
-        if not e <= t:
-            bad[f] = (e, t)
+        nb = max(np.linalg.norm(rb[f]), natural.get(f, 0.0) * np.sqrt(n))
+        e = np.linalg.norm(ra[f] - rb[f]) / nb if nb > 0 else np.linalg.norm(ra[f])
+        if not e <= tol:
+            bad[f] = e
     assert not bad, bad
-
-
-def spread(a, b):
-    """Per-field relative L2 difference of two downloads."""
-    return [rel(a[1][f], b[1][f]) for f in range(a[1].shape[0])]
 
 
 def test_batch_rejects_what_it_does_not_run(gpu):
@@ -108,6 +122,11 @@ def test_batch_rejects_what_it_does_not_run(gpu):
         b.upload(0, pos[:, :-1], r[:, :-1], idata[:, :-1])  # count differs from create
     with pytest.raises(capi.MPMError):
         b.set_tuning(workgroups=12)
+    with pytest.raises(capi.MPMError):
+        b.set_option(capi.OPT_PROFILE, 1)  # the only batch option is the deterministic step
+    b2, _ = ensemble.make_batch(SEEDS[:1])
+    with pytest.raises(capi.MPMError):
+        b2.set_option(capi.OPT_DETERMINISTIC, 1)  # after init
 
 
 def test_batch_one_step_nodal_matches_contexts(gpu):
@@ -125,40 +144,46 @@ def test_batch_one_step_nodal_matches_contexts(gpu):
         g.close()
 
 
+def det_context_of(sc):
+    s, pos, r, idata, _ = sc
+    g = capi.Context(s)
+    g.upload(pos, r, idata)
+    g.set_option(capi.OPT_DETERMINISTIC, 1)
+    g.init_internal_forces()
+    return g
+
+
 def test_batch_25_steps_match_contexts(gpu):
-    say(f"[batch] 25 steps, particle parity, seeds {SEEDS}")
-    b, sc = ensemble.make_batch(SEEDS)
+    """A deterministic batch against deterministic contexts: each scene's
+    particles, nodal fields and time after 25 steps are the context's bits
+    (the same step, every sum in the same fixed order)."""
+    say(f"[batch] deterministic, 25 steps, bitwise against contexts, seeds {SEEDS}")
+    b, sc = ensemble.make_batch(SEEDS, deterministic=True)
     b.step(25)
     for k, x in enumerate(sc):
-        g = context_of(x)
+        g = det_context_of(x)
         g.step(25)
-        g2 = context_of(x)
-        g2.step(25)
-        ref = g.download()
-        compare_particles(b.download(k), ref, 1e-10, floor=spread(g2.download(), ref))
+        assert same_bits(b.download(k), g.download()), k
+        for c in (N_MASS, N_VEL, N_VEL + 1, N_VEL + 2, N_DVEL, N_MASS_OLD):
+            assert b.nodal(k, c).tobytes() == g.nodal(c).tobytes(), (k, c)
         tb, tg = b.time(k), g.time()
         assert tb[1] == tg[1] == 25
-        assert abs(tb[0] - tg[0]) <= 1e-12 * tg[0]
+        assert tb[0] == tg[0] and tb[2] == tg[2]
         g.close()
-        g2.close()
 
 
-def test_batch_matches_the_oracle(gpu):
+@pytest.mark.parametrize("deterministic", [False, True])
+def test_batch_matches_the_oracle(gpu, deterministic):
     seeds = small_seeds(2, max_target=3000, max_n=20)
-    say(f"[batch] 10 steps against the CPU oracle, seeds {seeds}")
-    b, sc = ensemble.make_batch(seeds)
+    say(f"[batch] 10 steps against the CPU oracle, seeds {seeds}, deterministic {deterministic}")
+    b, sc = ensemble.make_batch(seeds, deterministic=deterministic)
     b.step(10)
     for k, (s, pos, r, idata, _) in enumerate(sc):
         o = oracle.Oracle(s)
         o.upload(pos, r, idata)
         o.init_internal_forces()
         o.step(10)
-        g = context_of(sc[k])
-        g.step(10)
-        g2 = context_of(sc[k])
-        g2.step(10)
-        compare_particles(b.download(k), o.download(), 1e-10,
-                          floor=spread(g2.download(), g.download()))
+        compare_to_oracle(b.download(k), o.download(), s, o.time()[0], 1e-10)
         assert abs(b.time(k)[0] - o.time()[0]) <= 1e-12 * o.time()[0]
         mass = r[R_MASS].sum()
         assert b.download(k)[1][R_MASS].sum() == mass  # particle masses never change
@@ -170,32 +195,28 @@ def test_batch_matches_the_oracle(gpu):
          graph_steps=1),
 ])
 def test_any_decomposition_gives_the_same_steps(gpu, tuning):
-    say(f"[batch] decomposition {tuning} vs default")
-    a, _ = ensemble.make_batch(SEEDS)
-    a2, _ = ensemble.make_batch(SEEDS)
-    b, _ = ensemble.make_batch(SEEDS, tuning=tuning)
+    say(f"[batch] deterministic, decomposition {tuning} vs default, bitwise")
+    a, _ = ensemble.make_batch(SEEDS, deterministic=True)
+    b, _ = ensemble.make_batch(SEEDS, tuning=tuning, deterministic=True)
     assert b.tuning() == tuning
-    for x in (a, a2, b):
+    for x in (a, b):
         x.step(10)
     for k in range(len(SEEDS)):
-        ref = a.download(k)
-        compare_particles(b.download(k), ref, 1e-12, floor=spread(a2.download(k), ref))
+        assert same_bits(b.download(k), a.download(k)), k
 
 
 def test_autotune_leaves_the_state_unchanged(gpu):
     say("[batch] autotune then step vs step")
-    a, _ = ensemble.make_batch(SEEDS)
-    a2, _ = ensemble.make_batch(SEEDS)
-    b, _ = ensemble.make_batch(SEEDS)
+    a, _ = ensemble.make_batch(SEEDS, deterministic=True)
+    b, _ = ensemble.make_batch(SEEDS, deterministic=True)
     best, ms = b.autotune(trial_steps=2)
     say(f"    best {best} {ms:.3f} ms/step")
     assert ms > 0 and b.tuning() == best
     assert b.time(0)[1] == 0
-    for x in (a, a2, b):
+    for x in (a, b):
         x.step(6)
     for k in range(len(SEEDS)):
-        ref = a.download(k)
-        compare_particles(b.download(k), ref, 1e-12, floor=spread(a2.download(k), ref))
+        assert same_bits(b.download(k), a.download(k)), k
 
 
 def cfl_dt(s, pos, r):
@@ -239,3 +260,18 @@ def test_full_ensemble_properties(gpu):
         t0, want = states[k]
         assert abs(dt - want) <= 1e-14 * want, (k, dt, want)
         assert t == t0 + dt
+
+
+def test_deterministic_batch_reruns_are_bitwise_equal(gpu):
+    """MPM_OPT_DETERMINISTIC: two runs of the same batch give the same bits."""
+    say(f"[batch] deterministic, two runs of 20 steps, seeds {SEEDS}")
+    a, _ = ensemble.make_batch(SEEDS, deterministic=True)
+    b, _ = ensemble.make_batch(SEEDS, deterministic=True)
+    a.step(20)
+    b.step(20)
+    for k in range(len(SEEDS)):
+        for x, y in zip(a.download(k), b.download(k)):
+            assert x.tobytes() == y.tobytes(), k
+        for c in (N_MASS, N_VEL, N_VEL + 1, N_VEL + 2, N_DVEL, N_MASS_OLD):
+            assert a.nodal(k, c).tobytes() == b.nodal(k, c).tobytes(), (k, c)
+        assert a.time(k) == b.time(k)
