@@ -2,6 +2,8 @@
 // built from the bodies of mpm_tiles.h (sort, tile P2G, fused particle
 // pipeline), the generic sorted particle kernel and their launchers.
 
+#include <atomic>
+
 #include <hipcub/hipcub.hpp>
 
 #include "mpm_tiles.h"
@@ -190,17 +192,19 @@ inline unsigned blocks_for(int64_t n) { return (unsigned)((n + kBlock - 1) / kBl
 // 33 k particles: 88.5 -> 70.3 us per step; dam break, 524 k: 240 us with 4
 // wavefronts against 282 us with 8).
 bool few_tiles(int64_t np) {
-    static int cus[64] = {0};
+    // compute units per device, queried once (contexts on several host
+    // threads may ask at once: every writer stores the same value)
+    static std::atomic<int> cus[64];
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
-    if (cus[dev] <= 0) {
-        int n = 0;
+    int n = cus[dev].load(std::memory_order_relaxed);
+    if (n <= 0) {
         if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
             n <= 0)
             n = 256;
-        cus[dev] = n;
+        cus[dev].store(n, std::memory_order_relaxed);
     }
-    return np < (int64_t)cus[dev] * 512;
+    return np < (int64_t)n * 512;
 }
 
 }  // namespace
