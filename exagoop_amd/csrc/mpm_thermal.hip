@@ -552,15 +552,17 @@ __global__ __launch_bounds__(kThreads) void k_th_g2p(GridP g, PState P, const in
 // tile); otherwise one thread per particle with global atomics.
 // interpolate_from_grid_temperature over the cell-sorted state, one
 // workgroup per non-empty tile (uniform order): the tile's node block of T
-// and delta T is staged in LDS, each thread takes particles of the tile.  A
-// particle outside its tile (not redistributed this step) reads the nodes
-// from global memory; the arithmetic is that of k_th_g2p.  Sorted position i
+// and delta T is staged in LDS, each thread takes particles of the tile and
+// gathers both with the tile factors of the mechanical G2P.  A particle
+// outside its tile (not redistributed this step) reads the nodes from global
+// memory.  Sorted position i
 // reads slot perm[i] and writes slot i (sorted) or perm[i] (in place).
 template <int O, bool UT, bool UQ>
 __global__ __launch_bounds__(kBlock) void k_th_g2p_tile(GridP g, PState P, SortState S,
                                                         const double *thin, double *thout,
                                                         int sorted_out, const double *tn) {
     __shared__ double nodes[kTNN * 2];
+    __shared__ SplineTables tab;
     const int tile = active_tile(S, blockIdx.x);
     if (tile < 0) return;  // uniform over the block
     const int beg = S.cell_start[tile * kCellsPerTile];
@@ -581,6 +583,12 @@ __global__ __launch_bounds__(kBlock) void k_th_g2p_tile(GridP g, PState P, SortS
         nodes[2 * e] = ok ? Tg[q] : 0.0;
         nodes[2 * e + 1] = ok ? DTg[q] : 0.0;
     }
+    if (O > 1) {
+        const int *src = reinterpret_cast<const int *>(&kTab);
+        int *dst = reinterpret_cast<int *>(&tab);
+        for (int e = threadIdx.x; e < (int)(sizeof(SplineTables) / sizeof(int)); e += kBlock)
+            dst[e] = src[e];
+    }
     __syncthreads();
     for (int i = beg + (int)threadIdx.x; i < end; i += kBlock) {
         const int64_t j = S.perm[i];
@@ -596,65 +604,91 @@ __global__ __launch_bounds__(kBlock) void k_th_g2p_tile(GridP g, PState P, SortS
                 bounds(iv[d], 0, g.hi[d], O, g.per[d], mn[d], mx[d]);
                 inside &= iv[d] >= T0[d] && iv[d] < T0[d] + kTileCells;
             }
-            // node (I, J, K) of the stencil, component 0 (T) or 1 (delta T)
-            auto nv = [&](int comp, int I, int J, int K) {
-                if (inside)
-                    return nodes[2 * (((K - T0[2] + 1) * kTN + (J - T0[1] + 1)) * kTN +
-                                      (I - T0[0] + 1)) + comp];
-                return nodal_read(g, comp ? DTg : Tg, I, J, K);
-            };
-            if (UT) {
-                double value;
-                if (O == 1) {
-                    const int ii = iv[0], jj = iv[1], kk = iv[2];
-                    const double a = (x[0] - (g.plo[0] + ii * g.dx[0])) / g.dx[0];
-                    const double b = (x[1] - (g.plo[1] + jj * g.dx[1])) / g.dx[1];
-                    const double c = (x[2] - (g.plo[2] + kk * g.dx[2])) / g.dx[2];
-                    value = (1.0 - a) * (1.0 - b) * (1.0 - c) * nv(1, ii, jj, kk);
-                    value += a * (1.0 - b) * (1.0 - c) * nv(1, ii + 1, jj, kk);
-                    value += (1.0 - a) * b * (1.0 - c) * nv(1, ii, jj + 1, kk);
-                    value += a * b * (1.0 - c) * nv(1, ii + 1, jj + 1, kk);
-                    value += (1.0 - a) * (1.0 - b) * c * nv(1, ii, jj, kk + 1);
-                    value += a * (1.0 - b) * c * nv(1, ii + 1, jj, kk + 1);
-                    value += (1.0 - a) * b * c * nv(1, ii, jj + 1, kk + 1);
-                    value += a * b * c * nv(1, ii + 1, jj + 1, kk + 1);
-                } else {
-                    double wv[3][4];
-                    for (int d = 0; d < 3; ++d)
-                        for (int l = mn[d]; l < mx[d]; ++l)
-                            wv[d][l - mn[d]] = interp1d(O, g.per[d], iv[d] + l, 0, g.hi[d], x[d],
-                                                        g.plo[d], g.dx[d]);
-                    value = 0.0;
-                    for (int nz = mn[2]; nz < mx[2]; ++nz)
-                        for (int ny = mn[1]; ny < mx[1]; ++ny)
-                            for (int nx = mn[0]; nx < mx[0]; ++nx)
-                                value += wv[0][nx - mn[0]] * wv[1][ny - mn[1]] *
-                                         wv[2][nz - mn[2]] *
-                                         nv(1, iv[0] + nx, iv[1] + ny, iv[2] + nz);
-                }
-                f[MPM_TH_TEMPERATURE] += value;
-            }
-            if (UQ) {
-                double bv[3][4], bd[3][4];
-                for (int d = 0; d < 3; ++d)
-                    for (int l = mn[d]; l < mx[d]; ++l)
-                        basis1d(O, g.per[d], iv[d], l, 0, g.hi[d], x[d], g.plo[d], g.dx[d],
-                                bv[d][l - mn[d]], bd[d][l - mn[d]]);
-                double gT[3] = {0.0, 0.0, 0.0};
-                for (int nz = mn[2]; nz < mx[2]; ++nz)
-                    for (int ny = mn[1]; ny < mx[1]; ++ny)
-                        for (int nx = mn[0]; nx < mx[0]; ++nx) {
-                            const double vx = bv[0][nx - mn[0]], vy = bv[1][ny - mn[1]],
-                                         vz = bv[2][nz - mn[2]];
-                            const double gr[3] = {bd[0][nx - mn[0]] * vy * vz,
-                                                  vx * bd[1][ny - mn[1]] * vz,
-                                                  vx * vy * bd[2][nz - mn[2]]};
-                            const double Tn = nv(0, iv[0] + nx, iv[1] + ny, iv[2] + nz);
-                            for (int d = 0; d < 3; ++d) gT[d] += Tn * gr[d];
+            // The mechanical G2P's factors (tile_factors: interpolation
+            // weight w, basis b and its derivative d of each offset, zero
+            // outside compute_bounds) and one pass over the stencil for both
+            // sums (z, then y, then x).  The nodes come
+            // from the tile's block in LDS, or from global memory for a
+            // particle outside its tile (not redistributed this step).
+            constexpr int W = (O == 1) ? 2 : 4;
+            constexpr int L0 = (O == 1) ? 0 : -1;
+            const SplineTables &TT = (O > 1) ? tab : kTab;
+            double value = 0.0, gT[3] = {0.0, 0.0, 0.0};
+            if (inside) {
+                double fw[2][W], fb[2][W], fd[2][W];
+#pragma unroll
+                for (int d = 0; d < 2; ++d)
+#pragma unroll
+                    for (int k = 0; k < W; ++k) {
+                        const int l = L0 + k;
+                        tile_factors<O>(TT, g.per[d], iv[d], l, g.hi[d], x[d], g.plo[d], g.dx[d],
+                                        g.dxi[d], l >= mn[d] && l < mx[d], fw[d][k], fb[d][k],
+                                        fd[d][k]);
+                    }
+                const double *blk =
+                    nodes + 2 * (((iv[2] - T0[2] + L0 + 1) * kTN + (iv[1] - T0[1] + L0 + 1)) * kTN +
+                                 (iv[0] - T0[0] + L0 + 1));
+#pragma unroll 1
+                for (int kz = 0; kz < W; ++kz) {
+                    const int lz = L0 + kz;
+                    double wz, bz, dz;
+                    tile_factors<O>(TT, g.per[2], iv[2], lz, g.hi[2], x[2], g.plo[2], g.dx[2],
+                                    g.dxi[2], lz >= mn[2] && lz < mx[2], wz, bz, dz);
+#pragma unroll
+                    for (int ky = 0; ky < W; ++ky) {
+                        // y and z factors first: products of x and y factors
+                        // alone would be loop invariant and held in registers
+                        // across the z loop.
+                        const double wyz = fw[1][ky] * wz;
+                        const double p0 = fb[1][ky] * bz, p1 = fd[1][ky] * bz,
+                                     p2 = fb[1][ky] * dz;
+#pragma unroll
+                        for (int kx = 0; kx < W; ++kx) {
+                            const double *q = blk + 2 * ((kz * kTN + ky) * kTN + kx);
+                            if (UT) value += fw[0][kx] * wyz * q[1];
+                            if (UQ) {
+                                const double gr[3] = {fd[0][kx] * p0, fb[0][kx] * p1,
+                                                      fb[0][kx] * p2};
+                                for (int d = 0; d < 3; ++d) gT[d] += q[0] * gr[d];
+                            }
                         }
+                    }
+                }
+            } else {
+                // Outside the tile (rare): the same factors and sums, node
+                // by node, from global memory.
+#pragma unroll 1
+                for (int lz = mn[2]; lz < mx[2]; ++lz) {
+                    double wz, bz, dz;
+                    tile_factors<O>(TT, g.per[2], iv[2], lz, g.hi[2], x[2], g.plo[2], g.dx[2],
+                                    g.dxi[2], true, wz, bz, dz);
+#pragma unroll 1
+                    for (int ly = mn[1]; ly < mx[1]; ++ly) {
+                        double wy, by, dy;
+                        tile_factors<O>(TT, g.per[1], iv[1], ly, g.hi[1], x[1], g.plo[1],
+                                        g.dx[1], g.dxi[1], true, wy, by, dy);
+                        const double wyz = wy * wz;
+                        const double p0 = by * bz, p1 = dy * bz, p2 = by * dz;
+#pragma unroll 1
+                        for (int lx = mn[0]; lx < mx[0]; ++lx) {
+                            double wx, bx, dx;
+                            tile_factors<O>(TT, g.per[0], iv[0], lx, g.hi[0], x[0], g.plo[0],
+                                            g.dx[0], g.dxi[0], true, wx, bx, dx);
+                            const int I = iv[0] + lx, J = iv[1] + ly, K = iv[2] + lz;
+                            if (UT) value += wx * wyz * nodal_read(g, DTg, I, J, K);
+                            if (UQ) {
+                                const double Tn = nodal_read(g, Tg, I, J, K);
+                                const double gr[3] = {dx * p0, bx * p1, bx * p2};
+                                for (int d = 0; d < 3; ++d) gT[d] += Tn * gr[d];
+                            }
+                        }
+                    }
+                }
+            }
+            if (UT) f[MPM_TH_TEMPERATURE] += value;
+            if (UQ)
                 for (int d = 0; d < 3; ++d)
                     f[MPM_TH_HEAT_FLUX + d] = -f[MPM_TH_CONDUCTIVITY] * gT[d];
-            }
         }
         const int64_t o = sorted_out ? (int64_t)i : j;
         for (int c = 0; c < MPM_NTHERMAL; ++c) thout[c * cap + o] = f[c];
