@@ -82,7 +82,6 @@ struct mpm_ctx {
     int64_t th_cap = 0;
     double *tn = nullptr;
     double *tscr = nullptr;
-    double *tpkt = nullptr;  // thermal P2G packets by sorted position (kPktComps * S.pcap)
     // Temperature wall functions and their per-step face tables (2 values per
     // face node).
     mpm_wall_temp_udf_fn tudf_fn[3][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};
@@ -126,8 +125,8 @@ bool walls_at_rest(const mpm_specs *s);
 // tiled deposit (epoch and fallback as for the momentum deposit).
 int thermal_on_upload(mpm_ctx *c);
 int thermal_init(mpm_ctx *c, bool tiled);
-int thermal_grid(mpm_ctx *c, bool tiled, int epoch, int fallback, hipStream_t st);
-int thermal_particles(mpm_ctx *c, bool musl, const int *perm, const int *count, int epoch,
+int thermal_grid(mpm_ctx *c, bool tiled, int fallback, hipStream_t st);
+int thermal_particles(mpm_ctx *c, bool musl, const int *perm, const int *count,
                       int fallback, hipStream_t st);
 void thermal_swap(mpm_ctx *c);
 // The temperature wall functions evaluated at time t into the face tables.

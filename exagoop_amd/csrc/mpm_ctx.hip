@@ -942,7 +942,7 @@ int mpm_step(mpm_ctx *c, int nsteps) {
                     step);
         c->all_dirty = false;
         if (c->thermal) {
-            int rc = thermal_grid(c, sorted, step, nr > 1, st);
+            int rc = thermal_grid(c, sorted, nr > 1, st);
             if (rc) return rc;
         }
         if (prof) HIPCHK(hipEventRecord(c->ev[4], st));
@@ -961,8 +961,8 @@ int mpm_step(mpm_ctx *c, int nsteps) {
         }
         if (c->thermal) {
             int rc = thermal_particles(c, musl, sorted ? c->S.perm : nullptr,
-                                       sorted ? c->S.cell_start + c->S.nkeys : nullptr, step,
-                                       nr > 1, st);
+                                       sorted ? c->S.cell_start + c->S.nkeys : nullptr, nr > 1,
+                                       st);
             if (rc) return rc;
             if (sorted) thermal_swap(c);
         }

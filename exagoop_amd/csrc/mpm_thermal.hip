@@ -29,18 +29,14 @@ enum { TN_MSP = 0, TN_MSPT = 1, TN_T = 2, TN_SRC = 3, TN_DT = 4, TN_MC = 5, TN_C
 
 inline int blocks(int64_t n) { return (int)((n + kThreads - 1) / kThreads); }
 
-// deposit_onto_grid_temperature (mpm_particle_grid_ops.cpp:656-884), one
-// thread per particle slot.  Deposits land on the valid nodes and, in
-// periodic directions, on the node one period away (the reference's ghost-
-// particle images), as the momentum deposit does.
+// deposit_onto_grid_temperature (mpm_particle_grid_ops.cpp:656-884) of
+// particle slot i (phase 0) with global atomics.  Deposits land on the valid
+// nodes and, in periodic directions, on the node one period away (the
+// reference's ghost-particle images), as the momentum deposit does.
 template <bool MT, bool SRC>
-__global__ __launch_bounds__(kThreads) void k_th_p2g(GridP g, PState P, const double *th,
-                                                     double *tn) {
-    const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
-    if (i >= P.np) return;
-    if (P.id[i] < 0) return;
+__device__ void th_deposit_one(const GridP &g, const PState &P, const double *th, double *tn,
+                               int64_t i) {
     const int64_t cap = P.cap;
-    if (P.idata[MPM_I_PHASE * cap + i] != 0) return;
     double x[3];
     int iv[3], mn[3], mx[3];
     for (int d = 0; d < 3; ++d) {
@@ -48,11 +44,6 @@ __global__ __launch_bounds__(kThreads) void k_th_p2g(GridP g, PState P, const do
         iv[d] = (int)floor((x[d] - g.plo[d]) * g.dxi[d]);
         bounds(iv[d], 0, g.hi[d], g.od[d], g.per[d], mn[d], mx[d]);
     }
-    double bv[3][4], bd[3][4];
-    for (int d = 0; d < 3; ++d)
-        for (int l = mn[d]; l < mx[d]; ++l)
-            basis1d(g.od[d], g.per[d], iv[d], l, 0, g.hi[d], x[d], g.plo[d], g.dx[d],
-                    bv[d][l - mn[d]], bd[d][l - mn[d]]);
     const double m = P.r[MPM_R_MASS * cap + i], V = P.r[MPM_R_VOLUME * cap + i];
     const double T = th[MPM_TH_TEMPERATURE * cap + i];
     const double cp = th[MPM_TH_SPECIFIC_HEAT * cap + i];
@@ -61,20 +52,32 @@ __global__ __launch_bounds__(kThreads) void k_th_p2g(GridP g, PState P, const do
     const double q[3] = {th[(MPM_TH_HEAT_FLUX + 0) * cap + i], th[(MPM_TH_HEAT_FLUX + 1) * cap + i],
                          th[(MPM_TH_HEAT_FLUX + 2) * cap + i]};
     const int64_t nn = g.nn;
+    // The 1-D factors are evaluated per node (rolled loops, no per-lane
+    // arrays): the function is inlined into the tiled deposit's rare
+    // fallback, and its registers count against that kernel's.
+#pragma unroll 1
     for (int n = mn[2]; n < mx[2]; ++n) {
         int tz[2];
         bool iz[2];
         const int cz = targets(iv[2] + n, g.n[2], g.per[2], tz, iz);
+        double vz, dz;
+        basis1d(g.od[2], g.per[2], iv[2], n, 0, g.hi[2], x[2], g.plo[2], g.dx[2], vz, dz);
+#pragma unroll 1
         for (int mm = mn[1]; mm < mx[1]; ++mm) {
             int ty[2];
             bool iy[2];
             const int cy = targets(iv[1] + mm, g.n[1], g.per[1], ty, iy);
+            double vy, dy;
+            basis1d(g.od[1], g.per[1], iv[1], mm, 0, g.hi[1], x[1], g.plo[1], g.dx[1], vy, dy);
+#pragma unroll 1
             for (int l = mn[0]; l < mx[0]; ++l) {
                 int tx[2];
                 bool ix[2];
                 const int cx = targets(iv[0] + l, g.n[0], g.per[0], tx, ix);
                 if (!cx || !cy || !cz) continue;
-                const double vx = bv[0][l - mn[0]], vy = bv[1][mm - mn[1]], vz = bv[2][n - mn[2]];
+                double vx, dxv;
+                basis1d(g.od[0], g.per[0], iv[0], l, 0, g.hi[0], x[0], g.plo[0], g.dx[0], vx,
+                        dxv);
                 const double N = vx * vy * vz;
                 double a0 = 0.0, a1 = 0.0, a2 = 0.0, s = 0.0;
                 if (MT) {
@@ -83,8 +86,7 @@ __global__ __launch_bounds__(kThreads) void k_th_p2g(GridP g, PState P, const do
                     a2 = m * kc * N;
                 }
                 if (SRC) {
-                    const double gr[3] = {bd[0][l - mn[0]] * vy * vz, vx * bd[1][mm - mn[1]] * vz,
-                                          vx * vy * bd[2][n - mn[2]]};
+                    const double gr[3] = {dxv * vy * vz, vx * dy * vz, vx * vy * dz};
                     double net = 0.0;
                     for (int d = 0; d < 3; ++d) net += q[d] * gr[d];
                     s = net * V + V * Q * N;
@@ -105,51 +107,298 @@ __global__ __launch_bounds__(kThreads) void k_th_p2g(GridP g, PState P, const do
     }
 }
 
-// The same deposit for the cell-sorted state, through the tiled momentum
-// deposit (p2g_tile): thermal P2G packets at the sorted positions in the
-// layout of the momentum packets, so that the packet's "mass" deposits
-// m c_p N, its momentum components m c_p T N and m k N, and its first force
-// component a N + B grad N with a = V Q and B's first row V q (the sum of
-// the internal and external source).  Zero for a particle with phase != 0.
+// One thread per particle slot.
 template <bool MT, bool SRC>
-__global__ __launch_bounds__(kThreads) void k_th_packets(PState P, SortState S, const double *th,
-                                                         double *pkt) {
-    const int64_t s = (int64_t)blockIdx.x * kThreads + threadIdx.x;
-    if (s >= (int64_t)S.cell_start[S.nkeys]) return;
-    const int64_t j = S.perm[s], cap = P.cap, pc = S.pcap;
-    double v[13];
-    for (int e = 0; e < 13; ++e) v[e] = 0.0;
-    if (P.idata[MPM_I_PHASE * cap + j] == 0) {
-        const double m = P.r[MPM_R_MASS * cap + j], V = P.r[MPM_R_VOLUME * cap + j];
-        if (MT) {
-            const double cp = th[MPM_TH_SPECIFIC_HEAT * cap + j];
-            v[0] = m * cp;
-            v[1] = m * cp * th[MPM_TH_TEMPERATURE * cap + j];
-            v[2] = m * th[MPM_TH_CONDUCTIVITY * cap + j];
-        }
-        if (SRC) {
-            v[4] = V * th[MPM_TH_HEAT_SOURCE * cap + j];
-            for (int d = 0; d < 3; ++d) v[7 + d] = V * th[(MPM_TH_HEAT_FLUX + d) * cap + j];
-        }
-    }
-    for (int d = 0; d < 3; ++d) pkt[d * pc + s] = P.pos[d * cap + j];
-    for (int e = 0; e < 13; ++e) pkt[(3 + e) * pc + s] = v[e];
+__global__ __launch_bounds__(kThreads) void k_th_p2g(GridP g, PState P, const double *th,
+                                                     double *tn) {
+    const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (i >= P.np) return;
+    if (P.id[i] < 0) return;
+    if (P.idata[MPM_I_PHASE * P.cap + i] != 0) return;
+    th_deposit_one<MT, SRC>(g, P, th, tn, i);
 }
 
-// The tiled deposit's sums into the thermal components; the accumulators are
-// left zero, as the momentum grid pass leaves them.
-template <bool MT, bool SRC>
-__global__ __launch_bounds__(kThreads) void k_th_take(GridP g, double *acc, double *tn) {
-    const int64_t q = (int64_t)blockIdx.x * kThreads + threadIdx.x;
-    const int64_t nn = g.nn;
-    if (q >= nn) return;
-    if (MT) {
-        tn[TN_MSP * nn + q] = acc[q];
-        tn[TN_MSPT * nn + q] = acc[nn + q];
-        tn[TN_MC * nn + q] = acc[2 * nn + q];
+// The same deposit over the cell-sorted state, one workgroup per non-empty
+// tile, with the momentum deposit's scheme (p2g_tile): wavefront w takes the
+// 16 cells of z-slice w, in rounds of four particles per cell; the 64 lanes
+// stage 16 particles (lane part d < 3 evaluates direction d's four factor
+// pairs, parts 0, 1 and 3 read the thermal fields through the permutation)
+// into the wavefront's LDS staging area, then lane (ky, kz) of each cell's
+// 16 lanes accumulates the four nodes along x of stencil row (ky, kz) in
+// registers, adds a finished cell row into the tile's LDS node block (LDS
+// FP64 atomics) and the block goes to the nodes with one global FP64 atomic
+// per non-zero node component.  Four sums instead of the momentum deposit's
+// seven: m c_p N, m c_p T N, m k N and the source V Q N + V q.grad N, with
+// the momentum deposit's factors and order of operations (the source as its
+// first force component with a = V Q and B's first row V q).  FALLBACK: a
+// particle outside its cell (not redistributed this step) deposits with
+// th_deposit_one; without it such a particle raises the error flag.
+constexpr int kThNC = 4;     // accumulated sums: m c_p, m c_p T, m k, source
+constexpr int kThPkV = 24;   // packet: 24 factor pairs, then m c_p, m c_p T, m k, V Q, V q
+constexpr int kThPk = 32;    // packet stride (doubles)
+constexpr int kThPkArea = kSlots * kThPk + 2 * (kSlots / 4 - 1);
+// Packet slot s starts at th_pk_base(s): two extra doubles per cell put the
+// four cells' packets, read by four lane groups at once, in different LDS
+// bank quads.
+__device__ __forceinline__ int th_pk_base(int s) { return s * kThPk + 2 * (s >> 2); }
+struct ThP2gLds {
+    double slab[kThNC * kTNN];
+    double pk[4][kThPkArea];
+    int cs[kCellsPerTile + 1];
+    int perm[kPermLds];
+    SplineTables tab;
+};
+
+template <int O, bool MT, bool SRC, bool FALLBACK>
+__global__ __launch_bounds__(kBlock) void k_th_p2g_cells(GridP g, PState P, SortState S,
+                                                         const double *th, double *tn,
+                                                         int *err) {
+    static_assert(kBlock == 256, "four wavefronts per tile, one z-slice each");
+    __shared__ ThP2gLds L;
+    const int tile = active_tile(S, blockIdx.x);
+    if (tile < 0) return;  // uniform over the block
+    const int tid = threadIdx.x;
+    const int pbeg = S.cell_start[tile * kCellsPerTile];
+    const int pnum = min(S.cell_start[tile * kCellsPerTile + kCellsPerTile] - pbeg, kPermLds);
+    if (tid <= kCellsPerTile) L.cs[tid] = S.cell_start[tile * kCellsPerTile + tid];
+    for (int e = tid; e < pnum; e += kBlock) L.perm[e] = S.perm[pbeg + e];
+    if (O != 1) {
+        const int *src = reinterpret_cast<const int *>(&kTab);
+        int *dst = reinterpret_cast<int *>(&L.tab);
+        for (int e = tid; e < (int)(sizeof(SplineTables) / sizeof(int)); e += kBlock)
+            dst[e] = src[e];
     }
-    if (SRC) tn[TN_SRC * nn + q] = acc[4 * nn + q];
-    for (int c = 0; c < 7; ++c) acc[c * nn + q] = 0.0;
+    for (int e = tid; e < kThNC * kTNN; e += kBlock) L.slab[e] = 0.0;
+    __syncthreads();
+    if (L.cs[kCellsPerTile] == L.cs[0]) return;  // empty tile (uniform over the block)
+    const int wave = tid >> 6, lane = tid & 63;
+    double *pkw = L.pk[wave];
+    const int tx = tile % g.ntc[0];
+    const int ty = (tile / g.ntc[0]) % g.ntc[1];
+    const int tz = tile / (g.ntc[0] * g.ntc[1]);
+    const int T0[3] = {tx * kTileCells, ty * kTileCells, tz * kTileCells};
+    const int64_t cap = P.cap;
+    const SplineTables &TT = (O != 1) ? L.tab : kTab;
+    const int cz = wave;
+    // accumulation role: group q (cell x inside the tile), lane (ky, kz)
+    const int q = lane >> 4;
+    const int ky = lane & 3, kz = (lane >> 2) & 3;
+    // staging role: packet slot s (cell x = s >> 2, particle s & 3 of the
+    // round), part 0..3
+    const int s_slot = lane & 15, s_part = lane >> 4;
+    const int s_cell = s_slot >> 2, s_t = s_slot & 3;
+    // largest cell count of each row over the wavefront's four cells
+    int mc[kTileCells];
+#pragma unroll
+    for (int r = 0; r < kTileCells; ++r) {
+        const int c0 = kTileCells * (r + kTileCells * cz) + q;
+        int m = L.cs[c0 + 1] - L.cs[c0];
+        m = max(m, __shfl_xor(m, 16, 64));
+        m = max(m, __shfl_xor(m, 32, 64));
+        mc[r] = m;
+    }
+    auto mc_of = [&](int r) {
+        return r == 0 ? mc[0] : (r == 1 ? mc[1] : (r == 2 ? mc[2] : mc[3]));
+    };
+    auto advance = [&](int &ry, int &base) {
+        base += 4;
+        while (ry < kTileCells && base >= mc_of(ry)) {
+            ++ry;
+            base = 0;
+        }
+    };
+    // The permutation entry of this lane's slot in round (ry, base), -1 for
+    // an empty slot; then the slot's inputs: part 0 the phase, m, c_p, T and
+    // k; part 1 V and Q; part 3 V and q; parts 0..2 one position component.
+    auto fetch_j = [&](int ry, int base) -> int {
+        const int c0 = kTileCells * (ry + kTileCells * cz) + s_cell;
+        const int sbeg = L.cs[c0], scnt = L.cs[c0 + 1] - sbeg;
+        const int t = base + s_t;
+        if (t >= scnt) return -1;
+        const int o = sbeg + t - pbeg;
+        return o < pnum ? L.perm[o] : S.perm[sbeg + t];
+    };
+    struct Raw {
+        double xd, val[4];
+        int j, phase;
+    };
+    auto load = [&](int j) {
+        Raw R;
+        R.j = j;
+        R.phase = 0;
+        R.xd = 0.0;
+        for (int e = 0; e < 4; ++e) R.val[e] = 0.0;
+        if (j >= 0) {
+            if (s_part == 0) R.phase = P.idata[MPM_I_PHASE * cap + j];
+            if (s_part < 3) R.xd = P.pos[s_part * cap + j];
+            if (MT && s_part == 0) {
+                R.val[0] = P.r[MPM_R_MASS * cap + j];
+                R.val[1] = th[MPM_TH_SPECIFIC_HEAT * cap + j];
+                R.val[2] = th[MPM_TH_TEMPERATURE * cap + j];
+                R.val[3] = th[MPM_TH_CONDUCTIVITY * cap + j];
+            }
+            if (SRC && s_part == 1) {
+                R.val[0] = P.r[MPM_R_VOLUME * cap + j];
+                R.val[1] = th[MPM_TH_HEAT_SOURCE * cap + j];
+            }
+            if (SRC && s_part == 3) {
+                R.val[0] = P.r[MPM_R_VOLUME * cap + j];
+                for (int d = 0; d < 3; ++d) R.val[1 + d] = th[(MPM_TH_HEAT_FLUX + d) * cap + j];
+            }
+        }
+        return R;
+    };
+    int ry = 0, base = -4;
+    advance(ry, base);
+    double a4[4][kThNC];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int c = 0; c < kThNC; ++c) a4[k][c] = 0.0;
+    Raw cur;
+    if (ry < kTileCells) cur = load(fetch_j(ry, base));
+    while (ry < kTileCells) {
+        int nry = ry, nbase = base;
+        advance(nry, nbase);
+        Raw nxt;
+        if (nry < kTileCells) nxt = load(fetch_j(nry, nbase));
+        // ---- staging: this round's 16 packets.  A particle with phase != 0
+        // deposits nothing; the phase comes from the slot's part-0 lane.
+        {
+            const int ph0 = __shfl(cur.phase, s_slot, 64);
+            const bool act = cur.j >= 0 && ph0 == 0;
+            const int ic = (s_part == 0) ? T0[0] + s_cell
+                                         : ((s_part == 1) ? T0[1] + ry : T0[2] + cz);
+            const bool out =
+                s_part < 3 && act &&
+                (int)floor((cur.xd - pick3(g.plo, s_part)) * pick3(g.dxi, s_part)) != ic;
+            const unsigned long long ob = __ballot(out);
+            const bool slot_out = ((ob >> s_slot) | (ob >> (s_slot + 16)) |
+                                   (ob >> (s_slot + 32))) & 1ull;
+            bool use = act;
+            if (slot_out) {
+                if (FALLBACK) {
+                    if (s_part == 3 && act) th_deposit_one<MT, SRC>(g, P, th, tn, cur.j);
+                } else {
+                    *err = 1;
+                }
+                use = false;
+            }
+            double *pw = pkw + th_pk_base(s_slot);
+            if (s_part < 3) {
+                double f[8];
+                if (use) {
+                    p2g_factors4<O>(TT, g, s_part, ic, cur.xd, f);
+                } else {
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) f[e] = 0.0;
+                }
+#pragma unroll
+                for (int e = 0; e < 8; ++e) pw[kPkFac + s_part * 8 + e] = f[e];
+            }
+            // packet values with the arithmetic of the reference's deposit
+            // terms: m c_p, m c_p T, m k (part 0); V Q (part 1); V q (part 3)
+            if (MT && s_part == 0) {
+                const double mcp = cur.val[0] * cur.val[1];
+                pw[kThPkV + 0] = use ? mcp : 0.0;
+                pw[kThPkV + 1] = use ? mcp * cur.val[2] : 0.0;
+                pw[kThPkV + 2] = use ? cur.val[0] * cur.val[3] : 0.0;
+            }
+            if (SRC && s_part == 1) pw[kThPkV + 3] = use ? cur.val[0] * cur.val[1] : 0.0;
+            if (SRC && s_part == 3)
+                for (int d = 0; d < 3; ++d)
+                    pw[kThPkV + 4 + d] = use ? cur.val[0] * cur.val[1 + d] : 0.0;
+        }
+        // The staging area is private to the wavefront and LDS operations of
+        // one wavefront execute in issue order: the compiler only has to keep
+        // the packet writes ahead of the reads below.
+        asm volatile("" ::: "memory");
+        // ---- accumulation: particle base + tt of each of the row's cells
+        const int nt = min(4, mc_of(ry) - base);
+        for (int tt = 0; tt < nt; ++tt) {
+#pragma clang fp contract(fast)
+            const double *pr = pkw + th_pk_base(q * 4 + tt);
+            double wx[4], dwx[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                wx[k] = pr[kPkFac + 2 * k];
+                dwx[k] = pr[kPkFac + 2 * k + 1];
+            }
+            const double wy = pr[kPkFac + 8 + 2 * ky], dwy = pr[kPkFac + 8 + 2 * ky + 1];
+            const double wz = pr[kPkFac + 16 + 2 * kz], dwz = pr[kPkFac + 16 + 2 * kz + 1];
+            const double wyz = wy * wz;
+            if (MT) {
+                double Mr[3];
+                for (int c = 0; c < 3; ++c) Mr[c] = pr[kThPkV + c] * wyz;
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+                    for (int c = 0; c < 3; ++c) a4[k][c] += wx[k] * Mr[c];
+            }
+            if (SRC) {
+                const double dwyz = dwy * wz, wydz = wy * dwz;
+                const double C = pr[kThPkV + 3] * wyz + pr[kThPkV + 5] * dwyz +
+                                 pr[kThPkV + 6] * wydz;
+                const double E = pr[kThPkV + 4] * wyz;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    // two fused multiply-adds into the accumulator
+                    a4[k][3] += wx[k] * C;
+                    a4[k][3] += dwx[k] * E;
+                }
+            }
+        }
+        asm volatile("" ::: "memory");  // reads of this round before the next writes
+        if (nry != ry) {
+            // Row done: lane (q, ky, kz) adds its four nodes (x = q + k,
+            // y = ry + ky, z = cz + kz) into the tile's block.
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int idx = ((cz + kz) * kTN + (ry + ky)) * kTN + (q + k);
+#pragma unroll
+                for (int c = 0; c < kThNC; ++c) {
+                    if ((c < 3 && !MT) || (c == 3 && !SRC)) continue;
+                    __hip_atomic_fetch_add(&L.slab[c * kTNN + idx], a4[k][c], __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+                    a4[k][c] = 0.0;
+                }
+            }
+        }
+        ry = nry;
+        base = nbase;
+        cur = nxt;
+    }
+    __syncthreads();
+    // Flush the node block: one global atomic per non-zero node component,
+    // periodic image targets as in deposit_global.
+    const int64_t nn = g.nn;
+    const int tnc[kThNC] = {TN_MSP, TN_MSPT, TN_MC, TN_SRC};
+    for (int e = tid; e < kTNN; e += kBlock) {
+        const int ax = e % kTN, ay = (e / kTN) % kTN, az = e / (kTN * kTN);
+        double v[kThNC];
+        bool any = false;
+        for (int c = 0; c < kThNC; ++c) {
+            const bool on = c < 3 ? MT : SRC;
+            v[c] = on ? L.slab[c * kTNN + e] : 0.0;
+            any |= (v[c] != 0.0);
+        }
+        if (!any) continue;
+        int tx0, tx1, ty0, ty1, tz0, tz1;
+        const int cx = targets2(T0[0] + ax - 1, g.n[0], g.per[0], tx0, tx1);
+        const int cy = targets2(T0[1] + ay - 1, g.n[1], g.per[1], ty0, ty1);
+        const int cz2 = targets2(T0[2] + az - 1, g.n[2], g.per[2], tz0, tz1);
+#pragma unroll
+        for (int qz = 0; qz < 2; ++qz)
+#pragma unroll
+            for (int qy = 0; qy < 2; ++qy)
+#pragma unroll
+                for (int qx = 0; qx < 2; ++qx) {
+                    if (qz >= cz2 || qy >= cy || qx >= cx) continue;
+                    const int64_t node =
+                        nidx(g, qx ? tx1 : tx0, qy ? ty1 : ty0, qz ? tz1 : tz0);
+                    for (int c = 0; c < kThNC; ++c)
+                        if (v[c] != 0.0) unsafeAtomicAdd(&tn[tnc[c] * nn + node], v[c]);
+                }
+    }
 }
 
 // FabArray::SumBoundary over periodic direction d (AMReX, called at
@@ -695,27 +944,35 @@ __global__ __launch_bounds__(kBlock) void k_th_g2p_tile(GridP g, PState P, SortS
     }
 }
 
-void launch_th_p2g(mpm_ctx *c, bool mass_temp, bool source, bool tiled, int epoch, int fallback,
+void launch_th_p2g(mpm_ctx *c, bool mass_temp, bool source, bool tiled, int fallback,
                    hipStream_t st) {
     const GridP &g = c->g;
     const int64_t np = c->P.np, nn = g.nn;
     if (np > 0 && tiled) {
-        SortState S = c->S;
-        S.pkt = c->tpkt;
-        S.pkt_valid = 1;
-        if (mass_temp && source)
-            k_th_packets<true, true><<<blocks(np), kThreads, 0, st>>>(c->P, S, c->th, c->tpkt);
-        else if (mass_temp)
-            k_th_packets<true, false><<<blocks(np), kThreads, 0, st>>>(c->P, S, c->th, c->tpkt);
-        else
-            k_th_packets<false, true><<<blocks(np), kThreads, 0, st>>>(c->P, S, c->th, c->tpkt);
-        launch_p2g_tiles(g, c->ph, c->P, S, c->N, epoch, 0, fallback, c->sc, st);
-        if (mass_temp && source)
-            k_th_take<true, true><<<blocks(nn), kThreads, 0, st>>>(g, c->N.acc, c->tn);
-        else if (mass_temp)
-            k_th_take<true, false><<<blocks(nn), kThreads, 0, st>>>(g, c->N.acc, c->tn);
-        else
-            k_th_take<false, true><<<blocks(nn), kThreads, 0, st>>>(g, c->N.acc, c->tn);
+        const dim3 gr((unsigned)((c->S.ntiles + 7) / 8 * 8)), bl(kBlock);
+        const int O = g.uniform ? g.od[0] : 0;
+        int *err = &c->sc->err;
+#define XMPM_THP(o, mt, src)                                                                  \
+    do {                                                                                      \
+        if (fallback)                                                                         \
+            k_th_p2g_cells<o, mt, src, true><<<gr, bl, 0, st>>>(g, c->P, c->S, c->th, c->tn,  \
+                                                                 err);                        \
+        else                                                                                  \
+            k_th_p2g_cells<o, mt, src, false><<<gr, bl, 0, st>>>(g, c->P, c->S, c->th, c->tn, \
+                                                                  err);                       \
+    } while (0)
+#define XMPM_THPO(o)                                                                          \
+    do {                                                                                      \
+        if (mass_temp && source) XMPM_THP(o, true, true);                                     \
+        else if (mass_temp) XMPM_THP(o, true, false);                                         \
+        else XMPM_THP(o, false, true);                                                        \
+    } while (0)
+        if (O == 0) XMPM_THPO(0);
+        else if (O == 1) XMPM_THPO(1);
+        else if (O == 2) XMPM_THPO(2);
+        else XMPM_THPO(3);
+#undef XMPM_THPO
+#undef XMPM_THP
     } else if (np > 0) {
         if (mass_temp && source)
             k_th_p2g<true, true><<<blocks(np), kThreads, 0, st>>>(g, c->P, c->th, c->tn);
@@ -817,15 +1074,11 @@ int check_specs(const mpm_thermal_specs *t) {
 int alloc_particle_thermal(mpm_ctx *c) {
     (void)hipFree(c->th);
     (void)hipFree(c->th_alt);
-    (void)hipFree(c->tpkt);
-    c->th = c->th_alt = c->tpkt = nullptr;
+    c->th = c->th_alt = nullptr;
     const int64_t cap = c->P.cap > 0 ? c->P.cap : 1;
     const size_t bytes = (size_t)MPM_NTHERMAL * cap * sizeof(double);
     if (hipMalloc(&c->th, bytes) != hipSuccess || hipMalloc(&c->th_alt, bytes) != hipSuccess)
         return set_err(MPM_E_OOM, "thermal particle buffers");
-    if (c->S.pcap > 0 &&
-        hipMalloc(&c->tpkt, (size_t)kPktComps * c->S.pcap * sizeof(double)) != hipSuccess)
-        return set_err(MPM_E_OOM, "thermal packet buffer");
     launch_zero(c->th, (int64_t)MPM_NTHERMAL * cap, c->stream);
     launch_zero(c->th_alt, (int64_t)MPM_NTHERMAL * cap, c->stream);
     c->th_cap = cap;
@@ -861,7 +1114,7 @@ int thermal_init(mpm_ctx *c, bool tiled) {
     hipStream_t st = c->stream;
     const int64_t nn = c->g.nn;
     launch_zero(c->tn, TN_COMPS * nn, st);
-    launch_th_p2g(c, true, false, tiled && c->tpkt, (int)c->steps, 1, st);
+    launch_th_p2g(c, true, false, tiled, 1, st);
     k_th_nodal<false><<<blocks(nn), kThreads, 0, st>>>(c->g, c->s.mass_tolerance, c->tn, nullptr);
     launch_th_walls(c, true, false, LS_NONE, st);
     launch_th_g2p(c, true, true, nullptr, nullptr, c->th, st);
@@ -872,10 +1125,10 @@ int thermal_init(mpm_ctx *c, bool tiled) {
 // momentum grid pass has finished the nodal mass: Reset_Nodaldata_to_Zero's
 // share of the thermal components, deposit, normalise + backup + update,
 // walls, delta.
-int thermal_grid(mpm_ctx *c, bool tiled, int epoch, int fallback, hipStream_t st) {
+int thermal_grid(mpm_ctx *c, bool tiled, int fallback, hipStream_t st) {
     const int64_t nn = c->g.nn;
     launch_zero(c->tn, TN_COMPS * nn, st);
-    launch_th_p2g(c, true, true, tiled && c->tpkt, epoch, fallback, st);
+    launch_th_p2g(c, true, true, tiled, fallback, st);
     k_th_nodal<true><<<blocks(nn), kThreads, 0, st>>>(c->g, c->s.mass_tolerance, c->tn, c->sc);
     launch_th_walls(c, true, true, c->s.stress_update_scheme == MPM_MUSL ? LS_DIRICHLET : LS_ALL,
                     st);
@@ -888,7 +1141,7 @@ int thermal_grid(mpm_ctx *c, bool tiled, int epoch, int fallback, hipStream_t st
 // heat flux (main.cpp:130-135).  With perm (the sorted step) the last G2P
 // writes the second buffer by sorted position; thermal_swap follows the
 // particle buffers' swap.
-int thermal_particles(mpm_ctx *c, bool musl, const int *perm, const int *count, int epoch,
+int thermal_particles(mpm_ctx *c, bool musl, const int *perm, const int *count,
                       int fallback, hipStream_t st) {
     double *out = perm ? c->th_alt : c->th;
     const bool tiles = perm != nullptr && c->g.uniform;
@@ -904,7 +1157,7 @@ int thermal_particles(mpm_ctx *c, bool musl, const int *perm, const int *count, 
     // clears the thermal mass, its temperature sum and the source.
     launch_zero(c->tn + TN_MSP * nn, 2 * nn, st);
     launch_zero(c->tn + TN_SRC * nn, nn, st);
-    launch_th_p2g(c, false, true, perm != nullptr && c->tpkt, epoch, fallback, st);
+    launch_th_p2g(c, false, true, perm != nullptr, fallback, st);
     launch_th_walls(c, false, true, LS_ALL, st);
     if (tiles) launch_th_g2p_tiles(c, false, true, true, out, st);
     else launch_th_g2p(c, false, true, perm, count, out, st);
@@ -941,8 +1194,6 @@ int thermal_udf_tables(mpm_ctx *c, double t) {
 }
 
 void thermal_free(mpm_ctx *c) {
-    (void)hipFree(c->tpkt);
-    c->tpkt = nullptr;
     for (int d = 0; d < 3; ++d)
         for (int h = 0; h < 2; ++h) {
             (void)hipFree(c->tudf_dev[d][h]);

@@ -206,6 +206,44 @@ def test_levelset_body_conditions_match_the_oracle(gpu, scheme):
     compare(g, o, 1e-10)
 
 
+# Redistribution every five steps with particles drifting through the
+# periodic x faces: between redistributions particles lie outside their tile
+# (and outside the domain), so the tiled thermal deposit sends them through
+# its per-particle fallback.  Mixed orders: z with 4 cells makes cubic and
+# quadratic fall back to linear there (per-direction orders in the tile
+# deposit and G2P).
+REDIST = [
+    ("drift_usl_o2", dict(order=2, periodic=(1, 0, 0), speed=0.5), 3.0),
+    ("drift_musl_o3", dict(order=3, periodic=(1, 0, 1), speed=0.5, scheme=MUSL), 3.0),
+    ("mixed_order_usl", dict(order=3, n=(12, 10, 4), speed=0.3), 0.0),
+]
+
+
+@pytest.mark.parametrize("name,kw,drift", REDIST, ids=[c[0] for c in REDIST])
+def test_thermal_redistribution_interval_matches_the_oracle(gpu, name, kw, drift):
+    say(f"[thermal] num_redist=5 {name}")
+    s, pos, r, idata = scenes.random_block(57, frac=0.9, **kw)
+    s.num_redist = 5
+    r[capi.R_VEL] += drift
+    th = thermal_fields(pos.shape[1], 21)
+    g, o = pair(s, pos, r, idata, th, mixed_walls(s))
+    for c in (g, o):
+        c.step(1)
+    compare(g, o, 1e-12)
+    outside = 0
+    for _ in range(4):
+        for c in (g, o):
+            c.step(3)
+        pg, _, _, _ = g.download()
+        outside += int(np.sum((pg < np.array(s.plo)[:, None]) |
+                              (pg > np.array(s.phi)[:, None])))
+        for c in (g, o):
+            c.step(2)
+    compare(g, o, 1e-10)
+    if drift:
+        assert outside > 0  # particles outside their tile deposited through the fallback
+
+
 def test_thermal_fields_follow_the_particles_through_removals(gpu):
     """Particles leaving a non-periodic box are removed (Redistribute); the
     thermal record follows the survivors by upload index on both sides."""
