@@ -966,11 +966,11 @@ int mpm_step(mpm_ctx *c, int nsteps) {
             if (rc) return rc;
             if (sorted) thermal_swap(c);
         }
-        // The tile kernel needs every particle inside its tile: particles
-        // redistributed every step.
-        if (sorted && c->fast && nr == 1 && particle_fast_ok(c->g, c->ls, c->udf)) {
+        // The tile kernel; with redistribution every few steps its form for
+        // particles outside their tile (outside the domain) runs.
+        if (sorted && c->fast && particle_fast_ok(c->g, c->ls, c->udf)) {
             launch_particle_fast(c->g, c->ph, c->P, c->Q, c->S, c->N, musl ? 0 : 1, redist ? 1 : 0,
-                                 c->sc, st);
+                                 nr > 1 ? 1 : 0, c->sc, st);
             std::swap(c->P, c->Q);
             c->dt_next_valid = c->fuse_dt != 0;
         } else if (sorted) {

@@ -212,10 +212,12 @@ void launch_particle_sorted(const GridP &g, const PhysP &ph, const LsP &ls, cons
                             const PState &Pin, const PState &Pout, const SortState &S,
                             const NState &N, int flags, const Scalars *sc, hipStream_t s);
 // Specialised particle kernel (uniform order, no level set, no UDF); also
-// accumulates the next step's dt minimum.
+// accumulates the next step's dt minimum.  outside: particles are
+// redistributed every few steps (a particle outside the domain lies outside
+// its tile and gathers from the nodal arrays).
 bool particle_fast_ok(const GridP &g, const LsP &ls, const UdfP &udf);
 void launch_particle_fast(const GridP &g, const PhysP &ph, const PState &Pin, const PState &Pout,
-                          const SortState &S, const NState &N, int vel, int redist, Scalars *sc,
-                          hipStream_t s);
+                          const SortState &S, const NState &N, int vel, int redist, int outside,
+                          Scalars *sc, hipStream_t s);
 
 }  // namespace xmpm

@@ -204,7 +204,7 @@ int coarse_begin(mpm_ctx *c, int nfine, int advance = 0, const PState *in = null
 // The particle kernel of the coarse step and the time advance.
 int coarse_finish(mpm_ctx *c, int nfine) {
     hipStream_t st = c->stream;
-    launch_particle_fast(c->g, coarse_ph(c, nfine), c->P, c->Q, c->S, c->N, 1, 1, c->sc, st);
+    launch_particle_fast(c->g, coarse_ph(c, nfine), c->P, c->Q, c->S, c->N, 1, 1, 0, c->sc, st);
     std::swap(c->P, c->Q);
     c->dt_next_valid = c->fuse_dt != 0;
     c->dirty = false;
@@ -697,7 +697,7 @@ int mpm_step_learned(mpm_ctx *c, mpm_mlp *m, int ncoarse, int nfine, const doubl
         HIPCHK(hipEventRecord(c->ml_ev[it & 1], st));
         hipLaunchKernelGGL(k_apply_dev, dim3(512), dim3(256), 0, st, c->ml_nodes, cnt, c->ml_y,
                            m->sc.dv_ref, c->N, c->g.nn, flag);
-        launch_particle_fast(c->g, coarse_ph(c, nfine), in, out, c->S, c->N, 1, 1, c->sc, st);
+        launch_particle_fast(c->g, coarse_ph(c, nfine), in, out, c->S, c->N, 1, 1, 0, c->sc, st);
         return check_launch();
     };
     // Speculation: step it + 1 (from Q into R) is enqueued as if step it

@@ -168,7 +168,7 @@ __global__ __launch_bounds__(kBlock) void k_p2g_gather(GridP g, BlkGather bg,
 
 // Fused particle pipeline over the non-empty tiles, one workgroup each (see
 // particle_tile); the next step's dt candidates go into the running minimum.
-template <int O, bool VEL, bool DELTA, int NT>
+template <int O, bool VEL, bool DELTA, int NT, bool OUT>
 __global__ __launch_bounds__(NT) void k_particle_tile(GridP g, PhysP ph, PState Pin,
                                                       PState Pout, SortState S, NState N,
                                                       int redist, Scalars *sc) {
@@ -176,8 +176,8 @@ __global__ __launch_bounds__(NT) void k_particle_tile(GridP g, PhysP ph, PState 
     const int tile = active_tile(S, blockIdx.x);
     if (tile < 0) return;  // uniform over the block
     double cmin = __builtin_huge_val();
-    particle_tile<O, VEL, DELTA, NT>(g, ph, Pin, Pout, S, N, redist, sc->dt, sc->time, &sc->err,
-                                     tile, lds, cmin);
+    particle_tile<O, VEL, DELTA, NT, OUT>(g, ph, Pin, Pout, S, N, redist, sc->dt, sc->time,
+                                          &sc->err, tile, lds, cmin);
     dt_wave_min(cmin, sc);  // every lane reaches this point
 }
 
@@ -301,20 +301,25 @@ bool particle_fast_ok(const GridP &g, const LsP &ls, const UdfP &udf) {
 }
 
 void launch_particle_fast(const GridP &g, const PhysP &ph, const PState &Pin, const PState &Pout,
-                          const SortState &S, const NState &N, int vel, int redist, Scalars *sc,
-                          hipStream_t s) {
+                          const SortState &S, const NState &N, int vel, int redist, int outside,
+                          Scalars *sc, hipStream_t s) {
     if (Pin.np <= 0) return;
     const dim3 gr((unsigned)((S.ntiles + 7) / 8 * 8)), bl(kBlock), bl2(2 * kBlock);
     const int O = g.od[0];
     const bool few = S.waves ? S.waves == 8 : few_tiles(Pin.np);
+    // outside: particles are redistributed every few steps, so one outside
+    // the domain may lie outside its tile (one form, a workgroup of 256).
 #define XMPM_PT(o, v, dl)                                                                     \
     do {                                                                                      \
-        if (few)                                                                              \
-            hipLaunchKernelGGL((k_particle_tile<o, v, dl, 2 * kBlock>), gr, bl2, 0, s, g, ph, \
+        if (outside)                                                                          \
+            hipLaunchKernelGGL((k_particle_tile<o, v, dl, kBlock, true>), gr, bl, 0, s, g, ph, \
                                Pin, Pout, S, N, redist, sc);                                  \
+        else if (few)                                                                         \
+            hipLaunchKernelGGL((k_particle_tile<o, v, dl, 2 * kBlock, false>), gr, bl2, 0, s,  \
+                               g, ph, Pin, Pout, S, N, redist, sc);                           \
         else                                                                                  \
-            hipLaunchKernelGGL((k_particle_tile<o, v, dl, kBlock>), gr, bl, 0, s, g, ph, Pin,  \
-                               Pout, S, N, redist, sc);                                       \
+            hipLaunchKernelGGL((k_particle_tile<o, v, dl, kBlock, false>), gr, bl, 0, s, g,    \
+                               ph, Pin, Pout, S, N, redist, sc);                              \
     } while (0)
     if (ph.delta) {
         if (O == 1) { if (vel) XMPM_PT(1, true, true); else XMPM_PT(1, false, true); }

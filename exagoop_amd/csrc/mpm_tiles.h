@@ -823,13 +823,16 @@ __device__ __forceinline__ void tile_factors(const SplineTables &T, int per, int
 // G2P of one particle from the LDS node block of its tile: PIC velocity
 // vals0 (and FLIP increment vals1 when VEL) and the velocity gradient L
 // (row-major).  The z offsets run as a loop and the W x W (y, x) offsets
-// of a plane are unrolled; the x and y factors stay in registers.  Returns
-// false, with zero sums, for a particle outside the tile.
-template <int O, bool VEL>
+// of a plane are unrolled; the x and y factors stay in registers.  A
+// particle outside the tile (outside the domain between redistributions:
+// the sort bins it with the nearest cell): OUT, the same sums node by node
+// from the nodal arrays N (rolled loops, factors per node); else false,
+// with zero sums.
+template <int O, bool VEL, bool OUT = false>
 __device__ __forceinline__ bool g2p_block(const GridP &g, const SplineTables &T,
                                           const double *nodes, const int T0[3],
                                           const double x[3], double vals0[3], double vals1[3],
-                                          double L[9]) {
+                                          double L[9], const NState *N = nullptr) {
 #pragma clang fp contract(fast)
     constexpr int W = (O == 1) ? 2 : 4;
     constexpr int L0 = (O == 1) ? 0 : -1;
@@ -843,7 +846,51 @@ __device__ __forceinline__ bool g2p_block(const GridP &g, const SplineTables &T,
     }
     for (int d = 0; d < 3; ++d) vals0[d] = vals1[d] = 0.0;
     for (int q = 0; q < 9; ++q) L[q] = 0.0;
-    if (!inside) return false;
+    if (!inside) {
+        if (!OUT) return false;
+        const int64_t nn = g.nn;
+#pragma unroll 1
+        for (int lz = mn[2]; lz < mx[2]; ++lz) {
+            double wz, bz, dz;
+            tile_factors<O>(T, g.per[2], iv[2], lz, g.hi[2], x[2], g.plo[2], g.dx[2], g.dxi[2],
+                            true, wz, bz, dz);
+#pragma unroll 1
+            for (int ly = mn[1]; ly < mx[1]; ++ly) {
+                double wy, by, dy;
+                tile_factors<O>(T, g.per[1], iv[1], ly, g.hi[1], x[1], g.plo[1], g.dx[1],
+                                g.dxi[1], true, wy, by, dy);
+                const double wyz = wy * wz;
+                const double p0 = by * bz, p1 = dy * bz, p2 = by * dz;
+#pragma unroll 1
+                for (int lx = mn[0]; lx < mx[0]; ++lx) {
+                    double wx, bx, dxv;
+                    tile_factors<O>(T, g.per[0], iv[0], lx, g.hi[0], x[0], g.plo[0], g.dx[0],
+                                    g.dxi[0], true, wx, bx, dxv);
+                    bool ok = true;
+                    const int I = wrap_read(iv[0] + lx, g.n[0], g.per[0], ok);
+                    const int J = wrap_read(iv[1] + ly, g.n[1], g.per[1], ok);
+                    const int K = wrap_read(iv[2] + lz, g.n[2], g.per[2], ok);
+                    if (!ok) continue;
+                    const int64_t qn = nidx(g, I, J, K);
+                    const double gr0 = dxv * p0, gr1 = bx * p1, gr2 = bx * p2;
+                    for (int a = 0; a < 3; ++a) {
+                        const double nv = N->vel[a * nn + qn];
+                        L[a * 3 + 0] += nv * gr0;
+                        L[a * 3 + 1] += nv * gr1;
+                        L[a * 3 + 2] += nv * gr2;
+                    }
+                    if (VEL) {
+                        const double w = wx * wyz;
+                        for (int a = 0; a < 3; ++a) {
+                            vals0[a] += w * N->vel[a * nn + qn];
+                            vals1[a] += w * N->dvel[a * nn + qn];
+                        }
+                    }
+                }
+            }
+        }
+        return true;
+    }
     double fw[2][W], fb[2][W], fd[2][W];
 #pragma unroll
     for (int d = 0; d < 2; ++d)
@@ -897,7 +944,7 @@ __device__ __forceinline__ bool g2p_block(const GridP &g, const SplineTables &T,
 // in LDS, then each thread takes one particle of the tile: G2P (g2p_block),
 // F, D, move + wall BCs, redistribution, volume, stress, written in sorted
 // order.  The next step's dt candidates are min-reduced into cmin.
-template <int O, bool VEL, bool DELTA, int NT = kBlock>
+template <int O, bool VEL, bool DELTA, int NT = kBlock, bool OUT = false>
 __device__ __forceinline__ void particle_tile(const GridP &g, const PhysP &ph, const PState &Pin,
                                               const PState &Pout, const SortState &S,
                                               const NState &N, int redist, double dt,
@@ -989,8 +1036,10 @@ __device__ __forceinline__ void particle_tile(const GridP &g, const PhysP &ph, c
         // ahead of the rest of the batch.
         double vals0[3], vals1[3], L[9];
         // Every particle lies in its tile when particles are redistributed
-        // each step (the only configuration this kernel is launched for).
-        const bool in_tile = g2p_block<O, VEL>(g, O > 1 ? tab : kTab, nodes, T0, x, vals0, vals1, L);
+        // each step; between redistributions (OUT) a particle outside the
+        // domain gathers from the nodal arrays.
+        const bool in_tile =
+            g2p_block<O, VEL, OUT>(g, O > 1 ? tab : kTab, nodes, T0, x, vals0, vals1, L, &N);
         if (!rigid) {
             if (!in_tile) *err = 1;
             if (VEL) {
