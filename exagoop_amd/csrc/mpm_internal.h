@@ -85,6 +85,10 @@ struct Scalars {
     unsigned long long dtmin_next_bits;  // same, accumulated by the step for the next step
     int err;
     int pad;
+    // Particles the tile deposit found outside their cell (outside the
+    // domain between redistributions), listed for the deposit after the tile
+    // pass: counters of alternate launches (SortState::ogen).
+    int ocount[2];
 };
 
 // Grid-kernel stage flags.
@@ -179,6 +183,11 @@ struct SortState {
     // Wavefronts per tile of the tile kernels (MPM_OPT_TILE_WAVES): 4 or 8,
     // 0 to choose by the particle count.
     int waves = 0;
+    // The tile deposit's list of particles outside their cell: entries in
+    // rank (free between the scatter and the next sort), their count in
+    // ocount[ogen & 1] (Scalars::ocount); ogen counts the launches that list.
+    int *ocount = nullptr;
+    int ogen = 0;
 };
 constexpr int kPktComps = 16;
 size_t sort_scan_bytes(int nkeys);

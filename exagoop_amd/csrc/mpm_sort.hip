@@ -134,15 +134,33 @@ __global__ __launch_bounds__(kBlock) void k_cell_order(SortState S) {
 }
 
 // deposit_onto_grid_momentum over the non-empty tiles, one workgroup each
-// (see p2g_tile).
-template <int O, bool FORCES, bool FALLBACK, bool GATHER, bool DET, int NW>
+// (see p2g_tile).  DEFER: particles outside their cell are listed
+// (SortState::ocount, rank) for k_deposit_outside; the launch clears the
+// counter the next listing launch uses.
+template <int O, bool FORCES, bool FALLBACK, bool GATHER, bool DET, int NW, bool DEFER>
 __global__ __launch_bounds__(NW * 64) void k_p2g_cells(GridP g, PhysP ph, PState P, SortState S,
                                                        double *acc, int epoch, Scalars *sc) {
     __shared__ P2gLdsOf<FORCES, GATHER, DET, NW> lds;
+    if (DEFER && blockIdx.x == 0 && threadIdx.x == 0) S.ocount[(S.ogen + 1) & 1] = 0;
     const int pos = active_pos(S, blockIdx.x);
     if (pos < 0) return;  // uniform over the block
-    p2g_tile<O, FORCES, FALLBACK, GATHER, DET, NW>(g, ph, P, S, acc, epoch, &sc->err,
-                                                   S.tiles[1 + pos], lds, pos, S.pass);
+    p2g_tile<O, FORCES, FALLBACK, GATHER, DET, NW, DEFER>(g, ph, P, S, acc, epoch, &sc->err,
+                                                          S.tiles[1 + pos], lds, pos, S.pass);
+}
+
+// The deposits of the particles a DEFER tile launch listed, one thread per
+// entry (deposit_global, as the inline fallback makes them).
+template <bool FORCES, bool GATHER>
+__global__ __launch_bounds__(kBlock) void k_deposit_outside(GridP g, PhysP ph, PState P,
+                                                            SortState S, double *acc, int epoch) {
+    // the count as the tile launch's atomics left it (a vector load)
+    const int n = __hip_atomic_load(&S.ocount[S.ogen & 1], __ATOMIC_RELAXED,
+                                    __HIP_MEMORY_SCOPE_AGENT);
+    for (int k = blockIdx.x * kBlock + threadIdx.x; k < n; k += gridDim.x * kBlock) {
+        double x[3], vals[13];
+        fallback_vals<FORCES, GATHER>(ph, P, S, S.rank[k], x, vals);
+        deposit_global(g, x, vals, FORCES, acc, S.tile_epoch, epoch);
+    }
 }
 
 // The deterministic P2G's gather pass: node (I, J, K) adds its blocks
@@ -241,14 +259,18 @@ void launch_p2g_tiles(const GridP &g, const PhysP &ph, const PState &P, SortStat
     const bool det = S.blk && !S.pkt_valid;
     const bool few = S.waves ? S.waves == 8 : few_tiles(P.np);
     if (det) ++S.pass;
+    // The atomic forms list the particles outside their cell for
+    // k_deposit_outside (DEFER = fallback); the deterministic forms deposit
+    // them inline.
+    S.ocount = sc->ocount;
 #define XMPM_PGW(o, f, fb, ga, w)                                                             \
-    hipLaunchKernelGGL((k_p2g_cells<o, f, fb, ga, false, w>), gr, (w) == 8 ? bl8 : bl, 0, s, \
-                       g, ph, P, S, N.acc, epoch, sc)
+    hipLaunchKernelGGL((k_p2g_cells<o, f, fb, ga, false, w, fb>), gr, (w) == 8 ? bl8 : bl, 0, \
+                       s, g, ph, P, S, N.acc, epoch, sc)
 #define XMPM_PG(o, f, fb)                                                                     \
     do {                                                                                      \
         if (det)                                                                              \
-            hipLaunchKernelGGL((k_p2g_cells<o, f, fb, f, true, 4>), gr, bl, 0, s, g, ph, P, S, \
-                               N.acc, epoch, sc);                                             \
+            hipLaunchKernelGGL((k_p2g_cells<o, f, fb, f, true, 4, false>), gr, bl, 0, s, g, ph, \
+                               P, S, N.acc, epoch, sc);                                       \
         else if (f && !S.pkt_valid) {                                                         \
             if (few) XMPM_PGW(o, f, fb, true, 8); else XMPM_PGW(o, f, fb, true, 4);           \
         } else {                                                                              \
@@ -270,6 +292,19 @@ void launch_p2g_tiles(const GridP &g, const PhysP &ph, const PState &P, SortStat
 #undef XMPM_PGO
 #undef XMPM_PG
 #undef XMPM_PGW
+    if (fallback && !det) {
+        const dim3 go(64);
+        if (momentum_only)
+            hipLaunchKernelGGL((k_deposit_outside<false, false>), go, bl, 0, s, g, ph, P, S,
+                               N.acc, epoch);
+        else if (!S.pkt_valid)
+            hipLaunchKernelGGL((k_deposit_outside<true, true>), go, bl, 0, s, g, ph, P, S, N.acc,
+                               epoch);
+        else
+            hipLaunchKernelGGL((k_deposit_outside<true, false>), go, bl, 0, s, g, ph, P, S,
+                               N.acc, epoch);
+        ++S.ogen;
+    }
     if (det) {
         BlkGather bg;
         bg.blk = S.blk;

@@ -295,6 +295,36 @@ __device__ __forceinline__ void p2g_factors4(const SplineTables &T, const GridP 
     }
 }
 
+// Position and deposit values (deposit_global's vals) of a particle the tile
+// deposit finds outside its cell: FORCES and GATHER, the packet of particle
+// slot as scatter_one builds it; FORCES without GATHER, the sorted packet at
+// position slot; otherwise (momentum only) m and m v of particle slot.
+template <bool FORCES, bool GATHER>
+__device__ __forceinline__ void fallback_vals(const PhysP &ph, const PState &P,
+                                              const SortState &S, int64_t slot, double x[3],
+                                              double vals[13]) {
+    const int64_t cap = P.cap, pc = S.pcap;
+    if (FORCES && GATHER) {
+        const int64_t j = slot;
+        for (int d = 0; d < 3; ++d) x[d] = P.pos[d * cap + j];
+        const double m = P.r[MPM_R_MASS * cap + j];
+        vals[0] = m;
+        for (int d = 0; d < 3; ++d) vals[1 + d] = m * P.r[(MPM_R_VEL + d) * cap + j];
+        for (int d = 0; d < 3; ++d) vals[4 + d] = m * ph.grav[d] + 0.0;
+        const double V = P.r[MPM_R_VOLUME * cap + j];
+        for (int e = 0; e < 6; ++e) vals[7 + e] = -V * P.r[(MPM_R_STRESS + e) * cap + j];
+    } else if (FORCES) {
+        for (int d = 0; d < 3; ++d) x[d] = S.pkt[d * pc + slot];
+        for (int e = 0; e < 13; ++e) vals[e] = S.pkt[(3 + e) * pc + slot];
+    } else {
+        const int64_t j = slot;
+        for (int d = 0; d < 3; ++d) x[d] = P.pos[d * cap + j];
+        for (int e = 0; e < 13; ++e) vals[e] = 0.0;
+        vals[0] = P.r[MPM_R_MASS * cap + j];
+        for (int d = 0; d < 3; ++d) vals[1 + d] = vals[0] * P.r[(MPM_R_VEL + d) * cap + j];
+    }
+}
+
 // Particle packet of the P2G staging area (doubles): 12 factor pairs
 // (value, derivative) for directions x, y, z and offsets -1..2, then m, m v,
 // the body force density a = m g (+ slab load) and B = -V sigma (xx, xy, xz,
@@ -344,7 +374,9 @@ struct StageRaw {
 // count allows three waves per SIMD).  At the end the block is flushed with
 // one global FP64 atomic per non-zero node component.  FALLBACK: a particle
 // outside its cell (not redistributed this step) deposits through
-// deposit_global; without it such a particle raises the error flag.
+// deposit_global, or (DEFER) is listed for k_deposit_outside, which keeps
+// deposit_global's registers and stack out of the tile kernel; without
+// FALLBACK such a particle raises the error flag.
 
 // LDS of one P2G tile (NC accumulated components).  DET false: the tile's
 // node block, shared by the wavefronts (LDS atomic adds).  DET true: one
@@ -379,7 +411,8 @@ struct PartLds {
 // pass in S.tile_slot; no atomics (but those of the FALLBACK global path).
 // NW wavefronts per tile: 4 (one z-slice each) or 8 (a z-slice's first or
 // second pair of cell rows each, half the chain of rounds per wavefront).
-template <int O, bool FORCES, bool FALLBACK, bool GATHER = false, bool DET = false, int NW = 4>
+template <int O, bool FORCES, bool FALLBACK, bool GATHER = false, bool DET = false, int NW = 4,
+          bool DEFER = false>
 __device__ __forceinline__ void p2g_tile(const GridP &g, const PhysP &ph, const PState &P,
                                          const SortState &S, double *acc, int epoch, int *err,
                                          int tile, P2gLdsOf<FORCES, GATHER, DET, NW> &L,
@@ -562,32 +595,15 @@ __device__ __forceinline__ void p2g_tile(const GridP &g, const PhysP &ph, const 
             if (slot_out) {
                 if (FALLBACK) {
                     if (s_part == 3 && cur.act) {
-                        double x[3], vals[13];
-                        if (FORCES && GATHER) {
-                            // the packet of particle j as scatter_one builds it
-                            // (no slab loads on this path)
-                            const int64_t j = cur.slot;
-                            for (int d = 0; d < 3; ++d) x[d] = P.pos[d * cap + j];
-                            const double m = P.r[MPM_R_MASS * cap + j];
-                            vals[0] = m;
-                            for (int d = 0; d < 3; ++d)
-                                vals[1 + d] = m * P.r[(MPM_R_VEL + d) * cap + j];
-                            for (int d = 0; d < 3; ++d) vals[4 + d] = m * ph.grav[d] + 0.0;
-                            const double V = P.r[MPM_R_VOLUME * cap + j];
-                            for (int e = 0; e < 6; ++e)
-                                vals[7 + e] = -V * P.r[(MPM_R_STRESS + e) * cap + j];
-                        } else if (FORCES) {
-                            for (int d = 0; d < 3; ++d) x[d] = S.pkt[d * pc + cur.slot];
-                            for (int e = 0; e < 13; ++e) vals[e] = S.pkt[(3 + e) * pc + cur.slot];
+                        if (DEFER) {
+                            // listed for the deposit after the tile pass
+                            const int k = atomicAdd(&S.ocount[S.ogen & 1], 1);
+                            S.rank[k] = (int)cur.slot;
                         } else {
-                            const int64_t j = cur.slot;
-                            for (int d = 0; d < 3; ++d) x[d] = P.pos[d * cap + j];
-                            for (int e = 0; e < 13; ++e) vals[e] = 0.0;
-                            vals[0] = P.r[MPM_R_MASS * cap + j];
-                            for (int d = 0; d < 3; ++d)
-                                vals[1 + d] = vals[0] * P.r[(MPM_R_VEL + d) * cap + j];
+                            double x[3], vals[13];
+                            fallback_vals<FORCES, GATHER>(ph, P, S, cur.slot, x, vals);
+                            deposit_global(g, x, vals, FORCES, acc, S.tile_epoch, epoch);
                         }
-                        deposit_global(g, x, vals, FORCES, acc, S.tile_epoch, epoch);
                     }
                 } else {
                     *err = 1;
