@@ -34,6 +34,11 @@ The disks object runs the elastic-disk collision (two linear-elastic disks,
 64x64x4 cells, 33 008 particles, quadratic B-splines, dt = 1e-4): 200 timed
 steps on the GPU and the oracle over the same steps.
 
+The redist10 object runs the same dam break with the reference's default
+redistribution interval (num_redist = 10, mpm_specs.H:89): particles are
+redistributed every tenth step, so between redistributions the tile kernels
+run their forms for particles outside their tile.
+
 The thermal object runs the same dam break with the thermal module on
 (uniform temperature, adiabatic walls): milliseconds per step and the
 thermal stages' share of them.
@@ -86,6 +91,7 @@ def main():
     ap.add_argument("--no-thermal", action="store_true")
     ap.add_argument("--no-disks", action="store_true")
     ap.add_argument("--no-det", action="store_true")
+    ap.add_argument("--no-redist10", action="store_true")
     ap.add_argument("--thermal-steps", type=int, default=50)
     ap.add_argument("--batch-scenes", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=500)
@@ -151,6 +157,10 @@ def main():
     if not args.no_det:
         det = deterministic_leg(s, pos, r, idata, args.warmup, args.steps, ev_ms / args.steps)
 
+    redist10 = None
+    if not args.no_redist10:
+        redist10 = redist_leg(s, pos, r, idata, args.warmup, args.steps, ev_ms / args.steps)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_leg(s, pos, r, idata, args.cpu_warmup, args.cpu_steps, args.cpu_1t_steps)
@@ -190,6 +200,7 @@ def main():
                    "constitutive": "Tait EOS K=2e6 gamma=7 mu=1e-3", "cfl": 0.3},
         "roofline": roofline,
         "deterministic": det,
+        "redist10": redist10,
         "cpu_baseline": cpu,
         "learned": learned,
         "batch": batch,
@@ -225,6 +236,36 @@ def deterministic_leg(s, pos, r, idata, warmup, nsteps, plain_ms):
            "sort_ms": round(kms[3], 5)}
     print(f"[bench] deterministic P2G: {ms:.4f} ms/step ({out['vs_default_time']}x the default "
           f"step), p2g+gather {kms[0]:.4f} ms, sort {kms[3]:.4f} ms", file=sys.stderr, flush=True)
+    return out
+
+
+def redist_leg(s, pos, r, idata, warmup, nsteps, plain_ms):
+    """The same dam break with num_redist = 10, the reference's default:
+    milliseconds per step over the same steps and the per-kernel times."""
+    import copy
+    from exagoop_amd import capi
+    s10 = copy.deepcopy(s)
+    s10.num_redist = 10
+    g = capi.Context(s10)
+    g.upload(pos, r, idata)
+    g.init_internal_forces()
+    g.step(warmup)
+    g.step(nsteps)
+    ev_ms, _ = g.last_step_timing()
+    g.set_option(capi.OPT_PROFILE, 1)
+    g.step(min(nsteps, 20))
+    _, kms = g.last_step_timing()
+    g.close()
+    ms = ev_ms / nsteps
+    out = {"num_redist": 10, "ms_per_step": round(ms, 5),
+           "value": round(pos.shape[1] * 1e3 / ms, 1), "unit": "particle-updates/s",
+           "vs_default_time": round(ms / plain_ms, 3),
+           "kernels_ms": {"p2g": round(kms[0], 5), "grid": round(kms[1], 5),
+                          "g2p_move_stress": round(kms[2], 5)},
+           "sort_ms": round(kms[3], 5)}
+    print(f"[bench] num_redist 10: {ms:.4f} ms/step ({out['vs_default_time']}x the every-step "
+          f"redistribution), p2g {kms[0]:.4f} ms, particle {kms[2]:.4f} ms", file=sys.stderr,
+          flush=True)
     return out
 
 
