@@ -52,9 +52,8 @@ __device__ void th_deposit_one(const GridP &g, const PState &P, const double *th
     const double q[3] = {th[(MPM_TH_HEAT_FLUX + 0) * cap + i], th[(MPM_TH_HEAT_FLUX + 1) * cap + i],
                          th[(MPM_TH_HEAT_FLUX + 2) * cap + i]};
     const int64_t nn = g.nn;
-    // The 1-D factors are evaluated per node (rolled loops, no per-lane
-    // arrays): the function is inlined into the tiled deposit's rare
-    // fallback, and its registers count against that kernel's.
+    // The 1-D factors are evaluated per node: rolled loops, no per-lane
+    // arrays, few registers.
 #pragma unroll 1
     for (int n = mn[2]; n < mx[2]; ++n) {
         int tz[2];
@@ -131,8 +130,10 @@ __global__ __launch_bounds__(kThreads) void k_th_p2g(GridP g, PState P, const do
 // seven: m c_p N, m c_p T N, m k N and the source V Q N + V q.grad N, with
 // the momentum deposit's factors and order of operations (the source as its
 // first force component with a = V Q and B's first row V q).  FALLBACK: a
-// particle outside its cell (not redistributed this step) deposits with
-// th_deposit_one; without it such a particle raises the error flag.
+// particle outside its cell (outside the domain between redistributions) is
+// listed, as the momentum tile deposit lists them (SortState::ocount, rank),
+// for k_th_deposit_outside; without it such a particle raises the error
+// flag.
 constexpr int kThNC = 4;     // accumulated sums: m c_p, m c_p T, m k, source
 constexpr int kThPkV = 24;   // packet: 24 factor pairs, then m c_p, m c_p T, m k, V Q, V q
 constexpr int kThPk = 32;    // packet stride (doubles)
@@ -155,6 +156,7 @@ __global__ __launch_bounds__(kBlock) void k_th_p2g_cells(GridP g, PState P, Sort
                                                          int *err) {
     static_assert(kBlock == 256, "four wavefronts per tile, one z-slice each");
     __shared__ ThP2gLds L;
+    if (FALLBACK && blockIdx.x == 0 && threadIdx.x == 0) S.ocount[(S.ogen + 1) & 1] = 0;
     const int tile = active_tile(S, blockIdx.x);
     if (tile < 0) return;  // uniform over the block
     const int tid = threadIdx.x;
@@ -278,7 +280,10 @@ __global__ __launch_bounds__(kBlock) void k_th_p2g_cells(GridP g, PState P, Sort
             bool use = act;
             if (slot_out) {
                 if (FALLBACK) {
-                    if (s_part == 3 && act) th_deposit_one<MT, SRC>(g, P, th, tn, cur.j);
+                    if (s_part == 3 && act) {
+                        const int k = atomicAdd(&S.ocount[S.ogen & 1], 1);
+                        S.rank[k] = cur.j;
+                    }
                 } else {
                     *err = 1;
                 }
@@ -399,6 +404,18 @@ __global__ __launch_bounds__(kBlock) void k_th_p2g_cells(GridP g, PState P, Sort
                         if (v[c] != 0.0) unsafeAtomicAdd(&tn[tnc[c] * nn + node], v[c]);
                 }
     }
+}
+
+// The deposits of the particles a FALLBACK k_th_p2g_cells launch listed, one
+// thread per entry.
+template <bool MT, bool SRC>
+__global__ __launch_bounds__(kBlock) void k_th_deposit_outside(GridP g, PState P, SortState S,
+                                                               const double *th, double *tn) {
+    // the count as the tile launch's atomics left it (a vector load)
+    const int n = __hip_atomic_load(&S.ocount[S.ogen & 1], __ATOMIC_RELAXED,
+                                    __HIP_MEMORY_SCOPE_AGENT);
+    for (int k = blockIdx.x * kBlock + threadIdx.x; k < n; k += gridDim.x * kBlock)
+        th_deposit_one<MT, SRC>(g, P, th, tn, S.rank[k]);
 }
 
 // FabArray::SumBoundary over periodic direction d (AMReX, called at
@@ -967,12 +984,23 @@ void launch_th_p2g(mpm_ctx *c, bool mass_temp, bool source, bool tiled, int fall
         else if (mass_temp) XMPM_THP(o, true, false);                                         \
         else XMPM_THP(o, false, true);                                                        \
     } while (0)
+        c->S.ocount = c->sc->ocount;
         if (O == 0) XMPM_THPO(0);
         else if (O == 1) XMPM_THPO(1);
         else if (O == 2) XMPM_THPO(2);
         else XMPM_THPO(3);
 #undef XMPM_THPO
 #undef XMPM_THP
+        if (fallback) {
+            const dim3 go(64);
+            if (mass_temp && source)
+                k_th_deposit_outside<true, true><<<go, bl, 0, st>>>(g, c->P, c->S, c->th, c->tn);
+            else if (mass_temp)
+                k_th_deposit_outside<true, false><<<go, bl, 0, st>>>(g, c->P, c->S, c->th, c->tn);
+            else
+                k_th_deposit_outside<false, true><<<go, bl, 0, st>>>(g, c->P, c->S, c->th, c->tn);
+            ++c->S.ogen;
+        }
     } else if (np > 0) {
         if (mass_temp && source)
             k_th_p2g<true, true><<<blocks(np), kThreads, 0, st>>>(g, c->P, c->th, c->tn);
