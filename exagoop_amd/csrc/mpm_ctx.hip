@@ -949,7 +949,11 @@ int mpm_step(mpm_ctx *c, int nsteps) {
         const int tail = P_MOVE | (redist ? P_REDIST : 0) | P_VOLUME | P_STRESS;
         int last = P_G2P_VEL | P_G2P_GRAD | tail;
         if (musl) {
-            launch_particle(c->g, c->ph, c->ls, c->udf, c->P, c->N, P_G2P_VEL, 0.0, 0.0, c->sc, st);
+            if (sorted && c->fast && particle_fast_ok(c->g, c->ls, c->udf))
+                launch_g2p_vel_tiles(c->g, c->ph, c->P, c->S, c->N, nr > 1 ? 1 : 0, c->sc, st);
+            else
+                launch_particle(c->g, c->ph, c->ls, c->udf, c->P, c->N, P_G2P_VEL, 0.0, 0.0, c->sc,
+                                st);
             // Images of the re-projection P2G carry the velocities of the
             // neighbour fill at the top of the step (updateNeighbors is not
             // called in between, main.cpp:120-140).

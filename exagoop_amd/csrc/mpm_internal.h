@@ -220,6 +220,10 @@ void launch_p2g_tiles(const GridP &g, const PhysP &ph, const PState &P, SortStat
 void launch_particle_sorted(const GridP &g, const PhysP &ph, const LsP &ls, const UdfP &udf,
                             const PState &Pin, const PState &Pout, const SortState &S,
                             const NState &N, int flags, const Scalars *sc, hipStream_t s);
+// MUSL's first G2P (velocities only) over the tiles, in place (uniform
+// order, the state sorted this step; outside: as launch_particle_fast).
+void launch_g2p_vel_tiles(const GridP &g, const PhysP &ph, const PState &P, const SortState &S,
+                          const NState &N, int outside, Scalars *sc, hipStream_t s);
 // Specialised particle kernel (uniform order, no level set, no UDF); also
 // accumulates the next step's dt minimum.  outside: particles are
 // redistributed every few steps (a particle outside the domain lies outside

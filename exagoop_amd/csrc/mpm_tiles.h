@@ -843,8 +843,8 @@ __device__ __forceinline__ void tile_factors(const SplineTables &T, int per, int
 // particle outside the tile (outside the domain between redistributions:
 // the sort bins it with the nearest cell): OUT, the same sums node by node
 // from the nodal arrays N (rolled loops, factors per node); else false,
-// with zero sums.
-template <int O, bool VEL, bool OUT = false>
+// with zero sums.  GRAD false: no velocity gradient (L stays zero).
+template <int O, bool VEL, bool OUT = false, bool GRAD = true>
 __device__ __forceinline__ bool g2p_block(const GridP &g, const SplineTables &T,
                                           const double *nodes, const int T0[3],
                                           const double x[3], double vals0[3], double vals1[3],
@@ -889,12 +889,13 @@ __device__ __forceinline__ bool g2p_block(const GridP &g, const SplineTables &T,
                     if (!ok) continue;
                     const int64_t qn = nidx(g, I, J, K);
                     const double gr0 = dxv * p0, gr1 = bx * p1, gr2 = bx * p2;
-                    for (int a = 0; a < 3; ++a) {
-                        const double nv = N->vel[a * nn + qn];
-                        L[a * 3 + 0] += nv * gr0;
-                        L[a * 3 + 1] += nv * gr1;
-                        L[a * 3 + 2] += nv * gr2;
-                    }
+                    if (GRAD)
+                        for (int a = 0; a < 3; ++a) {
+                            const double nv = N->vel[a * nn + qn];
+                            L[a * 3 + 0] += nv * gr0;
+                            L[a * 3 + 1] += nv * gr1;
+                            L[a * 3 + 2] += nv * gr2;
+                        }
                     if (VEL) {
                         const double w = wx * wyz;
                         for (int a = 0; a < 3; ++a) {
@@ -935,12 +936,13 @@ __device__ __forceinline__ bool g2p_block(const GridP &g, const SplineTables &T,
             for (int kx = 0; kx < W; ++kx) {
                 const double *q = py + kx * NCV;
                 const double gr0 = fd[0][kx] * p0, gr1 = fb[0][kx] * p1, gr2 = fb[0][kx] * p2;
-                for (int a = 0; a < 3; ++a) {
-                    const double nv = q[a];
-                    L[a * 3 + 0] += nv * gr0;
-                    L[a * 3 + 1] += nv * gr1;
-                    L[a * 3 + 2] += nv * gr2;
-                }
+                if (GRAD)
+                    for (int a = 0; a < 3; ++a) {
+                        const double nv = q[a];
+                        L[a * 3 + 0] += nv * gr0;
+                        L[a * 3 + 1] += nv * gr1;
+                        L[a * 3 + 2] += nv * gr2;
+                    }
                 if (VEL) {
                     const double w = fw[0][kx] * wyz;
                     for (int a = 0; a < 3; ++a) {
