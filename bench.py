@@ -39,6 +39,10 @@ redistribution interval (num_redist = 10, mpm_specs.H:89): particles are
 redistributed every tenth step, so between redistributions the tile kernels
 run their forms for particles outside their tile.
 
+The musl object runs the same dam break with the MUSL scheme (the second
+P2G re-projects the updated particle velocities before the particle
+kernel).
+
 The thermal object runs the same dam break with the thermal module on
 (uniform temperature, adiabatic walls): milliseconds per step and the
 thermal stages' share of them.
@@ -92,6 +96,7 @@ def main():
     ap.add_argument("--no-disks", action="store_true")
     ap.add_argument("--no-det", action="store_true")
     ap.add_argument("--no-redist10", action="store_true")
+    ap.add_argument("--no-musl", action="store_true")
     ap.add_argument("--thermal-steps", type=int, default=50)
     ap.add_argument("--batch-scenes", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=500)
@@ -159,7 +164,14 @@ def main():
 
     redist10 = None
     if not args.no_redist10:
-        redist10 = redist_leg(s, pos, r, idata, args.warmup, args.steps, ev_ms / args.steps)
+        redist10 = variant_leg(s, pos, r, idata, args.warmup, args.steps, ev_ms / args.steps,
+                               "num_redist 10", num_redist=10)
+
+    musl = None
+    if not args.no_musl and args.scheme == "USL":
+        from exagoop_amd.capi import MUSL
+        musl = variant_leg(s, pos, r, idata, args.warmup, args.steps, ev_ms / args.steps,
+                           "MUSL", stress_update_scheme=MUSL)
 
     cpu = None
     if not args.no_cpu:
@@ -201,6 +213,7 @@ def main():
         "roofline": roofline,
         "deterministic": det,
         "redist10": redist10,
+        "musl": musl,
         "cpu_baseline": cpu,
         "learned": learned,
         "batch": batch,
@@ -239,14 +252,16 @@ def deterministic_leg(s, pos, r, idata, warmup, nsteps, plain_ms):
     return out
 
 
-def redist_leg(s, pos, r, idata, warmup, nsteps, plain_ms):
-    """The same dam break with num_redist = 10, the reference's default:
-    milliseconds per step over the same steps and the per-kernel times."""
+def variant_leg(s, pos, r, idata, warmup, nsteps, plain_ms, label, **changes):
+    """The same dam break with the specs fields in changes: milliseconds per
+    step over the same steps and the per-kernel times (the particle segment
+    holds every stage between the grid pass and the particle kernel's end)."""
     import copy
     from exagoop_amd import capi
-    s10 = copy.deepcopy(s)
-    s10.num_redist = 10
-    g = capi.Context(s10)
+    sv = copy.deepcopy(s)
+    for k, v in changes.items():
+        setattr(sv, k, v)
+    g = capi.Context(sv)
     g.upload(pos, r, idata)
     g.init_internal_forces()
     g.step(warmup)
@@ -257,15 +272,14 @@ def redist_leg(s, pos, r, idata, warmup, nsteps, plain_ms):
     _, kms = g.last_step_timing()
     g.close()
     ms = ev_ms / nsteps
-    out = {"num_redist": 10, "ms_per_step": round(ms, 5),
-           "value": round(pos.shape[1] * 1e3 / ms, 1), "unit": "particle-updates/s",
-           "vs_default_time": round(ms / plain_ms, 3),
-           "kernels_ms": {"p2g": round(kms[0], 5), "grid": round(kms[1], 5),
-                          "g2p_move_stress": round(kms[2], 5)},
-           "sort_ms": round(kms[3], 5)}
-    print(f"[bench] num_redist 10: {ms:.4f} ms/step ({out['vs_default_time']}x the every-step "
-          f"redistribution), p2g {kms[0]:.4f} ms, particle {kms[2]:.4f} ms", file=sys.stderr,
-          flush=True)
+    out = dict(changes)
+    out.update({"ms_per_step": round(ms, 5), "value": round(pos.shape[1] * 1e3 / ms, 1),
+                "unit": "particle-updates/s", "vs_default_time": round(ms / plain_ms, 3),
+                "kernels_ms": {"p2g": round(kms[0], 5), "grid": round(kms[1], 5),
+                               "particle_segment": round(kms[2], 5)},
+                "sort_ms": round(kms[3], 5)})
+    print(f"[bench] {label}: {ms:.4f} ms/step ({out['vs_default_time']}x the default step), "
+          f"p2g {kms[0]:.4f} ms, particle segment {kms[2]:.4f} ms", file=sys.stderr, flush=True)
     return out
 
 
