@@ -10,8 +10,9 @@ import subprocess
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libexagoop_amd.so")
-SOURCES = ["mpm_kernels.hip", "mpm_sort.hip", "mpm_ctx.hip", "mpm_mlp.hip", "mpm_learn.hip",
-           "mpm_batch.hip", "mpm_thermal.hip", "mpm_noalloc.hip", "mpm_io.cpp", "mpm_geometry.cpp"]
+SOURCES = ["mpm_kernels.hip", "mpm_sort.hip", "mpm_partile.hip", "mpm_ctx.hip", "mpm_mlp.hip",
+           "mpm_learn.hip", "mpm_batch.hip", "mpm_thermal.hip", "mpm_noalloc.hip", "mpm_io.cpp",
+           "mpm_geometry.cpp"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

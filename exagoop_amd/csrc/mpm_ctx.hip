@@ -949,7 +949,7 @@ int mpm_step(mpm_ctx *c, int nsteps) {
         const int tail = P_MOVE | (redist ? P_REDIST : 0) | P_VOLUME | P_STRESS;
         int last = P_G2P_VEL | P_G2P_GRAD | tail;
         if (musl) {
-            if (sorted && c->fast && particle_fast_ok(c->g, c->ls, c->udf))
+            if (sorted && c->fast && c->g.uniform)
                 launch_g2p_vel_tiles(c->g, c->ph, c->P, c->S, c->N, nr > 1 ? 1 : 0, c->sc, st);
             else
                 launch_particle(c->g, c->ph, c->ls, c->udf, c->P, c->N, P_G2P_VEL, 0.0, 0.0, c->sc,
@@ -970,11 +970,13 @@ int mpm_step(mpm_ctx *c, int nsteps) {
             if (rc) return rc;
             if (sorted) thermal_swap(c);
         }
-        // The tile kernel; with redistribution every few steps its form for
-        // particles outside their tile (outside the domain) runs.
-        if (sorted && c->fast && particle_fast_ok(c->g, c->ls, c->udf)) {
-            launch_particle_fast(c->g, c->ph, c->P, c->Q, c->S, c->N, musl ? 0 : 1, redist ? 1 : 0,
-                                 nr > 1 ? 1 : 0, c->sc, st);
+        // The tile kernel (uniform order; mixed orders interpolate each
+        // velocity component with its own direction's order, in the generic
+        // kernel); with redistribution every few steps its form for particles
+        // outside their tile (outside the domain) runs.
+        if (sorted && c->fast && c->g.uniform) {
+            launch_particle_fast(c->g, c->ph, c->ls, c->udf, c->P, c->Q, c->S, c->N, musl ? 0 : 1,
+                                 redist ? 1 : 0, nr > 1 ? 1 : 0, c->sc, st);
             std::swap(c->P, c->Q);
             c->dt_next_valid = c->fuse_dt != 0;
         } else if (sorted) {

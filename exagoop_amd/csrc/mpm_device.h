@@ -455,17 +455,19 @@ __device__ inline void stage_g2p(const GridP &g, const PhysP &ph, const NState &
     }
 }
 
-// moveParticles (mpm_particle_timestep.cpp:329-576) for one particle.
-__device__ inline void stage_move(const GridP &g, const PhysP &ph, const LsP &ls,
-                                  const UdfP &udf, Rec &R, double dt) {
-    double *x = R.x;
-    for (int d = 0; d < 3; ++d) x[d] += R.vp[d] * dt;
+// moveParticles (mpm_particle_timestep.cpp:329-576) for one particle:
+// position x and velocity v updated from the grid velocity vp; level-set
+// bodies (phase 0 only), then the walls with their velocities or UDF tables.
+__device__ inline void move_particle(const GridP &g, const PhysP &ph, const LsP &ls,
+                                     const UdfP &udf, double x[3], double v[3],
+                                     const double vp[3], int phase, double dt) {
+    for (int d = 0; d < 3; ++d) x[d] += vp[d] * dt;
     double rin[3], rout[3];
     for (int d = 0; d < 3; ++d) {
-        rin[d] = R.v[d];
+        rin[d] = v[d];
         rout[d] = rin[d];
     }
-    if (ls.nb > 0 && R.phase == 0) {
+    if (ls.nb > 0 && phase == 0) {
         int hit = -1;
         double minphi = kTiny;
         for (int b = 0; b < ls.nb; ++b) {
@@ -485,7 +487,7 @@ __device__ inline void stage_move(const GridP &g, const PhysP &ph, const LsP &ls
             int mod = applybc(rin, rout, ls.mu[hit], nrm, ls.bc[hit]);
             if (mod > 0)
                 for (int d = 0; d < 3; ++d) x[d] += 2.0 * fabs(minphi) * nrm[d];
-            for (int d = 0; d < 3; ++d) R.v[d] = rout[d];
+            for (int d = 0; d < 3; ++d) v[d] = rout[d];
         }
     }
     double wv[3] = {0, 0, 0};
@@ -521,8 +523,14 @@ __device__ inline void stage_move(const GridP &g, const PhysP &ph, const LsP &ls
         int mod = applybc(rin, rout, side ? ph.mu_hi[dir] : ph.mu_lo[dir], nrm,
                           side ? ph.bchi[dir] : ph.bclo[dir]);
         if (mod > 0) x[dir] = 2.0 * (side ? g.phi[dir] : g.plo[dir]) - x[dir];
-        for (int q = 0; q < 3; ++q) R.v[q] = rout[q] + wv[q];
+        for (int q = 0; q < 3; ++q) v[q] = rout[q] + wv[q];
     }
+}
+
+// moveParticles for the record of one particle.
+__device__ inline void stage_move(const GridP &g, const PhysP &ph, const LsP &ls,
+                                  const UdfP &udf, Rec &R, double dt) {
+    move_particle(g, ph, ls, udf, R.x, R.v, R.vp, R.phase, dt);
 }
 
 // Redistribute's effect on one particle: periodic wrap (AMReX

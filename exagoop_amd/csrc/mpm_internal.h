@@ -220,17 +220,21 @@ void launch_p2g_tiles(const GridP &g, const PhysP &ph, const PState &P, SortStat
 void launch_particle_sorted(const GridP &g, const PhysP &ph, const LsP &ls, const UdfP &udf,
                             const PState &Pin, const PState &Pout, const SortState &S,
                             const NState &N, int flags, const Scalars *sc, hipStream_t s);
+// Few tiles: fewer particles than one full tile (4^3 cells x 8) per compute
+// unit; the tile kernels then run 8 wavefronts per tile.
+bool few_tiles(int64_t np);
 // MUSL's first G2P (velocities only) over the tiles, in place (uniform
 // order, the state sorted this step; outside: as launch_particle_fast).
 void launch_g2p_vel_tiles(const GridP &g, const PhysP &ph, const PState &P, const SortState &S,
                           const NState &N, int outside, Scalars *sc, hipStream_t s);
-// Specialised particle kernel (uniform order, no level set, no UDF); also
-// accumulates the next step's dt minimum.  outside: particles are
-// redistributed every few steps (a particle outside the domain lies outside
-// its tile and gathers from the nodal arrays).
-bool particle_fast_ok(const GridP &g, const LsP &ls, const UdfP &udf);
-void launch_particle_fast(const GridP &g, const PhysP &ph, const PState &Pin, const PState &Pout,
-                          const SortState &S, const NState &N, int vel, int redist, int outside,
-                          Scalars *sc, hipStream_t s);
+// The fused particle kernel over the tiles (uniform order, the state sorted
+// this step; level-set bodies and UDF wall tables); also accumulates the next
+// step's dt minimum.  outside: particles are redistributed every few steps
+// (a particle outside the domain lies outside its tile and gathers from the
+// nodal arrays).
+void launch_particle_fast(const GridP &g, const PhysP &ph, const LsP &ls, const UdfP &udf,
+                          const PState &Pin, const PState &Pout, const SortState &S,
+                          const NState &N, int vel, int redist, int outside, Scalars *sc,
+                          hipStream_t s);
 
 }  // namespace xmpm

@@ -138,8 +138,7 @@ int learned_ok(const mpm_ctx *c) {
     if (c->s.stress_update_scheme != MPM_USL)
         return set_err(MPM_E_UNSUPPORTED, "the learned step runs the USL scheme only");
     const int nr = c->s.num_redist > 0 ? c->s.num_redist : 1;
-    if (!c->sorted || !c->fast || nr != 1 || !particle_fast_ok(c->g, c->ls, c->udf) ||
-        c->any_udf)
+    if (!c->sorted || !c->fast || nr != 1 || !c->g.uniform || c->ls.nb > 0 || c->any_udf)
         return set_err(MPM_E_UNSUPPORTED,
                        "the learned step needs the specialised sorted step (redistribution "
                        "every step, uniform order, no level set or wall UDF)");
@@ -204,7 +203,8 @@ int coarse_begin(mpm_ctx *c, int nfine, int advance = 0, const PState *in = null
 // The particle kernel of the coarse step and the time advance.
 int coarse_finish(mpm_ctx *c, int nfine) {
     hipStream_t st = c->stream;
-    launch_particle_fast(c->g, coarse_ph(c, nfine), c->P, c->Q, c->S, c->N, 1, 1, 0, c->sc, st);
+    launch_particle_fast(c->g, coarse_ph(c, nfine), c->ls, c->udf, c->P, c->Q, c->S, c->N, 1, 1,
+                         0, c->sc, st);
     std::swap(c->P, c->Q);
     c->dt_next_valid = c->fuse_dt != 0;
     c->dirty = false;
@@ -697,7 +697,8 @@ int mpm_step_learned(mpm_ctx *c, mpm_mlp *m, int ncoarse, int nfine, const doubl
         HIPCHK(hipEventRecord(c->ml_ev[it & 1], st));
         hipLaunchKernelGGL(k_apply_dev, dim3(512), dim3(256), 0, st, c->ml_nodes, cnt, c->ml_y,
                            m->sc.dv_ref, c->N, c->g.nn, flag);
-        launch_particle_fast(c->g, coarse_ph(c, nfine), in, out, c->S, c->N, 1, 1, 0, c->sc, st);
+        launch_particle_fast(c->g, coarse_ph(c, nfine), c->ls, c->udf, in, out, c->S, c->N, 1,
+                             1, 0, c->sc, st);
         return check_launch();
     };
     // Speculation: step it + 1 (from Q into R) is enqueued as if step it

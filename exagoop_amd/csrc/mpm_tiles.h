@@ -956,18 +956,21 @@ __device__ __forceinline__ bool g2p_block(const GridP &g, const SplineTables &T,
     return true;
 }
 
-// Fused per-particle pipeline for the common case (every direction order O,
-// no level-set body, no wall UDF) over one cell tile: the tile's
-// 7^3 node block of v (and dv) and the spline coefficient tables are staged
-// in LDS, then each thread takes one particle of the tile: G2P (g2p_block),
-// F, D, move + wall BCs, redistribution, volume, stress, written in sorted
-// order.  The next step's dt candidates are min-reduced into cmin.
-template <int O, bool VEL, bool DELTA, int NT = kBlock, bool OUT = false>
+// Fused per-particle pipeline over one cell tile: the tile's 7^3 node block
+// of v (and dv) and the spline coefficient tables are staged in LDS, then
+// each thread takes one particle of the tile: G2P (g2p_block), F, D, move +
+// wall BCs, redistribution, volume, stress, written in sorted order.  The
+// next step's dt candidates are min-reduced into cmin.  O: the order of
+// every direction.  GEN: level-set bodies and UDF wall tables in the move
+// (ls, udf); otherwise walls with constant velocities only.
+template <int O, bool VEL, bool DELTA, int NT = kBlock, bool OUT = false, bool GEN = false>
 __device__ __forceinline__ void particle_tile(const GridP &g, const PhysP &ph, const PState &Pin,
                                               const PState &Pout, const SortState &S,
                                               const NState &N, int redist, double dt,
                                               double time, int *err, int tile,
-                                              PartLds<VEL ? 6 : 3> &L, double &cmin) {
+                                              PartLds<VEL ? 6 : 3> &L, double &cmin,
+                                              const LsP *ls = nullptr,
+                                              const UdfP *udf = nullptr) {
     constexpr int NCV = VEL ? 6 : 3;
     auto &nodes = L.nodes;
     auto &tab = L.tab;
@@ -1111,10 +1114,13 @@ __device__ __forceinline__ void particle_tile(const GridP &g, const PhysP &ph, c
             for (int d = 0; d < 3; ++d) xn[d] = stream_load(&Pin.pos[d * ci + jn]);
         }
         double *R = Pout.r;
-        // moveParticles without level sets or UDF tables
-        // (mpm_particle_timestep.cpp:329-576), then Redistribute.  Each
+        // moveParticles (mpm_particle_timestep.cpp:329-576; GEN: with
+        // level-set bodies and UDF wall tables), then Redistribute.  Each
         // output is stored as soon as it is final to keep live ranges short.
-        {
+        if (GEN) {
+            move_particle(g, ph, *ls, *udf, x, v, vp, phase, dt);
+            if (redist) stage_redistribute(g, x, id);
+        } else {
             for (int d = 0; d < 3; ++d) x[d] += vp[d] * dt;
             double rin[3], rout[3], wv[3] = {0, 0, 0};
             for (int d = 0; d < 3; ++d) rin[d] = rout[d] = v[d];

@@ -186,18 +186,31 @@ def test_individual_entry_points(gpu):
     compare_particles(g, o, 1e-12)
 
 
-def test_levelset_body(gpu):
-    say("[parity] level-set sphere body")
-    s, pos, r, idata = scenes.random_block(9, order=2, frac=0.7, speed=1.0)
+def sphere_phi(s, centre_frac, radius):
+    """Signed distance to a sphere on the nodes (i fastest)."""
     n = s.ncells
     dx = s.phi[0] / n[0]
     I, J, K = np.meshgrid(np.arange(n[0] + 1), np.arange(n[1] + 1), np.arange(n[2] + 1),
                           indexing="ij")
     X, Y, Z = I * dx, J * dx, K * dx
-    c = np.array([0.5 * s.phi[0], 0.5 * s.phi[1], 0.3 * s.phi[2]])
-    phi = np.sqrt((X - c[0]) ** 2 + (Y - c[1]) ** 2 + (Z - c[2]) ** 2) - 0.12
-    phi = phi.transpose(2, 1, 0).ravel()  # i fastest
+    c = np.array([centre_frac[d] * s.phi[d] for d in range(3)])
+    phi = np.sqrt((X - c[0]) ** 2 + (Y - c[1]) ** 2 + (Z - c[2]) ** 2) - radius
+    return phi.transpose(2, 1, 0).ravel()
+
+
+# The sorted step runs the level-set bodies and the wall UDF tables in the
+# tile particle kernel (MPM_OPT_FAST = 1, the default) or in the generic
+# sorted kernel (0): both against the oracle.
+FAST = [1, 0]
+
+
+@pytest.mark.parametrize("fast", FAST)
+def test_levelset_body(gpu, fast):
+    say(f"[parity] level-set sphere body, fast={fast}")
+    s, pos, r, idata = scenes.random_block(9, order=2, frac=0.7, speed=1.0)
+    phi = sphere_phi(s, (0.5, 0.5, 0.3), 0.12)
     g, o = pair(s, pos, r, idata)
+    g.set_option(capi.OPT_FAST, fast)
     for ctx in (g, o):
         ctx.add_levelset_body(phi, BC_PARTIALSLIP, 0.3, (0.0, 0.1, 0.0))
         ctx.init_internal_forces()
@@ -206,8 +219,29 @@ def test_levelset_body(gpu):
     compare_particles(g, o, 1e-11)
 
 
-def test_wall_udf(gpu):
-    say("[parity] wall UDF")
+@pytest.mark.parametrize("fast", FAST)
+def test_levelset_body_musl_order3_20_steps(gpu, fast):
+    """A block falling onto a slip-wall sphere (MUSL, cubic, as the
+    reference's dam break with obstacles runs): particles reach the body."""
+    say(f"[parity] level-set body, MUSL order 3, 20 steps, fast={fast}")
+    s, pos, r, idata = scenes.random_block(31, order=3, frac=0.6, speed=0.5, scheme=MUSL)
+    r[capi.R_VEL + 1] -= 3.0
+    phi = sphere_phi(s, (0.5, 0.25, 0.5), 0.1)
+    g, o = pair(s, pos, r, idata)
+    g.set_option(capi.OPT_FAST, fast)
+    for ctx in (g, o):
+        ctx.add_levelset_body(phi, BC_SLIP, 0.0)
+        ctx.init_internal_forces()
+    for _ in range(4):
+        for ctx in (g, o):
+            ctx.step(5)
+        compare_particles(g, o, 1e-10)
+    compare_nodal(g, o, 1e-10)
+
+
+@pytest.mark.parametrize("fast", FAST)
+def test_wall_udf(gpu, fast):
+    say(f"[parity] wall UDF, fast={fast}")
     s, pos, r, idata = scenes.random_block(4, order=3, frac=0.95, speed=1.5)
     s.bclo[2] = BC_NOSLIP
     s.bchi[0] = BC_SLIP
@@ -219,6 +253,7 @@ def test_wall_udf(gpu):
         return (0.1 * z, 0.0, 0.0)
 
     g, o = pair(s, pos, r, idata)
+    g.set_option(capi.OPT_FAST, fast)
     for ctx in (g, o):
         ctx.set_wall_udf(2, 0, twist)
         ctx.set_wall_udf(0, 1, shear)

@@ -174,10 +174,16 @@ def test_dam_break_64x32x32_200_steps(gpu):
 
 
 # ------------------------------------------------- boundary data equivalents
-def test_levelset_plane_on_wall_nodes(gpu):
+# The tile particle kernel (MPM_OPT_FAST = 1, the default) and the generic
+# sorted kernel (0) for level-set bodies and wall functions.
+FAST = [1, 0]
+
+
+@pytest.mark.parametrize("fast", FAST)
+def test_levelset_plane_on_wall_nodes(gpu, fast):
     """The level-set plane whose inside is the lo-x nodes (the oracle gives
     the wall's bits with it), on the GPU against the oracle."""
-    say("[parity] level-set plane on the wall nodes")
+    say(f"[parity] level-set plane on the wall nodes, fast={fast}")
     s, pos, r, idata = scenes.random_block(13, order=3, frac=0.8, speed=1.0,
                                            bcs=[(0, BC_SLIP), (BC_SLIP, BC_SLIP),
                                                 (BC_NOSLIP, BC_SLIP)])
@@ -187,6 +193,7 @@ def test_levelset_plane_on_wall_nodes(gpu):
                           indexing="ij")
     phi = (I * dx - 0.5 * dx).transpose(2, 1, 0).ravel()
     g, o = pair(s, pos, r, idata)
+    g.set_option(capi.OPT_FAST, fast)
     for c in (g, o):
         c.add_levelset_body(phi, BC_PARTIALSLIP, 0.3)
         c.init_internal_forces()
@@ -197,12 +204,14 @@ def test_levelset_plane_on_wall_nodes(gpu):
     compare_particles(g, o, 1e-10)
 
 
-def test_constant_wall_function(gpu):
-    say("[parity] constant wall function")
+@pytest.mark.parametrize("fast", FAST)
+def test_constant_wall_function(gpu, fast):
+    say(f"[parity] constant wall function, fast={fast}")
     s, pos, r, idata = scenes.random_block(14, order=2, frac=0.9, speed=1.0,
                                            bcs=[(BC_NOSLIP, BC_SLIP), (BC_SLIP, BC_SLIP),
                                                 (BC_SLIP, BC_SLIP)])
     g, o = pair(s, pos, r, idata)
+    g.set_option(capi.OPT_FAST, fast)
     for c in (g, o):
         c.set_wall_udf(0, 0, lambda x, y, z, t: (0.3, -0.2, 0.15))
         c.init_internal_forces()
