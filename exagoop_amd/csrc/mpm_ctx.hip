@@ -956,9 +956,15 @@ int mpm_step(mpm_ctx *c, int nsteps) {
                                 st);
             // Images of the re-projection P2G carry the velocities of the
             // neighbour fill at the top of the step (updateNeighbors is not
-            // called in between, main.cpp:120-140).
-            if (c->P.vsnap || !sorted) launch_p2g(c->g, c->ph, c->P, c->N, 0, 1, 0, c->P.vsnap, st);
-            else launch_p2g_tiles(c->g, c->ph, c->P, c->S, c->N, step, 1, nr > 1, c->sc, st);
+            // called in between, main.cpp:120-140): the tile deposit's image
+            // form when every particle lies in its cell (redistribution every
+            // step), the atomic one-thread-per-particle deposit otherwise and
+            // in the deterministic step.
+            const bool img_tiles = c->P.vsnap && nr == 1 && !c->S.blk;
+            if (!sorted || (c->P.vsnap && !img_tiles))
+                launch_p2g(c->g, c->ph, c->P, c->N, 0, 1, 0, c->P.vsnap, st);
+            else
+                launch_p2g_tiles(c->g, c->ph, c->P, c->S, c->N, step, 1, nr > 1, c->sc, st);
             launch_grid(c->g, c->ph, c->ls, c->udf, c->N, G_FINISH_VEL | G_BCS, 0.0, c->sc, st, te,
                         step);
             last = P_G2P_GRAD | tail;

@@ -137,16 +137,18 @@ __global__ __launch_bounds__(kBlock) void k_cell_order(SortState S) {
 // deposit_onto_grid_momentum over the non-empty tiles, one workgroup each
 // (see p2g_tile).  DEFER: particles outside their cell are listed
 // (SortState::ocount, rank) for k_deposit_outside; the launch clears the
-// counter the next listing launch uses.
-template <int O, bool FORCES, bool FALLBACK, bool GATHER, bool DET, int NW, bool DEFER>
+// counter the next listing launch uses.  IMG: MUSL's re-projection with
+// periodic images carrying the top-of-step velocity (P.vsnap).
+template <int O, bool FORCES, bool FALLBACK, bool GATHER, bool DET, int NW, bool DEFER,
+          bool IMG = false>
 __global__ __launch_bounds__(NW * 64) void k_p2g_cells(GridP g, PhysP ph, PState P, SortState S,
                                                        double *acc, int epoch, Scalars *sc) {
-    __shared__ P2gLdsOf<FORCES, GATHER, DET, NW> lds;
+    __shared__ P2gLdsOf<FORCES, GATHER, DET, NW, IMG> lds;
     if (DEFER && blockIdx.x == 0 && threadIdx.x == 0) S.ocount[(S.ogen + 1) & 1] = 0;
     const int pos = active_pos(S, blockIdx.x);
     if (pos < 0) return;  // uniform over the block
-    p2g_tile<O, FORCES, FALLBACK, GATHER, DET, NW, DEFER>(g, ph, P, S, acc, epoch, &sc->err,
-                                                          S.tiles[1 + pos], lds, pos, S.pass);
+    p2g_tile<O, FORCES, FALLBACK, GATHER, DET, NW, DEFER, IMG>(
+        g, ph, P, S, acc, epoch, &sc->err, S.tiles[1 + pos], lds, pos, S.pass);
 }
 
 // The deposits of the particles a DEFER tile launch listed, one thread per
@@ -271,11 +273,29 @@ void launch_p2g_tiles(const GridP &g, const PhysP &ph, const PState &P, SortStat
         else if (g.od[0] == 2) XMPM_PG(2, f, fb);                                             \
         else XMPM_PG(3, f, fb);                                                               \
     } while (0)
-    if (momentum_only) {
+    // MUSL's re-projection with periodic images (P.vsnap): every particle in
+    // its cell (redistributed this step), the atomic form.
+    const bool img = momentum_only && P.vsnap && !fallback && !det;
+#define XMPM_PGI(o)                                                                           \
+    do {                                                                                      \
+        if (few)                                                                              \
+            hipLaunchKernelGGL((k_p2g_cells<o, false, false, false, false, 8, false, true>),  \
+                               gr, bl8, 0, s, g, ph, P, S, N.acc, epoch, sc);                 \
+        else                                                                                  \
+            hipLaunchKernelGGL((k_p2g_cells<o, false, false, false, false, 4, false, true>),  \
+                               gr, bl, 0, s, g, ph, P, S, N.acc, epoch, sc);                  \
+    } while (0)
+    if (img) {
+        if (!g.uniform) XMPM_PGI(0);
+        else if (g.od[0] == 1) XMPM_PGI(1);
+        else if (g.od[0] == 2) XMPM_PGI(2);
+        else XMPM_PGI(3);
+    } else if (momentum_only) {
         if (fallback) XMPM_PGO(false, true); else XMPM_PGO(false, false);
     } else {
         if (fallback) XMPM_PGO(true, true); else XMPM_PGO(true, false);
     }
+#undef XMPM_PGI
 #undef XMPM_PGO
 #undef XMPM_PG
 #undef XMPM_PGW

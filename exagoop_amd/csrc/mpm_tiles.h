@@ -395,9 +395,10 @@ struct P2gLds {
     int perm[NPERM];
     SplineTables tab;
 };
-// The LDS of p2g_tile<O, FORCES, FALLBACK, GATHER, DET>.
-template <bool FORCES, bool GATHER, bool DET, int NW = 4>
-using P2gLdsOf = P2gLds<FORCES ? 7 : 3, DET, (GATHER || !FORCES) ? kPermLds : 1, NW>;
+// The LDS of p2g_tile<O, FORCES, FALLBACK, GATHER, DET, NW, DEFER, IMG>.
+template <bool FORCES, bool GATHER, bool DET, int NW = 4, bool IMG = false>
+using P2gLdsOf =
+    P2gLds<FORCES ? 7 : (IMG ? 6 : 3), DET, (GATHER || !FORCES) ? kPermLds : 1, NW>;
 
 // LDS of one particle tile (NCV node components).
 template <int NCV>
@@ -411,15 +412,21 @@ struct PartLds {
 // pass in S.tile_slot; no atomics (but those of the FALLBACK global path).
 // NW wavefronts per tile: 4 (one z-slice each) or 8 (a z-slice's first or
 // second pair of cell rows each, half the chain of rounds per wavefront).
+// IMG (momentum only, MUSL's re-projection with periodic faces): a deposit
+// that lands on a periodic image node carries the velocity of the particle
+// at the top of the step (P.vsnap), as the reference's neighbour particles
+// do; the tile accumulates both momentum sums and the flush picks one per
+// target.
 template <int O, bool FORCES, bool FALLBACK, bool GATHER = false, bool DET = false, int NW = 4,
-          bool DEFER = false>
+          bool DEFER = false, bool IMG = false>
 __device__ __forceinline__ void p2g_tile(const GridP &g, const PhysP &ph, const PState &P,
                                          const SortState &S, double *acc, int epoch, int *err,
-                                         int tile, P2gLdsOf<FORCES, GATHER, DET, NW> &L,
+                                         int tile, P2gLdsOf<FORCES, GATHER, DET, NW, IMG> &L,
                                          int pos = 0, int pass = 0) {
     static_assert(NW == 4 || (NW == 8 && !DET), "8 wavefronts per tile: atomic form only");
+    static_assert(!IMG || (!FORCES && !FALLBACK && !DET), "image momentum: re-projection only");
     constexpr int NT = NW * 64;
-    constexpr int NC = FORCES ? 7 : 3;
+    constexpr int NC = FORCES ? 7 : (IMG ? 6 : 3);
     constexpr int kSlabN = DET ? 4 * NC * kZN : NC * kTNN;
     auto &slab = L.slab;
     auto &pk = L.pk;
@@ -468,7 +475,8 @@ __device__ __forceinline__ void p2g_tile(const GridP &g, const PhysP &ph, const 
     const int s_slot = lane & 15, s_part = lane >> 4;
     const int s_cell = s_slot >> 2, s_t = s_slot & 3;
     const int v_first = (s_part == 0) ? 3 : 4 + 3 * s_part;
-    const int v_n = FORCES ? ((s_part == 0) ? 4 : 3) : ((s_part == 0) ? 4 : 0);
+    const int v_n =
+        FORCES ? ((s_part == 0) ? 4 : 3) : ((s_part == 0) ? 4 : ((IMG && s_part == 1) ? 3 : 0));
     // largest cell count of each row over the wavefront's four cells
     int mc[kTileCells];
 #pragma unroll
@@ -507,6 +515,12 @@ __device__ __forceinline__ void p2g_tile(const GridP &g, const PhysP &ph, const 
                         R.val[0] = P.r[MPM_R_MASS * cap + j];
                         for (int d = 0; d < 3; ++d)
                             R.val[1 + d] = R.val[0] * P.r[(MPM_R_VEL + d) * cap + j];
+                    }
+                    if (IMG && s_part == 1) {
+                        // m times the velocity at the top of the step: the
+                        // packet's slots kPkA.. (no forces in this form)
+                        const double m = P.r[MPM_R_MASS * cap + j];
+                        for (int d = 0; d < 3; ++d) R.val[d] = m * P.vsnap[d * cap + j];
                     }
                 }
             }
@@ -690,6 +704,13 @@ __device__ __forceinline__ void p2g_tile(const GridP &g, const PhysP &ph, const 
 #pragma unroll
                 for (int k = 0; k < 4; ++k)
                     for (int d = 0; d < 3; ++d) a4[k][d] += wx[k] * Pr[d];
+                if (IMG) {
+                    double Pi[3];
+                    for (int d = 0; d < 3; ++d) Pi[d] = pr[kPkA + d] * wyz;
+#pragma unroll
+                    for (int k = 0; k < 4; ++k)
+                        for (int d = 0; d < 3; ++d) a4[k][3 + d] += wx[k] * Pi[d];
+                }
             }
         }
         asm volatile("" ::: "memory");  // reads of this round before the next writes
@@ -766,7 +787,17 @@ __device__ __forceinline__ void p2g_tile(const GridP &g, const PhysP &ph, const 
                 for (int qx = 0; qx < 2; ++qx) {
                     if (qz >= cz2 || qy >= cy || qx >= cx) continue;
                     const int I = qx ? tx1 : tx0, J = qy ? ty1 : ty0, K = qz ? tz1 : tz0;
-                    if (!DET) {
+                    if (IMG) {
+                        // a periodic image target: the second target, or the
+                        // first when the block node lies outside the domain
+                        const bool image = qx || qy || qz || I != T0[0] + ax - 1 ||
+                                           J != T0[1] + ay - 1 || K != T0[2] + az - 1;
+                        const int64_t node = nidx(g, I, J, K);
+                        for (int d = 0; d < 3; ++d) {
+                            const double val = v[image ? 3 + d : d];
+                            if (val != 0.0) unsafeAtomicAdd(&acc[(1 + d) * nn + node], val);
+                        }
+                    } else if (!DET) {
                         const int64_t node = nidx(g, I, J, K);
                         for (int c = 0; c < NC; ++c)
                             if (v[c] != 0.0)
