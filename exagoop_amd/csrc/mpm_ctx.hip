@@ -957,10 +957,9 @@ int mpm_step(mpm_ctx *c, int nsteps) {
             // Images of the re-projection P2G carry the velocities of the
             // neighbour fill at the top of the step (updateNeighbors is not
             // called in between, main.cpp:120-140): the tile deposit's image
-            // form when every particle lies in its cell (redistribution every
-            // step), the atomic one-thread-per-particle deposit otherwise and
-            // in the deterministic step.
-            const bool img_tiles = c->P.vsnap && nr == 1 && !c->S.blk;
+            // form, or the atomic one-thread-per-particle deposit in the
+            // deterministic step.
+            const bool img_tiles = c->P.vsnap && !c->S.blk;
             if (!sorted || (c->P.vsnap && !img_tiles))
                 launch_p2g(c->g, c->ph, c->P, c->N, 0, 1, 0, c->P.vsnap, st);
             else

@@ -152,17 +152,26 @@ __global__ __launch_bounds__(NW * 64) void k_p2g_cells(GridP g, PhysP ph, PState
 }
 
 // The deposits of the particles a DEFER tile launch listed, one thread per
-// entry (deposit_global, as the inline fallback makes them).
-template <bool FORCES, bool GATHER>
+// entry (deposit_global, as the inline fallback makes them).  IMG: periodic
+// image targets receive the momentum at the top of the step (P.vsnap).
+template <bool FORCES, bool GATHER, bool IMG = false>
 __global__ __launch_bounds__(kBlock) void k_deposit_outside(GridP g, PhysP ph, PState P,
                                                             SortState S, double *acc, int epoch) {
     // the count as the tile launch's atomics left it (a vector load)
     const int n = __hip_atomic_load(&S.ocount[S.ogen & 1], __ATOMIC_RELAXED,
                                     __HIP_MEMORY_SCOPE_AGENT);
     for (int k = blockIdx.x * kBlock + threadIdx.x; k < n; k += gridDim.x * kBlock) {
+        const int64_t e = S.rank[k];
         double x[3], vals[13];
-        fallback_vals<FORCES, GATHER>(ph, P, S, S.rank[k], x, vals);
-        deposit_global(g, x, vals, FORCES, acc, S.tile_epoch, epoch);
+        fallback_vals<FORCES, GATHER>(ph, P, S, e, x, vals);
+        if (IMG) {
+            const double m = P.r[MPM_R_MASS * P.cap + e];
+            double mvi[3];
+            for (int d = 0; d < 3; ++d) mvi[d] = m * P.vsnap[d * P.cap + e];
+            deposit_global(g, x, vals, FORCES, acc, S.tile_epoch, epoch, mvi);
+        } else {
+            deposit_global(g, x, vals, FORCES, acc, S.tile_epoch, epoch);
+        }
     }
 }
 
@@ -273,17 +282,21 @@ void launch_p2g_tiles(const GridP &g, const PhysP &ph, const PState &P, SortStat
         else if (g.od[0] == 2) XMPM_PG(2, f, fb);                                             \
         else XMPM_PG(3, f, fb);                                                               \
     } while (0)
-    // MUSL's re-projection with periodic images (P.vsnap): every particle in
-    // its cell (redistributed this step), the atomic form.
-    const bool img = momentum_only && P.vsnap && !fallback && !det;
-#define XMPM_PGI(o)                                                                           \
+    // MUSL's re-projection with periodic images (P.vsnap), the atomic form
+    // (particles outside their cell deferred, with their image momentum).
+    const bool img = momentum_only && P.vsnap && !det;
+#define XMPM_PGIF(o, fb)                                                                      \
     do {                                                                                      \
         if (few)                                                                              \
-            hipLaunchKernelGGL((k_p2g_cells<o, false, false, false, false, 8, false, true>),  \
-                               gr, bl8, 0, s, g, ph, P, S, N.acc, epoch, sc);                 \
+            hipLaunchKernelGGL((k_p2g_cells<o, false, fb, false, false, 8, fb, true>), gr,    \
+                               bl8, 0, s, g, ph, P, S, N.acc, epoch, sc);                     \
         else                                                                                  \
-            hipLaunchKernelGGL((k_p2g_cells<o, false, false, false, false, 4, false, true>),  \
-                               gr, bl, 0, s, g, ph, P, S, N.acc, epoch, sc);                  \
+            hipLaunchKernelGGL((k_p2g_cells<o, false, fb, false, false, 4, fb, true>), gr,    \
+                               bl, 0, s, g, ph, P, S, N.acc, epoch, sc);                      \
+    } while (0)
+#define XMPM_PGI(o)                                                                           \
+    do {                                                                                      \
+        if (fallback) XMPM_PGIF(o, true); else XMPM_PGIF(o, false);                           \
     } while (0)
     if (img) {
         if (!g.uniform) XMPM_PGI(0);
@@ -296,12 +309,16 @@ void launch_p2g_tiles(const GridP &g, const PhysP &ph, const PState &P, SortStat
         if (fallback) XMPM_PGO(true, true); else XMPM_PGO(true, false);
     }
 #undef XMPM_PGI
+#undef XMPM_PGIF
 #undef XMPM_PGO
 #undef XMPM_PG
 #undef XMPM_PGW
     if (fallback && !det) {
         const dim3 go(64);
-        if (momentum_only)
+        if (img)
+            hipLaunchKernelGGL((k_deposit_outside<false, false, true>), go, bl, 0, s, g, ph, P,
+                               S, N.acc, epoch);
+        else if (momentum_only)
             hipLaunchKernelGGL((k_deposit_outside<false, false>), go, bl, 0, s, g, ph, P, S,
                                N.acc, epoch);
         else if (!S.pkt_valid)

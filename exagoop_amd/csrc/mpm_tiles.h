@@ -179,10 +179,12 @@ __device__ __forceinline__ void scatter_one(const PState &P, const SortState &S,
 // One particle's deposit straight to the global accumulators (particles
 // outside their tile's node block), from its P2G packet values: mass m,
 // momentum mv, body force density a and B = -V sigma (xx, xy, xz, yy, yz,
-// zz).  Per node: m N, mv N and a N + B grad N.
+// zz).  Per node: m N, mv N and a N + B grad N.  mv_img (MUSL's re-projection
+// with periodic images): the momentum that periodic image targets receive.
 __device__ __attribute__((noinline)) void deposit_global(const GridP &g, const double x[3],
                                                          const double *vals, bool forces,
-                                                         double *acc, int *tile_epoch, int epoch) {
+                                                         double *acc, int *tile_epoch, int epoch,
+                                                         const double *mv_img = nullptr) {
     const double m = vals[0];
     const double *mv = vals + 1, *a = vals + 4, *B = vals + 7;
     const int sym[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};
@@ -229,8 +231,13 @@ __device__ __attribute__((noinline)) void deposit_global(const GridP &g, const d
                     for (int qy = 0; qy < cy; ++qy)
                         for (int qx = 0; qx < cx; ++qx) {
                             const int64_t node = nidx(g, tx[qx], ty[qy], tz[qz]);
-                            for (int c = forces ? 0 : 1; c < (forces ? 7 : 4); ++c)
-                                unsafeAtomicAdd(&acc[c * nn + node], val[c]);
+                            const bool image = ix[qx] || iy[qy] || iz[qz];
+                            for (int c = forces ? 0 : 1; c < (forces ? 7 : 4); ++c) {
+                                const double a_ = (mv_img && image && c >= 1 && c <= 3)
+                                                      ? mv_img[c - 1] * N
+                                                      : val[c];
+                                unsafeAtomicAdd(&acc[c * nn + node], a_);
+                            }
                             tile_epoch[node_tile(g, tx[qx], ty[qy], tz[qz])] = epoch;
                         }
             }
@@ -424,7 +431,8 @@ __device__ __forceinline__ void p2g_tile(const GridP &g, const PhysP &ph, const 
                                          int tile, P2gLdsOf<FORCES, GATHER, DET, NW, IMG> &L,
                                          int pos = 0, int pass = 0) {
     static_assert(NW == 4 || (NW == 8 && !DET), "8 wavefronts per tile: atomic form only");
-    static_assert(!IMG || (!FORCES && !FALLBACK && !DET), "image momentum: re-projection only");
+    static_assert(!IMG || (!FORCES && !DET && DEFER == FALLBACK),
+                  "image momentum: the re-projection, its fallback deferred");
     constexpr int NT = NW * 64;
     constexpr int NC = FORCES ? 7 : (IMG ? 6 : 3);
     constexpr int kSlabN = DET ? 4 * NC * kZN : NC * kTNN;
