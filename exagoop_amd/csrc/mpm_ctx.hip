@@ -957,13 +957,9 @@ int mpm_step(mpm_ctx *c, int nsteps) {
             // Images of the re-projection P2G carry the velocities of the
             // neighbour fill at the top of the step (updateNeighbors is not
             // called in between, main.cpp:120-140): the tile deposit's image
-            // form, or the atomic one-thread-per-particle deposit in the
-            // deterministic step.
-            const bool img_tiles = c->P.vsnap && !c->S.blk;
-            if (!sorted || (c->P.vsnap && !img_tiles))
-                launch_p2g(c->g, c->ph, c->P, c->N, 0, 1, 0, c->P.vsnap, st);
-            else
-                launch_p2g_tiles(c->g, c->ph, c->P, c->S, c->N, step, 1, nr > 1, c->sc, st);
+            // form (P.vsnap set).
+            if (!sorted) launch_p2g(c->g, c->ph, c->P, c->N, 0, 1, 0, c->P.vsnap, st);
+            else launch_p2g_tiles(c->g, c->ph, c->P, c->S, c->N, step, 1, nr > 1, c->sc, st);
             launch_grid(c->g, c->ph, c->ls, c->udf, c->N, G_FINISH_VEL | G_BCS, 0.0, c->sc, st, te,
                         step);
             last = P_G2P_GRAD | tail;

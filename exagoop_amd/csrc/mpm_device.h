@@ -97,10 +97,12 @@ __device__ __forceinline__ int floor4(int x) { return x >= 0 ? x / 4 : -((3 - x)
 
 // Block sources of node index I along one direction for the deterministic
 // P2G: the pairs (cell tile t, block offset a) whose block node, unwrapped
-// index G = kTileCells t - 1 + a, the tile flush sends to I (targets2): G = I
-// and, in a periodic direction, the images G = I - n and G = I + n; in that
-// order, then by increasing t.
-__device__ __forceinline__ int block_sources(int I, int n, int per, int ntc, int t[6], int a[6]) {
+// index G = kTileCells t - 1 + a, the tile flush sends to I (targets2): in a
+// periodic direction the image G = I - n, then G = I, then the image
+// G = I + n; each by increasing t.  img (optional): whether the source is an
+// image.
+__device__ __forceinline__ int block_sources(int I, int n, int per, int ntc, int t[6], int a[6],
+                                             bool *img = nullptr) {
     static_assert(kTileCells == 4, "floor4");
     int c = 0;
     for (int s = 0; s < 3; ++s) {
@@ -112,6 +114,7 @@ __device__ __forceinline__ int block_sources(int I, int n, int per, int ntc, int
         for (int tt = lo; tt <= hi && c < 6; ++tt) {
             t[c] = tt;
             a[c] = G - 4 * tt + 1;
+            if (img) img[c] = s != 1;
             ++c;
         }
     }
@@ -122,15 +125,19 @@ __device__ __forceinline__ int block_sources(int I, int n, int per, int ntc, int
 // bg.pass, in a fixed order (z, then y, then x sources): out[0..NC), NC =
 // bg.nc.  Returns whether any block contributed.  SAME_LAUNCH: the stamps
 // and blocks were written earlier in the same launch by other workgroups
-// (read the stamps with agent-scope atomic loads).
-template <int NC, bool SAME_LAUNCH = false>
+// (read the stamps with agent-scope atomic loads).  IMG: blocks of 2 NC
+// components, the second NC summed from image sources (MUSL's re-projection
+// with periodic images).
+template <int NC, bool SAME_LAUNCH = false, bool IMG = false>
 __device__ __forceinline__ bool blk_gather(const GridP &g, const BlkGather &bg, int I, int J,
                                            int K, double out[NC]) {
     constexpr int kBN = kTileCells + 3;
+    constexpr int NB = IMG ? 2 * NC : NC;
     int tx[6], ax[6], ty[6], ay[6], tz[6], az[6];
-    const int cx = block_sources(I, g.n[0], g.per[0], g.ntc[0], tx, ax);
-    const int cy = block_sources(J, g.n[1], g.per[1], g.ntc[1], ty, ay);
-    const int cz = block_sources(K, g.n[2], g.per[2], g.ntc[2], tz, az);
+    bool mx[6], my[6], mz[6];
+    const int cx = block_sources(I, g.n[0], g.per[0], g.ntc[0], tx, ax, IMG ? mx : nullptr);
+    const int cy = block_sources(J, g.n[1], g.per[1], g.ntc[1], ty, ay, IMG ? my : nullptr);
+    const int cz = block_sources(K, g.n[2], g.per[2], g.ntc[2], tz, az, IMG ? mz : nullptr);
 #pragma unroll
     for (int c = 0; c < NC; ++c) out[c] = 0.0;
     bool any = false;
@@ -144,10 +151,11 @@ __device__ __forceinline__ bool blk_gather(const GridP &g, const BlkGather &bg, 
                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
                                 : bg.slot[tile];
                 if ((int)(e >> 32) != bg.pass) continue;
-                const double *b = bg.blk + (int64_t)(uint32_t)e * NC * kTNNodes +
+                const double *b = bg.blk + (int64_t)(uint32_t)e * NB * kTNNodes +
                                   (az[iz] * kBN + ay[iy]) * kBN + ax[ix];
+                const int c0 = (IMG && (mx[ix] || my[iy] || mz[iz])) ? NC : 0;
 #pragma unroll
-                for (int c = 0; c < NC; ++c) out[c] += b[c * kTNNodes];
+                for (int c = 0; c < NC; ++c) out[c] += b[(c0 + c) * kTNNodes];
                 any = true;
             }
     return any;

@@ -179,7 +179,7 @@ __global__ __launch_bounds__(kBlock) void k_deposit_outside(GridP g, PhysP ph, P
 // (blk_gather, a fixed order) to the accumulators, which hold only the
 // FALLBACK path's global deposits.  Node tiles the tile flush did not mark
 // in this epoch received only zeros.
-template <int NC>
+template <int NC, bool IMG = false>
 __global__ __launch_bounds__(kBlock) void k_p2g_gather(GridP g, BlkGather bg,
                                                        const int *tile_epoch, int epoch,
                                                        double *acc) {
@@ -190,7 +190,7 @@ __global__ __launch_bounds__(kBlock) void k_p2g_gather(GridP g, BlkGather bg,
     const int K = (int)(node / ((int64_t)(g.n[0] + 1) * (g.n[1] + 1)));
     if (tile_epoch[node_tile(g, I, J, K)] != epoch) return;
     double s[NC];
-    if (!blk_gather<NC>(g, bg, I, J, K, s)) return;
+    if (!blk_gather<NC, false, IMG>(g, bg, I, J, K, s)) return;
     const int64_t nn = g.nn;
 #pragma unroll
     for (int c = 0; c < NC; ++c) acc[(NC == 7 ? c : c + 1) * nn + node] += s[c];
@@ -282,12 +282,16 @@ void launch_p2g_tiles(const GridP &g, const PhysP &ph, const PState &P, SortStat
         else if (g.od[0] == 2) XMPM_PG(2, f, fb);                                             \
         else XMPM_PG(3, f, fb);                                                               \
     } while (0)
-    // MUSL's re-projection with periodic images (P.vsnap), the atomic form
-    // (particles outside their cell deferred, with their image momentum).
-    const bool img = momentum_only && P.vsnap && !det;
+    // MUSL's re-projection with periodic images (P.vsnap): the atomic form
+    // (particles outside their cell deferred, with their image momentum) or
+    // the deterministic one (both momentum sums in the blocks).
+    const bool img = momentum_only && P.vsnap;
 #define XMPM_PGIF(o, fb)                                                                      \
     do {                                                                                      \
-        if (few)                                                                              \
+        if (det)                                                                              \
+            hipLaunchKernelGGL((k_p2g_cells<o, false, fb, false, true, 4, false, true>), gr,  \
+                               bl, 0, s, g, ph, P, S, N.acc, epoch, sc);                      \
+        else if (few)                                                                         \
             hipLaunchKernelGGL((k_p2g_cells<o, false, fb, false, false, 8, fb, true>), gr,    \
                                bl8, 0, s, g, ph, P, S, N.acc, epoch, sc);                     \
         else                                                                                  \
@@ -334,8 +338,11 @@ void launch_p2g_tiles(const GridP &g, const PhysP &ph, const PState &P, SortStat
         bg.blk = S.blk;
         bg.slot = S.tile_slot;
         bg.pass = S.pass;
-        bg.nc = momentum_only ? 3 : 7;
-        if (momentum_only)
+        bg.nc = momentum_only ? (img ? 6 : 3) : 7;
+        if (img)
+            hipLaunchKernelGGL((k_p2g_gather<3, true>), dim3(blocks_for(g.nn)), dim3(kBlock), 0, s,
+                               g, bg, S.tile_epoch, epoch, N.acc);
+        else if (momentum_only)
             hipLaunchKernelGGL(k_p2g_gather<3>, dim3(blocks_for(g.nn)), dim3(kBlock), 0, s, g, bg,
                                S.tile_epoch, epoch, N.acc);
         else

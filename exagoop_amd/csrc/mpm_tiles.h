@@ -431,8 +431,8 @@ __device__ __forceinline__ void p2g_tile(const GridP &g, const PhysP &ph, const 
                                          int tile, P2gLdsOf<FORCES, GATHER, DET, NW, IMG> &L,
                                          int pos = 0, int pass = 0) {
     static_assert(NW == 4 || (NW == 8 && !DET), "8 wavefronts per tile: atomic form only");
-    static_assert(!IMG || (!FORCES && !DET && DEFER == FALLBACK),
-                  "image momentum: the re-projection, its fallback deferred");
+    static_assert(!IMG || (!FORCES && (DET || DEFER == FALLBACK)),
+                  "image momentum: the re-projection (atomic form: fallback deferred)");
     constexpr int NT = NW * 64;
     constexpr int NC = FORCES ? 7 : (IMG ? 6 : 3);
     constexpr int kSlabN = DET ? 4 * NC * kZN : NC * kTNN;
@@ -624,7 +624,14 @@ __device__ __forceinline__ void p2g_tile(const GridP &g, const PhysP &ph, const 
                         } else {
                             double x[3], vals[13];
                             fallback_vals<FORCES, GATHER>(ph, P, S, cur.slot, x, vals);
-                            deposit_global(g, x, vals, FORCES, acc, S.tile_epoch, epoch);
+                            if (IMG) {
+                                double mvi[3];
+                                for (int d = 0; d < 3; ++d)
+                                    mvi[d] = vals[0] * P.vsnap[d * cap + cur.slot];
+                                deposit_global(g, x, vals, FORCES, acc, S.tile_epoch, epoch, mvi);
+                            } else {
+                                deposit_global(g, x, vals, FORCES, acc, S.tile_epoch, epoch);
+                            }
                         }
                     }
                 } else {
@@ -795,7 +802,7 @@ __device__ __forceinline__ void p2g_tile(const GridP &g, const PhysP &ph, const 
                 for (int qx = 0; qx < 2; ++qx) {
                     if (qz >= cz2 || qy >= cy || qx >= cx) continue;
                     const int I = qx ? tx1 : tx0, J = qy ? ty1 : ty0, K = qz ? tz1 : tz0;
-                    if (IMG) {
+                    if (IMG && !DET) {
                         // a periodic image target: the second target, or the
                         // first when the block node lies outside the domain
                         const bool image = qx || qy || qz || I != T0[0] + ax - 1 ||

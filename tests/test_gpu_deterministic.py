@@ -92,15 +92,15 @@ def run_bits(s, pos, r, idata, nsteps):
     return p, rr, ii, ids, nodal, t
 
 
-BITWISE = [CASES[0], CASES[1], CASES[2], CASES[4]]
+BITWISE = [CASES[0], CASES[1], CASES[2], CASES[3], CASES[4]]
 
 
 @pytest.mark.parametrize("name,kw", BITWISE, ids=[c[0] for c in BITWISE])
 def test_deterministic_step_is_bitwise_reproducible(gpu, name, kw):
     """Two runs of 60 steps on the same input: byte-identical particles and
-    nodal state.  (MUSL with periodic faces is not among them: its
-    re-projection deposit gives the periodic images their start-of-step
-    velocities through the atomic one-thread-per-particle path.)"""
+    nodal state (MUSL with periodic faces included: the re-projection's
+    blocks carry both momentum sums, the gather takes the start-of-step one
+    from image sources)."""
     say(f"[deterministic] {name}: two runs, bitwise")
     s, pos, r, idata = scenes.random_block(31, **kw)
     a = run_bits(s, pos, r, idata, 60)
