@@ -368,11 +368,12 @@ int mpm_energies_noalloc(int64_t np, int64_t stride, const double *rdata, const 
  * each kernel of mpm_step (read back with mpm_last_step_timing);
  * MPM_OPT_KEEP_FORCES stores nodal forces for mpm_get_nodal (default 1);
  * MPM_OPT_SORTED selects the cell-sorted step (default 1) or the unsorted
- * one-thread-per-particle atomic step (0); MPM_OPT_FAST selects the
- * specialised particle kernel where it applies (default 1) or the generic
- * one (0); MPM_OPT_FUSED_DT lets the specialised particle kernel reduce the
- * next step's dt candidates on its way out (default 1) instead of a separate
- * pass over the particles at the top of the next step (0);
+ * one-thread-per-particle atomic step (0); MPM_OPT_FAST selects the tile
+ * kernels of the particle side where they apply (uniform orders, with or
+ * without level-set bodies and wall UDF tables; default 1) or the generic
+ * sorted kernel (0); MPM_OPT_FUSED_DT lets the tile particle kernel reduce
+ * the next step's dt candidates on its way out (default 1) instead of a
+ * separate pass over the particles at the top of the next step (0);
  * MPM_OPT_DETERMINISTIC (default 0) makes the sorted step's P2G free of
  * floating-point atomics, so that equal inputs give equal bits run to run:
  * each cell's particles are taken in a fixed order, the four wavefronts of a
@@ -381,9 +382,9 @@ int mpm_energies_noalloc(int64_t np, int64_t stride, const double *rdata, const 
  * order-dependent atomics, mpm_particle_grid_ops.cpp:484-578).  It applies
  * without the thermal module and external slab loads, to particles inside
  * their sorted tile (redistributed every step, or not yet outside the
- * domain), and not to MUSL's re-projection deposit with periodic faces (the
- * images' start-of-step velocities go through the atomic per-particle
- * deposit); it allocates one 7 x 343-double node block per cell tile.
+ * domain), MUSL's re-projection deposit with periodic faces included (its
+ * blocks carry the current and the start-of-step momentum, the images take
+ * the latter); it allocates one 7 x 343-double node block per cell tile.
  * MPM_OPT_TILE_WAVES: wavefronts per cell tile of the sorted step's P2G and
  * fused particle kernels, 4 or 8 (8: shorter per-tile chains for scenes with
  * fewer than one full tile per compute unit), 0 (default) to choose by the
