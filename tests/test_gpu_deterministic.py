@@ -81,6 +81,35 @@ def test_deterministic_step_matches_oracle(gpu, name, kw):
     assert not bad, bad
 
 
+def test_deterministic_musl_periodic_redistribution_interval(gpu):
+    """MUSL with periodic faces, redistribution every 5 steps and a drift
+    through the periodic x faces: particles outside their cell deposit
+    inline, their image targets with the start-of-step momentum."""
+    say("[deterministic] MUSL periodic, num_redist 5, drift: 20 steps vs oracle")
+    s, pos, r, idata = scenes.random_block(37, order=2, periodic=(1, 0, 1), speed=0.5,
+                                           scheme=MUSL)
+    s.num_redist = 5
+    r[capi.R_VEL] += 3.0
+    g, o = det_context(s, pos, r, idata), oracle.Oracle(s)
+    o.upload(pos, r, idata)
+    for c in (g, o):
+        c.init_internal_forces()
+    outside = 0
+    for _ in range(4):
+        for c in (g, o):
+            c.step(3)
+        pg, _, _, _ = g.download()
+        outside += int(np.sum((pg < np.array(s.plo)[:, None]) | (pg > np.array(s.phi)[:, None])))
+        for c in (g, o):
+            c.step(2)
+    assert outside > 0
+    pg, rg, _, _ = g.download()
+    po, ro, _, _ = o.download()
+    assert rel(pg, po) <= 1e-10, rel(pg, po)
+    bad = {f: rel(rg[f], ro[f]) for f in range(rg.shape[0]) if not rel(rg[f], ro[f]) <= 1e-10}
+    assert not bad, bad
+
+
 def run_bits(s, pos, r, idata, nsteps):
     g = det_context(s, pos, r, idata)
     g.init_internal_forces()

@@ -73,6 +73,8 @@ CASES = [
     ("neohookean_order2", dict(order=2, model=2), {}),
     ("musl_order2", dict(order=2, scheme=MUSL), {}),
     ("musl_periodic", dict(order=2, scheme=MUSL, periodic=(1, 0, 0)), {}),
+    ("musl_periodic_mixed_order", dict(order=3, n=(12, 10, 4), scheme=MUSL, periodic=(1, 0, 0)),
+     {}),
     ("partialslip_walls", dict(order=3, frac=0.9, speed=2.0,
                                bcs=[(BC_PARTIALSLIP, BC_NOSLIP), (BC_SLIP, BC_PARTIALSLIP),
                                     (BC_NOSLIP, BC_SLIP)]), {"wall_vel_lo": [0, 0.2, 0] + [0] * 6}),
