@@ -142,23 +142,29 @@ constexpr int kThPkArea = kSlots * kThPk + 2 * (kSlots / 4 - 1);
 // four cells' packets, read by four lane groups at once, in different LDS
 // bank quads.
 __device__ __forceinline__ int th_pk_base(int s) { return s * kThPk + 2 * (s >> 2); }
+// DET: one block of kZN nodes per wavefront (its cells' four z-levels),
+// summed in wavefront order at the flush, as the deterministic momentum
+// deposit does.
+template <bool DET>
 struct ThP2gLds {
-    double slab[kThNC * kTNN];
+    double slab[DET ? 4 * kThNC * kZN : kThNC * kTNN];
     double pk[4][kThPkArea];
     int cs[kCellsPerTile + 1];
     int perm[kPermLds];
     SplineTables tab;
 };
 
-template <int O, bool MT, bool SRC, bool FALLBACK>
+template <int O, bool MT, bool SRC, bool FALLBACK, bool DET>
 __global__ __launch_bounds__(kBlock) void k_th_p2g_cells(GridP g, PState P, SortState S,
                                                          const double *th, double *tn,
                                                          int *err) {
     static_assert(kBlock == 256, "four wavefronts per tile, one z-slice each");
-    __shared__ ThP2gLds L;
+    constexpr int kSlabN = DET ? 4 * kThNC * kZN : kThNC * kTNN;
+    __shared__ ThP2gLds<DET> L;
     if (FALLBACK && blockIdx.x == 0 && threadIdx.x == 0) S.ocount[(S.ogen + 1) & 1] = 0;
-    const int tile = active_tile(S, blockIdx.x);
-    if (tile < 0) return;  // uniform over the block
+    const int pos = active_pos(S, blockIdx.x);
+    if (pos < 0) return;  // uniform over the block
+    const int tile = S.tiles[1 + pos];
     const int tid = threadIdx.x;
     const int pbeg = S.cell_start[tile * kCellsPerTile];
     const int pnum = min(S.cell_start[tile * kCellsPerTile + kCellsPerTile] - pbeg, kPermLds);
@@ -170,7 +176,7 @@ __global__ __launch_bounds__(kBlock) void k_th_p2g_cells(GridP g, PState P, Sort
         for (int e = tid; e < (int)(sizeof(SplineTables) / sizeof(int)); e += kBlock)
             dst[e] = src[e];
     }
-    for (int e = tid; e < kThNC * kTNN; e += kBlock) L.slab[e] = 0.0;
+    for (int e = tid; e < kSlabN; e += kBlock) L.slab[e] = 0.0;
     __syncthreads();
     if (L.cs[kCellsPerTile] == L.cs[0]) return;  // empty tile (uniform over the block)
     const int wave = tid >> 6, lane = tid & 63;
@@ -358,12 +364,22 @@ __global__ __launch_bounds__(kBlock) void k_th_p2g_cells(GridP g, PState P, Sort
             // y = ry + ky, z = cz + kz) into the tile's block.
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
-                const int idx = ((cz + kz) * kTN + (ry + ky)) * kTN + (q + k);
 #pragma unroll
                 for (int c = 0; c < kThNC; ++c) {
                     if ((c < 3 && !MT) || (c == 3 && !SRC)) continue;
-                    __hip_atomic_fetch_add(&L.slab[c * kTNN + idx], a4[k][c], __ATOMIC_RELAXED,
-                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+                    if (DET) {
+                        // this wavefront's own block (z-level kz); lanes q and
+                        // q + 1 share nodes, so still atomic adds, in an order
+                        // the code fixes
+                        const int idx = (kz * kTN + (ry + ky)) * kTN + (q + k);
+                        __hip_atomic_fetch_add(&L.slab[wave * kThNC * kZN + c * kZN + idx],
+                                               a4[k][c], __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_WAVEFRONT);
+                    } else {
+                        const int idx = ((cz + kz) * kTN + (ry + ky)) * kTN + (q + k);
+                        __hip_atomic_fetch_add(&L.slab[c * kTNN + idx], a4[k][c],
+                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    }
                     a4[k][c] = 0.0;
                 }
             }
@@ -373,6 +389,24 @@ __global__ __launch_bounds__(kBlock) void k_th_p2g_cells(GridP g, PState P, Sort
         cur = nxt;
     }
     __syncthreads();
+    if (DET) {
+        // The wavefront blocks summed in wavefront order, the whole node
+        // block stored at list position pos and stamped with the pass (the
+        // gather pass adds it to the nodes).
+        double *blk = S.blk + (int64_t)pos * kThNC * kTNN;
+        if (tid == 0) S.tile_slot[tile] = ((int64_t)S.pass << 32) | (uint32_t)pos;
+        for (int e = tid; e < kTNN; e += kBlock) {
+            const int ax = e % kTN, ay = (e / kTN) % kTN, az = e / (kTN * kTN);
+            const int w0 = az > 3 ? az - 3 : 0, w1 = az < 3 ? az : 3;
+            for (int c = 0; c < kThNC; ++c) {
+                double v = 0.0;
+                for (int w = w0; w <= w1; ++w)
+                    v += L.slab[w * kThNC * kZN + c * kZN + ((az - w) * kTN + ay) * kTN + ax];
+                blk[c * kTNN + e] = v;
+            }
+        }
+        return;
+    }
     // Flush the node block: one global atomic per non-zero node component,
     // periodic image targets as in deposit_global.
     const int64_t nn = g.nn;
@@ -404,6 +438,27 @@ __global__ __launch_bounds__(kBlock) void k_th_p2g_cells(GridP g, PState P, Sort
                         if (v[c] != 0.0) unsafeAtomicAdd(&tn[tnc[c] * nn + node], v[c]);
                 }
     }
+}
+
+// The deterministic deposit's gather pass: node (I, J, K) adds its tile
+// blocks (blk_gather, a fixed order) to the thermal components, which hold
+// only the deferred fallback's deposits.
+template <bool MT, bool SRC>
+__global__ __launch_bounds__(kThreads) void k_th_gather(GridP g, BlkGather bg, double *tn) {
+    const int64_t node = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (node >= g.nn) return;
+    const int I = (int)(node % (g.n[0] + 1));
+    const int J = (int)((node / (g.n[0] + 1)) % (g.n[1] + 1));
+    const int K = (int)(node / ((int64_t)(g.n[0] + 1) * (g.n[1] + 1)));
+    double s[kThNC];
+    if (!blk_gather<kThNC>(g, bg, I, J, K, s)) return;
+    const int64_t nn = g.nn;
+    if (MT) {
+        tn[TN_MSP * nn + node] += s[0];
+        tn[TN_MSPT * nn + node] += s[1];
+        tn[TN_MC * nn + node] += s[2];
+    }
+    if (SRC) tn[TN_SRC * nn + node] += s[3];
 }
 
 // The deposits of the particles a FALLBACK k_th_p2g_cells launch listed, one
@@ -969,14 +1024,18 @@ void launch_th_p2g(mpm_ctx *c, bool mass_temp, bool source, bool tiled, int fall
         const dim3 gr((unsigned)((c->S.ntiles + 7) / 8 * 8)), bl(kBlock);
         const int O = g.uniform ? g.od[0] : 0;
         int *err = &c->sc->err;
+#define XMPM_THPD(o, mt, src, fb)                                                             \
+    do {                                                                                      \
+        if (det)                                                                              \
+            k_th_p2g_cells<o, mt, src, fb, true><<<gr, bl, 0, st>>>(g, c->P, c->S, c->th,     \
+                                                                     c->tn, err);             \
+        else                                                                                  \
+            k_th_p2g_cells<o, mt, src, fb, false><<<gr, bl, 0, st>>>(g, c->P, c->S, c->th,    \
+                                                                      c->tn, err);            \
+    } while (0)
 #define XMPM_THP(o, mt, src)                                                                  \
     do {                                                                                      \
-        if (fallback)                                                                         \
-            k_th_p2g_cells<o, mt, src, true><<<gr, bl, 0, st>>>(g, c->P, c->S, c->th, c->tn,  \
-                                                                 err);                        \
-        else                                                                                  \
-            k_th_p2g_cells<o, mt, src, false><<<gr, bl, 0, st>>>(g, c->P, c->S, c->th, c->tn, \
-                                                                  err);                       \
+        if (fallback) XMPM_THPD(o, mt, src, true); else XMPM_THPD(o, mt, src, false);         \
     } while (0)
 #define XMPM_THPO(o)                                                                          \
     do {                                                                                      \
@@ -985,12 +1044,17 @@ void launch_th_p2g(mpm_ctx *c, bool mass_temp, bool source, bool tiled, int fall
         else XMPM_THP(o, false, true);                                                        \
     } while (0)
         c->S.ocount = c->sc->ocount;
+        // MPM_OPT_DETERMINISTIC: tile blocks and a gather pass in a fixed
+        // order (the stamps of this pass)
+        const bool det = c->S.blk != nullptr;
+        if (det) ++c->S.pass;
         if (O == 0) XMPM_THPO(0);
         else if (O == 1) XMPM_THPO(1);
         else if (O == 2) XMPM_THPO(2);
         else XMPM_THPO(3);
 #undef XMPM_THPO
 #undef XMPM_THP
+#undef XMPM_THPD
         if (fallback) {
             const dim3 go(64);
             if (mass_temp && source)
@@ -1000,6 +1064,19 @@ void launch_th_p2g(mpm_ctx *c, bool mass_temp, bool source, bool tiled, int fall
             else
                 k_th_deposit_outside<false, true><<<go, bl, 0, st>>>(g, c->P, c->S, c->th, c->tn);
             ++c->S.ogen;
+        }
+        if (det) {
+            BlkGather bg;
+            bg.blk = c->S.blk;
+            bg.slot = c->S.tile_slot;
+            bg.pass = c->S.pass;
+            bg.nc = kThNC;
+            if (mass_temp && source)
+                k_th_gather<true, true><<<blocks(nn), kThreads, 0, st>>>(g, bg, c->tn);
+            else if (mass_temp)
+                k_th_gather<true, false><<<blocks(nn), kThreads, 0, st>>>(g, bg, c->tn);
+            else
+                k_th_gather<false, true><<<blocks(nn), kThreads, 0, st>>>(g, bg, c->tn);
         }
     } else if (np > 0) {
         if (mass_temp && source)
