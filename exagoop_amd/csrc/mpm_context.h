@@ -95,12 +95,12 @@ struct mpm_ctx {
 
 namespace xmpm {
 
-// The deterministic P2G applies to the sorted step without packets (the
-// thermal deposit included, which has a deterministic form of its own): not
-// with external slab loads, whose packets are written at the scatter's
-// (atomic) positions.
+// The deterministic P2G applies to the whole sorted step (the thermal
+// deposit has a deterministic form of its own; the tile deposit gathers
+// slab loads itself, so no packets are written at the scatter's atomic
+// positions).
 inline void det_sync(mpm_ctx *c) {
-    const bool on = c->det && !c->ph.extloads;
+    const bool on = c->det != 0;
     c->S.blk = on ? c->det_blk : nullptr;
     c->S.tile_slot = on ? c->det_slot : nullptr;
 }

@@ -377,7 +377,9 @@ int upload_impl(mpm_ctx *c, int64_t np, const double *pos, const double *rdata, 
     HIPCHK(hipMalloc(&c->S.key, cap * sizeof(int)));
     HIPCHK(hipMalloc(&c->S.rank, cap * sizeof(int)));
     HIPCHK(hipMalloc(&c->S.perm, cap * sizeof(int)));
-    HIPCHK(hipMalloc(&c->S.pkt, kPktComps * cap * sizeof(double)));
+    // No packets: the tile deposit gathers the particle fields (slab loads
+    // included) through the permutation.
+    c->S.pkt = nullptr;
     c->S.pcap = cap;
     if (c->any_per && c->s.stress_update_scheme == MPM_MUSL)
         HIPCHK(hipMalloc(&P.vsnap, 3 * cap * sizeof(double)));
@@ -865,7 +867,7 @@ int mpm_init_internal_forces(mpm_ctx *c) {
     // path) replaces the atomic one-thread-per-particle deposit when enabled.
     if (c->sorted && c->P.np > 0) {
         det_sync(c);
-        launch_sort(c->g, c->ph, c->P, c->S, nullptr, c->ph.extloads ? 1 : 0, c->stream);
+        launch_sort(c->g, c->ph, c->P, c->S, nullptr, 0, c->stream);
         launch_p2g_tiles(c->g, c->ph, c->P, c->S, c->N, (int)c->steps, 0, 1, c->sc, c->stream);
     } else {
         launch_p2g(c->g, c->ph, c->P, c->N, 1, 1, 1, nullptr, c->stream);
@@ -928,7 +930,7 @@ int mpm_step(mpm_ctx *c, int nsteps) {
         if (prof) HIPCHK(hipEventRecord(c->ev[1], st));
         if (sorted) {
             det_sync(c);
-            launch_sort(c->g, c->ph, c->P, c->S, c->Q.id, c->ph.extloads ? 1 : 0, st, fold);
+            launch_sort(c->g, c->ph, c->P, c->S, c->Q.id, 0, st, fold);
         }
         if (prof) HIPCHK(hipEventRecord(c->ev[2], st));
         if (sorted) launch_p2g_tiles(c->g, c->ph, c->P, c->S, c->N, step, 0, nr > 1, c->sc, st);

@@ -192,7 +192,7 @@ int coarse_begin(mpm_ctx *c, int nfine, int advance = 0, const PState *in = null
                   c->dt_next_valid ? 1 : 0, advance, (double)nfine);
     }
     det_sync(c);
-    launch_sort(c->g, c->ph, Pin, c->S, Pout.id, c->ph.extloads ? 1 : 0, st, fold);
+    launch_sort(c->g, c->ph, Pin, c->S, Pout.id, 0, st, fold);
     launch_p2g_tiles(c->g, c->ph, Pin, c->S, c->N, step, 0, 0, c->sc, st);
     const int *te = (c->skip_ok && !c->all_dirty) ? c->S.tile_epoch : nullptr;
     launch_grid(c->g, c->ph, c->ls, c->udf, c->N, grid_flags_full(c), 0.0, c->sc, st, te, step);

@@ -317,7 +317,10 @@ __device__ __forceinline__ void fallback_vals(const PhysP &ph, const PState &P,
         const double m = P.r[MPM_R_MASS * cap + j];
         vals[0] = m;
         for (int d = 0; d < 3; ++d) vals[1 + d] = m * P.r[(MPM_R_VEL + d) * cap + j];
-        for (int d = 0; d < 3; ++d) vals[4 + d] = m * ph.grav[d] + 0.0;
+        const bool sl = ph.extloads && x[0] > ph.slab_lo[0] && x[0] < ph.slab_hi[0] &&
+                        x[1] > ph.slab_lo[1] && x[1] < ph.slab_hi[1] && x[2] > ph.slab_lo[2] &&
+                        x[2] < ph.slab_hi[2];
+        for (int d = 0; d < 3; ++d) vals[4 + d] = m * ph.grav[d] + (sl ? ph.extforce[d] : 0.0);
         const double V = P.r[MPM_R_VOLUME * cap + j];
         for (int e = 0; e < 6; ++e) vals[7 + e] = -V * P.r[(MPM_R_STRESS + e) * cap + j];
     } else if (FORCES) {
@@ -653,14 +656,24 @@ __device__ __forceinline__ void p2g_tile(const GridP &g, const PhysP &ph, const 
             }
             if (FORCES && GATHER) {
                 // packet values from the gathered fields, with the arithmetic
-                // of scatter_one: m, m v (part 0); m g (part 1); -V sigma
-                // (parts 2, 3)
+                // of scatter_one: m, m v (part 0); m g plus the slab load
+                // (part 1; the slot's position from its part 0..2 lanes);
+                // -V sigma (parts 2, 3)
+                bool sl = false;
+                if (ph.extloads) {  // uniform over the wavefront
+                    const double px = __shfl(cur.xd, s_slot, 64);
+                    const double py = __shfl(cur.xd, s_slot + 16, 64);
+                    const double pz = __shfl(cur.xd, s_slot + 32, 64);
+                    sl = px > ph.slab_lo[0] && px < ph.slab_hi[0] && py > ph.slab_lo[1] &&
+                         py < ph.slab_hi[1] && pz > ph.slab_lo[2] && pz < ph.slab_hi[2];
+                }
                 double pv[4];
                 if (s_part == 0) {
                     pv[0] = cur.val[0];
                     for (int d = 0; d < 3; ++d) pv[1 + d] = cur.val[0] * cur.val[1 + d];
                 } else if (s_part == 1) {
-                    for (int d = 0; d < 3; ++d) pv[d] = cur.val[0] * ph.grav[d] + 0.0;
+                    for (int d = 0; d < 3; ++d)
+                        pv[d] = cur.val[0] * ph.grav[d] + (sl ? ph.extforce[d] : 0.0);
                     pv[3] = 0.0;
                 } else {
                     for (int e = 0; e < 3; ++e) pv[e] = -cur.val[0] * cur.val[1 + e];

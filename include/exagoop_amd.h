@@ -380,7 +380,7 @@ int mpm_energies_noalloc(int64_t np, int64_t stride, const double *rdata, const 
  * tile sum private slabs in a fixed order, and each node adds the node blocks
  * of its tiles in a fixed order (an extra pass; the reference deposits with
  * order-dependent atomics, mpm_particle_grid_ops.cpp:484-578).  It applies
- * (thermal deposit included) without external slab loads, to particles inside
+ * (thermal deposit and external slab loads included) to particles inside
  * their sorted tile (redistributed every step, or not yet outside the
  * domain), MUSL's re-projection deposit with periodic faces included (its
  * blocks carry the current and the start-of-step momentum, the images take
