@@ -140,14 +140,14 @@ __global__ __launch_bounds__(kBlock) void k_cell_order(SortState S) {
 // counter the next listing launch uses.  IMG: MUSL's re-projection with
 // periodic images carrying the top-of-step velocity (P.vsnap).
 template <int O, bool FORCES, bool FALLBACK, bool GATHER, bool DET, int NW, bool DEFER,
-          bool IMG = false>
+          bool IMG = false, bool SLAB = false>
 __global__ __launch_bounds__(NW * 64) void k_p2g_cells(GridP g, PhysP ph, PState P, SortState S,
                                                        double *acc, int epoch, Scalars *sc) {
     __shared__ P2gLdsOf<FORCES, GATHER, DET, NW, IMG> lds;
     if (DEFER && blockIdx.x == 0 && threadIdx.x == 0) S.ocount[(S.ogen + 1) & 1] = 0;
     const int pos = active_pos(S, blockIdx.x);
     if (pos < 0) return;  // uniform over the block
-    p2g_tile<O, FORCES, FALLBACK, GATHER, DET, NW, DEFER, IMG>(
+    p2g_tile<O, FORCES, FALLBACK, GATHER, DET, NW, DEFER, IMG, SLAB>(
         g, ph, P, S, acc, epoch, &sc->err, S.tiles[1 + pos], lds, pos, S.pass);
 }
 
@@ -261,18 +261,28 @@ void launch_p2g_tiles(const GridP &g, const PhysP &ph, const PState &P, SortStat
     // k_deposit_outside (DEFER = fallback); the deterministic forms deposit
     // them inline.
     S.ocount = sc->ocount;
-#define XMPM_PGW(o, f, fb, ga, w)                                                             \
-    hipLaunchKernelGGL((k_p2g_cells<o, f, fb, ga, false, w, fb>), gr, (w) == 8 ? bl8 : bl, 0, \
-                       s, g, ph, P, S, N.acc, epoch, sc)
+    // Slab loads: the gathering force forms with the slab term (SLAB, a
+    // form of its own so the plain one carries none of its code).
+    const bool slab = ph.extloads && !momentum_only;
+#define XMPM_PGW(o, f, fb, ga, w, sl)                                                         \
+    hipLaunchKernelGGL((k_p2g_cells<o, f, fb, ga, false, w, fb, false, sl>), gr,              \
+                       (w) == 8 ? bl8 : bl, 0, s, g, ph, P, S, N.acc, epoch, sc)
 #define XMPM_PG(o, f, fb)                                                                     \
     do {                                                                                      \
-        if (det)                                                                              \
+        if (det && slab)                                                                      \
+            hipLaunchKernelGGL((k_p2g_cells<o, f, fb, f, true, 4, false, false, f>), gr, bl,  \
+                               0, s, g, ph, P, S, N.acc, epoch, sc);                          \
+        else if (det)                                                                         \
             hipLaunchKernelGGL((k_p2g_cells<o, f, fb, f, true, 4, false>), gr, bl, 0, s, g, ph, \
                                P, S, N.acc, epoch, sc);                                       \
-        else if (f && !S.pkt_valid) {                                                         \
-            if (few) XMPM_PGW(o, f, fb, true, 8); else XMPM_PGW(o, f, fb, true, 4);           \
+        else if (f && !S.pkt_valid && slab) {                                                 \
+            if (few) XMPM_PGW(o, f, fb, true, 8, f); else XMPM_PGW(o, f, fb, true, 4, f);     \
+        } else if (f && !S.pkt_valid) {                                                       \
+            if (few) XMPM_PGW(o, f, fb, true, 8, false);                                      \
+            else XMPM_PGW(o, f, fb, true, 4, false);                                          \
         } else {                                                                              \
-            if (few) XMPM_PGW(o, f, fb, false, 8); else XMPM_PGW(o, f, fb, false, 4);         \
+            if (few) XMPM_PGW(o, f, fb, false, 8, false);                                     \
+            else XMPM_PGW(o, f, fb, false, 4, false);                                         \
         }                                                                                     \
     } while (0)
 #define XMPM_PGO(f, fb)                                                                       \

@@ -422,13 +422,15 @@ struct PartLds {
 // pass in S.tile_slot; no atomics (but those of the FALLBACK global path).
 // NW wavefronts per tile: 4 (one z-slice each) or 8 (a z-slice's first or
 // second pair of cell rows each, half the chain of rounds per wavefront).
+// SLAB: external slab loads in the gathering force deposit (the slot's
+// position shared by its staging lanes).
 // IMG (momentum only, MUSL's re-projection with periodic faces): a deposit
 // that lands on a periodic image node carries the velocity of the particle
 // at the top of the step (P.vsnap), as the reference's neighbour particles
 // do; the tile accumulates both momentum sums and the flush picks one per
 // target.
 template <int O, bool FORCES, bool FALLBACK, bool GATHER = false, bool DET = false, int NW = 4,
-          bool DEFER = false, bool IMG = false>
+          bool DEFER = false, bool IMG = false, bool SLAB = false>
 __device__ __forceinline__ void p2g_tile(const GridP &g, const PhysP &ph, const PState &P,
                                          const SortState &S, double *acc, int epoch, int *err,
                                          int tile, P2gLdsOf<FORCES, GATHER, DET, NW, IMG> &L,
@@ -436,6 +438,7 @@ __device__ __forceinline__ void p2g_tile(const GridP &g, const PhysP &ph, const 
     static_assert(NW == 4 || (NW == 8 && !DET), "8 wavefronts per tile: atomic form only");
     static_assert(!IMG || (!FORCES && (DET || DEFER == FALLBACK)),
                   "image momentum: the re-projection (atomic form: fallback deferred)");
+    static_assert(!SLAB || (FORCES && GATHER), "slab loads: the gathering force deposit");
     constexpr int NT = NW * 64;
     constexpr int NC = FORCES ? 7 : (IMG ? 6 : 3);
     constexpr int kSlabN = DET ? 4 * NC * kZN : NC * kTNN;
@@ -660,7 +663,7 @@ __device__ __forceinline__ void p2g_tile(const GridP &g, const PhysP &ph, const 
                 // (part 1; the slot's position from its part 0..2 lanes);
                 // -V sigma (parts 2, 3)
                 bool sl = false;
-                if (ph.extloads) {  // uniform over the wavefront
+                if (SLAB) {
                     const double px = __shfl(cur.xd, s_slot, 64);
                     const double py = __shfl(cur.xd, s_slot + 16, 64);
                     const double pz = __shfl(cur.xd, s_slot + 32, 64);
