@@ -365,9 +365,10 @@ struct StageRaw {
 };
 
 // deposit_onto_grid_momentum (mpm_particle_grid_ops.cpp:333-587) over one
-// cell tile.  FORCES: mass, momentum and forces (P2G(1,1,1)) from the sorted
-// packets the sort wrote; otherwise momentum only (the MUSL re-projection
-// P2G(0,1,0)) from the particle state through perm.
+// cell tile.  FORCES: mass, momentum and forces (P2G(1,1,1)), gathered from
+// the particle state through perm (GATHER, the one-scene step) or read from
+// the sorted packets the sort wrote (the batched ensemble); otherwise
+// momentum only (the MUSL re-projection P2G(0,1,0)) through perm.
 //
 // Wavefront w takes the 16 cells of z-slice w of the tile, one row of four
 // cells (x = 0..3) per value of cy.  Each row's particles are staged in rounds
